@@ -1,0 +1,240 @@
+"""ctypes front end of the CPU oracle (``oracle/vd_oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py``, never by the ``vorothree_amd`` package. The oracle restates
+``Tessellation<3, Cell3DFaces, Algorithm3DGrid>`` of mdt-re/vorothree on the CPU
+(``src/tessellation.rs``, ``src/algorithm/algo_3d_grid.rs``, ``src/cell/cell_3d_faces.rs``,
+``src/wall/wall_3d.rs``); every C function there cites the reference line it follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    sig = {
+        "or_tess_new": (_vp, [_dp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+        "or_tess_free": (None, [_vp]),
+        "or_tess_add_wall_plane": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
+        "or_tess_add_wall_sphere": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_double]),
+        "or_tess_set_generators": (None, [_vp, _dp, ctypes.c_size_t]),
+        "or_tess_count_generators": (ctypes.c_size_t, [_vp]),
+        "or_tess_get_generators": (None, [_vp, _dp]),
+        "or_tess_get_bins": (None, [_vp, _vp, _vp, _vp]),
+        "or_tess_bin_index": (ctypes.c_size_t, [_vp, _dp]),
+        "or_tess_shell_order": (ctypes.c_size_t, [_vp, _vp, _vp, ctypes.c_size_t]),
+        "or_tess_calculate": (None, [_vp, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]),
+        "or_tess_count_cells": (ctypes.c_size_t, [_vp]),
+        "or_tess_relax": (None, [_vp]),
+        "or_cell_sizes": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp]),
+        "or_cell_get": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]),
+        "or_tess_reduced": (None, [_vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]),
+        "or_cell_new": (_vp, [_dp, _dp]),
+        "or_cell_free": (None, [_vp]),
+        "or_cell_clip": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int32, _vp, _dp]),
+        "or_cell_volume": (ctypes.c_double, [_vp]),
+        "or_cell_centroid": (None, [_vp, _dp]),
+        "or_cell_counts": (None, [_vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _d3(x):
+    a = (ctypes.c_double * 3)(*[float(v) for v in x])
+    return a
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleCell:
+    """A cell as returned by the oracle: the four arrays of ``Cell3DFaces`` (cell_3d_faces.rs:23-34)."""
+
+    def __init__(self, vertices, face_counts, face_indices, face_neighbors):
+        self.vertices = vertices
+        self.face_counts = face_counts
+        self.face_indices = face_indices
+        self.face_neighbors = face_neighbors
+
+    def volume(self) -> float:
+        return cell_volume(self.vertices, self.face_counts, self.face_indices)
+
+
+def cell_volume(v, fc, fi) -> float:
+    """Python restatement of Cell3DFaces::volume (cell_3d_faces.rs:106-143), left fold as written."""
+    vol = 0.0
+    off = 0
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    for cnt in fc:
+        cnt = int(cnt)
+        if cnt >= 3:
+            a = v[fi[off]]
+            for i in range(1, cnt - 1):
+                b = v[fi[off + i]]
+                c = v[fi[off + i + 1]]
+                vol += (float(a[0]) * (float(b[1]) * float(c[2]) - float(b[2]) * float(c[1]))
+                        + float(a[1]) * (float(b[2]) * float(c[0]) - float(b[0]) * float(c[2]))
+                        + float(a[2]) * (float(b[0]) * float(c[1]) - float(b[1]) * float(c[0])))
+        off += cnt
+    return abs(vol / 6.0)
+
+
+class OracleTess:
+    """CPU restatement of ``Tessellation::<3, Cell3DFaces, Algorithm3DGrid>``."""
+
+    def __init__(self, lo, hi, nx, ny, nz):
+        self._lib = _load()
+        self.lo = tuple(float(x) for x in lo)
+        self.hi = tuple(float(x) for x in hi)
+        self.n = (int(nx), int(ny), int(nz))
+        self._h = self._lib.or_tess_new(_d3(lo), _d3(hi), int(nx), int(ny), int(nz))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.or_tess_free(h)
+            self._h = None
+
+    def add_wall_plane(self, wid, point, normal):
+        if self._lib.or_tess_add_wall_plane(self._h, int(wid), _d3(point), _d3(normal)) != 0:
+            raise ValueError("Wall ID must be <= -1000")
+
+    def add_wall_sphere(self, wid, center, radius):
+        if self._lib.or_tess_add_wall_sphere(self._h, int(wid), _d3(center), float(radius)) != 0:
+            raise ValueError("Wall ID must be <= -1000")
+
+    def set_generators(self, xyz):
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))
+        self._lib.or_tess_set_generators(self._h, a.ctypes.data_as(_dp), a.size // 3)
+
+    def count_generators(self) -> int:
+        return int(self._lib.or_tess_count_generators(self._h))
+
+    def generators(self) -> np.ndarray:
+        out = np.empty(3 * self.count_generators(), dtype=np.float64)
+        self._lib.or_tess_get_generators(self._h, out.ctypes.data_as(_dp))
+        return out
+
+    def bins(self):
+        n = self.count_generators()
+        nb = self.n[0] * self.n[1] * self.n[2]
+        bs = np.empty(nb + 1, dtype=np.uint32)
+        bi = np.empty(n, dtype=np.uint32)
+        gb = np.empty(n, dtype=np.uint32)
+        self._lib.or_tess_get_bins(self._h, _ptr(bs), _ptr(bi), _ptr(gb))
+        return bs, bi, gb
+
+    def bin_index(self, q) -> int:
+        return int(self._lib.or_tess_bin_index(self._h, _d3(q)))
+
+    def shell_order(self, cap=None):
+        total = int(self._lib.or_tess_shell_order(self._h, None, None, 0))
+        m = total if cap is None else min(cap, total)
+        dxyz = np.empty((m, 3), dtype=np.int32)
+        d2 = np.empty(m, dtype=np.float64)
+        self._lib.or_tess_shell_order(self._h, _ptr(dxyz), _ptr(d2), m)
+        return dxyz, d2
+
+    def calculate(self, subset=None, threads=1, brute=False):
+        if subset is None:
+            self._lib.or_tess_calculate(self._h, None, 0, int(threads), int(bool(brute)))
+        else:
+            s = np.ascontiguousarray(np.asarray(subset, dtype=np.int64))
+            self._lib.or_tess_calculate(self._h, _ptr(s), s.size, int(threads), int(bool(brute)))
+
+    def relax(self):
+        self._lib.or_tess_relax(self._h)
+
+    def count_cells(self) -> int:
+        return int(self._lib.or_tess_count_cells(self._h))
+
+    def get_cell(self, i):
+        sizes = np.zeros(3, dtype=np.uint64)
+        if self._lib.or_cell_sizes(self._h, int(i), _ptr(sizes)) != 0:
+            return None
+        nv, nf, ni = (int(x) for x in sizes)
+        v = np.empty(3 * nv, dtype=np.float64)
+        fc = np.empty(nf, dtype=np.uint32)
+        fi = np.empty(ni, dtype=np.uint32)
+        fn = np.empty(nf, dtype=np.int32)
+        self._lib.or_cell_get(self._h, int(i), _ptr(v), _ptr(fc), _ptr(fi), _ptr(fn))
+        return OracleCell(v, fc, fi, fn)
+
+    def reduced(self, idx=None):
+        """(volumes, centroids[n,3], face_counts, work[n,6]) for the listed cells (or all)."""
+        if idx is None:
+            m = self.count_cells()
+            ip = None
+        else:
+            idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+            m = idx.size
+            ip = _ptr(idx)
+        vol = np.empty(m, dtype=np.float64)
+        cen = np.empty((m, 3), dtype=np.float64)
+        nf = np.empty(m, dtype=np.uint32)
+        work = np.empty((m, 6), dtype=np.uint64)
+        self._lib.or_tess_reduced(self._h, ip, m, _ptr(vol), _ptr(cen), _ptr(nf), _ptr(work))
+        return vol, cen, nf, work
+
+
+class OracleStandaloneCell:
+    """A lone ``Cell3DFaces`` for the reference's cell unit tests (cell_3d_faces.rs:571-594)."""
+
+    def __init__(self, lo, hi):
+        self._lib = _load()
+        self._h = self._lib.or_cell_new(_d3(lo), _d3(hi))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.or_cell_free(h)
+            self._h = None
+
+    def clip(self, point, normal, nid, generator=None):
+        r = ctypes.c_double(0.0)
+        g = _d3(generator) if generator is not None else None
+        m = self._lib.or_cell_clip(self._h, _d3(point), _d3(normal), int(nid),
+                                   ctypes.cast(g, ctypes.c_void_p) if g is not None else None,
+                                   ctypes.byref(r))
+        return bool(m), r.value
+
+    def volume(self) -> float:
+        return float(self._lib.or_cell_volume(self._h))
+
+    def centroid(self):
+        out = (ctypes.c_double * 3)()
+        self._lib.or_cell_centroid(self._h, out)
+        return [out[0], out[1], out[2]]
+
+    def counts(self):
+        s = np.zeros(3, dtype=np.uint64)
+        self._lib.or_cell_counts(self._h, _ptr(s))
+        return tuple(int(x) for x in s)

@@ -1,0 +1,707 @@
+// CPU restatement of voronoid's Tessellation<3, Cell3DFaces, Algorithm3DGrid>::calculate() path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker that the HIP library is compared against. It
+// is loaded by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by nothing
+// that ships. It restates the reference algorithm step by step; every function cites the reference
+// file:line it follows (paths relative to the mdt-re/vorothree repository root).
+//
+// Numerics policy: compiled with -ffp-contract=off and without -ffast-math, so every expression is
+// evaluated as written, left to right, with IEEE-correct division and sqrt -- the same semantics as
+// the Rust reference, which never contracts a*b+c.
+//
+// One documented deviation from the reference: the bin shell order is sorted STABLY by its
+// min-distance key (ties keep the z-outer / y / x-inner generation order). The reference uses
+// sort_unstable_by (src/algorithm/algo_3d_grid.rs:76), whose tie order is that of Rust's ipnsort.
+//
+// Pinned by: the Voro++ fixtures (vertex counts and positions, face counts), closed forms (box,
+// lattices, plane-clipped boxes), a brute-force all-pairs clipping on small inputs, topology
+// invariants (reciprocity, Euler characteristic), and the grid unit-test values of the reference.
+// See tests/test_oracle_*.py.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <utility>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Algorithmic-work counters (flop accounting used as the roofline unit). Rules: each + - * / sqrt
+// counts 1; compares, clamps and max() count 0.
+// ---------------------------------------------------------------------------------------------
+struct Work {
+    uint64_t flops = 0;
+    uint64_t candidates = 0;   // visitor calls (tessellation.rs:446)
+    uint64_t bins = 0;         // in-range bins whose distance was evaluated (algo_3d_grid.rs:182)
+    uint64_t clips = 0;        // clip calls after the distance test
+    uint64_t cuts = 0;         // clips that modified the cell
+    uint64_t entries = 0;      // shell entries examined
+    void add(const Work &o) {
+        flops += o.flops; candidates += o.candidates; bins += o.bins;
+        clips += o.clips; cuts += o.cuts; entries += o.entries;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Polyhedral cell. Restates src/cell/cell_3d_faces.rs:23-74 (layout and box initialisation).
+// ---------------------------------------------------------------------------------------------
+struct Poly {
+    std::vector<double> v;        // xyz per vertex
+    std::vector<uint8_t> fcnt;    // vertex count per face (u8, as the reference)
+    std::vector<uint16_t> fidx;   // flattened face vertex indices (u16, as the reference)
+    std::vector<int32_t> fnb;     // neighbour id per face; negative = box side or wall
+    size_t nv() const { return v.size() / 3; }
+    bool empty() const { return v.empty(); }
+};
+
+inline int32_t box_side(int axis, bool is_max) {  // src/bounds/mod.rs:21-23
+    return -1 - (axis * 2 + (is_max ? 1 : 0));
+}
+
+void poly_box(Poly &c, const double lo[3], const double hi[3]) {  // cell_3d_faces.rs:37-74
+    const double xs[2] = {lo[0], hi[0]}, ys[2] = {lo[1], hi[1]}, zs[2] = {lo[2], hi[2]};
+    // vertex k: x from bit pattern of the reference's listing (0:(lo,lo,lo) 1:(hi,lo,lo)
+    // 2:(hi,hi,lo) 3:(lo,hi,lo) 4..7 the same at z=hi)
+    static const int xb[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+    static const int yb[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+    c.v.resize(24);
+    for (int k = 0; k < 8; ++k) {
+        c.v[3 * k + 0] = xs[xb[k]];
+        c.v[3 * k + 1] = ys[yb[k]];
+        c.v[3 * k + 2] = zs[k >> 2];
+    }
+    c.fcnt.assign(6, 4);
+    static const uint16_t faces[24] = {3, 2, 1, 0, 4, 5, 6, 7, 0, 1, 5, 4,
+                                       2, 3, 7, 6, 0, 4, 7, 3, 1, 2, 6, 5};
+    c.fidx.assign(faces, faces + 24);
+    c.fnb = {box_side(2, false), box_side(2, true), box_side(1, false),
+             box_side(1, true),  box_side(0, false), box_side(0, true)};
+}
+
+struct ClipScratch {  // per-thread buffers, the role of Cell3DFacesScratch (cell_3d_faces.rs:6-20)
+    Poly out;
+    std::vector<double> d;
+    std::vector<uint8_t> isx;
+    std::vector<int32_t> remap;
+    std::vector<std::pair<uint32_t, uint16_t>> xmap;
+    std::vector<std::pair<uint16_t, uint16_t>> seg;
+    std::vector<uint16_t> fbuf, lid, lmap;
+};
+
+constexpr double kTol = 1e-9;
+
+// Half-space clip. Restates clip_with_scratch, src/cell/cell_3d_faces.rs:288-503.
+// Keeps the side (x - p).n <= 1e-9. Returns {modified, max |v-g|^2 over the new vertices}; the
+// radius is 0.0 when no generator is given.
+std::pair<bool, double> poly_clip(Poly &c, const double p[3], const double n[3], int32_t nid,
+                                  ClipScratch &s, const double *g, Work *w) {
+    const size_t nv = c.nv();
+    s.d.resize(nv);
+    bool all_in = true, all_out = true;
+    for (size_t i = 0; i < nv; ++i) {  // :303-315
+        const double *q = &c.v[3 * i];
+        double di = (q[0] - p[0]) * n[0] + (q[1] - p[1]) * n[1] + (q[2] - p[2]) * n[2];
+        s.d[i] = di;
+        if (di > kTol) all_in = false;
+        else if (di < -kTol) all_out = false;
+    }
+    if (w) w->flops += 8 * nv;
+    if (all_in) return {false, 0.0};  // :317-319
+    if (all_out) {                    // :320-326
+        c.v.clear(); c.fcnt.clear(); c.fidx.clear(); c.fnb.clear();
+        return {true, 0.0};
+    }
+
+    Poly &o = s.out;
+    o.v.clear(); o.fcnt.clear(); o.fidx.clear(); o.fnb.clear();
+    s.isx.clear();
+    s.remap.assign(nv, -1);
+    s.xmap.clear();
+    s.seg.clear();
+    double max_d2 = 0.0;
+
+    for (size_t i = 0; i < nv; ++i) {  // keep inside vertices, :345-362
+        if (s.d[i] <= kTol) {
+            const double *q = &c.v[3 * i];
+            s.remap[i] = (int32_t)(o.v.size() / 3);
+            o.v.insert(o.v.end(), q, q + 3);
+            s.isx.push_back(0);
+            if (g) {
+                double dx = q[0] - g[0], dy = q[1] - g[1], dz = q[2] - g[2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > max_d2) max_d2 = d2;
+                if (w) w->flops += 8;
+            }
+        }
+    }
+
+    // Intersection vertex on edge (a -> b) in traversal direction, deduplicated by the unordered
+    // edge key (:391-417 and :422-448).
+    auto cross = [&](size_t a, size_t b) -> uint16_t {
+        uint32_t key = a < b ? ((uint32_t)a << 16 | (uint32_t)b) : ((uint32_t)b << 16 | (uint32_t)a);
+        for (auto &kv : s.xmap)
+            if (kv.first == key) return kv.second;
+        double t = s.d[a] / (s.d[a] - s.d[b]);
+        if (t < 0.0) t = 0.0;
+        if (t > 1.0) t = 1.0;
+        const double *A = &c.v[3 * a], *B = &c.v[3 * b];
+        double x = A[0] + t * (B[0] - A[0]);
+        double y = A[1] + t * (B[1] - A[1]);
+        double z = A[2] + t * (B[2] - A[2]);
+        uint16_t id = (uint16_t)(o.v.size() / 3);
+        o.v.push_back(x); o.v.push_back(y); o.v.push_back(z);
+        if (w) w->flops += 11;
+        if (g) {
+            double dx = x - g[0], dy = y - g[1], dz = z - g[2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > max_d2) max_d2 = d2;
+            if (w) w->flops += 8;
+        }
+        s.isx.push_back(1);
+        s.xmap.push_back({key, id});
+        return id;
+    };
+
+    size_t off = 0;
+    for (size_t f = 0; f < c.fcnt.size(); ++f) {  // face walk, :370-468
+        const size_t cnt = c.fcnt[f];
+        const uint16_t *fi = &c.fidx[off];
+        off += cnt;
+        s.fbuf.clear();
+        for (size_t i = 0; i < cnt; ++i) {
+            size_t a = fi[i], b = fi[(i + 1) % cnt];
+            bool a_in = s.d[a] <= kTol, b_in = s.d[b] <= kTol;
+            if (a_in) {
+                if (b_in) {
+                    if (s.remap[b] >= 0) s.fbuf.push_back((uint16_t)s.remap[b]);
+                } else {
+                    s.fbuf.push_back(cross(a, b));
+                }
+            } else if (b_in) {
+                s.fbuf.push_back(cross(a, b));
+                if (s.remap[b] >= 0) s.fbuf.push_back((uint16_t)s.remap[b]);
+            }
+        }
+        const size_t m = s.fbuf.size();
+        if (m >= 3) {  // :454-467
+            o.fcnt.push_back((uint8_t)m);
+            o.fnb.push_back(c.fnb[f]);
+            for (size_t i = 0; i < m; ++i) {
+                uint16_t u = s.fbuf[i], v = s.fbuf[(i + 1) % m];
+                if (s.isx[u] && s.isx[v]) s.seg.push_back({v, u});
+            }
+            o.fidx.insert(o.fidx.end(), s.fbuf.begin(), s.fbuf.end());
+        }
+    }
+
+    if (!s.seg.empty()) {  // lid polygon, :471-495
+        const uint16_t NONE = std::numeric_limits<uint16_t>::max();
+        s.lmap.assign(o.v.size() / 3, NONE);
+        for (auto &sg : s.seg) s.lmap[sg.first] = sg.second;
+        s.lid.clear();
+        uint16_t start = s.seg[0].first, cur = s.seg[0].second;
+        s.lid.push_back(start);
+        while (cur != start && s.lid.size() <= s.seg.size()) {
+            s.lid.push_back(cur);
+            cur = s.lmap[cur];
+            if (cur == NONE) break;
+        }
+        if (s.lid.size() >= 3) {
+            o.fcnt.push_back((uint8_t)s.lid.size());
+            o.fidx.insert(o.fidx.end(), s.lid.begin(), s.lid.end());
+            o.fnb.push_back(nid);
+        }
+    }
+    std::swap(c.v, o.v);
+    std::swap(c.fcnt, o.fcnt);
+    std::swap(c.fidx, o.fidx);
+    std::swap(c.fnb, o.fnb);
+    return {true, max_d2};
+}
+
+double poly_max_radius_sq(const Poly &c, const double g[3]) {  // cell_3d_faces.rs:271-286
+    double m = 0.0;
+    for (size_t k = 0; k < c.nv(); ++k) {
+        double dx = c.v[3 * k] - g[0], dy = c.v[3 * k + 1] - g[1], dz = c.v[3 * k + 2] - g[2];
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > m) m = d2;
+    }
+    return m;
+}
+
+// Fan triangulation over faces with >= 3 vertices, pivot at the face's first vertex. Calls
+// fn(det, v0, v1, v2) per triangle in face order. Shared traversal of volume (:106-143) and
+// centroid (:145-205).
+template <class F>
+void poly_fan(const Poly &c, F &&fn) {
+    size_t off = 0;
+    for (size_t f = 0; f < c.fcnt.size(); ++f) {
+        const size_t cnt = c.fcnt[f];
+        if (cnt >= 3) {
+            const double *a = &c.v[3 * c.fidx[off]];
+            for (size_t i = 1; i + 1 < cnt; ++i) {
+                const double *b = &c.v[3 * c.fidx[off + i]];
+                const double *q = &c.v[3 * c.fidx[off + i + 1]];
+                double cx = b[1] * q[2] - b[2] * q[1];
+                double cy = b[2] * q[0] - b[0] * q[2];
+                double cz = b[0] * q[1] - b[1] * q[0];
+                double det = a[0] * cx + a[1] * cy + a[2] * cz;
+                fn(det, a, b, q);
+            }
+        }
+        off += cnt;
+    }
+}
+
+double poly_volume(const Poly &c) {  // cell_3d_faces.rs:106-143
+    double vol = 0.0;
+    poly_fan(c, [&](double det, const double *, const double *, const double *) { vol += det; });
+    return std::fabs(vol / 6.0);
+}
+
+void poly_centroid(const Poly &c, double out[3]) {  // cell_3d_faces.rs:145-205
+    double sx = 0.0, sy = 0.0, sz = 0.0, tv = 0.0;
+    poly_fan(c, [&](double det, const double *a, const double *b, const double *q) {
+        tv += det;
+        sx += det * (a[0] + b[0] + q[0]);
+        sy += det * (a[1] + b[1] + q[1]);
+        sz += det * (a[2] + b[2] + q[2]);
+    });
+    if (std::fabs(tv) < 1e-9) { out[0] = out[1] = out[2] = 0.0; return; }
+    double f = 1.0 / (4.0 * tv);
+    out[0] = sx * f; out[1] = sy * f; out[2] = sz * f;
+}
+
+uint64_t poly_triangles(const Poly &c) {
+    uint64_t t = 0;
+    for (uint8_t k : c.fcnt) if (k >= 3) t += k - 2;
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Walls. Restates src/wall/mod.rs:15-49 and src/wall/wall_3d.rs:8-101 (Plane and Sphere only).
+// ---------------------------------------------------------------------------------------------
+struct Wall {
+    int32_t id;
+    int kind;       // 0 plane, 1 sphere
+    double a[3];    // plane point / sphere centre
+    double b[3];    // plane unit normal (pointing into the valid region)
+    double r;       // sphere radius
+};
+
+bool wall_contains(const Wall &w, const double q[3]) {
+    double dx = q[0] - w.a[0], dy = q[1] - w.a[1], dz = q[2] - w.a[2];
+    if (w.kind == 0) return (dx * w.b[0] + dy * w.b[1] + dz * w.b[2]) >= 0.0;  // wall_3d.rs:31-36
+    return (dx * dx + dy * dy + dz * dz) <= w.r * w.r;                          // :73-78
+}
+
+// Emits the cut plane (point, outward normal). Returns false when no plane is emitted.
+bool wall_cut(const Wall &w, const double g[3], double p[3], double n[3], Work *wk) {
+    if (w.kind == 0) {  // wall_3d.rs:38-42
+        for (int k = 0; k < 3; ++k) { p[k] = w.a[k]; n[k] = -w.b[k]; }
+        return true;
+    }
+    double dx = g[0] - w.a[0], dy = g[1] - w.a[1], dz = g[2] - w.a[2];  // :80-100
+    double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist == 0.0) return false;
+    double sc = w.r / dist;
+    p[0] = w.a[0] + dx * sc; p[1] = w.a[1] + dy * sc; p[2] = w.a[2] + dz * sc;
+    n[0] = dx / dist; n[1] = dy / dist; n[2] = dz / dist;
+    if (wk) wk->flops += 19;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Uniform grid. Restates src/algorithm/algo_3d_grid.rs:47-202.
+// ---------------------------------------------------------------------------------------------
+struct ShellEntry { int32_t dx, dy, dz; double d2; };
+
+struct Grid {
+    int64_t n[3];
+    double s[3], lim[3], lo[3];
+    std::vector<uint32_t> bin_start;  // CSR of the bin lists (each in ascending generator index)
+    std::vector<uint32_t> bin_items;
+    std::vector<uint32_t> gen_bin;
+    std::vector<ShellEntry> order;
+};
+
+void grid_new(Grid &G, int64_t nx, int64_t ny, int64_t nz, const double lo[3], const double hi[3]) {
+    G.n[0] = nx; G.n[1] = ny; G.n[2] = nz;
+    double cs[3];
+    for (int k = 0; k < 3; ++k) {
+        G.s[k] = (double)G.n[k] / (hi[k] - lo[k]);   // :48-50
+        cs[k] = 1.0 / G.s[k];                        // :52-54
+        G.lim[k] = (double)G.n[k] - 1e-5;            // :85-87
+        G.lo[k] = lo[k];
+    }
+    auto axis = [](int64_t d, double c) -> double {  // :61-66
+        if (d > 0) return (double)(d - 1) * c;
+        if (d < 0) return (double)(-d - 1) * c;
+        return 0.0;
+    };
+    G.order.clear();
+    G.order.reserve((size_t)(2 * nx + 1) * (2 * ny + 1) * (2 * nz + 1));
+    for (int64_t z = -nz; z <= nz; ++z)        // :68-75, z outer, x inner
+        for (int64_t y = -ny; y <= ny; ++y)
+            for (int64_t x = -nx; x <= nx; ++x) {
+                double mx = axis(x, cs[0]), my = axis(y, cs[1]), mz = axis(z, cs[2]);
+                G.order.push_back({(int32_t)x, (int32_t)y, (int32_t)z, mx * mx + my * my + mz * mz});
+            }
+    // :76 sorts unstably; this restatement sorts stably (ties keep generation order).
+    std::stable_sort(G.order.begin(), G.order.end(),
+                     [](const ShellEntry &a, const ShellEntry &b) { return a.d2 < b.d2; });
+}
+
+inline size_t axis_index(double x, double lo, double s, double lim) {  // :102-104
+    double v = (x - lo) * s;
+    if (v < 0.0) v = 0.0;       // f64::clamp; NaN passes through
+    if (v > lim) v = lim;
+    if (!(v == v)) return 0;    // `NaN as usize` is 0
+    return (size_t)v;           // truncation
+}
+
+size_t grid_bin_index(const Grid &G, const double q[3]) {  // :98-107
+    size_t ix = axis_index(q[0], G.lo[0], G.s[0], G.lim[0]);
+    size_t iy = axis_index(q[1], G.lo[1], G.s[1], G.lim[1]);
+    size_t iz = axis_index(q[2], G.lo[2], G.s[2], G.lim[2]);
+    return ix + iy * (size_t)G.n[0] + iz * (size_t)G.n[0] * (size_t)G.n[1];
+}
+
+void grid_set_generators(Grid &G, const std::vector<double> &gen) {  // :111-130
+    const size_t nb = (size_t)(G.n[0] * G.n[1] * G.n[2]);
+    const size_t cnt = gen.size() / 3;
+    G.gen_bin.resize(cnt);
+    G.bin_start.assign(nb + 1, 0);
+    for (size_t i = 0; i < cnt; ++i) {
+        G.gen_bin[i] = (uint32_t)grid_bin_index(G, &gen[3 * i]);
+        G.bin_start[G.gen_bin[i] + 1]++;
+    }
+    for (size_t b = 0; b < nb; ++b) G.bin_start[b + 1] += G.bin_start[b];
+    std::vector<uint32_t> fill(G.bin_start.begin(), G.bin_start.end() - 1);
+    G.bin_items.resize(cnt);
+    for (size_t i = 0; i < cnt; ++i) G.bin_items[fill[G.gen_bin[i]]++] = (uint32_t)i;  // push in i order
+}
+
+// visit_neighbors, :145-202. The visitor returns the new squared search radius.
+template <class V>
+void grid_visit(const Grid &G, const std::vector<double> &gen, size_t index, const double pos[3],
+                double &r2, V &&visitor, Work *w) {
+    const size_t b = G.gen_bin[index];
+    const int64_t nxy = G.n[0] * G.n[1];
+    const int64_t iz = (int64_t)b / nxy, rem = (int64_t)b % nxy;
+    const int64_t iy = rem / G.n[0], ix = rem % G.n[0];
+    const double csx = 1.0 / G.s[0], csy = 1.0 / G.s[1], csz = 1.0 / G.s[2];
+    const double rx = (pos[0] - G.lo[0]) * G.s[0] - (double)ix;
+    const double ry = (pos[1] - G.lo[1]) * G.s[1] - (double)iy;
+    const double rz = (pos[2] - G.lo[2]) * G.s[2] - (double)iz;
+    auto bd = [](int64_t d, double rel, double c) -> double {  // :182-184
+        if (d > 0) return ((double)d - rel) * c;
+        if (d < 0) return ((double)(-(d + 1)) + rel) * c;
+        return 0.0;
+    };
+    for (const ShellEntry &e : G.order) {
+        if (w) w->entries++;
+        if (e.d2 > 4.0 * r2) break;  // :170
+        const int64_t bx = ix + e.dx, by = iy + e.dy, bz = iz + e.dz;
+        if (bx < 0 || bx >= G.n[0] || by < 0 || by >= G.n[1] || bz < 0 || bz >= G.n[2]) continue;
+        double ex = std::max(bd(e.dx, rx, csx), 0.0);
+        double ey = std::max(bd(e.dy, ry, csy), 0.0);
+        double ez = std::max(bd(e.dz, rz, csz), 0.0);
+        if (w) { w->bins++; w->flops += 12; }
+        if (ex * ex + ey * ey + ez * ez <= 4.0 * r2) {  // :190
+            const size_t bi = (size_t)(bx + by * G.n[0] + bz * nxy);
+            for (uint32_t k = G.bin_start[bi]; k < G.bin_start[bi + 1]; ++k) {
+                const uint32_t j = G.bin_items[k];
+                if (j == index) continue;
+                r2 = visitor(j, &gen[3 * j], r2);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Tessellation. Restates src/tessellation.rs:13-67, 175-230, 414-475, 482-495.
+// ---------------------------------------------------------------------------------------------
+struct Tess {
+    double lo[3], hi[3];
+    std::vector<double> gen;
+    std::vector<Wall> walls;
+    Grid grid;
+    std::vector<Poly> cells;
+    std::vector<uint8_t> computed;
+    std::vector<Work> work;   // per cell
+};
+
+void tess_set_generators(Tess &T, const double *xyz, size_t n) {  // tessellation.rs:43-67
+    std::vector<double> keep;
+    keep.reserve(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const double *q = xyz + 3 * i;
+        bool in = true;
+        for (const Wall &w : T.walls)
+            if (!wall_contains(w, q)) { in = false; break; }
+        if (in) keep.insert(keep.end(), q, q + 3);
+    }
+    T.gen.swap(keep);
+    grid_set_generators(T.grid, T.gen);
+    T.cells.clear();
+    T.computed.clear();
+}
+
+void tess_add_wall(Tess &T, const Wall &w) {  // tessellation.rs:175-200
+    T.walls.push_back(w);
+    std::vector<double> keep;
+    const size_t n = T.gen.size() / 3;
+    for (size_t i = 0; i < n; ++i) {
+        const double *q = &T.gen[3 * i];
+        bool in = true;
+        for (const Wall &x : T.walls)
+            if (!wall_contains(x, q)) { in = false; break; }
+        if (in) keep.insert(keep.end(), q, q + 3);
+    }
+    if (keep.size() != T.gen.size()) {
+        T.gen.swap(keep);
+        grid_set_generators(T.grid, T.gen);
+    }
+}
+
+// compute_cell, tessellation.rs:414-475. brute != 0 visits every other generator in index order
+// with no grid and no early exit (used only to pin the grid walk).
+void compute_cell(const Tess &T, size_t i, Poly &cell, ClipScratch &s, Work *w, int brute) {
+    const double *g = &T.gen[3 * i];
+    poly_box(cell, T.lo, T.hi);
+    for (const Wall &wl : T.walls) {  // :429-436
+        double p[3], n[3];
+        if (wall_cut(wl, g, p, n, w)) {
+            if (w) w->clips++;
+            auto r = poly_clip(cell, p, n, wl.id, s, nullptr, w);
+            if (w && r.first) w->cuts++;
+        }
+        if (cell.empty()) return;
+    }
+    double r2 = poly_max_radius_sq(cell, g);  // :438
+    if (w) w->flops += 8 * cell.nv();
+    auto visitor = [&](uint32_t j, const double *q, double cur) -> double {  // :446-471
+        double dist_sq = 0.0, mid[3], nrm[3];
+        for (int k = 0; k < 3; ++k) {
+            double d = q[k] - g[k];
+            dist_sq += d * d;
+            mid[k] = g[k] + d * 0.5;
+            nrm[k] = d;
+        }
+        if (w) { w->candidates++; w->flops += 16; }
+        if (dist_sq > 4.0 * cur) return cur;
+        if (w) w->clips++;
+        auto r = poly_clip(cell, mid, nrm, (int32_t)j, s, g, w);
+        if (r.first) {
+            if (w) w->cuts++;
+            return cell.empty() ? 0.0 : r.second;
+        }
+        return cur;
+    };
+    if (!brute) {
+        grid_visit(T.grid, T.gen, i, g, r2, visitor, w);
+    } else {
+        const size_t n = T.gen.size() / 3;
+        for (size_t j = 0; j < n; ++j)
+            if (j != i) r2 = visitor((uint32_t)j, &T.gen[3 * j], 1e300);
+    }
+}
+
+void tess_calculate(Tess &T, const int64_t *subset, size_t m, int threads, int brute) {
+    const size_t n = T.gen.size() / 3;
+    T.cells.assign(n, Poly());
+    T.computed.assign(n, 0);
+    T.work.assign(n, Work());
+    const size_t total = subset ? m : n;
+    std::atomic<size_t> next{0};
+    auto run = [&]() {
+        ClipScratch s;
+        for (;;) {
+            size_t a = next.fetch_add(64);
+            if (a >= total) break;
+            size_t b = std::min(total, a + 64);
+            for (size_t k = a; k < b; ++k) {
+                size_t i = subset ? (size_t)subset[k] : k;
+                if (i >= n) continue;
+                Work w;
+                compute_cell(T, i, T.cells[i], s, &w, brute);
+                // reductions done by the caller side of the reference (volume/centroid) are part
+                // of the measured output here, as they are in the library's reduced mode
+                uint64_t tri = poly_triangles(T.cells[i]);
+                w.flops += 15 * tri + 1 + 27 * tri + 5;
+                T.work[i] = w;
+                T.computed[i] = 1;
+            }
+        }
+    };
+    if (threads <= 1) {
+        run();
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) pool.emplace_back(run);
+        for (auto &th : pool) th.join();
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// C ABI (ctypes) for the tests and the CPU baseline. Prefix "or_" = oracle.
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+void *or_tess_new(const double lo[3], const double hi[3], int64_t nx, int64_t ny, int64_t nz) {
+    Tess *T = new Tess();
+    for (int k = 0; k < 3; ++k) { T->lo[k] = lo[k]; T->hi[k] = hi[k]; }
+    grid_new(T->grid, nx, ny, nz, lo, hi);
+    grid_set_generators(T->grid, T->gen);
+    return T;
+}
+
+void or_tess_free(void *t) { delete (Tess *)t; }
+
+int or_tess_add_wall_plane(void *t, int32_t id, const double p[3], const double n[3]) {
+    if (id > -1000) return -1;  // wall/mod.rs:25-27 panics
+    Wall w{};
+    w.id = id; w.kind = 0;
+    double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);  // wall_3d.rs:23-27
+    if (len == 0.0) { w.b[0] = 0.0; w.b[1] = 0.0; w.b[2] = 1.0; }
+    else { w.b[0] = n[0] / len; w.b[1] = n[1] / len; w.b[2] = n[2] / len; }
+    for (int k = 0; k < 3; ++k) w.a[k] = p[k];
+    tess_add_wall(*(Tess *)t, w);
+    return 0;
+}
+
+int or_tess_add_wall_sphere(void *t, int32_t id, const double c[3], double r) {
+    if (id > -1000) return -1;
+    Wall w{};
+    w.id = id; w.kind = 1; w.r = r;
+    for (int k = 0; k < 3; ++k) w.a[k] = c[k];
+    tess_add_wall(*(Tess *)t, w);
+    return 0;
+}
+
+void or_tess_set_generators(void *t, const double *xyz, size_t n) {
+    tess_set_generators(*(Tess *)t, xyz, n);
+}
+
+size_t or_tess_count_generators(void *t) { return ((Tess *)t)->gen.size() / 3; }
+
+void or_tess_get_generators(void *t, double *out) {
+    Tess &T = *(Tess *)t;
+    std::memcpy(out, T.gen.data(), T.gen.size() * sizeof(double));
+}
+
+void or_tess_get_bins(void *t, uint32_t *bin_start, uint32_t *bin_items, uint32_t *gen_bin) {
+    Tess &T = *(Tess *)t;
+    if (bin_start) std::memcpy(bin_start, T.grid.bin_start.data(), T.grid.bin_start.size() * 4);
+    if (bin_items) std::memcpy(bin_items, T.grid.bin_items.data(), T.grid.bin_items.size() * 4);
+    if (gen_bin) std::memcpy(gen_bin, T.grid.gen_bin.data(), T.grid.gen_bin.size() * 4);
+}
+
+size_t or_tess_bin_index(void *t, const double q[3]) { return grid_bin_index(((Tess *)t)->grid, q); }
+
+// Full stable shell order (dx, dy, dz, min_d2) -> caller buffers of size cap.
+size_t or_tess_shell_order(void *t, int32_t *dxyz, double *d2, size_t cap) {
+    Tess &T = *(Tess *)t;
+    size_t m = std::min(cap, T.grid.order.size());
+    for (size_t k = 0; k < m; ++k) {
+        dxyz[3 * k] = T.grid.order[k].dx;
+        dxyz[3 * k + 1] = T.grid.order[k].dy;
+        dxyz[3 * k + 2] = T.grid.order[k].dz;
+        d2[k] = T.grid.order[k].d2;
+    }
+    return T.grid.order.size();
+}
+
+// Compute all cells (subset == NULL) or only the listed cell indices.
+void or_tess_calculate(void *t, const int64_t *subset, size_t m, int threads, int brute) {
+    tess_calculate(*(Tess *)t, subset, m, threads, brute);
+}
+
+size_t or_tess_count_cells(void *t) { return ((Tess *)t)->cells.size(); }
+
+// Tessellation::relax, tessellation.rs:482-495 (requires a full calculate first).
+void or_tess_relax(void *t) {
+    Tess &T = *(Tess *)t;
+    const size_t n = T.gen.size() / 3;
+    std::vector<double> ng(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        if (T.cells[i].empty()) {
+            for (int k = 0; k < 3; ++k) ng[3 * i + k] = T.gen[3 * i + k];
+        } else {
+            poly_centroid(T.cells[i], &ng[3 * i]);
+        }
+    }
+    tess_set_generators(T, ng.data(), n);
+}
+
+// sizes[0] = vertices, sizes[1] = faces, sizes[2] = face index entries. Returns 0 when the cell
+// was computed, -1 otherwise.
+int or_cell_sizes(void *t, size_t i, uint64_t sizes[3]) {
+    Tess &T = *(Tess *)t;
+    if (i >= T.cells.size() || !T.computed[i]) return -1;
+    const Poly &c = T.cells[i];
+    sizes[0] = c.nv(); sizes[1] = c.fcnt.size(); sizes[2] = c.fidx.size();
+    return 0;
+}
+
+int or_cell_get(void *t, size_t i, double *v, uint32_t *fc, uint32_t *fi, int32_t *fn) {
+    Tess &T = *(Tess *)t;
+    if (i >= T.cells.size() || !T.computed[i]) return -1;
+    const Poly &c = T.cells[i];
+    if (v) std::memcpy(v, c.v.data(), c.v.size() * 8);
+    if (fc) for (size_t k = 0; k < c.fcnt.size(); ++k) fc[k] = c.fcnt[k];
+    if (fi) for (size_t k = 0; k < c.fidx.size(); ++k) fi[k] = c.fidx[k];
+    if (fn) std::memcpy(fn, c.fnb.data(), c.fnb.size() * 4);
+    return 0;
+}
+
+// Per-cell reductions for the listed cells (or all when idx == NULL): volume, centroid, face
+// count; and the algorithmic-work counters [flops, candidates, bins, clips, cuts, entries].
+void or_tess_reduced(void *t, const int64_t *idx, size_t m, double *vol, double *cen,
+                     uint32_t *nf, uint64_t *work6) {
+    Tess &T = *(Tess *)t;
+    const size_t total = idx ? m : T.cells.size();
+    for (size_t k = 0; k < total; ++k) {
+        size_t i = idx ? (size_t)idx[k] : k;
+        const Poly &c = T.cells[i];
+        if (vol) vol[k] = poly_volume(c);
+        if (cen) poly_centroid(c, &cen[3 * k]);
+        if (nf) nf[k] = (uint32_t)c.fcnt.size();
+        if (work6) {
+            const Work &w = T.work[i];
+            work6[6 * k + 0] = w.flops; work6[6 * k + 1] = w.candidates; work6[6 * k + 2] = w.bins;
+            work6[6 * k + 3] = w.clips; work6[6 * k + 4] = w.cuts; work6[6 * k + 5] = w.entries;
+        }
+    }
+}
+
+// Standalone cell (for the reference's unit tests on Cell3DFaces).
+void *or_cell_new(const double lo[3], const double hi[3]) {
+    Poly *c = new Poly();
+    poly_box(*c, lo, hi);
+    return c;
+}
+void or_cell_free(void *c) { delete (Poly *)c; }
+int or_cell_clip(void *c, const double p[3], const double n[3], int32_t nid, const double *g,
+                 double *r_out) {
+    ClipScratch s;
+    auto r = poly_clip(*(Poly *)c, p, n, nid, s, g, nullptr);
+    if (r_out) *r_out = r.second;
+    return r.first ? 1 : 0;
+}
+double or_cell_volume(void *c) { return poly_volume(*(Poly *)c); }
+void or_cell_centroid(void *c, double out[3]) { poly_centroid(*(Poly *)c, out); }
+void or_cell_counts(void *c, uint64_t sizes[3]) {
+    Poly &p = *(Poly *)c;
+    sizes[0] = p.nv(); sizes[1] = p.fcnt.size(); sizes[2] = p.fidx.size();
+}
+
+}  // extern "C"

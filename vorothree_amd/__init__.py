@@ -1,0 +1,17 @@
+"""MI355X (gfx950) implementation of voronoid's Tessellation<3, Cell3DFaces, Algorithm3DGrid>.
+
+The compute path is libvoronoid_hip.so (HIP kernels behind the C ABI of include/vd.h); this
+package mirrors the reference's Rust API on top of it.
+"""
+from .tessellation import (  # noqa: F401
+    WALL_ID_MAX,
+    Algorithm3DGrid,
+    BoundingBox,
+    Cell3DFaces,
+    PlaneGeometry,
+    SphereGeometry,
+    Tessellation,
+    Wall,
+    box_side,
+    shell_prefix,
+)

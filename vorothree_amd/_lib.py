@@ -1,0 +1,81 @@
+"""ctypes binding of libvoronoid_hip.so (the C ABI declared in include/vd.h).
+
+The library is the only compute path: if it is missing or cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvoronoid_hip.so")
+
+VD_OK = 0
+VD_EINVAL = -1
+VD_ERANGE = -2
+VD_ENOMEM = -3
+VD_EDEVICE = -4
+VD_ECAPACITY = -5
+VD_ESTATE = -6
+VD_OPT_KEEP_CELLS = 1
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_u32 = ctypes.c_uint32
+_i32 = ctypes.c_int32
+
+# name -> (restype, argtypes); the list of exported entry points of include/vd.h
+SIGNATURES = {
+    "vd_tess_create": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, ctypes.POINTER(_vp)]),
+    "vd_tess_destroy": (None, [_vp]),
+    "vd_tess_add_wall_plane": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "vd_tess_add_wall_sphere": (ctypes.c_int, [_vp, _i32, _vp, ctypes.c_double]),
+    "vd_tess_clear_walls": (ctypes.c_int, [_vp]),
+    "vd_tess_set_generators": (ctypes.c_int, [_vp, _vp, _sz]),
+    "vd_tess_set_generators_device": (ctypes.c_int, [_vp, _vp, _sz]),
+    "vd_tess_calculate": (ctypes.c_int, [_vp]),
+    "vd_tess_relax": (ctypes.c_int, [_vp]),
+    "vd_tess_count_generators": (_sz, [_vp]),
+    "vd_tess_count_cells": (_sz, [_vp]),
+    "vd_tess_get_generators": (ctypes.c_int, [_vp, _vp, _sz]),
+    "vd_tess_get_volumes": (ctypes.c_int, [_vp, _vp, _sz]),
+    "vd_tess_get_centroids": (ctypes.c_int, [_vp, _vp, _sz]),
+    "vd_tess_get_face_counts": (ctypes.c_int, [_vp, _vp, _sz]),
+    "vd_tess_get_neighbors": (ctypes.c_int, [_vp, _vp, _sz, _vp, _sz]),
+    "vd_tess_get_cell": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "vd_tess_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
+    "vd_tess_get_stats": (ctypes.c_int, [_vp, _vp]),
+    "vd_shell_prefix": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _sz, _vp, _vp]),
+    "vd_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class VdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m vorothree_amd.build` "
+                "(the tessellation has no CPU fallback)")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != VD_OK:
+        msg = lib().vd_last_error()
+        raise VdError(rc, msg.decode() if msg else "")

@@ -1,0 +1,757 @@
+// Cell kernel for gfx950: Tessellation<3, Cell3DFaces, Algorithm3DGrid>::compute_cell for every
+// listed generator, one cell per 64-lane wavefront, persistent over an atomic work queue.
+//
+// Reference path (mdt-re/vorothree):
+//   src/tessellation.rs:414-475          compute_cell (walls, radius, neighbour visitor)
+//   src/algorithm/algo_3d_grid.rs:145-202 visit_neighbors (shell walk, early exit, bin test)
+//   src/cell/cell_3d_faces.rs:37-74       box cell
+//   src/cell/cell_3d_faces.rs:288-503     clip_with_scratch (half-space clip + lid)
+//   src/cell/cell_3d_faces.rs:106-205     volume / centroid (fan triangulation, left fold)
+//   src/cell/cell_3d_faces.rs:271-286     max_radius_sq
+//   src/wall/wall_3d.rs:38-42, 80-100     plane / sphere wall cut
+//
+// Results are bit-identical to the sequential algorithm: the same clip sequence, the same vertex
+// and face numbering, the same expression order (built with -ffp-contract=off), IEEE division and
+// square root, and the same left folds.
+//
+// Layout: the cell's polytope lives in LDS (structure of arrays, ping-pong buffers):
+//   vx/vy/vz[V]      vertex coordinates (f64)
+//   sedge[S]         per face slot: vertex (low 16 bits) and its successor in the face (high 16)
+//   sface[S]         per face slot: face number
+//   fstart[F+1]      first slot of each face (fstart[F] = S); fnb[F] neighbour id per face
+// "Slots" are the flattened face_indices of the reference; sedge carries the successor so a clip
+// reads each edge with one LDS access.
+//
+// Parallel clip (one plane): lanes over vertices compute the signed distances and the kept-vertex
+// numbering (ballot prefix counts); lanes over slots classify each edge, prefix-sum the output
+// entries per face, and list the plane-crossing slots; lanes over crossings find the first
+// occurrence of each edge key in slot order (that occurrence's direction fixes the intersection's
+// arithmetic, exactly as the reference's insertion-ordered search does) and number the new
+// vertices in that order; lanes over the new slots find the lid segments; the lid cycle is walked
+// by the whole wave in lock step.
+//
+// Neighbour walk: the shell order is consumed 64 entries at a time (one entry per lane: break
+// test, bin range, bin distance). Between two cuts the search radius is constant, so the
+// candidates up to the next cut can be tested in parallel with that radius: the wave builds a
+// window of up to 64 candidates in visiting order, tests each against the radius and the current
+// polytope, and clips with the first one that cuts. Every earlier candidate provably leaves the
+// cell unchanged, so the clip sequence equals the sequential one.
+#include "vd_internal.h"
+
+#include <cfloat>
+
+namespace vd {
+namespace {
+
+constexpr double kTol = 1e-9;
+
+__device__ inline int lane_id() { return (int)__lane_id(); }
+__device__ inline uint64_t lt_mask() { return (1ull << lane_id()) - 1ull; }
+__device__ inline uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
+__device__ inline int popc(uint64_t x) { return __popcll(x); }
+__device__ inline int ffs64(uint64_t x) { return x ? (int)__builtin_ctzll(x) : 64; }
+__device__ inline int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline uint32_t rlu(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ inline double rld(double v, int l) {
+    long long b = __double_as_longlong(v);
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b & 0xffffffffll), l);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((unsigned long long)b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline uint32_t first_lane_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ inline uint64_t bcast_u64(uint64_t v) {  // lane 0's value to every lane
+    const uint32_t lo = first_lane_u32((uint32_t)v), hi = first_lane_u32((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t x) {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (l >= d) x += y;
+    }
+    return x;
+}
+
+// Maximum with the reference's update rule `if d2 > m { m = d2 }` (NaN never wins).
+__device__ inline double gt_max(double m, double d2) { return d2 > m ? d2 : m; }
+
+__device__ inline double wave_max(double x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = gt_max(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
+__device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
+    double v = (x - lo) * s;  // algo_3d_grid.rs:102-104
+    if (v < 0.0) v = 0.0;
+    if (v > lim) v = lim;
+    if (!(v == v)) return 0;
+    return (uint32_t)v;
+}
+
+template <int VM_, int SM_, int FM_, int KM_>
+struct Cap {
+    static constexpr int VM = VM_;   // vertices
+    static constexpr int SM = SM_;   // face slots (flattened face indices)
+    static constexpr int FM = FM_;   // faces
+    static constexpr int KM = KM_;   // plane-crossing slots per clip
+};
+
+template <class C>
+struct Mem {
+    double vx[2][C::VM], vy[2][C::VM], vz[2][C::VM];
+    double dist[C::VM];
+    uint32_t sedge[2][C::SM];
+    uint32_t ckey[C::KM];
+    uint32_t cab[C::KM];
+    uint32_t lmap[C::VM];
+    uint32_t seg[C::SM];
+    int32_t fnb[2][C::FM];
+    uint32_t winq[64];
+    uint16_t sface[2][C::SM];
+    uint16_t fstart[2][C::FM + 1];
+    uint16_t remap[C::VM];
+    uint16_t P[C::SM + 1];
+    uint16_t cxid[C::KM];
+    uint16_t fidxn[C::SM];
+    int16_t fnew[C::FM];
+    uint16_t fnoff[C::FM];
+    uint16_t lid[C::SM];
+    uint16_t winent[64];
+    uint16_t winoff[64];
+};
+
+enum ClipRc : int { kNoChange = 0, kModified = 1, kOverflow = -1, kFace255 = -2 };
+
+struct Plane { double px, py, pz, nx, ny, nz; };
+
+// The polytope state of one wave; all members are wave-uniform.
+struct Poly { int cur, V, F, S; };
+
+template <class C>
+__device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi[3]) {
+    // cell_3d_faces.rs:37-74: vertex order (lo,lo,lo) (hi,lo,lo) (hi,hi,lo) (lo,hi,lo), then z=hi;
+    // faces z-, z+, y-, y+, x-, x+ with the listed index cycles; neighbours -5 -6 -3 -4 -1 -2.
+    const int l = lane_id();
+    if (l < 8) {
+        const int xb = (l == 1 || l == 2 || l == 5 || l == 6);
+        const int yb = (l == 2 || l == 3 || l == 6 || l == 7);
+        const int zb = l >> 2;
+        m.vx[0][l] = xb ? hi[0] : lo[0];
+        m.vy[0][l] = yb ? hi[1] : lo[1];
+        m.vz[0][l] = zb ? hi[2] : lo[2];
+    }
+    if (l < 24) {
+        // face f = l/4, position i = l%4, successor (i+1)%4; one nibble per index:
+        // z- 3,2,1,0 | z+ 4,5,6,7 | y- 0,1,5,4 | y+ 2,3,7,6 | x- 0,4,7,3 | x+ 1,2,6,5
+        const uint64_t lo16 = 0x6732'4510'7654'0123ull;  // faces 0-3, nibble k = index k
+        const uint32_t hi8 = 0x5621'3740u;               // faces 4-5
+        const int f = l >> 2, i = l & 3;
+        const int ka = 4 * f + i, kb = 4 * f + ((i + 1) & 3);
+        const uint32_t a = (uint32_t)((ka < 16 ? lo16 >> (4 * ka) : (uint64_t)hi8 >> (4 * (ka - 16))) & 0xf);
+        const uint32_t b = (uint32_t)((kb < 16 ? lo16 >> (4 * kb) : (uint64_t)hi8 >> (4 * (kb - 16))) & 0xf);
+        m.sedge[0][l] = a | (b << 16);
+        m.sface[0][l] = (uint16_t)f;
+    }
+    if (l < 7) m.fstart[0][l] = (uint16_t)(4 * l);
+    if (l < 6) m.fnb[0][l] = -(int32_t)((0x214365u >> (4 * l)) & 0xf);  // -5 -6 -3 -4 -1 -2
+    P.cur = 0; P.V = 8; P.F = 6; P.S = 24;
+    __syncthreads();
+}
+
+// Half-space clip keeping (x - p).n <= 1e-9 (cell_3d_faces.rs:288-503). Returns kNoChange,
+// kModified (r2 = max |v - g|^2 over the new vertices, or 0 without a generator), or a negative
+// code when this tier's capacity is exceeded.
+template <class C>
+__device__ int poly_clip(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, bool withgen,
+                         double gx, double gy, double gz, double &r2) {
+    const int l = lane_id();
+    const uint64_t lt = lt_mask();
+    const int c = P.cur, o = c ^ 1;
+
+    // 1. signed distances, early outs, kept-vertex numbering (:303-362)
+    uint64_t gtm = 0, ltm = 0;
+    int nk = 0;
+    double md2 = 0.0;
+    for (int base = 0; base < P.V; base += 64) {
+        const int v = base + l;
+        const bool act = v < P.V;
+        double x = 0.0, y = 0.0, z = 0.0, d = 0.0;
+        if (act) {
+            x = m.vx[c][v]; y = m.vy[c][v]; z = m.vz[c][v];
+            d = (x - pl.px) * pl.nx + (y - pl.py) * pl.ny + (z - pl.pz) * pl.nz;
+            m.dist[v] = d;
+        }
+        gtm |= ballot(act && d > kTol);
+        ltm |= ballot(act && d < -kTol);
+        const bool in = act && d <= kTol;
+        const uint64_t im = ballot(in);
+        if (in) {
+            const int ni = nk + popc(im & lt);
+            m.remap[v] = (uint16_t)ni;
+            m.vx[o][ni] = x; m.vy[o][ni] = y; m.vz[o][ni] = z;
+            if (withgen) {
+                const double dx = x - gx, dy = y - gy, dz = z - gz;
+                md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
+            }
+        }
+        nk += popc(im);
+    }
+    if (gtm == 0) return kNoChange;     // all inside (:317-319)
+    if (ltm == 0) {                     // all outside (:320-326)
+        P.V = P.F = P.S = 0;
+        r2 = 0.0;
+        return kModified;
+    }
+    __syncthreads();
+
+    // 2. per slot: output entry count and plane crossings (:378-452)
+    int K = 0, ptot = 0;
+    for (int base = 0; base < P.S; base += 64) {
+        const int s = base + l;
+        const bool act = s < P.S;
+        const uint32_t e = act ? m.sedge[c][s] : 0u;
+        const uint32_t a = e & 0xffffu, b = e >> 16;
+        const double da = m.dist[a], db = m.dist[b];
+        const bool ain = da <= kTol, bin = db <= kTol;
+        const uint64_t b0 = ballot(act && ain);            // in->in, in->out: 1 entry
+        const uint64_t b1 = ballot(act && !ain && bin);    // out->in: 2 entries
+        if (act) m.P[s] = (uint16_t)(ptot + popc(b0 & lt) + 2 * popc(b1 & lt));
+        ptot += popc(b0) + 2 * popc(b1);
+        const bool cr = act && (ain != bin);
+        const uint64_t cm = ballot(cr);
+        if (cr) {
+            const int ci = K + popc(cm & lt);
+            if (ci < C::KM) {
+                m.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
+                m.cab[ci] = a | (b << 16);
+            }
+        }
+        K += popc(cm);
+    }
+    if (l == 0) m.P[P.S] = (uint16_t)ptot;
+    if (K > C::KM || ptot > 0xffff) return kOverflow;
+    __syncthreads();
+
+    // 3. per face: surviving faces (>= 3 entries, :454-456) and their new offsets
+    int Fn = 0, Spre = 0;
+    bool big = false;
+    for (int base = 0; base < P.F; base += 64) {
+        const int f = base + l;
+        const bool act = f < P.F;
+        const int fs = act ? m.fstart[c][f] : 0, fe = act ? m.fstart[c][f + 1] : 0;
+        const int tot = act ? (int)m.P[fe] - (int)m.P[fs] : 0;
+        const bool sv = act && tot >= 3;
+        const uint64_t smk = ballot(sv);
+        const int nf = Fn + popc(smk & lt);
+        const uint32_t t = sv ? (uint32_t)tot : 0u;
+        const uint32_t inc = wave_incl_scan(t);
+        const int off = Spre + (int)(inc - t);
+        if (act) { m.fnew[f] = (int16_t)(sv ? nf : -1); m.fnoff[f] = (uint16_t)off; }
+        if (sv && nf < C::FM) { m.fstart[o][nf] = (uint16_t)off; m.fnb[o][nf] = m.fnb[c][f]; }
+        big |= ballot(sv && tot > 255) != 0;
+        Fn += popc(smk);
+        Spre += (int)rlu(inc, 63);
+    }
+    if (big) return kFace255;
+    if (Fn + 1 > C::FM || Spre > C::SM) return kOverflow;
+
+    // 4. first occurrence of each crossing edge in slot order numbers the new vertices (:391-417)
+    int nint = 0;
+    for (int base = 0; base < K; base += 64) {
+        const int ci = base + l;
+        const bool act = ci < K;
+        const uint32_t key = act ? m.ckey[ci] : 0xffffffffu;
+        int first = ci;
+        if (K <= 64) {
+            for (int cj = 0; cj < K; ++cj) {
+                const uint32_t kj = rlu(key, cj);
+                if (cj < first && kj == key) first = cj;
+            }
+        } else {
+            const int lim = min(K, base + 64);
+            for (int cj = 0; cj < lim; ++cj) {
+                const uint32_t kj = m.ckey[cj];
+                if (cj < first && kj == key) first = cj;
+            }
+        }
+        const bool isf = act && first == ci;
+        const uint64_t fm = ballot(isf);
+        const int xid = nk + nint + popc(fm & lt);
+        nint += popc(fm);
+        if (act) m.cxid[ci] = (uint16_t)(isf ? xid : (0x8000 | first));
+        if (isf && xid < C::VM) {
+            const uint32_t ab = m.cab[ci];
+            const uint32_t a = ab & 0xffffu, b = ab >> 16;
+            const double da = m.dist[a], db = m.dist[b];
+            double t = da / (da - db);
+            if (t < 0.0) t = 0.0;   // f64::clamp(0, 1)
+            if (t > 1.0) t = 1.0;
+            const double ax = m.vx[c][a], ay = m.vy[c][a], az = m.vz[c][a];
+            const double bx = m.vx[c][b], by = m.vy[c][b], bz = m.vz[c][b];
+            const double x = ax + t * (bx - ax);
+            const double y = ay + t * (by - ay);
+            const double z = az + t * (bz - az);
+            m.vx[o][xid] = x; m.vy[o][xid] = y; m.vz[o][xid] = z;
+            if (withgen) {
+                const double dx = x - gx, dy = y - gy, dz = z - gz;
+                md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
+            }
+        }
+    }
+    if (nk + nint > C::VM) return kOverflow;
+    __syncthreads();
+    for (int ci = l; ci < K; ci += 64) {
+        const uint16_t v = m.cxid[ci];
+        if (v & 0x8000) m.cxid[ci] = m.cxid[v & 0x7fff];
+    }
+    __syncthreads();
+
+    // 5. emit the surviving faces' entries (:386-451, :466)
+    int Kb = 0;
+    for (int base = 0; base < P.S; base += 64) {
+        const int s = base + l;
+        const bool act = s < P.S;
+        const uint32_t e = act ? m.sedge[c][s] : 0u;
+        const uint32_t a = e & 0xffffu, b = e >> 16;
+        const double da = m.dist[a], db = m.dist[b];
+        const bool ain = da <= kTol, bin = db <= kTol;
+        const bool cr = act && (ain != bin);
+        const uint64_t cm = ballot(cr);
+        const int ci = Kb + popc(cm & lt);
+        Kb += popc(cm);
+        const int f = act ? m.sface[c][s] : 0;
+        const int nf = act ? m.fnew[f] : -1;
+        if (act && nf >= 0 && (ain || bin)) {
+            const int pos = (int)m.fnoff[f] + ((int)m.P[s] - (int)m.P[m.fstart[c][f]]);
+            if (ain && bin) {
+                m.fidxn[pos] = m.remap[b];
+                m.sface[o][pos] = (uint16_t)nf;
+            } else {
+                m.fidxn[pos] = m.cxid[ci];
+                m.sface[o][pos] = (uint16_t)nf;
+                if (!ain) {
+                    m.fidxn[pos + 1] = m.remap[b];
+                    m.sface[o][pos + 1] = (uint16_t)nf;
+                }
+            }
+        }
+    }
+    if (l == 0) m.fstart[o][Fn] = (uint16_t)Spre;
+    __syncthreads();
+
+    // 6. successor of every new slot; lid segments (v, u) for intersection pairs (:459-465)
+    int M = 0;
+    for (int base = 0; base < Spre; base += 64) {
+        const int p = base + l;
+        const bool act = p < Spre;
+        const int fp = act ? m.sface[o][p] : 0;
+        const int fs = m.fstart[o][fp], fe = m.fstart[o][fp + 1];
+        const int nx = (p + 1 == fe) ? fs : p + 1;
+        const uint32_t ua = act ? m.fidxn[p] : 0u, vb = act ? m.fidxn[nx] : 0u;
+        if (act) m.sedge[o][p] = ua | (vb << 16);
+        const bool sg = act && (int)ua >= nk && (int)vb >= nk;
+        const uint64_t gm = ballot(sg);
+        if (sg) m.seg[M + popc(gm & lt)] = vb | (ua << 16);
+        M += popc(gm);
+    }
+
+    // 7. lid polygon (:471-495): map[first] = second (later segments overwrite), walk from seg[0]
+    if (M > 0) {
+        for (int t = l; t < nint; t += 64) m.lmap[t] = 0u;
+        __syncthreads();
+        for (int t = l; t < M; t += 64) {
+            const uint32_t sg = m.seg[t];
+            atomicMax(&m.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
+        }
+        __syncthreads();
+        const uint32_t s0 = m.seg[0];
+        const int start = (int)(s0 & 0xffffu);
+        int cv = (int)(s0 >> 16);
+        if (l == 0) m.lid[0] = (uint16_t)start;
+        int len = 1;
+        while (cv != start && len <= M) {
+            if (len >= C::SM) return kOverflow;
+            if (l == 0) m.lid[len] = (uint16_t)cv;
+            ++len;
+            const uint32_t mv = m.lmap[cv - nk];
+            cv = mv ? (int)(mv & 0xffffu) : 0xffff;
+            if (cv == 0xffff) break;
+        }
+        if (len >= 3) {
+            if (Spre + len > C::SM || len > 255) return len > 255 ? kFace255 : kOverflow;
+            __syncthreads();
+            for (int t = l; t < len; t += 64) {
+                const uint32_t u = m.lid[t], w = m.lid[(t + 1) % len];
+                m.sedge[o][Spre + t] = u | (w << 16);
+                m.sface[o][Spre + t] = (uint16_t)Fn;
+            }
+            if (l == 0) {
+                m.fstart[o][Fn] = (uint16_t)Spre;
+                m.fnb[o][Fn] = nid;
+                m.fstart[o][Fn + 1] = (uint16_t)(Spre + len);
+            }
+            Fn += 1;
+            Spre += len;
+        }
+    }
+    __syncthreads();
+    P.cur = o;
+    P.V = nk + nint;
+    P.F = Fn;
+    P.S = Spre;
+    r2 = withgen ? wave_max(md2) : 0.0;
+    return kModified;
+}
+
+template <class C>
+__device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double gy, double gz) {
+    double md = 0.0;  // cell_3d_faces.rs:271-286
+    for (int v = lane_id(); v < P.V; v += 64) {
+        const double dx = m.vx[P.cur][v] - gx, dy = m.vy[P.cur][v] - gy, dz = m.vz[P.cur][v] - gz;
+        md = gt_max(md, dx * dx + dy * dy + dz * dz);
+    }
+    return wave_max(md);
+}
+
+// Volume and centroid folds (cell_3d_faces.rs:106-205). Triangles are produced lane-parallel in
+// slot order and folded left in that order.
+template <class C>
+__device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]) {
+    const int l = lane_id();
+    const int c = P.cur;
+    double tv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int base = 0; base < P.S; base += 64) {
+        const int p = base + l;
+        const bool act = p < P.S;
+        double det = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+        bool tri = false;
+        if (act) {
+            const int f = m.sface[c][p];
+            const int fs = m.fstart[c][f], fe = m.fstart[c][f + 1];
+            const int cnt = fe - fs, i = p - fs;
+            tri = cnt >= 3 && i >= 1 && i <= cnt - 2;
+            if (tri) {
+                const uint32_t e = m.sedge[c][p];
+                const int i0 = (int)(m.sedge[c][fs] & 0xffffu);
+                const int i1 = (int)(e & 0xffffu), i2 = (int)(e >> 16);
+                const double ax = m.vx[c][i0], ay = m.vy[c][i0], az = m.vz[c][i0];
+                const double bx = m.vx[c][i1], by = m.vy[c][i1], bz = m.vz[c][i1];
+                const double qx = m.vx[c][i2], qy = m.vy[c][i2], qz = m.vz[c][i2];
+                const double cx = by * qz - bz * qy;
+                const double cy = bz * qx - bx * qz;
+                const double cz = bx * qy - by * qx;
+                det = ax * cx + ay * cy + az * cz;
+                wx = det * (ax + bx + qx);
+                wy = det * (ay + by + qy);
+                wz = det * (az + bz + qz);
+            }
+        }
+        uint64_t tm = ballot(tri);
+        while (tm) {
+            const int t = ffs64(tm);
+            tm &= tm - 1;
+            tv += rld(det, t);
+            sx += rld(wx, t);
+            sy += rld(wy, t);
+            sz += rld(wz, t);
+        }
+    }
+    vol = fabs(tv / 6.0);
+    if (fabs(tv) < 1e-9) {
+        cen[0] = cen[1] = cen[2] = 0.0;
+    } else {
+        const double f = 1.0 / (4.0 * tv);
+        cen[0] = sx * f; cen[1] = sy * f; cen[2] = sz * f;
+    }
+}
+
+// One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports.
+template <class C>
+__device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
+    const int l = lane_id();
+    const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
+    Poly P;
+    poly_box<C>(m, P, A.box_lo, A.box_hi);
+    int status = kCellOk;
+    double r2 = 0.0;
+
+    // walls in insertion order (tessellation.rs:429-436)
+    for (int w = 0; w < A.nwalls && status == kCellOk; ++w) {
+        const WallDev &W = A.walls[w];
+        Plane pl;
+        bool emit = true;
+        if (W.kind == 0) {  // wall_3d.rs:38-42
+            pl = Plane{W.a[0], W.a[1], W.a[2], -W.b[0], -W.b[1], -W.b[2]};
+        } else {            // wall_3d.rs:80-100
+            const double dx = gx - W.a[0], dy = gy - W.a[1], dz = gz - W.a[2];
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist == 0.0) {
+                emit = false;
+            } else {
+                const double sc = W.r / dist;
+                pl = Plane{W.a[0] + dx * sc, W.a[1] + dy * sc, W.a[2] + dz * sc,
+                           dx / dist, dy / dist, dz / dist};
+            }
+        }
+        if (emit) {
+            double unused;
+            const int rc = poly_clip<C>(m, P, pl, W.id, false, gx, gy, gz, unused);
+            if (rc == kOverflow) status = kCellCapacity;
+            else if (rc == kFace255) status = kCellFace255;
+        }
+        if (P.V == 0) break;
+    }
+
+    if (status == kCellOk && P.V > 0) {
+        r2 = poly_max_radius_sq<C>(m, P, gx, gy, gz);
+
+        // neighbour walk (algo_3d_grid.rs:145-202)
+        const GridGeom &G = A.grid;
+        const int ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
+        const int iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
+        const int iz = (int)axis_bin(gz, G.lo[2], G.s[2], G.lim[2]);
+        const double rx = (gx - G.lo[0]) * G.s[0] - (double)ix;
+        const double ry = (gy - G.lo[1]) * G.s[1] - (double)iy;
+        const double rz = (gz - G.lo[2]) * G.s[2] - (double)iz;
+
+        uint32_t e0 = 0;       // first shell entry of the loaded batch (lane k holds e0 + k)
+        int k0 = 0, o0 = 0;    // next unvisited entry of the batch and items of it already visited
+        bool load = true;
+        double bd2 = 0.0, bdist = 0.0;
+        uint32_t bs = 0, bc = 0;
+        bool binr = false, bend = false;
+        for (;;) {
+            if (load) {
+                const uint32_t e = e0 + (uint32_t)l;
+                bend = e >= A.nshell;
+                binr = false;
+                bc = 0;
+                if (!bend) {
+                    const ShellEntry se = A.shell[e];
+                    bd2 = se.d2;
+                    const int bx = ix + se.dx, by = iy + se.dy, bz = iz + se.dz;
+                    binr = bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2];
+                    if (binr) {
+                        const uint32_t b = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] +
+                                           (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
+                        bs = A.bin_start[b];
+                        bc = A.bin_start[b + 1] - bs;
+                        // algo_3d_grid.rs:182-190
+                        double ex = se.dx > 0 ? ((double)se.dx - rx) * G.cs[0]
+                                  : se.dx < 0 ? ((double)(-(se.dx + 1)) + rx) * G.cs[0] : 0.0;
+                        double ey = se.dy > 0 ? ((double)se.dy - ry) * G.cs[1]
+                                  : se.dy < 0 ? ((double)(-(se.dy + 1)) + ry) * G.cs[1] : 0.0;
+                        double ez = se.dz > 0 ? ((double)se.dz - rz) * G.cs[2]
+                                  : se.dz < 0 ? ((double)(-(se.dz + 1)) + rz) * G.cs[2] : 0.0;
+                        ex = fmax(ex, 0.0); ey = fmax(ey, 0.0); ez = fmax(ez, 0.0);
+                        bdist = ex * ex + ey * ey + ez * ez;
+                    }
+                } else {
+                    bd2 = (e == A.nshell) ? A.shell_next_d2 : DBL_MAX;
+                }
+                load = false;
+            }
+            const double lim = 4.0 * r2;
+            const bool committed = (l == k0) && (o0 > 0);
+            const bool brk = (l >= k0) && !committed && (bend || bd2 > lim);
+            const int kb = ffs64(ballot(brk));
+            const bool take = (l >= k0) && (l < kb) && (committed || (binr && bdist <= lim));
+            const uint32_t skip = (l == k0) ? (uint32_t)o0 : 0u;
+            const uint32_t cnt = take ? bc - skip : 0u;
+            const uint32_t inc = wave_incl_scan(cnt);
+            const uint32_t pos = inc - cnt;
+            const uint32_t tot = rlu(inc, 63);
+            const int nwin = tot < 64u ? (int)tot : 64;
+            if (take) {
+                for (uint32_t t = 0; t < cnt && pos + t < 64u; ++t) {
+                    m.winq[pos + t] = bs + skip + t;
+                    m.winent[pos + t] = (uint16_t)l;
+                    m.winoff[pos + t] = (uint16_t)(skip + t);
+                }
+            }
+            __syncthreads();
+
+            // candidates: tessellation.rs:446-471 (distance test, then clip)
+            bool pass = false;
+            uint32_t j = 0;
+            Plane cp{0, 0, 0, 0, 0, 0};
+            if (l < nwin) {
+                const double4 q = A.spos[m.winq[l]];
+                j = (uint32_t)__double_as_longlong(q.w);
+                const double d0 = q.x - gx, d1 = q.y - gy, d2 = q.z - gz;
+                const double dist_sq = d0 * d0 + d1 * d1 + d2 * d2;
+                cp = Plane{gx + d0 * 0.5, gy + d1 * 0.5, gz + d2 * 0.5, d0, d1, d2};
+                pass = (j != i) && !(dist_sq > lim);
+            }
+            uint64_t pm = ballot(pass);
+            int hit = 64;       // window lane of the candidate that cut the cell
+            while (pm) {
+                if (popc(pm) <= 2) {
+                    // few candidates left: clip the next one directly (its distance pass is the
+                    // all-inside test)
+                    const int c = ffs64(pm);
+                    pm &= pm - 1;
+                    const Plane pl{rld(cp.px, c), rld(cp.py, c), rld(cp.pz, c),
+                                   rld(cp.nx, c), rld(cp.ny, c), rld(cp.nz, c)};
+                    double nr2 = 0.0;
+                    const int rc = poly_clip<C>(m, P, pl, (int32_t)rlu(j, c), true, gx, gy, gz, nr2);
+                    if (rc == kNoChange) continue;
+                    if (rc == kOverflow) { status = kCellCapacity; }
+                    else if (rc == kFace255) { status = kCellFace255; }
+                    else { r2 = (P.V == 0) ? 0.0 : nr2; }
+                    hit = c;
+                    break;
+                }
+                // many candidates: test all of them against the polytope in parallel
+                bool cut = false;
+                const bool mine = (pm >> l) & 1ull;
+                uint64_t cutm = 0;
+                for (int v = 0; v < P.V; ++v) {
+                    const double x = m.vx[P.cur][v], y = m.vy[P.cur][v], z = m.vz[P.cur][v];
+                    if (mine && !cut) {
+                        const double dv = (x - cp.px) * cp.nx + (y - cp.py) * cp.ny + (z - cp.pz) * cp.nz;
+                        cut = dv > kTol;
+                    }
+                    cutm = ballot(cut);
+                    const uint64_t below = cutm ? ((cutm & (~cutm + 1ull)) - 1ull) : ~0ull;
+                    if ((pm & ~cutm & below) == 0) break;
+                }
+                if (cutm == 0) { pm = 0; break; }
+                const int c = ffs64(cutm);
+                const Plane pl{rld(cp.px, c), rld(cp.py, c), rld(cp.pz, c),
+                               rld(cp.nx, c), rld(cp.ny, c), rld(cp.nz, c)};
+                double nr2 = 0.0;
+                const int rc = poly_clip<C>(m, P, pl, (int32_t)rlu(j, c), true, gx, gy, gz, nr2);
+                if (rc == kOverflow) status = kCellCapacity;
+                else if (rc == kFace255) status = kCellFace255;
+                else r2 = (P.V == 0) ? 0.0 : nr2;
+                hit = c;
+                break;
+            }
+            if (status != kCellOk) break;
+
+            if (hit < 64) {
+                // resume right after the cutting candidate; its bin stays committed
+                const int k = m.winent[hit], off = m.winoff[hit];
+                if ((uint32_t)(off + 1) < rlu(bc, k)) { k0 = k; o0 = off + 1; }
+                else { k0 = k + 1; o0 = 0; }
+            } else if (tot > 64u) {
+                const int k = m.winent[63], off = m.winoff[63];
+                if ((uint32_t)(off + 1) < rlu(bc, k)) { k0 = k; o0 = off + 1; }
+                else { k0 = k + 1; o0 = 0; }
+            } else if (kb < 64) {
+                // the walk stops at entry kb: a real break, or the end of the prefix
+                if (rl((int)bend, kb)) {
+                    const uint32_t e = e0 + (uint32_t)kb;
+                    if (!(e == A.nshell && A.shell_next_d2 > lim)) status = kCellShell;
+                }
+                __syncthreads();
+                break;
+            } else {
+                k0 = 64;
+            }
+            if (k0 >= 64) { e0 += 64; k0 = 0; o0 = 0; load = true; }
+            __syncthreads();
+        }
+    }
+
+    // outputs
+    if (l == 0) A.status[i] = status;
+    if (status != kCellOk) { __syncthreads(); return; }
+    double vol, cen[3];
+    poly_reduce<C>(m, P, vol, cen);
+    if (l == 0) {
+        A.vol[i] = vol;
+        A.cen[3 * (size_t)i] = cen[0];
+        A.cen[3 * (size_t)i + 1] = cen[1];
+        A.cen[3 * (size_t)i + 2] = cen[2];
+        A.nface[i] = (uint32_t)P.F;
+        A.nvert[i] = (uint32_t)P.V;
+    }
+    if (A.vtx) {
+        uint64_t ov = 0, of = 0, oi = 0;
+        if (l == 0) {
+            ov = atomicAdd((unsigned long long *)&A.arena_top[0], (unsigned long long)P.V);
+            of = atomicAdd((unsigned long long *)&A.arena_top[1], (unsigned long long)P.F);
+            oi = atomicAdd((unsigned long long *)&A.arena_top[2], (unsigned long long)P.S);
+        }
+        ov = bcast_u64(ov);
+        of = bcast_u64(of);
+        oi = bcast_u64(oi);
+        if (ov + P.V > A.arena_cap[0] || of + P.F > A.arena_cap[1] || oi + P.S > A.arena_cap[2]) {
+            if (l == 0) A.status[i] = kCellArena;
+        } else {
+            const int c = P.cur;
+            for (int v = l; v < P.V; v += 64) {
+                A.vtx[3 * (ov + v)] = m.vx[c][v];
+                A.vtx[3 * (ov + v) + 1] = m.vy[c][v];
+                A.vtx[3 * (ov + v) + 2] = m.vz[c][v];
+            }
+            for (int f = l; f < P.F; f += 64) {
+                A.fcnt[of + f] = (uint8_t)(m.fstart[c][f + 1] - m.fstart[c][f]);
+                A.fnb[of + f] = m.fnb[c][f];
+            }
+            for (int s = l; s < P.S; s += 64) A.fidx[oi + s] = (uint16_t)(m.sedge[c][s] & 0xffffu);
+            if (l == 0) {
+                A.cell_off[3 * (size_t)i] = ov;
+                A.cell_off[3 * (size_t)i + 1] = of;
+                A.cell_off[3 * (size_t)i + 2] = oi;
+            }
+        }
+    } else if (A.fnb_only) {
+        // reduced mode keeps the neighbour ids (face order) only
+        uint64_t of = 0;
+        if (l == 0) of = atomicAdd((unsigned long long *)&A.arena_top[1], (unsigned long long)P.F);
+        of = bcast_u64(of);
+        if (of + P.F > A.arena_cap[1]) {
+            if (l == 0) A.status[i] = kCellArena;
+        } else {
+            for (int f = l; f < P.F; f += 64) A.fnb[of + f] = m.fnb[P.cur][f];
+            if (l == 0) A.cell_off[3 * (size_t)i + 1] = of;
+        }
+    }
+    __syncthreads();
+}
+
+template <class C>
+__global__ __launch_bounds__(64) void k_cells(TessArgs A) {
+    __shared__ Mem<C> m;
+    for (;;) {
+        uint32_t q = 0;
+        if (lane_id() == 0) q = atomicAdd(A.queue, 1u);
+        q = first_lane_u32(q);
+        if (q >= A.ncells) break;
+        process_cell<C>(m, A, A.cells[q]);
+    }
+}
+
+using Tier0 = Cap<96, 288, 48, 144>;
+using Tier1 = Cap<1024, 2560, 320, 1280>;
+
+}  // namespace
+
+int tier_max_vertices(int tier) { return tier == 0 ? Tier0::VM : Tier1::VM; }
+
+size_t tier_lds_bytes(int tier) { return tier == 0 ? sizeof(Mem<Tier0>) : sizeof(Mem<Tier1>); }
+
+hipError_t launch_cells(int tier, const TessArgs &a, int grid_waves, hipStream_t st) {
+    if (a.ncells == 0) return hipSuccess;
+    if (tier == 0) k_cells<Tier0><<<grid_waves, 64, 0, st>>>(a);
+    else k_cells<Tier1><<<grid_waves, 64, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t cells_occupancy(int tier, int *blocks_per_cu) {
+    if (tier == 0)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1>, 64, 0);
+}
+
+}  // namespace vd

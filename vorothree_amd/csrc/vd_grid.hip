@@ -1,0 +1,282 @@
+// Device-side generator handling for Algorithm3DGrid: wall admission filter, stable compaction,
+// stable counting-sort binning, bin-ordered position copy, and the Lloyd (relax) update.
+//
+// Reference: src/tessellation.rs:43-67 (set_generators), :175-200 (add_wall / prune),
+// src/algorithm/algo_3d_grid.rs:98-130 (get_bin_index, set_generators), src/tessellation.rs:482-495
+// (relax). All HBM-bound integer/byte work: one thread per element, coalesced where the access
+// pattern allows (generators and flags), 256-thread blocks.
+#include "vd_internal.h"
+
+namespace vd {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned blocks_for(uint64_t n, int per = kBlock) {
+    return (unsigned)((n + per - 1) / per);
+}
+
+// Wall::contains for every wall in insertion order (wall_3d.rs:31-36 plane, :73-78 sphere).
+__device__ inline bool wall_contains(const WallDev &w, double x, double y, double z) {
+    double dx = x - w.a[0], dy = y - w.a[1], dz = z - w.a[2];
+    if (w.kind == 0) return (dx * w.b[0] + dy * w.b[1] + dz * w.b[2]) >= 0.0;
+    return (dx * dx + dy * dy + dz * dz) <= w.r * w.r;
+}
+
+struct WallPack { WallDev w[kMaxWalls]; };
+
+__global__ void k_wall_flags(const double *__restrict__ xyz, uint32_t n, WallPack walls, int nw,
+                             uint32_t *__restrict__ flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
+    uint32_t in = 1;
+    for (int k = 0; k < nw; ++k)
+        if (!wall_contains(walls.w[k], x, y, z)) { in = 0; break; }
+    flags[i] = in;
+}
+
+// Keeps the flagged points in input order (the indices are compacted, tessellation.rs:58-60).
+__global__ void k_compact3(const double *__restrict__ xyz, const uint32_t *__restrict__ flags,
+                           const uint32_t *__restrict__ pos, uint32_t n, double *__restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i]) return;
+    size_t d = 3 * (size_t)pos[i], s = 3 * (size_t)i;
+    out[d] = xyz[s]; out[d + 1] = xyz[s + 1]; out[d + 2] = xyz[s + 2];
+}
+
+// get_bin_index per axis (algo_3d_grid.rs:102-104): clamp((x - min) * s, 0, n - 1e-5) as usize.
+// f64::clamp passes NaN through and `NaN as usize` is 0.
+__device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
+    double v = (x - lo) * s;
+    if (v < 0.0) v = 0.0;
+    if (v > lim) v = lim;
+    if (!(v == v)) return 0;
+    return (uint32_t)v;
+}
+
+__global__ void k_bin_count(const double *__restrict__ xyz, uint32_t n, GridGeom g,
+                            uint32_t *__restrict__ gen_bin, uint32_t *__restrict__ counts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    size_t o = 3 * (size_t)i;
+    uint32_t ix = axis_bin(xyz[o], g.lo[0], g.s[0], g.lim[0]);
+    uint32_t iy = axis_bin(xyz[o + 1], g.lo[1], g.s[1], g.lim[1]);
+    uint32_t iz = axis_bin(xyz[o + 2], g.lo[2], g.s[2], g.lim[2]);
+    uint32_t b = ix + iy * (uint32_t)g.n[0] + iz * (uint32_t)g.n[0] * (uint32_t)g.n[1];
+    gen_bin[i] = b;
+    atomicAdd(&counts[b], 1u);
+}
+
+__global__ void k_bin_scatter(const uint32_t *__restrict__ gen_bin, uint32_t n,
+                              const uint32_t *__restrict__ bin_start, uint32_t *__restrict__ fill,
+                              uint32_t *__restrict__ items) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = gen_bin[i];
+    uint32_t k = atomicAdd(&fill[b], 1u);
+    items[bin_start[b] + k] = i;
+}
+
+// The atomic scatter is unordered; each bin list must be ascending (the reference pushes in
+// index order, algo_3d_grid.rs:121-129). Bins hold a handful of items, so a per-bin insertion
+// sort by one thread is the cheap way to restore the order.
+__global__ void k_bin_sort(const uint32_t *__restrict__ bin_start, uint32_t nbins,
+                           uint32_t *__restrict__ items) {
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    uint32_t lo = bin_start[b], hi = bin_start[b + 1];
+    for (uint32_t k = lo + 1; k < hi; ++k) {
+        uint32_t v = items[k];
+        uint32_t j = k;
+        while (j > lo && items[j - 1] > v) { items[j] = items[j - 1]; --j; }
+        items[j] = v;
+    }
+}
+
+__global__ void k_fill_spos(const double *__restrict__ xyz, const uint32_t *__restrict__ items,
+                            uint32_t n, double4 *__restrict__ spos) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t i = items[k];
+    size_t o = 3 * (size_t)i;
+    double4 r;
+    r.x = xyz[o]; r.y = xyz[o + 1]; r.z = xyz[o + 2];
+    r.w = __longlong_as_double((long long)i);
+    spos[k] = r;
+}
+
+// Tessellation::relax (tessellation.rs:482-495): the centroid of each non-empty cell, the old
+// position for an empty one.
+__global__ void k_relax(const double *__restrict__ gen, const double *__restrict__ cen,
+                        const uint32_t *__restrict__ nvert, uint32_t n, double *__restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *src = nvert[i] == 0 ? gen : cen;
+    size_t o = 3 * (size_t)i;
+    out[o] = src[o]; out[o + 1] = src[o + 1]; out[o + 2] = src[o + 2];
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Sorts the listed cells by outcome of a cell launch: capacity overflow -> out_cap, shell prefix
+// too short -> out_shell; counts[0..3] = capacity, shell, arena-full, hard errors.
+__global__ void k_collect(const int32_t *__restrict__ status, const uint32_t *__restrict__ list,
+                          uint32_t n, uint32_t *__restrict__ out_cap, uint32_t *__restrict__ out_shell,
+                          uint32_t *__restrict__ counts) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t i = list[k];
+    int32_t s = status[i];
+    if (s == kCellOk) return;
+    if (s == kCellCapacity) out_cap[atomicAdd(&counts[0], 1u)] = i;
+    else if (s == kCellShell) out_shell[atomicAdd(&counts[1], 1u)] = i;
+    else if (s == kCellArena) atomicAdd(&counts[2], 1u);
+    else atomicAdd(&counts[3], 1u);
+}
+
+// ---- exclusive scan of u32 (3 phases: tile scan, scan of tile sums, add) -------------------
+constexpr int kScanTile = 2048;  // elements per 256-thread block (8 per thread)
+
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t &total) {
+    // Hillis-Steele over 256 threads in LDS.
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {
+        uint32_t x = t >= d ? sh[t - d] : 0u;
+        __syncthreads();
+        sh[t] += x;
+        __syncthreads();
+    }
+    total = sh[kBlock - 1];
+    uint32_t incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void k_scan_tiles(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                             uint32_t n, uint32_t *__restrict__ sums) {
+    __shared__ uint32_t sh[kBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * 8;
+    uint32_t v[8];
+    uint32_t local = 0;
+    for (int k = 0; k < 8; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        local += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan(local, sh, total);
+    for (int k = 0; k < 8; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(uint32_t *__restrict__ out, uint32_t n, const uint32_t *__restrict__ offs) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * 8;
+    uint32_t add = offs[blockIdx.x];
+    for (int k = 0; k < 8; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+}  // namespace
+
+size_t scan_tmp_words(uint32_t n) {
+    // tile sums and their scans at every level
+    size_t words = 0;
+    uint64_t m = n;
+    while (m > 1) {
+        m = (m + kScanTile - 1) / kScanTile;
+        words += 2 * m + 1;
+    }
+    return words + 2;
+}
+
+// Exclusive scan of n values; out may equal in. tmp holds scan_tmp_words(n) u32.
+hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp,
+                           size_t tmp_words, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (tmp_words < scan_tmp_words(n)) return hipErrorInvalidValue;
+    uint32_t tiles = (uint32_t)((n + kScanTile - 1) / kScanTile);
+    uint32_t *sums = tmp;
+    uint32_t *sums_scan = tmp + tiles;
+    k_scan_tiles<<<tiles, kBlock, 0, st>>>(in, out, n, sums);
+    if (tiles > 1) {
+        hipError_t e = launch_scan_u32(sums, sums_scan, tiles, tmp + 2 * tiles + 1,
+                                       tmp_words - (2 * tiles + 1), st);
+        if (e != hipSuccess) return e;
+        k_scan_add<<<tiles, kBlock, 0, st>>>(out, n, sums_scan);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_wall_flags(const double *xyz, uint32_t n, const WallDev *walls, int nwalls,
+                             uint32_t *flags, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    WallPack p{};
+    for (int k = 0; k < nwalls && k < kMaxWalls; ++k) p.w[k] = walls[k];
+    k_wall_flags<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, p, nwalls, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact3(const double *xyz, const uint32_t *flags, const uint32_t *pos,
+                           uint32_t n, double *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_compact3<<<blocks_for(n), kBlock, 0, st>>>(xyz, flags, pos, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_count(const double *xyz, uint32_t n, GridGeom g, uint32_t *gen_bin,
+                            uint32_t *counts, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_bin_count<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, g, gen_bin, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint32_t n, const uint32_t *bin_start,
+                              uint32_t *fill, uint32_t *items, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_bin_scatter<<<blocks_for(n), kBlock, 0, st>>>(gen_bin, n, bin_start, fill, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_sort(const uint32_t *bin_start, uint32_t nbins, uint32_t *items,
+                           hipStream_t st) {
+    if (nbins == 0) return hipSuccess;
+    k_bin_sort<<<blocks_for(nbins), kBlock, 0, st>>>(bin_start, nbins, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint32_t n, double4 *spos,
+                            hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_fill_spos<<<blocks_for(n), kBlock, 0, st>>>(xyz, items, n, spos);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_fill_u32<<<blocks_for(n), kBlock, 0, st>>>(p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n,
+                          uint32_t *out_cap, uint32_t *out_shell, uint32_t *counts, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_collect<<<blocks_for(n), kBlock, 0, st>>>(status, list, n, out_cap, out_shell, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint32_t n,
+                        double *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_relax<<<blocks_for(n), kBlock, 0, st>>>(gen, cen, nvert, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace vd

@@ -1,0 +1,113 @@
+// Internal interface between the C-ABI host code (vd_api.cpp) and the gfx950 kernels (*.hip).
+// Not installed; the public interface is include/vd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace vd {
+
+// One entry of the bin shell order of Algorithm3DGrid (src/algorithm/algo_3d_grid.rs:56-76):
+// offset (dx, dy, dz) and its minimum squared distance. 16 bytes, loaded 64 at a time.
+struct alignas(16) ShellEntry {
+    double d2;
+    int16_t dx, dy, dz, pad;
+};
+
+// A wall as the kernels see it (src/wall/wall_3d.rs:8-101). kind 0 = plane, 1 = sphere.
+struct WallDev {
+    int32_t id;
+    int32_t kind;
+    double a[3];   // plane point / sphere centre
+    double b[3];   // plane unit normal pointing into the valid region
+    double r;      // sphere radius
+};
+
+constexpr int kMaxWalls = 8;
+
+// Grid geometry, computed on the host exactly as Algorithm3DGrid::new (algo_3d_grid.rs:47-95).
+struct GridGeom {
+    double lo[3];
+    double s[3];     // bins per unit length
+    double cs[3];    // 1 / s
+    double lim[3];   // n - 1e-5
+    int32_t n[3];
+};
+
+// Everything one tessellation's cell kernel reads and writes. Device pointers.
+struct TessArgs {
+    GridGeom grid;
+    const double *gen;            // generators, xyz AoS, index order
+    const double4 *spos;          // generators in bin order: x, y, z, index (bit pattern of u64)
+    const uint32_t *bin_start;    // nbins + 1
+    const ShellEntry *shell;      // stably sorted prefix of the shell order
+    uint32_t nshell;
+    double shell_next_d2;         // key of the first entry not in the prefix (+inf if complete)
+    int32_t nwalls;
+    WallDev walls[kMaxWalls];
+    double box_lo[3], box_hi[3];
+    uint32_t ngen;
+
+    // work list: cell (generator) indices to compute
+    const uint32_t *cells;
+    uint32_t ncells;
+    uint32_t *queue;              // atomic work counter (zeroed before launch)
+
+    // reduced outputs, indexed by generator index
+    double *vol;
+    double *cen;                  // 3 per cell
+    uint32_t *nface;
+    uint32_t *nvert;              // vertex count (0 = empty cell)
+    int32_t *status;              // 0 ok, >0 needs a retry (1 capacity, 2 shell prefix), <0 error
+
+    // full cell export (optional: vtx == nullptr disables)
+    double *vtx;                  // arena of vertices, 3 doubles each
+    uint8_t *fcnt;                // arena of face vertex counts
+    int32_t *fnb;                 // arena of face neighbour ids (same indexing as fcnt)
+    uint16_t *fidx;               // arena of face vertex indices
+    uint64_t *arena_top;          // [0] vertices, [1] faces, [2] face indices
+    uint64_t arena_cap[3];
+    uint64_t *cell_off;           // 3 per cell: vertex, face, index offsets
+    int32_t fnb_only;             // without vtx: still export the neighbour ids (fnb arena)
+};
+
+// Status codes written per cell.
+enum : int32_t {
+    kCellOk = 0,
+    kCellCapacity = 1,     // polytope exceeded this tier's LDS capacity: retry in a larger tier
+    kCellShell = 2,        // walk ran past the shell prefix: retry with a longer prefix
+    kCellArena = 3,        // output arena full: grow and retry
+    kCellFace255 = -1,     // a face reached 256 vertices (the reference's u8 count would wrap)
+};
+
+// Launchers (vd_grid.hip, vd_cells.hip).
+hipError_t launch_wall_flags(const double *xyz, uint32_t n, const WallDev *walls, int nwalls,
+                             uint32_t *flags, hipStream_t st);
+hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp,
+                           size_t tmp_words, hipStream_t st);
+size_t scan_tmp_words(uint32_t n);
+hipError_t launch_compact3(const double *xyz, const uint32_t *flags, const uint32_t *pos,
+                           uint32_t n, double *out, hipStream_t st);
+hipError_t launch_bin_count(const double *xyz, uint32_t n, GridGeom g, uint32_t *gen_bin,
+                            uint32_t *counts, hipStream_t st);
+hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint32_t n, const uint32_t *bin_start,
+                              uint32_t *fill, uint32_t *items, hipStream_t st);
+hipError_t launch_bin_sort(const uint32_t *bin_start, uint32_t nbins, uint32_t *items,
+                           hipStream_t st);
+hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint32_t n, double4 *spos,
+                            hipStream_t st);
+hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint32_t n,
+                        double *out, hipStream_t st);
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
+hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n,
+                          uint32_t *out_cap, uint32_t *out_shell, uint32_t *counts, hipStream_t st);
+
+// tier: 0 = small LDS polytope (many waves per CU), 1 = large LDS polytope (one wave per CU)
+hipError_t launch_cells(int tier, const TessArgs &a, int grid_waves, hipStream_t st);
+hipError_t cells_occupancy(int tier, int *blocks_per_cu);
+int tier_max_vertices(int tier);
+size_t tier_lds_bytes(int tier);
+
+}  // namespace vd

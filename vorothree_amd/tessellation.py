@@ -1,0 +1,330 @@
+"""Python mirror of voronoid's public API for the 3D grid path, over the HIP C ABI.
+
+Names, argument meaning and error behaviour follow the reference crate (mdt-re/vorothree):
+
+* ``BoundingBox``                  src/bounds/mod.rs:2-12, ``box_side`` :21-23
+* ``Algorithm3DGrid(nx, ny, nz, bounds)``  src/algorithm/algo_3d_grid.rs:47-95
+* ``Wall(id, geometry)``           src/wall/mod.rs:24-32 (id > WALL_ID_MAX raises, the panic)
+* ``PlaneGeometry``, ``SphereGeometry``   src/wall/wall_3d.rs:8-101
+* ``Tessellation(bounds, algorithm)``     src/tessellation.rs:13-36 with ``set_generators`` :43-67,
+  ``add_wall`` :197-200, ``clear_walls`` :203-205, ``calculate`` :213-230, ``map`` :391-412,
+  ``relax`` :482-495, ``count_generators``/``count_cells`` :498-505, ``get_cell`` :514-516
+  (None when out of range), ``generators`` :519-521
+* ``Cell3DFaces`` accessors        src/cell/cell_3d_faces.rs:76-99, ``volume`` :106-143,
+  ``centroid`` :145-205, ``face_area`` :207-254, ``is_empty`` :528-530
+
+Every computation runs in libvoronoid_hip.so on the GPU; the Cell3DFaces methods evaluate the
+returned cell on the host with the reference formulas.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+WALL_ID_MAX = -1000
+
+
+def box_side(axis: int, is_max: bool) -> int:
+    """Id of a bounding-box face: x- -1, x+ -2, y- -3, y+ -4, z- -5, z+ -6."""
+    return -1 - (axis * 2 + (1 if is_max else 0))
+
+
+def _d3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class BoundingBox:
+    def __init__(self, min, max):  # noqa: A002 - reference field names
+        self.min = [float(x) for x in min]
+        self.max = [float(x) for x in max]
+
+
+class Algorithm3DGrid:
+    """Uniform bin grid; the index itself is built on the device by ``set_generators``."""
+
+    def __init__(self, nx: int, ny: int, nz: int, bounds: BoundingBox):
+        self.grid_res_x, self.grid_res_y, self.grid_res_z = int(nx), int(ny), int(nz)
+        self.bounds = bounds
+
+    def get_bin_index(self, x: float, y: float, z: float, bounds: BoundingBox) -> int:
+        """algo_3d_grid.rs:98-107 (host evaluation, same arithmetic as the device binning)."""
+        n = (self.grid_res_x, self.grid_res_y, self.grid_res_z)
+        idx = []
+        for a, v in enumerate((x, y, z)):
+            s = n[a] / (bounds.max[a] - bounds.min[a])
+            t = (v - bounds.min[a]) * s
+            lim = n[a] - 1e-5
+            if t < 0.0:
+                t = 0.0
+            if t > lim:
+                t = lim
+            idx.append(0 if t != t else int(t))
+        return idx[0] + idx[1] * n[0] + idx[2] * n[0] * n[1]
+
+
+class PlaneGeometry:
+    def __init__(self, point, normal):
+        self.point = [float(x) for x in point]
+        self.normal_raw = [float(x) for x in normal]
+
+
+class SphereGeometry:
+    def __init__(self, center, radius):
+        self.center = [float(x) for x in center]
+        self.radius = float(radius)
+
+
+class Wall:
+    def __init__(self, id: int, geometry):  # noqa: A002
+        if id > WALL_ID_MAX:
+            raise ValueError(f"Wall ID must be <= {WALL_ID_MAX}")
+        if not isinstance(geometry, (PlaneGeometry, SphereGeometry)):
+            raise TypeError("only PlaneGeometry and SphereGeometry walls run on the device")
+        self._id = int(id)
+        self.geometry = geometry
+
+    def id(self) -> int:
+        return self._id
+
+
+class Cell3DFaces:
+    """A computed cell: the four arrays of the reference struct (cell_3d_faces.rs:23-34)."""
+
+    def __init__(self, id, vertices, face_counts, face_indices, face_neighbors):  # noqa: A002
+        self._id = id
+        self._v = vertices
+        self._fc = face_counts
+        self._fi = face_indices
+        self._fn = face_neighbors
+
+    def id(self) -> int:
+        return self._id
+
+    def vertices(self) -> np.ndarray:
+        return self._v.copy()
+
+    def face_counts(self) -> np.ndarray:
+        return self._fc.copy()
+
+    def face_indices(self) -> np.ndarray:
+        return self._fi.copy()
+
+    def face_neighbors(self) -> np.ndarray:
+        return self._fn.copy()
+
+    def neighbors(self) -> np.ndarray:
+        return self._fn
+
+    def is_empty(self) -> bool:
+        return self._v.size == 0
+
+    def _fan(self):
+        v = self._v.reshape(-1, 3)
+        off = 0
+        for cnt in self._fc:
+            cnt = int(cnt)
+            if cnt >= 3:
+                a = v[self._fi[off]]
+                for i in range(1, cnt - 1):
+                    yield a, v[self._fi[off + i]], v[self._fi[off + i + 1]]
+            off += cnt
+
+    def volume(self) -> float:
+        vol = 0.0
+        for a, b, c in self._fan():
+            a0, a1, a2 = float(a[0]), float(a[1]), float(a[2])
+            b0, b1, b2 = float(b[0]), float(b[1]), float(b[2])
+            c0, c1, c2 = float(c[0]), float(c[1]), float(c[2])
+            vol += a0 * (b1 * c2 - b2 * c1) + a1 * (b2 * c0 - b0 * c2) + a2 * (b0 * c1 - b1 * c0)
+        return abs(vol / 6.0)
+
+    def centroid(self):
+        cx = cy = cz = tv = 0.0
+        for a, b, c in self._fan():
+            a0, a1, a2 = float(a[0]), float(a[1]), float(a[2])
+            b0, b1, b2 = float(b[0]), float(b[1]), float(b[2])
+            c0, c1, c2 = float(c[0]), float(c[1]), float(c[2])
+            crx = b1 * c2 - b2 * c1
+            cry = b2 * c0 - b0 * c2
+            crz = b0 * c1 - b1 * c0
+            det = a0 * crx + a1 * cry + a2 * crz
+            tv += det
+            cx += det * (a0 + b0 + c0)
+            cy += det * (a1 + b1 + c1)
+            cz += det * (a2 + b2 + c2)
+        if abs(tv) < 1e-9:
+            return [0.0, 0.0, 0.0]
+        f = 1.0 / (4.0 * tv)
+        return [cx * f, cy * f, cz * f]
+
+    def face_area(self, face_index: int) -> float:
+        if face_index >= len(self._fc):
+            return 0.0
+        off = int(sum(int(c) for c in self._fc[:face_index]))
+        cnt = int(self._fc[face_index])
+        if cnt < 3:
+            return 0.0
+        v = self._v.reshape(-1, 3)
+        p0 = v[self._fi[off]]
+        area = 0.0
+        for i in range(1, cnt - 1):
+            p1 = v[self._fi[off + i]]
+            p2 = v[self._fi[off + i + 1]]
+            u = [float(p1[k]) - float(p0[k]) for k in range(3)]
+            w = [float(p2[k]) - float(p0[k]) for k in range(3)]
+            cx = u[1] * w[2] - u[2] * w[1]
+            cy = u[2] * w[0] - u[0] * w[2]
+            cz = u[0] * w[1] - u[1] * w[0]
+            area += 0.5 * math.sqrt(cx * cx + cy * cy + cz * cz)
+        return area
+
+
+class Tessellation:
+    """``Tessellation::<3, Cell3DFaces, Algorithm3DGrid>`` on the GPU."""
+
+    def __init__(self, bounds: BoundingBox, algorithm: Algorithm3DGrid, keep_cells: bool = True):
+        self.bounds = bounds
+        self.algorithm = algorithm
+        self._walls = []
+        h = ctypes.c_void_p()
+        check(lib().vd_tess_create(_d3(bounds.min), _d3(bounds.max), algorithm.grid_res_x,
+                                   algorithm.grid_res_y, algorithm.grid_res_z, ctypes.byref(h)))
+        self._h = h
+        if not keep_cells:
+            check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_KEEP_CELLS, 0))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().vd_tess_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------------------------
+    def set_generators(self, generators) -> None:
+        a = np.ascontiguousarray(np.asarray(generators, dtype=np.float64).reshape(-1))
+        n = a.size // 3
+        check(lib().vd_tess_set_generators(self._h, _ptr(a), n))
+
+    def set_generators_device(self, ptr: int, n: int) -> None:
+        """Points already in device memory (e.g. a torch CUDA tensor's data_ptr())."""
+        check(lib().vd_tess_set_generators_device(self._h, ctypes.c_void_p(int(ptr)), int(n)))
+
+    def add_wall(self, wall: Wall) -> None:
+        g = wall.geometry
+        if isinstance(g, PlaneGeometry):
+            check(lib().vd_tess_add_wall_plane(self._h, wall.id(), _d3(g.point), _d3(g.normal_raw)))
+        else:
+            check(lib().vd_tess_add_wall_sphere(self._h, wall.id(), _d3(g.center), g.radius))
+        self._walls.append(wall)
+
+    def clear_walls(self) -> None:
+        check(lib().vd_tess_clear_walls(self._h))
+        self._walls = []
+
+    def calculate(self) -> None:
+        check(lib().vd_tess_calculate(self._h))
+
+    def relax(self) -> None:
+        check(lib().vd_tess_relax(self._h))
+
+    def set_keep_cells(self, keep: bool) -> None:
+        check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_KEEP_CELLS, 1 if keep else 0))
+
+    # -- queries -------------------------------------------------------------------------------
+    def count_generators(self) -> int:
+        return int(lib().vd_tess_count_generators(self._h))
+
+    def count_cells(self) -> int:
+        return int(lib().vd_tess_count_cells(self._h))
+
+    def generators(self) -> np.ndarray:
+        n = self.count_generators()
+        out = np.empty(3 * n, dtype=np.float64)
+        check(lib().vd_tess_get_generators(self._h, _ptr(out), n))
+        return out
+
+    def get_generator(self, index: int):
+        return self.generators()[3 * index:3 * index + 3]
+
+    def volumes(self) -> np.ndarray:
+        n = self.count_cells()
+        out = np.empty(n, dtype=np.float64)
+        check(lib().vd_tess_get_volumes(self._h, _ptr(out), n))
+        return out
+
+    def centroids(self) -> np.ndarray:
+        n = self.count_cells()
+        out = np.empty((n, 3), dtype=np.float64)
+        check(lib().vd_tess_get_centroids(self._h, _ptr(out), n))
+        return out
+
+    def face_counts(self) -> np.ndarray:
+        """Number of faces per cell."""
+        n = self.count_cells()
+        out = np.empty(n, dtype=np.uint32)
+        check(lib().vd_tess_get_face_counts(self._h, _ptr(out), n))
+        return out
+
+    def neighbors(self):
+        """(offsets[n+1], ids) CSR of every cell's face_neighbors in face order."""
+        n = self.count_cells()
+        off = np.empty(n + 1, dtype=np.uint64)
+        check(lib().vd_tess_get_neighbors(self._h, _ptr(off), n, None, 0))
+        ids = np.empty(int(off[-1]), dtype=np.int32)
+        check(lib().vd_tess_get_neighbors(self._h, _ptr(off), n, _ptr(ids), ids.size))
+        return off, ids
+
+    def get_cell(self, index: int):
+        if index < 0 or index >= self.count_cells():
+            return None
+        sizes = np.zeros(3, dtype=np.uint64)
+        check(lib().vd_tess_get_cell(self._h, int(index), _ptr(sizes), None, None, None, None))
+        nv, nf, ni = (int(x) for x in sizes)
+        v = np.empty(3 * nv, dtype=np.float64)
+        fc = np.empty(nf, dtype=np.uint32)
+        fi = np.empty(ni, dtype=np.uint32)
+        fn = np.empty(nf, dtype=np.int32)
+        check(lib().vd_tess_get_cell(self._h, int(index), _ptr(sizes), _ptr(v), _ptr(fc), _ptr(fi), _ptr(fn)))
+        return Cell3DFaces(int(index), v, fc, fi, fn)
+
+    def cells(self):
+        return [self.get_cell(i) for i in range(self.count_cells())]
+
+    def map(self, f):
+        """Tessellation::map: compute every cell and apply f to it."""
+        self.calculate()
+        return [f(self.get_cell(i)) for i in range(self.count_cells())]
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.float64)
+        check(lib().vd_tess_get_stats(self._h, _ptr(out)))
+        return out
+
+
+def shell_prefix(bounds: BoundingBox, nx: int, ny: int, nz: int, kmax: int):
+    """Host-side stable shell-order prefix (see vd_shell_prefix in include/vd.h)."""
+    count = ctypes.c_size_t(0)
+    nxt = ctypes.c_double(0.0)
+    check(lib().vd_shell_prefix(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, kmax, None, None, 0,
+                                ctypes.byref(count), ctypes.byref(nxt)))
+    m = count.value
+    dxyz = np.empty((m, 3), dtype=np.int32)
+    keys = np.empty(m, dtype=np.float64)
+    check(lib().vd_shell_prefix(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, kmax, _ptr(dxyz),
+                                _ptr(keys), m, ctypes.byref(count), ctypes.byref(nxt)))
+    return dxyz, keys, nxt.value
