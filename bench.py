@@ -1,0 +1,148 @@
+"""Benchmark: Voronoi cells/s of the plain GPU path on one MI355X.
+
+Workload (default): unit cube, 1,000,000 uniform f64 generators (seeded PCG32, seed 2), grid
+100^3 bins, one SphereGeometry wall centred (0.5, 0.5, 0.5) with r = 0.5 (about 524k surviving
+cells). One step = set_generators from HBM-resident points (wall filter, compaction, binning) plus
+calculate() (every cell, full cells kept, as the reference's calculate stores them). The points
+are uploaded once before the timed region.
+
+Prints one JSON line. ``roofline`` is the FP64 roofline of the cell kernel (the path has no dense
+contraction, so it is bound by FP64 vector work and latency, not by HBM or matrix cores): the
+algorithmic flops of the launch, counted per cell by the oracle with the same rules, divided by
+the kernel time measured with HIP events on the library's stream. ``cpu_baseline`` times the CPU
+oracle (a C++ restatement of the reference's calculate) on the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Voronoi cells/s (f64, 1M uniform) batched vs plain GPU; p50/p99 serve latency"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--n", type=int, default=1_000_000)
+    p.add_argument("--grid", type=int, default=100)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--cpu-threads", type=int, default=16)
+    p.add_argument("--reduced", action="store_true", help="keep reductions only, not full cells")
+    return p.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": a.gpus}), flush=True)
+        sys.exit(2)
+
+    from vorothree_amd import Algorithm3DGrid, BoundingBox, SphereGeometry, Tessellation, Wall
+    from vorothree_amd import _hip
+    from vorothree_amd import workloads as W
+
+    w = W.config2(n=a.n, grid=a.grid)
+    b = BoundingBox(w.lo, w.hi)
+    t = Tessellation(b, Algorithm3DGrid(*w.grid, b), keep_cells=not a.reduced)
+    t.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5)))
+    dpts = _hip.DeviceArray(w.points)
+    _hip.synchronize()
+
+    def step():
+        t.set_generators_device(dpts.ptr.value, a.n)
+        t.calculate()
+
+    for _ in range(a.warmup):
+        step()
+    _hip.synchronize()
+    kern_us = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+        kern_us.append(float(t.stats()[4]))
+    _hip.synchronize()
+    dt = time.perf_counter() - t0
+    st = t.stats()
+    cells = t.count_cells()
+    ms = 1000.0 * dt / a.steps
+    value = cells / (dt / a.steps)
+    kern_s = float(np.mean(kern_us)) * 1e-6
+
+    # algorithmic work of one cell launch, counted by the oracle over the same cells
+    cpu = None
+    flops_per_cell = None
+    if not a.no_cpu:
+        from oracle import OracleTess
+        o = OracleTess(w.lo, w.hi, *w.grid)
+        o.add_wall_sphere(-1000, (0.5, 0.5, 0.5), 0.5)
+        o.set_generators(w.points)
+        c0 = time.perf_counter()
+        o.calculate(threads=a.cpu_threads)
+        cdt = time.perf_counter() - c0
+        _, _, _, work = o.reduced()
+        flops_per_cell = float(work[:, 0].mean())
+        cpu = {"value": o.count_cells() / cdt, "unit": "cells/s", "cores": a.cpu_threads,
+               "kind": "port",
+               "sample": f"the whole workload: all {o.count_cells()} cells of the timed "
+                         f"tessellation, one calculate() on {a.cpu_threads} threads "
+                         f"(std::thread, dynamic chunks of 64 cells); host nproc={os.cpu_count()}"}
+        del o
+
+    # algorithmic HBM bytes of the launch: generator + bin-list read, cell + reductions written
+    nv = nf = ns = 0
+    if not a.reduced:
+        fc = t.face_counts()
+        nf = int(fc.sum())
+    vol = t.volumes()
+    bytes_launch = cells * (24 + 8 + 8 + 24 + 4 + 4 + 24) + nf * 5
+    roof = None
+    if flops_per_cell is not None:
+        achieved = cells * flops_per_cell / kern_s / 1e12
+        roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "kernel": "k_cells<Tier0>", "kernel_ms": kern_s * 1e3,
+                "flops_per_cell": flops_per_cell,
+                "hbm": {"achieved": bytes_launch / kern_s / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": bytes_launch / kern_s / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_launch": bytes_launch}}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG32 uniform points)",
+        "config": {"workload": "plain GPU, one tessellation: unit cube, 1M uniform f64 generators, "
+                               "grid 100^3, sphere wall c=(0.5,0.5,0.5) r=0.5",
+                   "generators": a.n, "cells": cells, "grid": list(w.grid),
+                   "keep_cells": not a.reduced, "path": "set_generators(device) + calculate()",
+                   "cells_retried_large_tier": st[1], "shell_prefix_entries": st[3],
+                   "volume_sum": float(vol.sum())},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
