@@ -41,11 +41,29 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--cpu-threads", type=int, default=16)
     p.add_argument("--reduced", action="store_true", help="keep reductions only, not full cells")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     return p.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    """One hipMemcpy device-to-device of a known size, so FETCH_SIZE / WRITE_SIZE counters read
+    under rocprofv3 can be scaled to bytes."""
+    from vorothree_amd import _hip
+    n = mib * (1 << 20) // 8
+    src = _hip.DeviceArray(np.ones(n))
+    dst = _hip.DeviceArray(np.zeros(n))
+    _hip.synchronize()
+    rc = _hip.hip().hipMemcpy(dst.ptr, src.ptr, n * 8, 3)
+    _hip.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "rc": rc}), flush=True)
 
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}), flush=True)

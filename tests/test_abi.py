@@ -1,0 +1,73 @@
+"""C ABI of libvoronoid_hip.so without a GPU: the library loads, exports every entry point that
+include/vd.h declares, and its host-only shell-order builder equals the oracle's stable order."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import OracleTess
+from vorothree_amd import BoundingBox, Wall, shell_prefix
+from vorothree_amd import _lib
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "vd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), "Python binding and header disagree"
+
+
+def test_last_error_and_invalid_arguments():
+    lib = _lib.lib()
+    out = ctypes.c_void_p()
+    lo = (ctypes.c_double * 3)(0, 0, 0)
+    hi = (ctypes.c_double * 3)(1, 1, 1)
+    rc = lib.vd_tess_create(lo, hi, 0, 1, 1, ctypes.byref(out))   # zero bins: rejected before any device call
+    assert rc == _lib.VD_EINVAL and not out.value
+    assert b"grid" in lib.vd_last_error()
+
+
+def test_wall_id_check():
+    # src/wall/mod.rs:24-27 panics for id > WALL_ID_MAX
+    from vorothree_amd import SphereGeometry
+    with pytest.raises(ValueError):
+        Wall(-999, SphereGeometry((0, 0, 0), 1.0))
+    Wall(-1000, SphereGeometry((0, 0, 0), 1.0))
+
+
+@pytest.mark.parametrize("grid,lo,hi,k", [
+    ((10, 10, 10), (0, 0, 0), (10, 10, 10), 7),
+    ((3, 4, 5), (0, 0, 0), (1, 2, 0.5), 3),
+    ((6, 6, 6), (-10, -10, -10), (10, 10, 10), 40),
+    ((20, 7, 9), (0, 0, 0), (1.3, 0.4, 0.9), 5),
+    ((1, 1, 1), (0, 0, 0), (1, 1, 1), 2),
+])
+def test_shell_prefix_matches_oracle(grid, lo, hi, k):
+    d, keys, nxt = shell_prefix(BoundingBox(lo, hi), *grid, k)
+    o = OracleTess(lo, hi, *grid)
+    od, ok = o.shell_order()
+    m = len(keys)
+    assert m >= 1
+    assert np.array_equal(od[:m], d)
+    assert np.array_equal(ok[:m].view(np.uint64), keys.view(np.uint64))
+    if m < len(ok):
+        assert ok[m] == nxt and ok[m] > ok[m - 1] or ok[m] == nxt
+    else:
+        assert np.isinf(nxt)
+
+
+def test_shell_prefix_grows_to_the_full_table():
+    b = BoundingBox((0, 0, 0), (1, 1, 1))
+    sizes = [len(shell_prefix(b, 8, 8, 8, k)[1]) for k in (2, 4, 8, 16)]
+    assert sizes == sorted(sizes) and sizes[-1] == 17 ** 3

@@ -20,7 +20,7 @@
 //   sface[S]         per face slot: face number
 //   fstart[F+1]      first slot of each face (fstart[F] = S); fnb[F] neighbour id per face
 // "Slots" are the flattened face_indices of the reference; sedge carries the successor so a clip
-// reads each edge with one LDS access.
+// reads each edge with one LDS access. While V <= 64, lane v also keeps vertex v in registers.
 //
 // Parallel clip (one plane): lanes over vertices compute the signed distances and the kept-vertex
 // numbering (ballot prefix counts); lanes over slots classify each edge, prefix-sum the output
@@ -28,14 +28,18 @@
 // occurrence of each edge key in slot order (that occurrence's direction fixes the intersection's
 // arithmetic, exactly as the reference's insertion-ordered search does) and number the new
 // vertices in that order; lanes over the new slots find the lid segments; the lid cycle is walked
-// by the whole wave in lock step.
+// by the whole wave in lock step. Polytopes with V <= 64, S <= 192 and F <= 64 take a fast path
+// that keeps the slot data of a clip in registers (three 64-slot chunks) and uses DPP scans;
+// larger ones take the generic multi-chunk path. Both produce the same polytope.
 //
 // Neighbour walk: the shell order is consumed 64 entries at a time (one entry per lane: break
-// test, bin range, bin distance). Between two cuts the search radius is constant, so the
-// candidates up to the next cut can be tested in parallel with that radius: the wave builds a
-// window of up to 64 candidates in visiting order, tests each against the radius and the current
-// polytope, and clips with the first one that cuts. Every earlier candidate provably leaves the
-// cell unchanged, so the clip sequence equals the sequential one.
+// test, bin range, bin distance). A window of up to 64 candidates is gathered in visiting order
+// under the current search radius R^2. Candidates are then clipped in order; a clip whose
+// distance pass finds every vertex inside leaves the cell unchanged, exactly like the reference.
+// After a cut the radius shrinks, so the rest of the window is re-filtered with the new radius
+// (break test and bin test per entry, distance test per candidate) instead of being rebuilt; if a
+// cut ever grows the radius (possible by rounding), the window is rebuilt from the position after
+// the cut, because entries dropped earlier might then pass.
 #include "vd_internal.h"
 
 #include <cfloat>
@@ -67,24 +71,53 @@ __device__ inline uint64_t bcast_u64(uint64_t v) {  // lane 0's value to every l
     const uint32_t lo = first_lane_u32((uint32_t)v), hi = first_lane_u32((uint32_t)(v >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
+// Value of v in lane src (per-lane source, all lanes must be active).
+__device__ inline uint32_t lane_get(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+__device__ inline double lane_get_d(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = lane_get((uint32_t)(b & 0xffffffffll), src);
+    const uint32_t hi = lane_get((uint32_t)((unsigned long long)b >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
+template <int CTRL, int ROW>
+__device__ inline uint32_t dpp0(uint32_t x) {  // x from the DPP source lane, 0 where none
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW, 0xf, false);
+}
+
+// Inclusive prefix sum over the 64 lanes (row shifts, then row broadcasts 15 and 31).
 __device__ inline uint32_t wave_incl_scan(uint32_t x) {
-    const int l = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (l >= d) x += y;
-    }
+    x += dpp0<0x111, 0xf>(x);
+    x += dpp0<0x112, 0xf>(x);
+    x += dpp0<0x114, 0xf>(x);
+    x += dpp0<0x118, 0xf>(x);
+    x += dpp0<0x142, 0xa>(x);
+    x += dpp0<0x143, 0xc>(x);
     return x;
 }
 
 // Maximum with the reference's update rule `if d2 > m { m = d2 }` (NaN never wins).
 __device__ inline double gt_max(double m, double d2) { return d2 > m ? d2 : m; }
 
+template <int CTRL, int ROW>
+__device__ inline double dpp0_d(double x) {
+    const long long b = __double_as_longlong(x);
+    const uint32_t lo = dpp0<CTRL, ROW>((uint32_t)(b & 0xffffffffll));
+    const uint32_t hi = dpp0<CTRL, ROW>((uint32_t)((unsigned long long)b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Maximum over the wave of non-negative values (0.0 fills lanes without a DPP source).
 __device__ inline double wave_max(double x) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x = gt_max(x, __shfl_xor(x, d, 64));
-    return x;
+    x = gt_max(x, dpp0_d<0x111, 0xf>(x));
+    x = gt_max(x, dpp0_d<0x112, 0xf>(x));
+    x = gt_max(x, dpp0_d<0x114, 0xf>(x));
+    x = gt_max(x, dpp0_d<0x118, 0xf>(x));
+    x = gt_max(x, dpp0_d<0x142, 0xa>(x));
+    x = gt_max(x, dpp0_d<0x143, 0xc>(x));
+    return rld(x, 63);
 }
 
 __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
@@ -113,6 +146,7 @@ struct Mem {
     uint32_t lmap[C::VM];
     uint32_t seg[C::SM];
     int32_t fnb[2][C::FM];
+    int32_t fbase[C::FM];
     uint32_t winq[64];
     uint16_t sface[2][C::SM];
     uint16_t fstart[2][C::FM + 1];
@@ -120,19 +154,35 @@ struct Mem {
     uint16_t P[C::SM + 1];
     uint16_t cxid[C::KM];
     uint16_t fidxn[C::SM];
+    uint16_t nextp[C::SM];
     int16_t fnew[C::FM];
     uint16_t fnoff[C::FM];
+    uint16_t nend[C::FM];
     uint16_t lid[C::SM];
     uint16_t winent[64];
     uint16_t winoff[64];
 };
 
-enum ClipRc : int { kNoChange = 0, kModified = 1, kOverflow = -1, kFace255 = -2 };
+enum ClipRc : int { kNoChange = 0, kModified = 1, kOverflow = -1, kFace255 = -2, kGeneric = -3 };
 
 struct Plane { double px, py, pz, nx, ny, nz; };
 
 // The polytope state of one wave; all members are wave-uniform.
 struct Poly { int cur, V, F, S; };
+
+// Lane v's copy of vertex v (valid while V <= 64).
+struct VReg { double x, y, z; bool ok; };
+
+template <class C>
+__device__ inline void vreg_load(Mem<C> &m, const Poly &P, VReg &r) {
+    const int l = lane_id();
+    r.ok = P.V <= 64;
+    if (r.ok && l < P.V) {
+        r.x = m.vx[P.cur][l];
+        r.y = m.vy[P.cur][l];
+        r.z = m.vz[P.cur][l];
+    }
+}
 
 template <class C>
 __device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi[3]) {
@@ -169,8 +219,8 @@ __device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi
 // kModified (r2 = max |v - g|^2 over the new vertices, or 0 without a generator), or a negative
 // code when this tier's capacity is exceeded.
 template <class C>
-__device__ int poly_clip(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, bool withgen,
-                         double gx, double gy, double gz, double &r2) {
+__device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, bool withgen,
+                            double gx, double gy, double gz, double &r2) {
     const int l = lane_id();
     const uint64_t lt = lt_mask();
     const int c = P.cur, o = c ^ 1;
@@ -410,6 +460,246 @@ __device__ int poly_clip(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, bool w
     return kModified;
 }
 
+
+// Fast clip for V <= 64, S <= 192, F <= 64 (the common case): vertex distances from the register
+// copy, slot data of the clip held in registers (three 64-slot chunks), crossings and lid cycle
+// handled with lane shuffles. Same result as clip_generic. Returns kGeneric (with the current
+// polytope untouched) when the crossings or the lid do not fit its register layout.
+template <class C>
+__device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t nid, bool withgen,
+                         double gx, double gy, double gz, double &r2) {
+    const int l = lane_id();
+    const uint64_t lt = lt_mask();
+    const int c = P.cur, o = c ^ 1;
+    const bool va = l < P.V;
+
+    // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362)
+    const double d = va ? (r.x - pl.px) * pl.nx + (r.y - pl.py) * pl.ny + (r.z - pl.pz) * pl.nz : 0.0;
+    const uint64_t gtm = ballot(va && d > kTol);
+    if (gtm == 0) return kNoChange;
+    const uint64_t ltm = ballot(va && d < -kTol);
+    if (ltm == 0) {
+        P.V = P.F = P.S = 0;
+        r2 = 0.0;
+        r.ok = true;
+        return kModified;
+    }
+    const uint64_t inm = ballot(va && d <= kTol);
+    const int nk = popc(inm);
+    double md2 = 0.0;
+    if (va) m.dist[l] = d;
+    if ((inm >> l) & 1ull) {
+        const int ni = popc(inm & lt);
+        m.vx[o][ni] = r.x; m.vy[o][ni] = r.y; m.vz[o][ni] = r.z;
+        if (withgen) {
+            const double dx = r.x - gx, dy = r.y - gy, dz = r.z - gz;
+            md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
+        }
+    }
+
+    // 2. slots: output entry counts (prefix in slot order) and plane crossings (:378-452)
+    const int nch = (P.S + 63) >> 6;
+    uint32_t ea[3] = {0u, 0u, 0u};
+    uint32_t ef[3] = {0u, 0u, 0u};
+    uint32_t eP[3] = {0u, 0u, 0u};
+    int eci[3] = {0, 0, 0};
+    int K = 0, ptot = 0;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        if (ch < nch) {
+            const int s = 64 * ch + l;
+            const bool sa = s < P.S;
+            const uint32_t e = sa ? m.sedge[c][s] : 0u;
+            ea[ch] = e;
+            ef[ch] = sa ? m.sface[c][s] : 0u;
+            const uint32_t a = e & 0xffffu, b = e >> 16;
+            const bool ain = (inm >> a) & 1ull, bin = (inm >> b) & 1ull;
+            const uint64_t b0 = ballot(sa && ain);
+            const uint64_t b1 = ballot(sa && !ain && bin);
+            eP[ch] = (uint32_t)(ptot + popc(b0 & lt) + 2 * popc(b1 & lt));
+            ptot += popc(b0) + 2 * popc(b1);
+            if (sa) m.P[s] = (uint16_t)eP[ch];
+            const bool cr = sa && (ain != bin);
+            const uint64_t cm = ballot(cr);
+            eci[ch] = K + popc(cm & lt);
+            if (cr && eci[ch] < 64) {
+                m.ckey[eci[ch]] = a < b ? ((a << 16) | b) : ((b << 16) | a);
+                m.cab[eci[ch]] = a | (b << 16);
+            }
+            K += popc(cm);
+        }
+    }
+    if (K > 64) return kGeneric;
+    if (l == 0) m.P[P.S] = (uint16_t)ptot;
+    __syncthreads();
+
+    // 3. faces: survivors (>= 3 entries, :454-456) and their new offsets
+    const bool fa = l < P.F;
+    const int fs = fa ? m.fstart[c][l] : 0, fe = fa ? m.fstart[c][l + 1] : 0;
+    const int pfs = fa ? (int)m.P[fs] : 0;
+    const int tot = fa ? (int)m.P[fe] - pfs : 0;
+    const bool sv = fa && tot >= 3;
+    const uint64_t svm = ballot(sv);
+    int Fn = popc(svm);
+    const uint32_t tv = sv ? (uint32_t)tot : 0u;
+    const uint32_t inc = wave_incl_scan(tv);
+    const int off = (int)(inc - tv);
+    int Spre = (int)rlu(inc, 63);
+    if (ballot(sv && tot > 255)) return kFace255;
+    if (Fn + 1 > C::FM || Spre > C::SM) return kOverflow;
+    if (fa) {
+        const int nf = popc(svm & lt);
+        m.fnew[l] = (int16_t)(sv ? nf : -1);
+        m.fbase[l] = off - pfs;
+        m.fnoff[l] = (uint16_t)off;
+        m.nend[l] = (uint16_t)(off + tot);
+        if (sv) {
+            m.fstart[o][nf] = (uint16_t)off;
+            m.fnb[o][nf] = m.fnb[c][l];
+        }
+    }
+    if (l == 0) m.fstart[o][Fn] = (uint16_t)Spre;
+
+    // 4. crossings: the first occurrence of an edge key in slot order creates the vertex
+    //    (:391-417); new vertices are numbered in that order after the kept ones
+    const bool ka = l < K;
+    const uint32_t key = ka ? m.ckey[l] : 0xffffffffu;
+    int first = l;
+    for (int cj = 0; cj < K; ++cj) {
+        const uint32_t kj = rlu(key, cj);
+        if (cj < first && kj == key) first = cj;
+    }
+    const bool isf = ka && first == l;
+    const uint64_t fm = ballot(isf);
+    const int nint = popc(fm);
+    const int myx = nk + popc(fm & lt);
+    const int xid = (int)lane_get((uint32_t)myx, first);
+    if (nk + nint > C::VM) return kOverflow;
+    if (isf) {
+        const uint32_t ab = m.cab[l];
+        const uint32_t a = ab & 0xffffu, b = ab >> 16;
+        const double da = m.dist[a], db = m.dist[b];
+        double t = da / (da - db);
+        if (t < 0.0) t = 0.0;   // f64::clamp(0, 1)
+        if (t > 1.0) t = 1.0;
+        const double ax = m.vx[c][a], ay = m.vy[c][a], az = m.vz[c][a];
+        const double bx = m.vx[c][b], by = m.vy[c][b], bz = m.vz[c][b];
+        const double x = ax + t * (bx - ax);
+        const double y = ay + t * (by - ay);
+        const double z = az + t * (bz - az);
+        m.vx[o][myx] = x; m.vy[o][myx] = y; m.vz[o][myx] = z;
+        if (withgen) {
+            const double dx = x - gx, dy = y - gy, dz = z - gz;
+            md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
+        }
+    }
+    __syncthreads();
+
+    // 5. emit the entries of every surviving face (:386-451), with each entry's successor slot
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        if (ch < nch) {
+            const int s = 64 * ch + l;
+            const uint32_t a = ea[ch] & 0xffffu, b = ea[ch] >> 16;
+            const bool ain = (inm >> a) & 1ull, bin = (inm >> b) & 1ull;
+            const int xv = (int)lane_get((uint32_t)xid, eci[ch] & 63);
+            if (s < P.S && (ain || bin)) {
+                const int f = (int)ef[ch];
+                const int nf = m.fnew[f];
+                if (nf >= 0) {
+                    const int pos = m.fbase[f] + (int)eP[ch];
+                    const int ns = m.fnoff[f], ne = m.nend[f];
+                    const uint16_t rb = (uint16_t)popc(inm & ((1ull << b) - 1ull));
+                    m.fidxn[pos] = (ain && bin) ? rb : (uint16_t)xv;
+                    m.sface[o][pos] = (uint16_t)nf;
+                    m.nextp[pos] = (uint16_t)(pos + 1 == ne ? ns : pos + 1);
+                    if (!ain) {
+                        m.fidxn[pos + 1] = rb;
+                        m.sface[o][pos + 1] = (uint16_t)nf;
+                        m.nextp[pos + 1] = (uint16_t)(pos + 2 == ne ? ns : pos + 2);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // 6. successor of every new slot; lid segments (v, u) for intersection pairs (:459-465)
+    int M = 0;
+    for (int base = 0; base < Spre; base += 64) {
+        const int p = base + l;
+        const bool pa = p < Spre;
+        const uint32_t ua = pa ? m.fidxn[p] : 0u;
+        const int np = pa ? m.nextp[p] : 0;
+        const uint32_t vb = pa ? m.fidxn[np] : 0u;
+        if (pa) m.sedge[o][p] = ua | (vb << 16);
+        const bool sg = pa && (int)ua >= nk && (int)vb >= nk;
+        const uint64_t gm = ballot(sg);
+        if (sg) m.seg[M + popc(gm & lt)] = vb | (ua << 16);
+        M += popc(gm);
+    }
+
+    // 7. lid polygon (:471-495); lane j holds the map entry of new vertex nk + j
+    if (M > 0) {
+        if (l < nint) m.lmap[l] = 0u;
+        __syncthreads();
+        for (int t = l; t < M; t += 64) {
+            const uint32_t sg = m.seg[t];
+            atomicMax(&m.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
+        }
+        __syncthreads();
+        const uint32_t mv = (l < nint) ? m.lmap[l] : 0u;
+        const uint32_t s0 = m.seg[0];
+        const int start = (int)(s0 & 0xffffu);
+        int cv = (int)(s0 >> 16);
+        uint32_t lidv = (uint32_t)start;
+        int len = 1;
+        while (cv != start && len <= M) {
+            if (len >= 64) return kGeneric;
+            if (l == len) lidv = (uint32_t)cv;
+            ++len;
+            const uint32_t e = rlu(mv, cv - nk);
+            cv = e ? (int)(e & 0xffffu) : 0xffff;
+            if (cv == 0xffff) break;
+        }
+        if (len >= 3) {
+            if (Spre + len > C::SM || Fn + 1 > C::FM) return kOverflow;
+            const uint32_t nxt = lane_get(lidv, (l + 1 >= len) ? 0 : l + 1);
+            if (l < len) {
+                m.sedge[o][Spre + l] = lidv | (nxt << 16);
+                m.sface[o][Spre + l] = (uint16_t)Fn;
+            }
+            if (l == 0) {
+                m.fstart[o][Fn] = (uint16_t)Spre;
+                m.fnb[o][Fn] = nid;
+                m.fstart[o][Fn + 1] = (uint16_t)(Spre + len);
+            }
+            Fn += 1;
+            Spre += len;
+        }
+    }
+    __syncthreads();
+    P.cur = o;
+    P.V = nk + nint;
+    P.F = Fn;
+    P.S = Spre;
+    r2 = withgen ? wave_max(md2) : 0.0;
+    vreg_load(m, P, r);
+    return kModified;
+}
+
+template <class C>
+__device__ int clip_any(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t nid, bool withgen,
+                        double gx, double gy, double gz, double &r2) {
+    if (r.ok && P.S <= 192 && P.F <= 64) {
+        const int rc = clip_fast<C>(m, P, r, pl, nid, withgen, gx, gy, gz, r2);
+        if (rc != kGeneric) return rc;
+    }
+    const int rc = clip_generic<C>(m, P, pl, nid, withgen, gx, gy, gz, r2);
+    if (rc == kModified) vreg_load(m, P, r);
+    return rc;
+}
+
 template <class C>
 __device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double gy, double gz) {
     double md = 0.0;  // cell_3d_faces.rs:271-286
@@ -479,6 +769,8 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
     const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
     Poly P;
     poly_box<C>(m, P, A.box_lo, A.box_hi);
+    VReg r;
+    vreg_load(m, P, r);
     int status = kCellOk;
     double r2 = 0.0;
 
@@ -502,7 +794,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
         }
         if (emit) {
             double unused;
-            const int rc = poly_clip<C>(m, P, pl, W.id, false, gx, gy, gz, unused);
+            const int rc = clip_any<C>(m, P, r, pl, W.id, false, gx, gy, gz, unused);
             if (rc == kOverflow) status = kCellCapacity;
             else if (rc == kFace255) status = kCellFace255;
         }
@@ -522,7 +814,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
         const double rz = (gz - G.lo[2]) * G.s[2] - (double)iz;
 
         uint32_t e0 = 0;       // first shell entry of the loaded batch (lane k holds e0 + k)
-        int k0 = 0, o0 = 0;    // next unvisited entry of the batch and items of it already visited
+        int k0 = 0, o0 = 0;    // next unvisited entry of the batch, items of it already visited
         bool load = true;
         double bd2 = 0.0, bdist = 0.0;
         uint32_t bs = 0, bc = 0;
@@ -533,6 +825,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                 bend = e >= A.nshell;
                 binr = false;
                 bc = 0;
+                bdist = 0.0;
                 if (!bend) {
                     const ShellEntry se = A.shell[e];
                     bd2 = se.d2;
@@ -558,11 +851,14 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                 }
                 load = false;
             }
-            const double lim = 4.0 * r2;
+
+            // gather the next window of candidates under the current radius r2b
+            const double r2b = r2;
+            double lim = 4.0 * r2;
             const bool committed = (l == k0) && (o0 > 0);
             const bool brk = (l >= k0) && !committed && (bend || bd2 > lim);
-            const int kb = ffs64(ballot(brk));
-            const bool take = (l >= k0) && (l < kb) && (committed || (binr && bdist <= lim));
+            int kbc = ffs64(ballot(brk));   // first entry that ends the walk (64: none here)
+            const bool take = (l >= k0) && (l < kbc) && (committed || (binr && bdist <= lim));
             const uint32_t skip = (l == k0) ? (uint32_t)o0 : 0u;
             const uint32_t cnt = take ? bc - skip : 0u;
             const uint32_t inc = wave_incl_scan(cnt);
@@ -577,85 +873,82 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                 }
             }
             __syncthreads();
-
-            // candidates: tessellation.rs:446-471 (distance test, then clip)
-            bool pass = false;
-            uint32_t j = 0;
-            Plane cp{0, 0, 0, 0, 0, 0};
-            if (l < nwin) {
-                const double4 q = A.spos[m.winq[l]];
-                j = (uint32_t)__double_as_longlong(q.w);
-                const double d0 = q.x - gx, d1 = q.y - gy, d2 = q.z - gz;
-                const double dist_sq = d0 * d0 + d1 * d1 + d2 * d2;
-                cp = Plane{gx + d0 * 0.5, gy + d1 * 0.5, gz + d2 * 0.5, d0, d1, d2};
-                pass = (j != i) && !(dist_sq > lim);
+            const bool wa = l < nwin;
+            const uint32_t wq = wa ? m.winq[l] : 0u;
+            const int wk = wa ? (int)m.winent[l] : 0;
+            const int woff = wa ? (int)m.winoff[l] : 0;
+            const double wbd = lane_get_d(bdist, wk);
+            uint32_t wj = 0xffffffffu;
+            double wd0 = 0.0, wd1 = 0.0, wd2 = 0.0, wdsq = 0.0;
+            if (wa) {
+                const double4 q = A.spos[wq];
+                wj = (uint32_t)__double_as_longlong(q.w);
+                wd0 = q.x - gx; wd1 = q.y - gy; wd2 = q.z - gz;
+                wdsq = wd0 * wd0 + wd1 * wd1 + wd2 * wd2;   // tessellation.rs:447-456
             }
-            uint64_t pm = ballot(pass);
-            int hit = 64;       // window lane of the candidate that cut the cell
-            while (pm) {
-                if (popc(pm) <= 2) {
-                    // few candidates left: clip the next one directly (its distance pass is the
-                    // all-inside test)
-                    const int c = ffs64(pm);
-                    pm &= pm - 1;
-                    const Plane pl{rld(cp.px, c), rld(cp.py, c), rld(cp.pz, c),
-                                   rld(cp.nx, c), rld(cp.ny, c), rld(cp.nz, c)};
-                    double nr2 = 0.0;
-                    const int rc = poly_clip<C>(m, P, pl, (int32_t)rlu(j, c), true, gx, gy, gz, nr2);
-                    if (rc == kNoChange) continue;
-                    if (rc == kOverflow) { status = kCellCapacity; }
-                    else if (rc == kFace255) { status = kCellFace255; }
-                    else { r2 = (P.V == 0) ? 0.0 : nr2; }
-                    hit = c;
+            bool wvalid = wa;
+            int last = -1;
+            bool rebuild = false;
+            // visit the window's candidates in order (tessellation.rs:446-471)
+            for (;;) {
+                lim = 4.0 * r2;
+                const uint64_t pm = ballot(wvalid && l > last && wj != i && !(wdsq > lim)) ;
+                if (pm == 0) break;
+                const int cl = ffs64(pm);
+                const double d0 = rld(wd0, cl), d1 = rld(wd1, cl), d2 = rld(wd2, cl);
+                const Plane pl{gx + d0 * 0.5, gy + d1 * 0.5, gz + d2 * 0.5, d0, d1, d2};
+                last = cl;
+                double nr2 = 0.0;
+                const int rc = clip_any<C>(m, P, r, pl, (int32_t)rlu(wj, cl), true, gx, gy, gz, nr2);
+                if (rc == kNoChange) continue;
+                if (rc < 0) {
+                    status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
                     break;
                 }
-                // many candidates: test all of them against the polytope in parallel
-                bool cut = false;
-                const bool mine = (pm >> l) & 1ull;
-                uint64_t cutm = 0;
-                for (int v = 0; v < P.V; ++v) {
-                    const double x = m.vx[P.cur][v], y = m.vy[P.cur][v], z = m.vz[P.cur][v];
-                    if (mine && !cut) {
-                        const double dv = (x - cp.px) * cp.nx + (y - cp.py) * cp.ny + (z - cp.pz) * cp.nz;
-                        cut = dv > kTol;
-                    }
-                    cutm = ballot(cut);
-                    const uint64_t below = cutm ? ((cutm & (~cutm + 1ull)) - 1ull) : ~0ull;
-                    if ((pm & ~cutm & below) == 0) break;
+                if (P.V == 0) nr2 = 0.0;
+                const int kc = rl(wk, cl), offc = rl(woff, cl);
+                r2 = nr2;
+                if (nr2 > r2b) {
+                    // the radius grew: entries dropped from this window may now pass, rebuild
+                    if ((uint32_t)(offc + 1) < rlu(bc, kc)) { k0 = kc; o0 = offc + 1; }
+                    else { k0 = kc + 1; o0 = 0; }
+                    rebuild = true;
+                    break;
                 }
-                if (cutm == 0) { pm = 0; break; }
-                const int c = ffs64(cutm);
-                const Plane pl{rld(cp.px, c), rld(cp.py, c), rld(cp.pz, c),
-                               rld(cp.nx, c), rld(cp.ny, c), rld(cp.nz, c)};
-                double nr2 = 0.0;
-                const int rc = poly_clip<C>(m, P, pl, (int32_t)rlu(j, c), true, gx, gy, gz, nr2);
-                if (rc == kOverflow) status = kCellCapacity;
-                else if (rc == kFace255) status = kCellFace255;
-                else r2 = (P.V == 0) ? 0.0 : nr2;
-                hit = c;
-                break;
+                // the radius shrank: re-filter the rest of the window with it
+                lim = 4.0 * r2;
+                kbc = ffs64(ballot((l > kc) && (bend || bd2 > lim)));
+                wvalid = wvalid && (l <= cl || wk == kc || (wk < kbc && wbd <= lim));
             }
             if (status != kCellOk) break;
 
-            if (hit < 64) {
-                // resume right after the cutting candidate; its bin stays committed
-                const int k = m.winent[hit], off = m.winoff[hit];
-                if ((uint32_t)(off + 1) < rlu(bc, k)) { k0 = k; o0 = off + 1; }
-                else { k0 = k + 1; o0 = 0; }
-            } else if (tot > 64u) {
-                const int k = m.winent[63], off = m.winoff[63];
-                if ((uint32_t)(off + 1) < rlu(bc, k)) { k0 = k; o0 = off + 1; }
-                else { k0 = k + 1; o0 = 0; }
-            } else if (kb < 64) {
-                // the walk stops at entry kb: a real break, or the end of the prefix
-                if (rl((int)bend, kb)) {
-                    const uint32_t e = e0 + (uint32_t)kb;
-                    if (!(e == A.nshell && A.shell_next_d2 > lim)) status = kCellShell;
+            bool walk_end = false;
+            if (!rebuild) {
+                if (tot > 64u) {
+                    // the window stopped inside the shell order: continue after its last item
+                    const int k63 = rl(wk, 63), off63 = rl(woff, 63);
+                    if (rl((int)wvalid, 63)) {
+                        if ((uint32_t)(off63 + 1) < rlu(bc, k63)) { k0 = k63; o0 = off63 + 1; }
+                        else { k0 = k63 + 1; o0 = 0; }
+                    } else if (k63 >= kbc) {
+                        walk_end = true;
+                    } else {
+                        k0 = k63 + 1; o0 = 0;
+                    }
+                } else if (kbc < 64) {
+                    walk_end = true;
+                } else {
+                    k0 = 64;
+                }
+            }
+            if (walk_end) {
+                // the walk stops at entry kbc: a real break, or the end of the shell prefix
+                if (rl((int)bend, kbc)) {
+                    const uint32_t e = e0 + (uint32_t)kbc;
+                    if (!(e == A.nshell && A.shell_next_d2 > 4.0 * r2)) status = kCellShell;
                 }
                 __syncthreads();
                 break;
-            } else {
-                k0 = 64;
             }
             if (k0 >= 64) { e0 += 64; k0 = 0; o0 = 0; load = true; }
             __syncthreads();
