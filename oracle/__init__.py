@@ -189,7 +189,10 @@ class OracleTess:
         return OracleCell(v, fc, fi, fn)
 
     def reduced(self, idx=None):
-        """(volumes, centroids[n,3], face_counts, work[n,6]) for the listed cells (or all)."""
+        """(volumes, centroids[n,3], face_counts, work[n,9]) for the listed cells (or all).
+
+        work columns: flops, candidates, bins, clips, cuts, shell entries, largest vertex
+        count, largest face-index count, largest face count while the cell was built."""
         if idx is None:
             m = self.count_cells()
             ip = None
@@ -200,7 +203,7 @@ class OracleTess:
         vol = np.empty(m, dtype=np.float64)
         cen = np.empty((m, 3), dtype=np.float64)
         nf = np.empty(m, dtype=np.uint32)
-        work = np.empty((m, 6), dtype=np.uint64)
+        work = np.empty((m, 9), dtype=np.uint64)
         self._lib.or_tess_reduced(self._h, ip, m, _ptr(vol), _ptr(cen), _ptr(nf), _ptr(work))
         return vol, cen, nf, work
 

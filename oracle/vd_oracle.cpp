@@ -16,7 +16,7 @@
 // Pinned by: the Voro++ fixtures (vertex counts and positions, face counts), closed forms (box,
 // lattices, plane-clipped boxes), a brute-force all-pairs clipping on small inputs, topology
 // invariants (reciprocity, Euler characteristic), and the grid unit-test values of the reference.
-// See tests/test_oracle_*.py.
+// See tests/test_oracle.py.
 
 #include <algorithm>
 #include <atomic>
@@ -31,23 +31,6 @@
 namespace {
 
 // ---------------------------------------------------------------------------------------------
-// Algorithmic-work counters (flop accounting used as the roofline unit). Rules: each + - * / sqrt
-// counts 1; compares, clamps and max() count 0.
-// ---------------------------------------------------------------------------------------------
-struct Work {
-    uint64_t flops = 0;
-    uint64_t candidates = 0;   // visitor calls (tessellation.rs:446)
-    uint64_t bins = 0;         // in-range bins whose distance was evaluated (algo_3d_grid.rs:182)
-    uint64_t clips = 0;        // clip calls after the distance test
-    uint64_t cuts = 0;         // clips that modified the cell
-    uint64_t entries = 0;      // shell entries examined
-    void add(const Work &o) {
-        flops += o.flops; candidates += o.candidates; bins += o.bins;
-        clips += o.clips; cuts += o.cuts; entries += o.entries;
-    }
-};
-
-// ---------------------------------------------------------------------------------------------
 // Polyhedral cell. Restates src/cell/cell_3d_faces.rs:23-74 (layout and box initialisation).
 // ---------------------------------------------------------------------------------------------
 struct Poly {
@@ -57,6 +40,27 @@ struct Poly {
     std::vector<int32_t> fnb;     // neighbour id per face; negative = box side or wall
     size_t nv() const { return v.size() / 3; }
     bool empty() const { return v.empty(); }
+};
+
+// Algorithmic-work counters (flop accounting used as the roofline unit). Rules: each + - * / sqrt
+// counts 1; compares, clamps and max() count 0.
+struct Work {
+    uint64_t flops = 0;
+    uint64_t candidates = 0;   // visitor calls (tessellation.rs:446)
+    uint64_t bins = 0;         // in-range bins whose distance was evaluated (algo_3d_grid.rs:182)
+    uint64_t clips = 0;        // clip calls after the distance test
+    uint64_t cuts = 0;         // clips that modified the cell
+    uint64_t entries = 0;      // shell entries examined
+    uint64_t maxv = 0, maxs = 0, maxf = 0;   // largest polytope met while the cell was built
+    void add(const Work &o) {
+        flops += o.flops; candidates += o.candidates; bins += o.bins;
+        clips += o.clips; cuts += o.cuts; entries += o.entries;
+    }
+    void size(const Poly &c) {
+        maxv = std::max<uint64_t>(maxv, c.nv());
+        maxs = std::max<uint64_t>(maxs, c.fidx.size());
+        maxf = std::max<uint64_t>(maxf, c.fcnt.size());
+    }
 };
 
 inline int32_t box_side(int axis, bool is_max) {  // src/bounds/mod.rs:21-23
@@ -221,6 +225,7 @@ std::pair<bool, double> poly_clip(Poly &c, const double p[3], const double n[3],
     std::swap(c.fcnt, o.fcnt);
     std::swap(c.fidx, o.fidx);
     std::swap(c.fnb, o.fnb);
+    if (w) w->size(c);
     return {true, max_d2};
 }
 
@@ -664,7 +669,8 @@ int or_cell_get(void *t, size_t i, double *v, uint32_t *fc, uint32_t *fi, int32_
 }
 
 // Per-cell reductions for the listed cells (or all when idx == NULL): volume, centroid, face
-// count; and the algorithmic-work counters [flops, candidates, bins, clips, cuts, entries].
+// count; and the algorithmic-work counters [flops, candidates, bins, clips, cuts, entries,
+// largest vertex count, largest face-index count, largest face count].
 void or_tess_reduced(void *t, const int64_t *idx, size_t m, double *vol, double *cen,
                      uint32_t *nf, uint64_t *work6) {
     Tess &T = *(Tess *)t;
@@ -677,8 +683,9 @@ void or_tess_reduced(void *t, const int64_t *idx, size_t m, double *vol, double 
         if (nf) nf[k] = (uint32_t)c.fcnt.size();
         if (work6) {
             const Work &w = T.work[i];
-            work6[6 * k + 0] = w.flops; work6[6 * k + 1] = w.candidates; work6[6 * k + 2] = w.bins;
-            work6[6 * k + 3] = w.clips; work6[6 * k + 4] = w.cuts; work6[6 * k + 5] = w.entries;
+            uint64_t *o = work6 + 9 * k;
+            o[0] = w.flops; o[1] = w.candidates; o[2] = w.bins; o[3] = w.clips; o[4] = w.cuts;
+            o[5] = w.entries; o[6] = w.maxv; o[7] = w.maxs; o[8] = w.maxf;
         }
     }
 }

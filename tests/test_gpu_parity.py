@@ -186,3 +186,32 @@ def test_relax_matches_oracle():
         o.relax()
         assert np.array_equal(bits(g.generators()), bits(o.generators()))
     check_all_cells(g, o)
+
+
+def fibonacci_sphere(n, r, c):
+    k = np.arange(n) + 0.5
+    phi = np.arccos(1.0 - 2.0 * k / n)
+    th = np.pi * (1.0 + 5 ** 0.5) * k
+    return np.stack([c[0] + r * np.cos(th) * np.sin(phi), c[1] + r * np.sin(th) * np.sin(phi),
+                     c[2] + r * np.cos(phi)], axis=1)
+
+
+@pytest.mark.parametrize("nshell", [70, 400])
+def test_large_cells_move_to_larger_tiers(nshell):
+    # a generator surrounded by nshell points on a sphere: its cell has about nshell faces
+    # (70 -> past the 64-vertex fast tier; 400 -> past the 256-vertex tier)
+    bg = W.PCG32(51).uniform(300, (0, 0, 0), (1, 1, 1))
+    pts = np.concatenate([[[0.5, 0.5, 0.5]], fibonacci_sphere(nshell, 0.2, (0.5, 0.5, 0.5)), bg])
+    g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (5, 5, 5), pts)
+    check_all_cells(g, o)
+    assert g.stats()[1] > 0, "no cell reached a larger tier"
+    assert g.face_counts()[0] >= nshell // 2
+
+
+def test_sparse_grid_extends_shell_prefix():
+    # 40 generators in a 48^3 grid: search radii span many bins, past the first shell prefix
+    pts = W.PCG32(52).uniform(40, (0, 0, 0), (1, 1, 1))
+    g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (48, 48, 48), pts)
+    vol = check_all_cells(g, o)
+    assert g.stats()[2] > 0, "the shell prefix was never extended"
+    assert abs(vol.sum() - 1.0) < 1e-12

@@ -15,6 +15,8 @@
 
 namespace {
 
+constexpr int kTiers = 3;   // polytope capacity tiers of the cell kernel (vd_cells.hip)
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string &msg) {
@@ -130,7 +132,7 @@ struct vd_tess {
     std::vector<vd::WallDev> walls;
     int device = 0;
     int cus = 1;
-    int occ[2] = {1, 1};
+    int occ[kTiers] = {1, 1, 1};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int keep_cells = 1;
@@ -320,27 +322,30 @@ int calculate(vd_tess *t) {
         if ((rc = ensure(t->d_fnb, t->cap[1] * 4))) return rc;
         VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(t->d_top.p), 0u, 6, t->stream));
 
-        // Tier 0 (small LDS polytope) over every cell in bin order. A cell that outgrows tier 0
-        // moves to tier 1; a cell whose walk runs past the shell prefix reruns in its tier after
-        // the prefix is extended. Reruns start from scratch and give the same result.
-        DevBuf L[2];                              // owned work lists per tier
-        const uint32_t *list[2] = {t->d_items.as<uint32_t>(), nullptr};
-        uint32_t nlist[2] = {(uint32_t)n, 0};
+        // Tier 0 (register fast path, small LDS) over every cell in bin order. A cell that
+        // outgrows its tier moves to the next one; a cell whose walk runs past the shell prefix
+        // reruns in its tier after the prefix is extended. Reruns start from scratch and give
+        // the same result.
+        constexpr int NT = kTiers;
+        DevBuf L[NT];                              // owned work lists per tier
+        const uint32_t *list[NT] = {t->d_items.as<uint32_t>()};
+        uint32_t nlist[NT] = {(uint32_t)n};
         bool arena_full = false;
         bool first = true;
-        for (int round = 0; round < 64 && (nlist[0] || nlist[1]); ++round) {
-            for (int tier = 0; tier < 2; ++tier) {
+        auto pending = [&]() { for (int k = 0; k < NT; ++k) if (nlist[k]) return true; return false; };
+        for (int round = 0; round < 64 && pending(); ++round) {
+            for (int tier = 0; tier < NT; ++tier) {
                 if (!nlist[tier]) continue;
                 float ms = 0.f;
                 if ((rc = run_cells(t, tier, list[tier], nlist[tier], first ? &ms : nullptr))) return rc;
                 if (first) { t->stats[4] = 1000.0 * ms; t->stats[0] = (double)nlist[0]; first = false; }
-                else if (tier == 1) t->stats[1] += nlist[1];
+                else if (tier > 0) t->stats[1] += nlist[tier];
             }
             // outcome per tier: counts[4*tier + {0 capacity, 1 shell, 2 arena, 3 error}]
-            uint32_t counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            DevBuf out[4];   // tier0 capacity, tier0 shell, tier1 capacity, tier1 shell
-            VD_HIP(vd::launch_fill_u32(t->d_counts4.as<uint32_t>(), 0u, 8, t->stream));
-            for (int tier = 0; tier < 2; ++tier) {
+            uint32_t counts[4 * NT] = {};
+            DevBuf out[2 * NT];   // per tier: capacity list, shell list
+            VD_HIP(vd::launch_fill_u32(t->d_counts4.as<uint32_t>(), 0u, 4 * NT, t->stream));
+            for (int tier = 0; tier < NT; ++tier) {
                 if (!nlist[tier]) continue;
                 if ((rc = ensure(out[2 * tier], (size_t)nlist[tier] * 4)) ||
                     (rc = ensure(out[2 * tier + 1], (size_t)nlist[tier] * 4)))
@@ -349,41 +354,48 @@ int calculate(vd_tess *t) {
                                           out[2 * tier].as<uint32_t>(), out[2 * tier + 1].as<uint32_t>(),
                                           t->d_counts4.as<uint32_t>() + 4 * tier, t->stream));
             }
-            VD_HIP(hipMemcpyAsync(counts, t->d_counts4.p, 32, hipMemcpyDeviceToHost, t->stream));
+            VD_HIP(hipMemcpyAsync(counts, t->d_counts4.p, sizeof(counts), hipMemcpyDeviceToHost, t->stream));
             VD_HIP(hipStreamSynchronize(t->stream));
-            if (counts[3] || counts[7]) return fail(VD_ECAPACITY, "a cell has a face with 256 or more vertices");
-            if (counts[2] || counts[6]) { arena_full = true; break; }
-            if (counts[4]) return fail(VD_ECAPACITY, "a cell exceeds the large polytope tier");
-            if (counts[1] || counts[5]) {
+            bool shell = false;
+            for (int tier = 0; tier < NT; ++tier) {
+                if (counts[4 * tier + 3]) return fail(VD_ECAPACITY, "a cell has a face with 256 or more vertices");
+                if (counts[4 * tier + 2]) arena_full = true;
+                shell |= counts[4 * tier + 1] != 0;
+            }
+            if (arena_full) break;
+            if (counts[4 * (NT - 1)]) return fail(VD_ECAPACITY, "a cell exceeds the largest polytope tier");
+            if (shell) {
                 if (t->shell.complete) return fail(VD_EDEVICE, "shell walk past a complete table");
                 if ((rc = upload_shell(t, t->shell_k * 2))) return rc;
                 t->stats[2] += 1;
             }
-            // next lists: tier 0 <- tier-0 shell failures; tier 1 <- tier-0 capacity + tier-1 shell
-            DevBuf next1;
-            const uint32_t n1 = counts[0] + counts[5];
-            if (n1) {
-                if ((rc = ensure(next1, (size_t)n1 * 4))) return rc;
-                if (counts[0])
-                    VD_HIP(hipMemcpyAsync(next1.p, out[0].p, counts[0] * 4, hipMemcpyDeviceToDevice, t->stream));
-                if (counts[5])
-                    VD_HIP(hipMemcpyAsync(next1.as<uint32_t>() + counts[0], out[3].p, counts[5] * 4,
+            // next list of tier k: its own shell failures + the capacity failures of tier k-1
+            DevBuf next[NT];
+            uint32_t nnext[NT] = {};
+            for (int tier = 0; tier < NT; ++tier) {
+                const uint32_t ns = counts[4 * tier + 1];
+                const uint32_t nc = tier > 0 ? counts[4 * (tier - 1)] : 0u;
+                nnext[tier] = ns + nc;
+                if (!nnext[tier]) continue;
+                if ((rc = ensure(next[tier], (size_t)nnext[tier] * 4))) return rc;
+                if (ns)
+                    VD_HIP(hipMemcpyAsync(next[tier].p, out[2 * tier + 1].p, ns * 4, hipMemcpyDeviceToDevice,
+                                          t->stream));
+                if (nc)
+                    VD_HIP(hipMemcpyAsync(next[tier].as<uint32_t>() + ns, out[2 * (tier - 1)].p, nc * 4,
                                           hipMemcpyDeviceToDevice, t->stream));
-                VD_HIP(hipStreamSynchronize(t->stream));
             }
-            release(L[0]);
-            release(L[1]);
-            L[0] = out[1];
-            out[1] = DevBuf();
-            L[1] = next1;
+            VD_HIP(hipStreamSynchronize(t->stream));
+            for (int tier = 0; tier < NT; ++tier) {
+                release(L[tier]);
+                L[tier] = next[tier];
+                next[tier] = DevBuf();
+                list[tier] = L[tier].as<uint32_t>();
+                nlist[tier] = nnext[tier];
+            }
             for (DevBuf &b : out) release(b);
-            list[0] = L[0].as<uint32_t>();
-            nlist[0] = counts[1];
-            list[1] = L[1].as<uint32_t>();
-            nlist[1] = n1;
         }
-        release(L[0]);
-        release(L[1]);
+        for (DevBuf &b : L) release(b);
         if (!arena_full) break;
         for (int k = 0; k < 3; ++k) t->cap[k] *= 2;
     }
@@ -456,14 +468,12 @@ int vd_tess_create(const double bmin[3], const double bmax[3], uint32_t nx, uint
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&t->ev0);
     if (e == hipSuccess) e = hipEventCreate(&t->ev1);
-    if (e == hipSuccess) e = vd::cells_occupancy(0, &t->occ[0]);
-    if (e == hipSuccess) e = vd::cells_occupancy(1, &t->occ[1]);
+    for (int k = 0; k < kTiers && e == hipSuccess; ++k) e = vd::cells_occupancy(k, &t->occ[k]);
     if (e != hipSuccess) {
         vd_tess_destroy(t);
         return fail(VD_EDEVICE, std::string("device setup: ") + hipGetErrorString(e));
     }
-    t->occ[0] = std::max(1, t->occ[0]);
-    t->occ[1] = std::max(1, t->occ[1]);
+    for (int k = 0; k < kTiers; ++k) t->occ[k] = std::max(1, t->occ[k]);
     int rc = ensure(t->d_gen, 16);
     if (!rc) rc = rebin(t);
     if (!rc) rc = hipStreamSynchronize(t->stream) == hipSuccess ? VD_OK : fail(VD_EDEVICE, "sync");

@@ -128,39 +128,47 @@ __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
     return (uint32_t)v;
 }
 
-template <int VM_, int SM_, int FM_, int KM_>
+template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int TM_>
 struct Cap {
-    static constexpr int VM = VM_;   // vertices
-    static constexpr int SM = SM_;   // face slots (flattened face indices)
-    static constexpr int FM = FM_;   // faces
-    static constexpr int KM = KM_;   // plane-crossing slots per clip
+    static constexpr int VM = VM_;      // vertices
+    static constexpr int SM = SM_;      // face slots (flattened face indices)
+    static constexpr int FM = FM_;      // faces
+    static constexpr int KM = KM_;      // plane-crossing slots per clip
+    static constexpr bool FAST = FAST_; // register fast path (VM <= 64) instead of the generic one
+    static constexpr int TM = TM_;      // triangles staged in LDS for the volume/centroid folds
 };
 
 template <class C>
 struct Mem {
     double vx[2][C::VM], vy[2][C::VM], vz[2][C::VM];
-    double dist[C::VM];
     uint32_t sedge[2][C::SM];
-    uint32_t ckey[C::KM];
-    uint32_t cab[C::KM];
-    uint32_t lmap[C::VM];
-    uint32_t seg[C::SM];
     int32_t fnb[2][C::FM];
-    int32_t fbase[C::FM];
     uint32_t winq[64];
     uint16_t sface[2][C::SM];
     uint16_t fstart[2][C::FM + 1];
-    uint16_t remap[C::VM];
-    uint16_t P[C::SM + 1];
-    uint16_t cxid[C::KM];
-    uint16_t fidxn[C::SM];
-    uint16_t nextp[C::SM];
-    int16_t fnew[C::FM];
-    uint16_t fnoff[C::FM];
-    uint16_t nend[C::FM];
-    uint16_t lid[C::SM];
     uint16_t winent[64];
     uint16_t winoff[64];
+    // Scratch of one clip; at the end of a cell the same bytes stage the fold terms.
+    union {
+        struct {
+            double dist[C::VM];
+            uint32_t ckey[C::KM];
+            uint32_t cab[C::KM];
+            uint32_t lmap[C::VM];
+            uint32_t seg[C::SM];
+            int32_t fbase[C::FM];
+            uint16_t remap[C::VM];
+            uint16_t P[C::SM + 1];
+            uint16_t cxid[C::KM];
+            uint16_t fidxn[C::SM];
+            uint16_t nextp[C::SM];
+            int16_t fnew[C::FM];
+            uint16_t fnoff[C::FM];
+            uint16_t nend[C::FM];
+            uint16_t lid[C::SM];
+        };
+        double tri[4][C::TM + 1];   // +1 keeps the four rows in different LDS banks
+    };
 };
 
 enum ClipRc : int { kNoChange = 0, kModified = 1, kOverflow = -1, kFace255 = -2, kGeneric = -3 };
@@ -689,15 +697,15 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
 }
 
 template <class C>
-__device__ int clip_any(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t nid, bool withgen,
-                        double gx, double gy, double gz, double &r2) {
-    if (r.ok && P.S <= 192 && P.F <= 64) {
+__device__ inline int clip_any(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t nid, bool withgen,
+                               double gx, double gy, double gz, double &r2) {
+    if constexpr (C::FAST) {
+        // a polytope that leaves the register layout moves the cell to a generic tier
         const int rc = clip_fast<C>(m, P, r, pl, nid, withgen, gx, gy, gz, r2);
-        if (rc != kGeneric) return rc;
+        return rc == kGeneric ? kOverflow : rc;
+    } else {
+        return clip_generic<C>(m, P, pl, nid, withgen, gx, gy, gz, r2);
     }
-    const int rc = clip_generic<C>(m, P, pl, nid, withgen, gx, gy, gz, r2);
-    if (rc == kModified) vreg_load(m, P, r);
-    return rc;
 }
 
 template <class C>
@@ -710,20 +718,25 @@ __device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double
     return wave_max(md);
 }
 
-// Volume and centroid folds (cell_3d_faces.rs:106-205). Triangles are produced lane-parallel in
-// slot order and folded left in that order.
+// Volume and centroid (cell_3d_faces.rs:106-205). Every face has >= 3 entries, so slot p of face
+// f at position 1 <= i <= count-2 is triangle p - 2f - 1 of the fan order. Triangle terms are
+// computed lane-parallel and staged in LDS; lanes 0-3 then fold det, det*x, det*y, det*z left in
+// that order. More than TM triangles fall back to a fold by lane reads.
 template <class C>
 __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]) {
     const int l = lane_id();
     const int c = P.cur;
+    const int T = P.S - 2 * P.F;
+    const bool staged = T <= C::TM;
     double tv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
     for (int base = 0; base < P.S; base += 64) {
         const int p = base + l;
         const bool act = p < P.S;
         double det = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
         bool tri = false;
+        int f = 0;
         if (act) {
-            const int f = m.sface[c][p];
+            f = m.sface[c][p];
             const int fs = m.fstart[c][f], fe = m.fstart[c][f + 1];
             const int cnt = fe - fs, i = p - fs;
             tri = cnt >= 3 && i >= 1 && i <= cnt - 2;
@@ -743,15 +756,30 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
                 wz = det * (az + bz + qz);
             }
         }
-        uint64_t tm = ballot(tri);
-        while (tm) {
-            const int t = ffs64(tm);
-            tm &= tm - 1;
-            tv += rld(det, t);
-            sx += rld(wx, t);
-            sy += rld(wy, t);
-            sz += rld(wz, t);
+        if (staged) {
+            if (tri) {
+                const int t = p - 2 * f - 1;
+                m.tri[0][t] = det; m.tri[1][t] = wx; m.tri[2][t] = wy; m.tri[3][t] = wz;
+            }
+        } else {
+            uint64_t tm = ballot(tri);
+            while (tm) {
+                const int t = ffs64(tm);
+                tm &= tm - 1;
+                tv += rld(det, t);
+                sx += rld(wx, t);
+                sy += rld(wy, t);
+                sz += rld(wz, t);
+            }
         }
+    }
+    if (staged) {
+        __syncthreads();
+        double acc = 0.0;
+        if (l < 4)
+            for (int t = 0; t < T; ++t) acc += m.tri[l][t];
+        tv = rld(acc, 0); sx = rld(acc, 1); sy = rld(acc, 2); sz = rld(acc, 3);
+        __syncthreads();
     }
     vol = fabs(tv / 6.0);
     if (fabs(tv) < 1e-9) {
@@ -762,7 +790,8 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
     }
 }
 
-// One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports.
+// One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports. The
+// walls and the neighbour candidates feed a single clip site (one inlined copy of the clip).
 template <class C>
 __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
     const int l = lane_id();
@@ -773,159 +802,144 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
     vreg_load(m, P, r);
     int status = kCellOk;
     double r2 = 0.0;
+    const GridGeom &G = A.grid;
 
-    // walls in insertion order (tessellation.rs:429-436)
-    for (int w = 0; w < A.nwalls && status == kCellOk; ++w) {
-        const WallDev &W = A.walls[w];
+    // walk state (algo_3d_grid.rs:145-202)
+    int ix = 0, iy = 0, iz = 0;
+    double rx = 0.0, ry = 0.0, rz = 0.0;
+    uint32_t e0 = 0;        // first shell entry of the loaded batch (lane k holds e0 + k)
+    int k0 = 0, o0 = 0;     // next unvisited entry of the batch, items of it already visited
+    bool load = true;
+    double bd2 = 0.0, bdist = 0.0;
+    uint32_t bs = 0, bc = 0;
+    bool binr = false, bend = false;
+    // window state: lane w holds candidate w
+    bool win = false;       // a window is open
+    double r2b = 0.0;       // radius when the window was gathered
+    uint32_t wtot = 0;      // candidates the window would have held without the 64 cap
+    int kbc = 64;           // first batch entry that ends the walk under the current radius
+    int last = -1;          // window lanes <= last are visited
+    int wk = 0, woff = 0;
+    uint32_t wj = 0xffffffffu;
+    double wd0 = 0.0, wd1 = 0.0, wd2 = 0.0, wdsq = 0.0, wbd = 0.0;
+    bool wvalid = false;
+
+    int wi = 0;             // next wall (tessellation.rs:429-436), then the neighbour walk
+    bool walking = false;
+
+    for (;;) {
         Plane pl;
-        bool emit = true;
-        if (W.kind == 0) {  // wall_3d.rs:38-42
-            pl = Plane{W.a[0], W.a[1], W.a[2], -W.b[0], -W.b[1], -W.b[2]};
-        } else {            // wall_3d.rs:80-100
-            const double dx = gx - W.a[0], dy = gy - W.a[1], dz = gz - W.a[2];
-            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-            if (dist == 0.0) {
-                emit = false;
-            } else {
+        int32_t nid = 0;
+        bool withgen = false;
+        int cl = -1;
+        if (wi < A.nwalls) {
+            const WallDev &W = A.walls[wi];
+            if (W.kind == 0) {  // wall_3d.rs:38-42
+                pl = Plane{W.a[0], W.a[1], W.a[2], -W.b[0], -W.b[1], -W.b[2]};
+            } else {            // wall_3d.rs:80-100
+                const double dx = gx - W.a[0], dy = gy - W.a[1], dz = gz - W.a[2];
+                const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+                if (dist == 0.0) {
+                    ++wi;           // no plane from this wall
+                    if (P.V == 0) break;
+                    continue;
+                }
                 const double sc = W.r / dist;
                 pl = Plane{W.a[0] + dx * sc, W.a[1] + dy * sc, W.a[2] + dz * sc,
                            dx / dist, dy / dist, dz / dist};
             }
-        }
-        if (emit) {
-            double unused;
-            const int rc = clip_any<C>(m, P, r, pl, W.id, false, gx, gy, gz, unused);
-            if (rc == kOverflow) status = kCellCapacity;
-            else if (rc == kFace255) status = kCellFace255;
-        }
-        if (P.V == 0) break;
-    }
-
-    if (status == kCellOk && P.V > 0) {
-        r2 = poly_max_radius_sq<C>(m, P, gx, gy, gz);
-
-        // neighbour walk (algo_3d_grid.rs:145-202)
-        const GridGeom &G = A.grid;
-        const int ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
-        const int iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
-        const int iz = (int)axis_bin(gz, G.lo[2], G.s[2], G.lim[2]);
-        const double rx = (gx - G.lo[0]) * G.s[0] - (double)ix;
-        const double ry = (gy - G.lo[1]) * G.s[1] - (double)iy;
-        const double rz = (gz - G.lo[2]) * G.s[2] - (double)iz;
-
-        uint32_t e0 = 0;       // first shell entry of the loaded batch (lane k holds e0 + k)
-        int k0 = 0, o0 = 0;    // next unvisited entry of the batch, items of it already visited
-        bool load = true;
-        double bd2 = 0.0, bdist = 0.0;
-        uint32_t bs = 0, bc = 0;
-        bool binr = false, bend = false;
-        for (;;) {
-            if (load) {
-                const uint32_t e = e0 + (uint32_t)l;
-                bend = e >= A.nshell;
-                binr = false;
-                bc = 0;
-                bdist = 0.0;
-                if (!bend) {
-                    const ShellEntry se = A.shell[e];
-                    bd2 = se.d2;
-                    const int bx = ix + se.dx, by = iy + se.dy, bz = iz + se.dz;
-                    binr = bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2];
-                    if (binr) {
-                        const uint32_t b = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] +
-                                           (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
-                        bs = A.bin_start[b];
-                        bc = A.bin_start[b + 1] - bs;
-                        // algo_3d_grid.rs:182-190
-                        double ex = se.dx > 0 ? ((double)se.dx - rx) * G.cs[0]
-                                  : se.dx < 0 ? ((double)(-(se.dx + 1)) + rx) * G.cs[0] : 0.0;
-                        double ey = se.dy > 0 ? ((double)se.dy - ry) * G.cs[1]
-                                  : se.dy < 0 ? ((double)(-(se.dy + 1)) + ry) * G.cs[1] : 0.0;
-                        double ez = se.dz > 0 ? ((double)se.dz - rz) * G.cs[2]
-                                  : se.dz < 0 ? ((double)(-(se.dz + 1)) + rz) * G.cs[2] : 0.0;
-                        ex = fmax(ex, 0.0); ey = fmax(ey, 0.0); ez = fmax(ez, 0.0);
-                        bdist = ex * ex + ey * ey + ez * ez;
+            nid = W.id;
+        } else {
+            if (!walking) {
+                if (P.V == 0) break;
+                r2 = poly_max_radius_sq<C>(m, P, gx, gy, gz);   // tessellation.rs:438
+                ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
+                iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
+                iz = (int)axis_bin(gz, G.lo[2], G.s[2], G.lim[2]);
+                rx = (gx - G.lo[0]) * G.s[0] - (double)ix;
+                ry = (gy - G.lo[1]) * G.s[1] - (double)iy;
+                rz = (gz - G.lo[2]) * G.s[2] - (double)iz;
+                walking = true;
+            }
+            if (!win) {
+                if (load) {
+                    const uint32_t e = e0 + (uint32_t)l;
+                    bend = e >= A.nshell;
+                    binr = false;
+                    bc = 0;
+                    bdist = 0.0;
+                    if (!bend) {
+                        const ShellEntry se = A.shell[e];
+                        bd2 = se.d2;
+                        const int bx = ix + se.dx, by = iy + se.dy, bz = iz + se.dz;
+                        binr = bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2];
+                        if (binr) {
+                            const uint32_t b = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] +
+                                               (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
+                            bs = A.bin_start[b];
+                            bc = A.bin_start[b + 1] - bs;
+                            // algo_3d_grid.rs:182-190
+                            double ex = se.dx > 0 ? ((double)se.dx - rx) * G.cs[0]
+                                      : se.dx < 0 ? ((double)(-(se.dx + 1)) + rx) * G.cs[0] : 0.0;
+                            double ey = se.dy > 0 ? ((double)se.dy - ry) * G.cs[1]
+                                      : se.dy < 0 ? ((double)(-(se.dy + 1)) + ry) * G.cs[1] : 0.0;
+                            double ez = se.dz > 0 ? ((double)se.dz - rz) * G.cs[2]
+                                      : se.dz < 0 ? ((double)(-(se.dz + 1)) + rz) * G.cs[2] : 0.0;
+                            ex = fmax(ex, 0.0); ey = fmax(ey, 0.0); ez = fmax(ez, 0.0);
+                            bdist = ex * ex + ey * ey + ez * ez;
+                        }
+                    } else {
+                        bd2 = (e == A.nshell) ? A.shell_next_d2 : DBL_MAX;
                     }
-                } else {
-                    bd2 = (e == A.nshell) ? A.shell_next_d2 : DBL_MAX;
+                    load = false;
                 }
-                load = false;
-            }
-
-            // gather the next window of candidates under the current radius r2b
-            const double r2b = r2;
-            double lim = 4.0 * r2;
-            const bool committed = (l == k0) && (o0 > 0);
-            const bool brk = (l >= k0) && !committed && (bend || bd2 > lim);
-            int kbc = ffs64(ballot(brk));   // first entry that ends the walk (64: none here)
-            const bool take = (l >= k0) && (l < kbc) && (committed || (binr && bdist <= lim));
-            const uint32_t skip = (l == k0) ? (uint32_t)o0 : 0u;
-            const uint32_t cnt = take ? bc - skip : 0u;
-            const uint32_t inc = wave_incl_scan(cnt);
-            const uint32_t pos = inc - cnt;
-            const uint32_t tot = rlu(inc, 63);
-            const int nwin = tot < 64u ? (int)tot : 64;
-            if (take) {
-                for (uint32_t t = 0; t < cnt && pos + t < 64u; ++t) {
-                    m.winq[pos + t] = bs + skip + t;
-                    m.winent[pos + t] = (uint16_t)l;
-                    m.winoff[pos + t] = (uint16_t)(skip + t);
+                // gather the next window of candidates under the current radius
+                r2b = r2;
+                const double lim = 4.0 * r2;
+                const bool committed = (l == k0) && (o0 > 0);
+                const bool brk = (l >= k0) && !committed && (bend || bd2 > lim);   // :170
+                kbc = ffs64(ballot(brk));
+                const bool take = (l >= k0) && (l < kbc) && (committed || (binr && bdist <= lim));
+                const uint32_t skip = (l == k0) ? (uint32_t)o0 : 0u;
+                const uint32_t cnt = take ? bc - skip : 0u;
+                const uint32_t inc = wave_incl_scan(cnt);
+                const uint32_t pos = inc - cnt;
+                wtot = rlu(inc, 63);
+                const int nwin = wtot < 64u ? (int)wtot : 64;
+                if (take) {
+                    for (uint32_t t = 0; t < cnt && pos + t < 64u; ++t) {
+                        m.winq[pos + t] = bs + skip + t;
+                        m.winent[pos + t] = (uint16_t)l;
+                        m.winoff[pos + t] = (uint16_t)(skip + t);
+                    }
                 }
-            }
-            __syncthreads();
-            const bool wa = l < nwin;
-            const uint32_t wq = wa ? m.winq[l] : 0u;
-            const int wk = wa ? (int)m.winent[l] : 0;
-            const int woff = wa ? (int)m.winoff[l] : 0;
-            const double wbd = lane_get_d(bdist, wk);
-            uint32_t wj = 0xffffffffu;
-            double wd0 = 0.0, wd1 = 0.0, wd2 = 0.0, wdsq = 0.0;
-            if (wa) {
-                const double4 q = A.spos[wq];
-                wj = (uint32_t)__double_as_longlong(q.w);
-                wd0 = q.x - gx; wd1 = q.y - gy; wd2 = q.z - gz;
-                wdsq = wd0 * wd0 + wd1 * wd1 + wd2 * wd2;   // tessellation.rs:447-456
-            }
-            bool wvalid = wa;
-            int last = -1;
-            bool rebuild = false;
-            // visit the window's candidates in order (tessellation.rs:446-471)
-            for (;;) {
-                lim = 4.0 * r2;
-                const uint64_t pm = ballot(wvalid && l > last && wj != i && !(wdsq > lim)) ;
-                if (pm == 0) break;
-                const int cl = ffs64(pm);
-                const double d0 = rld(wd0, cl), d1 = rld(wd1, cl), d2 = rld(wd2, cl);
-                const Plane pl{gx + d0 * 0.5, gy + d1 * 0.5, gz + d2 * 0.5, d0, d1, d2};
-                last = cl;
-                double nr2 = 0.0;
-                const int rc = clip_any<C>(m, P, r, pl, (int32_t)rlu(wj, cl), true, gx, gy, gz, nr2);
-                if (rc == kNoChange) continue;
-                if (rc < 0) {
-                    status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
-                    break;
+                __syncthreads();
+                const bool wa = l < nwin;
+                const uint32_t wq = wa ? m.winq[l] : 0u;
+                wk = wa ? (int)m.winent[l] : 0;
+                woff = wa ? (int)m.winoff[l] : 0;
+                wbd = lane_get_d(bdist, wk);
+                wj = 0xffffffffu;
+                wd0 = wd1 = wd2 = wdsq = 0.0;
+                if (wa) {
+                    const double4 q = A.spos[wq];
+                    wj = (uint32_t)__double_as_longlong(q.w);
+                    wd0 = q.x - gx; wd1 = q.y - gy; wd2 = q.z - gz;
+                    wdsq = wd0 * wd0 + wd1 * wd1 + wd2 * wd2;   // tessellation.rs:447-456
                 }
-                if (P.V == 0) nr2 = 0.0;
-                const int kc = rl(wk, cl), offc = rl(woff, cl);
-                r2 = nr2;
-                if (nr2 > r2b) {
-                    // the radius grew: entries dropped from this window may now pass, rebuild
-                    if ((uint32_t)(offc + 1) < rlu(bc, kc)) { k0 = kc; o0 = offc + 1; }
-                    else { k0 = kc + 1; o0 = 0; }
-                    rebuild = true;
-                    break;
-                }
-                // the radius shrank: re-filter the rest of the window with it
-                lim = 4.0 * r2;
-                kbc = ffs64(ballot((l > kc) && (bend || bd2 > lim)));
-                wvalid = wvalid && (l <= cl || wk == kc || (wk < kbc && wbd <= lim));
+                wvalid = wa;
+                last = -1;
+                win = true;
+                __syncthreads();
             }
-            if (status != kCellOk) break;
-
-            bool walk_end = false;
-            if (!rebuild) {
-                if (tot > 64u) {
-                    // the window stopped inside the shell order: continue after its last item
+            // next candidate of the window that passes the distance test (tessellation.rs:458)
+            const double lim = 4.0 * r2;
+            const uint64_t pm = ballot(wvalid && l > last && wj != i && !(wdsq > lim));
+            if (pm == 0) {
+                // window exhausted: find where the walk continues
+                win = false;
+                bool walk_end = false;
+                if (wtot > 64u) {
                     const int k63 = rl(wk, 63), off63 = rl(woff, 63);
                     if (rl((int)wvalid, 63)) {
                         if ((uint32_t)(off63 + 1) < rlu(bc, k63)) { k0 = k63; o0 = off63 + 1; }
@@ -940,20 +954,55 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                 } else {
                     k0 = 64;
                 }
-            }
-            if (walk_end) {
-                // the walk stops at entry kbc: a real break, or the end of the shell prefix
-                if (rl((int)bend, kbc)) {
-                    const uint32_t e = e0 + (uint32_t)kbc;
-                    if (!(e == A.nshell && A.shell_next_d2 > 4.0 * r2)) status = kCellShell;
+                if (walk_end) {
+                    // the walk stops at entry kbc: a real break, or the end of the shell prefix
+                    if (rl((int)bend, kbc)) {
+                        const uint32_t e = e0 + (uint32_t)kbc;
+                        if (!(e == A.nshell && A.shell_next_d2 > lim)) status = kCellShell;
+                    }
+                    break;
                 }
-                __syncthreads();
-                break;
+                if (k0 >= 64) { e0 += 64; k0 = 0; o0 = 0; load = true; }
+                continue;
             }
-            if (k0 >= 64) { e0 += 64; k0 = 0; o0 = 0; load = true; }
-            __syncthreads();
+            cl = ffs64(pm);
+            const double d0 = rld(wd0, cl), d1 = rld(wd1, cl), d2 = rld(wd2, cl);
+            pl = Plane{gx + d0 * 0.5, gy + d1 * 0.5, gz + d2 * 0.5, d0, d1, d2};
+            nid = (int32_t)rlu(wj, cl);
+            withgen = true;
+            last = cl;
         }
+
+        double nr2 = 0.0;
+        const int rc = clip_any<C>(m, P, r, pl, nid, withgen, gx, gy, gz, nr2);   // the clip site
+        if (rc < 0) {
+            status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
+            break;
+        }
+        if (cl < 0) {                   // a wall plane
+            ++wi;
+            if (P.V == 0) break;        // tessellation.rs:433-435
+            continue;
+        }
+        if (rc == kNoChange) continue;
+        // a cut (tessellation.rs:462-469)
+        if (P.V == 0) nr2 = 0.0;
+        const int kc = rl(wk, cl), offc = rl(woff, cl);
+        r2 = nr2;
+        if (nr2 > r2b) {
+            // the radius grew: entries dropped from this window may now pass, gather again
+            if ((uint32_t)(offc + 1) < rlu(bc, kc)) { k0 = kc; o0 = offc + 1; }
+            else { k0 = kc + 1; o0 = 0; }
+            if (k0 >= 64) { e0 += 64; k0 = 0; o0 = 0; load = true; }
+            win = false;
+            continue;
+        }
+        // the radius shrank or stayed: re-filter the rest of the window with it
+        const double lim = 4.0 * r2;
+        kbc = ffs64(ballot((l > kc) && (bend || bd2 > lim)));
+        wvalid = wvalid && (l <= cl || wk == kc || (wk < kbc && wbd <= lim));
     }
+    __syncthreads();
 
     // outputs
     if (l == 0) A.status[i] = status;
@@ -1025,26 +1074,38 @@ __global__ __launch_bounds__(64) void k_cells(TessArgs A) {
     }
 }
 
-using Tier0 = Cap<96, 288, 48, 144>;
-using Tier1 = Cap<1024, 2560, 320, 1280>;
+// Tier 0: register fast path, small LDS (many waves per CU). Tier 1: generic path for cells that
+// outgrow tier 0. Tier 2: generic path with the largest LDS polytope (one wave per CU).
+using Tier0 = Cap<64, 192, 48, 64, true, 127>;
+using Tier1 = Cap<256, 768, 128, 384, false, 255>;
+using Tier2 = Cap<1024, 2560, 320, 1280, false, 511>;
 
 }  // namespace
 
-int tier_max_vertices(int tier) { return tier == 0 ? Tier0::VM : Tier1::VM; }
+int tier_count() { return 3; }
 
-size_t tier_lds_bytes(int tier) { return tier == 0 ? sizeof(Mem<Tier0>) : sizeof(Mem<Tier1>); }
+int tier_max_vertices(int tier) {
+    return tier == 0 ? Tier0::VM : tier == 1 ? Tier1::VM : Tier2::VM;
+}
+
+size_t tier_lds_bytes(int tier) {
+    return tier == 0 ? sizeof(Mem<Tier0>) : tier == 1 ? sizeof(Mem<Tier1>) : sizeof(Mem<Tier2>);
+}
 
 hipError_t launch_cells(int tier, const TessArgs &a, int grid_waves, hipStream_t st) {
     if (a.ncells == 0) return hipSuccess;
     if (tier == 0) k_cells<Tier0><<<grid_waves, 64, 0, st>>>(a);
-    else k_cells<Tier1><<<grid_waves, 64, 0, st>>>(a);
+    else if (tier == 1) k_cells<Tier1><<<grid_waves, 64, 0, st>>>(a);
+    else k_cells<Tier2><<<grid_waves, 64, 0, st>>>(a);
     return hipGetLastError();
 }
 
 hipError_t cells_occupancy(int tier, int *blocks_per_cu) {
     if (tier == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0>, 64, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1>, 64, 0);
+    if (tier == 1)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2>, 64, 0);
 }
 
 }  // namespace vd

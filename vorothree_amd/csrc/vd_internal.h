@@ -104,9 +104,11 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n,
                           uint32_t *out_cap, uint32_t *out_shell, uint32_t *counts, hipStream_t st);
 
-// tier: 0 = small LDS polytope (many waves per CU), 1 = large LDS polytope (one wave per CU)
+// tier: 0 = register fast path with a small LDS polytope (many waves per CU), 1 = generic path
+// with a medium LDS polytope, 2 = generic path with the largest LDS polytope (one wave per CU)
 hipError_t launch_cells(int tier, const TessArgs &a, int grid_waves, hipStream_t st);
 hipError_t cells_occupancy(int tier, int *blocks_per_cu);
+int tier_count();
 int tier_max_vertices(int tier);
 size_t tier_lds_bytes(int tier);
 
