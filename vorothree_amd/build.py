@@ -29,7 +29,8 @@ def hipcc() -> str:
 
 
 def flags() -> list:
-    return [
+    extra = [f"-D{d}" for d in os.environ.get("VD_DEFINES", "").split() if d]
+    return [*extra,
         f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
         "-ffp-contract=off", "-fno-fast-math",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

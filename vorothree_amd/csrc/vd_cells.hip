@@ -128,7 +128,7 @@ __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
     return (uint32_t)v;
 }
 
-template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int TM_>
+template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int TM_, int WAVES_>
 struct Cap {
     static constexpr int VM = VM_;      // vertices
     static constexpr int SM = SM_;      // face slots (flattened face indices)
@@ -136,6 +136,7 @@ struct Cap {
     static constexpr int KM = KM_;      // plane-crossing slots per clip
     static constexpr bool FAST = FAST_; // register fast path (VM <= 64) instead of the generic one
     static constexpr int TM = TM_;      // triangles staged in LDS for the volume/centroid folds
+    static constexpr int WAVES = WAVES_;  // occupancy target (waves per SIMD) for the compiler
 };
 
 template <class C>
@@ -157,15 +158,15 @@ struct Mem {
             uint32_t lmap[C::VM];
             uint32_t seg[C::SM];
             int32_t fbase[C::FM];
-            uint16_t remap[C::VM];
+            uint16_t remap[C::FAST ? 1 : C::VM];   // generic path only
             uint16_t P[C::SM + 1];
-            uint16_t cxid[C::KM];
+            uint16_t cxid[C::FAST ? 1 : C::KM];    // generic path only
             uint16_t fidxn[C::SM];
             uint16_t nextp[C::SM];
             int16_t fnew[C::FM];
             uint16_t fnoff[C::FM];
             uint16_t nend[C::FM];
-            uint16_t lid[C::SM];
+            uint16_t lid[C::FAST ? 1 : C::SM];     // generic path only
         };
         double tri[4][C::TM + 1];   // +1 keeps the four rows in different LDS banks
     };
@@ -505,12 +506,12 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
         }
     }
 
-    // 2. slots: output entry counts (prefix in slot order) and plane crossings (:378-452)
+    // 2. slots: output entry counts (prefix in slot order) and plane crossings (:378-452).
+    //    Per chunk and lane two registers: ea = a | b << 6 | face << 12 (a, b < 64, face < 64)
+    //    and eq = output prefix | crossing index << 16.
     const int nch = (P.S + 63) >> 6;
     uint32_t ea[3] = {0u, 0u, 0u};
-    uint32_t ef[3] = {0u, 0u, 0u};
-    uint32_t eP[3] = {0u, 0u, 0u};
-    int eci[3] = {0, 0, 0};
+    uint32_t eq[3] = {0u, 0u, 0u};
     int K = 0, ptot = 0;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
@@ -518,21 +519,21 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
             const int s = 64 * ch + l;
             const bool sa = s < P.S;
             const uint32_t e = sa ? m.sedge[c][s] : 0u;
-            ea[ch] = e;
-            ef[ch] = sa ? m.sface[c][s] : 0u;
             const uint32_t a = e & 0xffffu, b = e >> 16;
+            ea[ch] = a | (b << 6) | ((sa ? (uint32_t)m.sface[c][s] : 0u) << 12);
             const bool ain = (inm >> a) & 1ull, bin = (inm >> b) & 1ull;
             const uint64_t b0 = ballot(sa && ain);
             const uint64_t b1 = ballot(sa && !ain && bin);
-            eP[ch] = (uint32_t)(ptot + popc(b0 & lt) + 2 * popc(b1 & lt));
+            const uint32_t pp = (uint32_t)(ptot + popc(b0 & lt) + 2 * popc(b1 & lt));
             ptot += popc(b0) + 2 * popc(b1);
-            if (sa) m.P[s] = (uint16_t)eP[ch];
+            if (sa) m.P[s] = (uint16_t)pp;
             const bool cr = sa && (ain != bin);
             const uint64_t cm = ballot(cr);
-            eci[ch] = K + popc(cm & lt);
-            if (cr && eci[ch] < 64) {
-                m.ckey[eci[ch]] = a < b ? ((a << 16) | b) : ((b << 16) | a);
-                m.cab[eci[ch]] = a | (b << 16);
+            const int ci = K + popc(cm & lt);
+            eq[ch] = pp | ((uint32_t)(ci & 0xffff) << 16);
+            if (cr && ci < 64) {
+                m.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
+                m.cab[ci] = a | (b << 16);
             }
             K += popc(cm);
         }
@@ -608,14 +609,14 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
     for (int ch = 0; ch < 3; ++ch) {
         if (ch < nch) {
             const int s = 64 * ch + l;
-            const uint32_t a = ea[ch] & 0xffffu, b = ea[ch] >> 16;
+            const uint32_t a = ea[ch] & 63u, b = (ea[ch] >> 6) & 63u;
             const bool ain = (inm >> a) & 1ull, bin = (inm >> b) & 1ull;
-            const int xv = (int)lane_get((uint32_t)xid, eci[ch] & 63);
+            const int xv = (int)lane_get((uint32_t)xid, (int)(eq[ch] >> 16) & 63);
             if (s < P.S && (ain || bin)) {
-                const int f = (int)ef[ch];
+                const int f = (int)(ea[ch] >> 12);
                 const int nf = m.fnew[f];
                 if (nf >= 0) {
-                    const int pos = m.fbase[f] + (int)eP[ch];
+                    const int pos = m.fbase[f] + (int)(eq[ch] & 0xffffu);
                     const int ns = m.fnoff[f], ne = m.nend[f];
                     const uint16_t rb = (uint16_t)popc(inm & ((1ull << b) - 1ull));
                     m.fidxn[pos] = (ain && bin) ? rb : (uint16_t)xv;
@@ -819,10 +820,10 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
     uint32_t wtot = 0;      // candidates the window would have held without the 64 cap
     int kbc = 64;           // first batch entry that ends the walk under the current radius
     int last = -1;          // window lanes <= last are visited
-    int wk = 0, woff = 0;
+    uint32_t wko = 0;       // batch lane of the candidate's entry | offset in its bin << 8
     uint32_t wj = 0xffffffffu;
-    double wd0 = 0.0, wd1 = 0.0, wd2 = 0.0, wdsq = 0.0, wbd = 0.0;
-    bool wvalid = false;
+    double wd0 = 0.0, wd1 = 0.0, wd2 = 0.0, wdsq = 0.0;
+    uint64_t wvm = 0;       // window lanes whose entry is still live
 
     int wi = 0;             // next wall (tessellation.rs:429-436), then the neighbour walk
     bool walking = false;
@@ -916,9 +917,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                 __syncthreads();
                 const bool wa = l < nwin;
                 const uint32_t wq = wa ? m.winq[l] : 0u;
-                wk = wa ? (int)m.winent[l] : 0;
-                woff = wa ? (int)m.winoff[l] : 0;
-                wbd = lane_get_d(bdist, wk);
+                wko = wa ? ((uint32_t)m.winent[l] | ((uint32_t)m.winoff[l] << 8)) : 0u;
                 wj = 0xffffffffu;
                 wd0 = wd1 = wd2 = wdsq = 0.0;
                 if (wa) {
@@ -927,21 +926,23 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                     wd0 = q.x - gx; wd1 = q.y - gy; wd2 = q.z - gz;
                     wdsq = wd0 * wd0 + wd1 * wd1 + wd2 * wd2;   // tessellation.rs:447-456
                 }
-                wvalid = wa;
+                wvm = ballot(wa);
                 last = -1;
                 win = true;
                 __syncthreads();
             }
             // next candidate of the window that passes the distance test (tessellation.rs:458)
             const double lim = 4.0 * r2;
-            const uint64_t pm = ballot(wvalid && l > last && wj != i && !(wdsq > lim));
+            const uint64_t above = last < 0 ? ~0ull : (last >= 63 ? 0ull : ~((2ull << last) - 1ull));
+            const uint64_t pm = wvm & above & ballot(wj != i && !(wdsq > lim));
             if (pm == 0) {
                 // window exhausted: find where the walk continues
                 win = false;
                 bool walk_end = false;
                 if (wtot > 64u) {
-                    const int k63 = rl(wk, 63), off63 = rl(woff, 63);
-                    if (rl((int)wvalid, 63)) {
+                    const uint32_t ko63 = rlu(wko, 63);
+                    const int k63 = (int)(ko63 & 0xffu), off63 = (int)(ko63 >> 8);
+                    if ((wvm >> 63) & 1ull) {
                         if ((uint32_t)(off63 + 1) < rlu(bc, k63)) { k0 = k63; o0 = off63 + 1; }
                         else { k0 = k63 + 1; o0 = 0; }
                     } else if (k63 >= kbc) {
@@ -987,7 +988,8 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
         if (rc == kNoChange) continue;
         // a cut (tessellation.rs:462-469)
         if (P.V == 0) nr2 = 0.0;
-        const int kc = rl(wk, cl), offc = rl(woff, cl);
+        const uint32_t koc = rlu(wko, cl);
+        const int kc = (int)(koc & 0xffu), offc = (int)(koc >> 8);
         r2 = nr2;
         if (nr2 > r2b) {
             // the radius grew: entries dropped from this window may now pass, gather again
@@ -1000,7 +1002,9 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
         // the radius shrank or stayed: re-filter the rest of the window with it
         const double lim = 4.0 * r2;
         kbc = ffs64(ballot((l > kc) && (bend || bd2 > lim)));
-        wvalid = wvalid && (l <= cl || wk == kc || (wk < kbc && wbd <= lim));
+        const int wk = (int)(wko & 0xffu);
+        const double wbd = lane_get_d(bdist, wk);   // bin distance of the candidate's entry
+        wvm &= ballot(l <= cl || wk == kc || (wk < kbc && wbd <= lim));
     }
     __syncthreads();
 
@@ -1063,7 +1067,8 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
 }
 
 template <class C>
-__global__ __launch_bounds__(64) void k_cells(TessArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C::WAVES, 8)))
+void k_cells(TessArgs A) {
     __shared__ Mem<C> m;
     for (;;) {
         uint32_t q = 0;
@@ -1076,9 +1081,12 @@ __global__ __launch_bounds__(64) void k_cells(TessArgs A) {
 
 // Tier 0: register fast path, small LDS (many waves per CU). Tier 1: generic path for cells that
 // outgrow tier 0. Tier 2: generic path with the largest LDS polytope (one wave per CU).
-using Tier0 = Cap<64, 192, 48, 64, true, 127>;
-using Tier1 = Cap<256, 768, 128, 384, false, 255>;
-using Tier2 = Cap<1024, 2560, 320, 1280, false, 511>;
+#ifndef VD_TIER0_WAVES
+#define VD_TIER0_WAVES 4
+#endif
+using Tier0 = Cap<64, 192, 48, 64, true, 107, VD_TIER0_WAVES>;
+using Tier1 = Cap<256, 768, 128, 384, false, 255, 1>;
+using Tier2 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
 
 }  // namespace
 
