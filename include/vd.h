@@ -117,6 +117,63 @@ int vd_shell_prefix(const double bmin[3], const double bmax[3], uint32_t nx, uin
                     uint32_t nz, uint32_t kmax, int32_t *dxyz, double *keys, size_t cap,
                     size_t *count, double *next_key);
 
+/* ---- Batched tessellations (no counterpart in the reference: many independent
+ * Tessellation::<3, Cell3DFaces, Algorithm3DGrid> computed in one persistent launch) ---------- */
+
+typedef struct vd_batch vd_batch;
+
+/* A wall of a batch member. kind 0: PlaneGeometry::new(a, b) (b is normalised as
+ * src/wall/wall_3d.rs:23-27 does); kind 1: SphereGeometry::new(a, r). id <= VD_WALL_ID_MAX. */
+typedef struct vd_wall {
+    int32_t id;
+    int32_t kind;
+    double a[3];
+    double b[3];
+    double r;
+} vd_wall;
+
+int vd_batch_create(vd_batch **out);
+void vd_batch_destroy(vd_batch *b);
+/* Removes every member. */
+int vd_batch_clear(vd_batch *b);
+/* Adds one tessellation: Tessellation::new(bounds, Algorithm3DGrid::new(nx, ny, nz, &bounds)),
+ * its walls (add_wall in order) and set_generators(xyz[0..3n]); *index receives its member
+ * number. nx = ny = nz = 0 picks the grid with vd_grid_for at the batch's generators-per-bin
+ * target (VD_BOPT_PER_BIN). Points are staged on the host and uploaded by the next
+ * vd_batch_upload / vd_batch_calculate. */
+int vd_batch_add(vd_batch *b, const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
+                 uint32_t nz, const vd_wall *walls, int nwalls, const double *xyz, size_t n,
+                 uint32_t *index);
+/* Uploads the staged points of every member to device memory (done once; calculate reuses it). */
+int vd_batch_upload(vd_batch *b);
+/* Every member's calculate() in one launch sequence: wall filter and binning of all members, then
+ * one persistent cell launch over all their cells. */
+int vd_batch_calculate(vd_batch *b);
+/* Every member's relax() (src/tessellation.rs:482-495); requires vd_batch_calculate first. */
+int vd_batch_relax(vd_batch *b);
+size_t vd_batch_count(const vd_batch *b);
+/* info[0] cells (generators kept), [1..3] grid nx ny nz, [4] first output slot of the member. */
+int vd_batch_member_info(const vd_batch *b, uint32_t t, uint64_t info[5]);
+int vd_batch_get_volumes(const vd_batch *b, uint32_t t, double *out, size_t n);
+int vd_batch_get_centroids(const vd_batch *b, uint32_t t, double *out3, size_t n);
+int vd_batch_get_face_counts(const vd_batch *b, uint32_t t, uint32_t *out, size_t n);
+int vd_batch_get_generators(const vd_batch *b, uint32_t t, double *out, size_t n);
+int vd_batch_get_neighbors(const vd_batch *b, uint32_t t, uint64_t *offsets, size_t n, int32_t *ids,
+                           size_t cap);
+int vd_batch_get_cell(const vd_batch *b, uint32_t t, size_t i, uint64_t sizes[3], double *vertices,
+                      uint32_t *face_counts, uint32_t *face_indices, int32_t *face_neighbors);
+#define VD_BOPT_KEEP_CELLS 1 /* as VD_OPT_KEEP_CELLS; default 0 for batches */
+#define VD_BOPT_PER_BIN 2    /* generators-per-bin target of vd_batch_add's grid choice, x1000 */
+int vd_batch_set_option(vd_batch *b, int key, int64_t value);
+int vd_batch_get_stats(const vd_batch *b, double out[8]);
+/* Times vd_batch_calculate for each candidate generators-per-bin target (grids re-derived with
+ * vd_grid_for for the members added with a zero grid), keeps the fastest, and returns it. */
+int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *best);
+
+/* The grid rule of the autotuner: bins per axis proportional to the box edges with about
+ * per_bin generators per bin, at least 1 per axis. */
+int vd_grid_for(const double bmin[3], const double bmax[3], size_t n, double per_bin, uint32_t out[3]);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char *vd_last_error(void);
 

@@ -6,6 +6,7 @@ package mirrors the reference's Rust API on top of it.
 from .tessellation import (  # noqa: F401
     WALL_ID_MAX,
     Algorithm3DGrid,
+    Batch,
     BoundingBox,
     Cell3DFaces,
     PlaneGeometry,
@@ -13,5 +14,6 @@ from .tessellation import (  # noqa: F401
     Tessellation,
     Wall,
     box_side,
+    grid_for,
     shell_prefix,
 )

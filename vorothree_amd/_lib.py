@@ -18,6 +18,14 @@ VD_EDEVICE = -4
 VD_ECAPACITY = -5
 VD_ESTATE = -6
 VD_OPT_KEEP_CELLS = 1
+VD_BOPT_KEEP_CELLS = 1
+VD_BOPT_PER_BIN = 2
+
+
+class VdWall(ctypes.Structure):
+    """struct vd_wall of include/vd.h."""
+    _fields_ = [("id", ctypes.c_int32), ("kind", ctypes.c_int32), ("a", ctypes.c_double * 3),
+                ("b", ctypes.c_double * 3), ("r", ctypes.c_double)]
 
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -46,6 +54,25 @@ SIGNATURES = {
     "vd_tess_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
     "vd_tess_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_shell_prefix": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _sz, _vp, _vp]),
+    "vd_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "vd_batch_destroy": (None, [_vp]),
+    "vd_batch_clear": (ctypes.c_int, [_vp]),
+    "vd_batch_add": (ctypes.c_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp, ctypes.c_int, _vp, _sz, _vp]),
+    "vd_batch_upload": (ctypes.c_int, [_vp]),
+    "vd_batch_calculate": (ctypes.c_int, [_vp]),
+    "vd_batch_relax": (ctypes.c_int, [_vp]),
+    "vd_batch_count": (_sz, [_vp]),
+    "vd_batch_member_info": (ctypes.c_int, [_vp, _u32, _vp]),
+    "vd_batch_get_volumes": (ctypes.c_int, [_vp, _u32, _vp, _sz]),
+    "vd_batch_get_centroids": (ctypes.c_int, [_vp, _u32, _vp, _sz]),
+    "vd_batch_get_face_counts": (ctypes.c_int, [_vp, _u32, _vp, _sz]),
+    "vd_batch_get_generators": (ctypes.c_int, [_vp, _u32, _vp, _sz]),
+    "vd_batch_get_neighbors": (ctypes.c_int, [_vp, _u32, _vp, _sz, _vp, _sz]),
+    "vd_batch_get_cell": (ctypes.c_int, [_vp, _u32, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "vd_batch_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
+    "vd_batch_get_stats": (ctypes.c_int, [_vp, _vp]),
+    "vd_batch_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    "vd_grid_for": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_double, _vp]),
     "vd_last_error": (ctypes.c_char_p, []),
 }
 

@@ -1,6 +1,10 @@
-// C ABI of libvoronoid_hip.so (declared in include/vd.h): handle state, device memory, the bin
-// shell-order prefix, the generator pipeline (wall filter -> compaction -> binning) and the cell
-// launches with their retry tiers.
+// C ABI of libvoronoid_hip.so (declared in include/vd.h).
+//
+// An Engine owns the device state of a set of tessellations ("members"): their input points, the
+// wall filter and compaction, the concatenated bins, the shell-order prefixes (one per distinct
+// grid geometry), the outputs, and the cell launches with their retry tiers. vd_tess (the
+// reference's Tessellation) is an Engine with one member; vd_batch holds many and computes them in
+// one persistent launch.
 #include "vd.h"
 #include "vd_internal.h"
 
@@ -30,6 +34,12 @@ int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess)                                                           \
             return fail(e_ == hipErrorOutOfMemory ? VD_ENOMEM : VD_EDEVICE,             \
                         std::string(#call) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+#define VD_TRY(expr)                  \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != VD_OK) return rc_; \
     } while (0)
 
 struct DevBuf {
@@ -122,213 +132,349 @@ void build_shell(const int64_t n[3], const double cs[3], uint32_t kmax, ShellPre
     out.complete = std::isinf(next);
 }
 
-}  // namespace
+// Algorithm3DGrid::new's scale factors (algo_3d_grid.rs:48-54, 85-87).
+vd::GridGeom make_grid(const double lo[3], const double hi[3], const int64_t n[3]) {
+    vd::GridGeom g{};
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = lo[a];
+        g.s[a] = (double)n[a] / (hi[a] - lo[a]);
+        g.cs[a] = 1.0 / g.s[a];
+        g.lim[a] = (double)n[a] - 1e-5;
+        g.n[a] = (int32_t)n[a];
+    }
+    return g;
+}
 
-struct vd_tess {
+void grid_for(const double lo[3], const double hi[3], size_t n, double per_bin, int64_t out[3]) {
+    double e[3], vol = 1.0;
+    for (int a = 0; a < 3; ++a) { e[a] = hi[a] - lo[a]; vol *= e[a]; }
+    const double h = std::cbrt(vol * per_bin / (double)std::max<size_t>(n, 1));
+    for (int a = 0; a < 3; ++a) out[a] = std::min<int64_t>(16384, std::max<int64_t>(1, (int64_t)std::llround(e[a] / h)));
+}
+
+int make_wall(const vd_wall &w, vd::WallDev &out) {
+    if (w.id > VD_WALL_ID_MAX) return fail(VD_EINVAL, "Wall ID must be <= -1000");   // wall/mod.rs:25-27
+    out = vd::WallDev{};
+    out.id = w.id;
+    out.kind = w.kind;
+    for (int k = 0; k < 3; ++k) out.a[k] = w.a[k];
+    if (w.kind == 0) {
+        // PlaneGeometry::new, wall_3d.rs:23-27
+        const double len = std::sqrt(w.b[0] * w.b[0] + w.b[1] * w.b[1] + w.b[2] * w.b[2]);
+        if (len == 0.0) { out.b[0] = 0.0; out.b[1] = 0.0; out.b[2] = 1.0; }
+        else { out.b[0] = w.b[0] / len; out.b[1] = w.b[1] / len; out.b[2] = w.b[2] / len; }
+    } else if (w.kind == 1) {
+        out.r = w.r;
+    } else {
+        return fail(VD_EINVAL, "unknown wall kind (0 plane, 1 sphere)");
+    }
+    return VD_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Engine
+// ------------------------------------------------------------------------------------------
+struct Member {
     double lo[3], hi[3];
     int64_t n[3];
-    uint64_t nbins = 0;
+    bool auto_grid = false;
     vd::GridGeom grid{};
     std::vector<vd::WallDev> walls;
-    int device = 0;
-    int cus = 1;
+    int shape = -1;
+    uint64_t raw_off = 0, nraw = 0, gen_off = 0, ngen = 0, bin_off = 0, nbins = 0;
+};
+
+struct Shape {   // one shell prefix per distinct grid geometry
+    int64_t n[3];
+    double cs[3];
+    ShellPrefix sp;
+    uint32_t off = 0;
+};
+
+struct Engine {
+    std::vector<Member> mem;
+    std::vector<Shape> shapes;
+    uint32_t shell_k = 7;
+    int device = 0, cus = 1;
     int occ[kTiers] = {1, 1, 1};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int keep_cells = 1;
 
-    // shell prefix
-    uint32_t shell_k = 0;
-    ShellPrefix shell;
-    DevBuf d_shell;
-
-    // generators and bins
-    size_t ngen = 0;
-    DevBuf d_gen, d_raw, d_flags, d_pos, d_scan;
-    DevBuf d_gen_bin, d_counts, d_bin_start, d_fill, d_items, d_spos;
-
-    // results
-    bool calculated = false;
-    size_t ncells = 0;
+    DevBuf d_desc, d_offs, d_raw, d_raw_tess, d_flags, d_pos, d_scan, d_gat;
+    DevBuf d_gen, d_gen_tess, d_gen_bin, d_counts, d_bin_start, d_fill, d_items, d_spos, d_shell;
     DevBuf d_vol, d_cen, d_nface, d_nvert, d_status, d_cell_off;
     DevBuf d_vtx, d_fcnt, d_fnb, d_fidx, d_top, d_queue, d_counts4;
+    uint64_t nraw = 0, ngen = 0, nbins = 0;
+    bool prepared = false, calculated = false;
     uint64_t cap[3] = {0, 0, 0};
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<vd::TessDesc> desc;
 };
 
-namespace {
-
-int scan(vd_tess *t, const uint32_t *in, uint32_t *out, uint32_t n) {
-    size_t words = vd::scan_tmp_words(n);
-    int rc = ensure(t->d_scan, words * 4);
-    if (rc) return rc;
-    VD_HIP(vd::launch_scan_u32(in, out, n, t->d_scan.as<uint32_t>(), words, t->stream));
+int engine_init(Engine &E) {
+    hipError_t e = hipGetDevice(&E.device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&E.cus, hipDeviceAttributeMultiprocessorCount, E.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&E.ev0);
+    if (e == hipSuccess) e = hipEventCreate(&E.ev1);
+    for (int k = 0; k < kTiers && e == hipSuccess; ++k) e = vd::cells_occupancy(k, &E.occ[k]);
+    if (e != hipSuccess) return fail(VD_EDEVICE, std::string("device setup: ") + hipGetErrorString(e));
+    for (int k = 0; k < kTiers; ++k) E.occ[k] = std::max(1, E.occ[k]);
     return VD_OK;
 }
 
-// Wall filter of n points in d_src (input order kept), result in t->d_gen; t->ngen updated.
-int filter_into_gen(vd_tess *t, const double *d_src, size_t n) {
-    int rc;
-    if (t->walls.empty()) {
-        if ((rc = ensure(t->d_gen, n * 24))) return rc;
-        if (n) VD_HIP(hipMemcpyAsync(t->d_gen.p, d_src, n * 24, hipMemcpyDeviceToDevice, t->stream));
-        t->ngen = n;
-        return VD_OK;
+void engine_free(Engine &E) {
+    if (E.stream) (void)hipStreamSynchronize(E.stream);
+    DevBuf *bufs[] = {&E.d_desc, &E.d_offs, &E.d_raw, &E.d_raw_tess, &E.d_flags, &E.d_pos, &E.d_scan,
+                      &E.d_gat, &E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start,
+                      &E.d_fill, &E.d_items, &E.d_spos, &E.d_shell, &E.d_vol, &E.d_cen, &E.d_nface,
+                      &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
+                      &E.d_top, &E.d_queue, &E.d_counts4};
+    for (DevBuf *b : bufs) release(*b);
+    if (E.ev0) (void)hipEventDestroy(E.ev0);
+    if (E.ev1) (void)hipEventDestroy(E.ev1);
+    if (E.stream) (void)hipStreamDestroy(E.stream);
+    E.ev0 = E.ev1 = nullptr;
+    E.stream = nullptr;
+}
+
+int scan(Engine &E, const uint32_t *in, uint32_t *out, uint32_t n) {
+    const size_t words = vd::scan_tmp_words(n);
+    VD_TRY(ensure(E.d_scan, words * 4));
+    VD_HIP(vd::launch_scan_u32(in, out, n, E.d_scan.as<uint32_t>(), words, E.stream));
+    return VD_OK;
+}
+
+// Builds (with the current shell_k) the shell prefix of every distinct grid.
+int engine_shells(Engine &E) {
+    E.shapes.clear();
+    for (Member &m : E.mem) {
+        int found = -1;
+        for (size_t k = 0; k < E.shapes.size(); ++k) {
+            const Shape &s = E.shapes[k];
+            if (s.n[0] == m.n[0] && s.n[1] == m.n[1] && s.n[2] == m.n[2] &&
+                std::memcmp(s.cs, m.grid.cs, sizeof(s.cs)) == 0) { found = (int)k; break; }
+        }
+        if (found < 0) {
+            Shape s;
+            for (int a = 0; a < 3; ++a) { s.n[a] = m.n[a]; s.cs[a] = m.grid.cs[a]; }
+            E.shapes.push_back(s);
+            found = (int)E.shapes.size() - 1;
+        }
+        m.shape = found;
     }
-    if ((rc = ensure(t->d_flags, (n + 1) * 4)) || (rc = ensure(t->d_pos, (n + 1) * 4))) return rc;
-    VD_HIP(vd::launch_fill_u32(t->d_flags.as<uint32_t>(), 0u, n + 1, t->stream));
-    VD_HIP(vd::launch_wall_flags(d_src, (uint32_t)n, t->walls.data(), (int)t->walls.size(),
-                                 t->d_flags.as<uint32_t>(), t->stream));
-    if ((rc = scan(t, t->d_flags.as<uint32_t>(), t->d_pos.as<uint32_t>(), (uint32_t)(n + 1)))) return rc;
-    uint32_t kept = 0;
-    VD_HIP(hipMemcpyAsync(&kept, t->d_pos.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, t->stream));
-    VD_HIP(hipStreamSynchronize(t->stream));
-    DevBuf out;
-    if (d_src == t->d_gen.p) {
-        if ((rc = ensure(out, (size_t)kept * 24))) return rc;
+    size_t total = 0;
+    for (Shape &s : E.shapes) {
+        build_shell(s.n, s.cs, E.shell_k, s.sp);
+        s.off = (uint32_t)total;
+        total += s.sp.e.size();
+    }
+    std::vector<vd::ShellEntry> all;
+    all.reserve(total);
+    for (const Shape &s : E.shapes) all.insert(all.end(), s.sp.e.begin(), s.sp.e.end());
+    VD_TRY(ensure(E.d_shell, std::max<size_t>(total, 1) * sizeof(vd::ShellEntry)));
+    if (total)
+        VD_HIP(hipMemcpyAsync(E.d_shell.p, all.data(), total * sizeof(vd::ShellEntry), hipMemcpyHostToDevice,
+                              E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    return VD_OK;
+}
+
+int upload_desc(Engine &E) {
+    const size_t M = E.mem.size();
+    if (E.shapes.empty() && M) VD_TRY(engine_shells(E));
+    E.desc.assign(M, vd::TessDesc{});
+    for (size_t t = 0; t < M; ++t) {
+        const Member &m = E.mem[t];
+        vd::TessDesc &d = E.desc[t];
+        d.grid = m.grid;
+        for (int a = 0; a < 3; ++a) { d.box_lo[a] = m.lo[a]; d.box_hi[a] = m.hi[a]; }
+        d.raw_off = m.raw_off;
+        d.gen_off = m.gen_off;
+        d.bin_off = m.bin_off;
+        d.nraw = (uint32_t)m.nraw;
+        d.ngen = (uint32_t)m.ngen;
+        const Shape &s = E.shapes[m.shape];
+        d.shell_off = s.off;
+        d.nshell = (uint32_t)s.sp.e.size();
+        d.shell_next_d2 = s.sp.next;
+        d.nwalls = (int32_t)m.walls.size();
+        for (size_t k = 0; k < m.walls.size(); ++k) d.walls[k] = m.walls[k];
+    }
+    VD_TRY(ensure(E.d_desc, std::max<size_t>(M, 1) * sizeof(vd::TessDesc)));
+    if (M)
+        VD_HIP(hipMemcpyAsync(E.d_desc.p, E.desc.data(), M * sizeof(vd::TessDesc), hipMemcpyHostToDevice,
+                              E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    return VD_OK;
+}
+
+// Member layout of the raw points (raw_off from nraw) and of the bins (bin_off from the grids).
+void engine_layout(Engine &E) {
+    uint64_t r = 0, b = 0;
+    for (Member &m : E.mem) {
+        m.raw_off = r;
+        r += m.nraw;
+        m.bin_off = b;
+        m.nbins = (uint64_t)m.n[0] * m.n[1] * m.n[2];
+        b += m.nbins;
+    }
+    E.nraw = r;
+    E.nbins = b;
+}
+
+// set_generators for every member from E.d_raw (tessellation.rs:43-67): wall filter in input order,
+// compaction, then the stable counting-sort binning (algo_3d_grid.rs:111-130).
+int engine_prepare(Engine &E) {
+    E.prepared = E.calculated = false;
+    const size_t M = E.mem.size();
+    const uint64_t n = E.nraw;
+    if (E.nbins + 1 >= 0xffffffffull || n >= 0xffffffffull) return fail(VD_EINVAL, "too many bins or points");
+    if (E.shapes.empty()) VD_TRY(engine_shells(E));
+    // member of every raw point
+    std::vector<uint64_t> offs(M + 1);
+    for (size_t t = 0; t < M; ++t) offs[t] = E.mem[t].raw_off;
+    offs[M] = n;
+    VD_TRY(ensure(E.d_offs, (M + 1) * 8));
+    VD_HIP(hipMemcpyAsync(E.d_offs.p, offs.data(), (M + 1) * 8, hipMemcpyHostToDevice, E.stream));
+    VD_TRY(ensure(E.d_raw_tess, (n + 1) * 4));
+    VD_HIP(vd::launch_member_of(E.d_offs.as<uint64_t>(), (uint32_t)M, E.d_raw_tess.as<uint32_t>(), n, E.stream));
+    bool walls = false;
+    for (const Member &m : E.mem) walls |= !m.walls.empty();
+    // descriptors with the raw layout (the wall filter reads the walls)
+    for (Member &m : E.mem) { m.gen_off = m.raw_off; m.ngen = m.nraw; }
+    VD_TRY(upload_desc(E));
+    VD_TRY(ensure(E.d_gen, (n + 1) * 24));
+    VD_TRY(ensure(E.d_gen_tess, (n + 1) * 4));
+    if (!walls) {
+        if (n) {
+            VD_HIP(hipMemcpyAsync(E.d_gen.p, E.d_raw.p, n * 24, hipMemcpyDeviceToDevice, E.stream));
+            VD_HIP(hipMemcpyAsync(E.d_gen_tess.p, E.d_raw_tess.p, n * 4, hipMemcpyDeviceToDevice, E.stream));
+        }
+        E.ngen = n;
     } else {
-        if ((rc = ensure(t->d_gen, (size_t)kept * 24))) return rc;
+        VD_TRY(ensure(E.d_flags, (n + 1) * 4));
+        VD_TRY(ensure(E.d_pos, (n + 1) * 4));
+        VD_HIP(vd::launch_fill_u32(E.d_flags.as<uint32_t>(), 0u, n + 1, E.stream));
+        VD_HIP(vd::launch_wall_flags(E.d_raw.as<double>(), n, E.d_desc.as<vd::TessDesc>(),
+                                     E.d_raw_tess.as<uint32_t>(), E.d_flags.as<uint32_t>(), E.stream));
+        VD_TRY(scan(E, E.d_flags.as<uint32_t>(), E.d_pos.as<uint32_t>(), (uint32_t)(n + 1)));
+        VD_HIP(vd::launch_compact(E.d_raw.as<double>(), E.d_raw_tess.as<uint32_t>(), E.d_flags.as<uint32_t>(),
+                                  E.d_pos.as<uint32_t>(), n, E.d_gen.as<double>(), E.d_gen_tess.as<uint32_t>(),
+                                  E.stream));
+        // kept count before every member boundary
+        VD_TRY(ensure(E.d_gat, (M + 1) * 4));
+        VD_HIP(vd::launch_gather_u32(E.d_pos.as<uint32_t>(), E.d_offs.as<uint64_t>(), (uint32_t)(M + 1),
+                                     E.d_gat.as<uint32_t>(), E.stream));
+        std::vector<uint32_t> kept(M + 1);
+        VD_HIP(hipMemcpyAsync(kept.data(), E.d_gat.p, (M + 1) * 4, hipMemcpyDeviceToHost, E.stream));
+        VD_HIP(hipStreamSynchronize(E.stream));
+        for (size_t t = 0; t < M; ++t) {
+            E.mem[t].gen_off = kept[t];
+            E.mem[t].ngen = kept[t + 1] - kept[t];
+        }
+        E.ngen = kept[M];
+        VD_TRY(upload_desc(E));
     }
-    double *dst = (d_src == t->d_gen.p) ? out.as<double>() : t->d_gen.as<double>();
-    VD_HIP(vd::launch_compact3(d_src, t->d_flags.as<uint32_t>(), t->d_pos.as<uint32_t>(), (uint32_t)n,
-                               dst, t->stream));
-    if (d_src == t->d_gen.p) {
-        VD_HIP(hipStreamSynchronize(t->stream));
-        release(t->d_gen);
-        t->d_gen = out;
-    }
-    t->ngen = kept;
+    // binning of all members at once
+    const uint64_t nb = E.nbins, ng = E.ngen;
+    VD_TRY(ensure(E.d_counts, (nb + 1) * 4));
+    VD_TRY(ensure(E.d_bin_start, (nb + 1) * 4));
+    VD_TRY(ensure(E.d_fill, (nb + 1) * 4));
+    VD_TRY(ensure(E.d_gen_bin, (ng + 1) * 4));
+    VD_TRY(ensure(E.d_items, (ng + 1) * 4));
+    VD_TRY(ensure(E.d_spos, (ng + 1) * 32));
+    VD_HIP(vd::launch_fill_u32(E.d_counts.as<uint32_t>(), 0u, nb + 1, E.stream));
+    VD_HIP(vd::launch_fill_u32(E.d_fill.as<uint32_t>(), 0u, nb + 1, E.stream));
+    VD_HIP(vd::launch_bin_count(E.d_gen.as<double>(), ng, E.d_desc.as<vd::TessDesc>(), E.d_gen_tess.as<uint32_t>(),
+                                E.d_gen_bin.as<uint32_t>(), E.d_counts.as<uint32_t>(), E.stream));
+    VD_TRY(scan(E, E.d_counts.as<uint32_t>(), E.d_bin_start.as<uint32_t>(), (uint32_t)(nb + 1)));
+    VD_HIP(vd::launch_bin_scatter(E.d_gen_bin.as<uint32_t>(), ng, E.d_bin_start.as<uint32_t>(),
+                                  E.d_fill.as<uint32_t>(), E.d_items.as<uint32_t>(), E.stream));
+    VD_HIP(vd::launch_bin_sort(E.d_bin_start.as<uint32_t>(), nb, E.d_items.as<uint32_t>(), E.stream));
+    VD_HIP(vd::launch_fill_spos(E.d_gen.as<double>(), E.d_items.as<uint32_t>(), ng, E.d_desc.as<vd::TessDesc>(),
+                                E.d_gen_tess.as<uint32_t>(), E.d_spos.as<double4>(), E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    E.prepared = true;
     return VD_OK;
 }
 
-// Algorithm3DGrid::set_generators (algo_3d_grid.rs:111-130) on the device.
-int rebin(vd_tess *t) {
-    int rc;
-    const size_t n = t->ngen;
-    const uint64_t nb = t->nbins;
-    if ((rc = ensure(t->d_counts, (nb + 1) * 4)) || (rc = ensure(t->d_bin_start, (nb + 1) * 4)) ||
-        (rc = ensure(t->d_fill, (nb + 1) * 4)) || (rc = ensure(t->d_gen_bin, (n + 1) * 4)) ||
-        (rc = ensure(t->d_items, (n + 1) * 4)) || (rc = ensure(t->d_spos, (n + 1) * 32)))
-        return rc;
-    VD_HIP(vd::launch_fill_u32(t->d_counts.as<uint32_t>(), 0u, nb + 1, t->stream));
-    VD_HIP(vd::launch_fill_u32(t->d_fill.as<uint32_t>(), 0u, nb + 1, t->stream));
-    VD_HIP(vd::launch_bin_count(t->d_gen.as<double>(), (uint32_t)n, t->grid, t->d_gen_bin.as<uint32_t>(),
-                                t->d_counts.as<uint32_t>(), t->stream));
-    if ((rc = scan(t, t->d_counts.as<uint32_t>(), t->d_bin_start.as<uint32_t>(), (uint32_t)(nb + 1)))) return rc;
-    VD_HIP(vd::launch_bin_scatter(t->d_gen_bin.as<uint32_t>(), (uint32_t)n, t->d_bin_start.as<uint32_t>(),
-                                  t->d_fill.as<uint32_t>(), t->d_items.as<uint32_t>(), t->stream));
-    VD_HIP(vd::launch_bin_sort(t->d_bin_start.as<uint32_t>(), (uint32_t)nb, t->d_items.as<uint32_t>(), t->stream));
-    VD_HIP(vd::launch_fill_spos(t->d_gen.as<double>(), t->d_items.as<uint32_t>(), (uint32_t)n,
-                                t->d_spos.as<double4>(), t->stream));
-    t->calculated = false;
-    t->ncells = 0;
-    return VD_OK;
-}
-
-int upload_shell(vd_tess *t, uint32_t k) {
-    build_shell(t->n, t->grid.cs, k, t->shell);
-    t->shell_k = k;
-    int rc = ensure(t->d_shell, t->shell.e.size() * sizeof(vd::ShellEntry));
-    if (rc) return rc;
-    VD_HIP(hipMemcpyAsync(t->d_shell.p, t->shell.e.data(), t->shell.e.size() * sizeof(vd::ShellEntry),
-                          hipMemcpyHostToDevice, t->stream));
-    VD_HIP(hipStreamSynchronize(t->stream));
-    return VD_OK;
-}
-
-int add_wall(vd_tess *t, const vd::WallDev &w) {
-    if (w.id > VD_WALL_ID_MAX) return fail(VD_EINVAL, "Wall ID must be <= -1000");
-    if ((int)t->walls.size() >= vd::kMaxWalls) return fail(VD_EINVAL, "too many walls (max 8)");
-    t->walls.push_back(w);
-    // prune_outside_generators (tessellation.rs:175-194): re-filter, re-bin if the count changed
-    if (t->ngen == 0) return VD_OK;
-    const size_t before = t->ngen;
-    int rc = filter_into_gen(t, t->d_gen.as<double>(), before);
-    if (rc) return rc;
-    if (t->ngen != before) return rebin(t);
-    return VD_OK;
-}
-
-int run_cells(vd_tess *t, int tier, const uint32_t *d_list, uint32_t nlist, float *ms) {
-    vd::TessArgs a{};
-    a.grid = t->grid;
-    a.gen = t->d_gen.as<double>();
-    a.spos = t->d_spos.as<double4>();
-    a.bin_start = t->d_bin_start.as<uint32_t>();
-    a.shell = t->d_shell.as<vd::ShellEntry>();
-    a.nshell = (uint32_t)t->shell.e.size();
-    a.shell_next_d2 = t->shell.next;
-    a.nwalls = (int32_t)t->walls.size();
-    for (size_t k = 0; k < t->walls.size(); ++k) a.walls[k] = t->walls[k];
-    for (int k = 0; k < 3; ++k) { a.box_lo[k] = t->lo[k]; a.box_hi[k] = t->hi[k]; }
-    a.ngen = (uint32_t)t->ngen;
+int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms) {
+    vd::CellArgs a{};
+    a.desc = E.d_desc.as<vd::TessDesc>();
+    a.gen_tess = E.d_gen_tess.as<uint32_t>();
+    a.gen = E.d_gen.as<double>();
+    a.spos = E.d_spos.as<double4>();
+    a.bin_start = E.d_bin_start.as<uint32_t>();
+    a.shell = E.d_shell.as<vd::ShellEntry>();
     a.cells = d_list;
     a.ncells = nlist;
-    a.queue = t->d_queue.as<uint32_t>();
-    a.vol = t->d_vol.as<double>();
-    a.cen = t->d_cen.as<double>();
-    a.nface = t->d_nface.as<uint32_t>();
-    a.nvert = t->d_nvert.as<uint32_t>();
-    a.status = t->d_status.as<int32_t>();
-    a.vtx = t->keep_cells ? t->d_vtx.as<double>() : nullptr;
-    a.fcnt = t->d_fcnt.as<uint8_t>();
-    a.fnb = t->d_fnb.as<int32_t>();
-    a.fidx = t->d_fidx.as<uint16_t>();
-    a.arena_top = t->d_top.as<uint64_t>();
-    for (int k = 0; k < 3; ++k) a.arena_cap[k] = t->cap[k];
-    a.cell_off = t->d_cell_off.as<uint64_t>();
-    a.fnb_only = 1;
-    VD_HIP(vd::launch_fill_u32(t->d_queue.as<uint32_t>(), 0u, 1, t->stream));
-    int waves = t->cus * t->occ[tier];
+    a.queue = E.d_queue.as<uint32_t>();
+    a.vol = E.d_vol.as<double>();
+    a.cen = E.d_cen.as<double>();
+    a.nface = E.d_nface.as<uint32_t>();
+    a.nvert = E.d_nvert.as<uint32_t>();
+    a.status = E.d_status.as<int32_t>();
+    a.vtx = E.keep_cells ? E.d_vtx.as<double>() : nullptr;
+    a.fcnt = E.d_fcnt.as<uint8_t>();
+    a.fnb = E.d_fnb.as<int32_t>();
+    a.fidx = E.d_fidx.as<uint16_t>();
+    a.arena_top = E.d_top.as<uint64_t>();
+    for (int k = 0; k < 3; ++k) a.arena_cap[k] = E.cap[k];
+    a.cell_off = E.d_cell_off.as<uint64_t>();
+    VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>(), 0u, 1, E.stream));
+    int waves = E.cus * E.occ[tier];
     if ((uint32_t)waves > nlist) waves = (int)nlist;
-    if (ms) VD_HIP(hipEventRecord(t->ev0, t->stream));
-    VD_HIP(vd::launch_cells(tier, a, waves, t->stream));
+    if (ms) VD_HIP(hipEventRecord(E.ev0, E.stream));
+    VD_HIP(vd::launch_cells(tier, a, waves, E.stream));
     if (ms) {
-        VD_HIP(hipEventRecord(t->ev1, t->stream));
-        VD_HIP(hipEventSynchronize(t->ev1));
-        VD_HIP(hipEventElapsedTime(ms, t->ev0, t->ev1));
+        VD_HIP(hipEventRecord(E.ev1, E.stream));
+        VD_HIP(hipEventSynchronize(E.ev1));
+        VD_HIP(hipEventElapsedTime(ms, E.ev0, E.ev1));
     }
     return VD_OK;
 }
 
-int calculate(vd_tess *t) {
+// calculate() of every member (tessellation.rs:213-230).
+int engine_calculate(Engine &E) {
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
-    int rc;
-    const size_t n = t->ngen;
-    for (double &s : t->stats) s = 0.0;
-    if ((rc = ensure(t->d_vol, n * 8)) || (rc = ensure(t->d_cen, n * 24)) || (rc = ensure(t->d_nface, n * 4)) ||
-        (rc = ensure(t->d_nvert, n * 4)) || (rc = ensure(t->d_status, n * 4)) ||
-        (rc = ensure(t->d_cell_off, n * 24)) || (rc = ensure(t->d_top, 64)) || (rc = ensure(t->d_queue, 64)) ||
-        (rc = ensure(t->d_counts4, 64)))
-        return rc;
-    if (t->cap[1] == 0 || t->cap[1] < n * 16) {
-        t->cap[0] = n * 40 + 4096;
-        t->cap[1] = n * 24 + 4096;
-        t->cap[2] = n * 120 + 4096;
+    if (!E.prepared) return fail(VD_ESTATE, "generators are not set");
+    const uint64_t n = E.ngen;
+    for (double &s : E.stats) s = 0.0;
+    VD_TRY(ensure(E.d_vol, n * 8));
+    VD_TRY(ensure(E.d_cen, n * 24));
+    VD_TRY(ensure(E.d_nface, n * 4));
+    VD_TRY(ensure(E.d_nvert, n * 4));
+    VD_TRY(ensure(E.d_status, n * 4));
+    VD_TRY(ensure(E.d_cell_off, n * 24));
+    VD_TRY(ensure(E.d_top, 64));
+    VD_TRY(ensure(E.d_queue, 64));
+    VD_TRY(ensure(E.d_counts4, 64));
+    if (E.cap[1] < n * 16) {
+        E.cap[0] = n * 40 + 4096;
+        E.cap[1] = n * 24 + 4096;
+        E.cap[2] = n * 120 + 4096;
     }
-    t->calculated = false;
-    t->ncells = n;
-    if (n == 0) { t->calculated = true; return VD_OK; }
-    if (t->shell.e.empty()) {
-        if ((rc = upload_shell(t, 7))) return rc;
-    }
+    E.calculated = false;
+    if (n == 0) { E.calculated = true; return VD_OK; }
     for (int attempt = 0; attempt < 8; ++attempt) {
-        if (t->keep_cells) {
-            if ((rc = ensure(t->d_vtx, t->cap[0] * 24)) || (rc = ensure(t->d_fcnt, t->cap[1])) ||
-                (rc = ensure(t->d_fidx, t->cap[2] * 2)))
-                return rc;
+        if (E.keep_cells) {
+            VD_TRY(ensure(E.d_vtx, E.cap[0] * 24));
+            VD_TRY(ensure(E.d_fcnt, E.cap[1]));
+            VD_TRY(ensure(E.d_fidx, E.cap[2] * 2));
         }
-        if ((rc = ensure(t->d_fnb, t->cap[1] * 4))) return rc;
-        VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(t->d_top.p), 0u, 6, t->stream));
+        VD_TRY(ensure(E.d_fnb, E.cap[1] * 4));
+        VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
 
         // Tier 0 (register fast path, small LDS) over every cell in bin order. A cell that
         // outgrows its tier moves to the next one; a cell whose walk runs past the shell prefix
-        // reruns in its tier after the prefix is extended. Reruns start from scratch and give
+        // reruns in its tier after the prefixes are extended. Reruns start from scratch and give
         // the same result.
         constexpr int NT = kTiers;
-        DevBuf L[NT];                              // owned work lists per tier
-        const uint32_t *list[NT] = {t->d_items.as<uint32_t>()};
+        DevBuf L[NT];
+        const uint32_t *list[NT] = {E.d_items.as<uint32_t>()};
         uint32_t nlist[NT] = {(uint32_t)n};
         bool arena_full = false;
         bool first = true;
@@ -337,25 +483,24 @@ int calculate(vd_tess *t) {
             for (int tier = 0; tier < NT; ++tier) {
                 if (!nlist[tier]) continue;
                 float ms = 0.f;
-                if ((rc = run_cells(t, tier, list[tier], nlist[tier], first ? &ms : nullptr))) return rc;
-                if (first) { t->stats[4] = 1000.0 * ms; t->stats[0] = (double)nlist[0]; first = false; }
-                else if (tier > 0) t->stats[1] += nlist[tier];
+                VD_TRY(run_cells(E, tier, list[tier], nlist[tier], first ? &ms : nullptr));
+                if (first) { E.stats[4] = 1000.0 * ms; E.stats[0] = (double)nlist[0]; first = false; }
+                else if (tier > 0) E.stats[1] += nlist[tier];
             }
             // outcome per tier: counts[4*tier + {0 capacity, 1 shell, 2 arena, 3 error}]
             uint32_t counts[4 * NT] = {};
             DevBuf out[2 * NT];   // per tier: capacity list, shell list
-            VD_HIP(vd::launch_fill_u32(t->d_counts4.as<uint32_t>(), 0u, 4 * NT, t->stream));
+            VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 4 * NT, E.stream));
             for (int tier = 0; tier < NT; ++tier) {
                 if (!nlist[tier]) continue;
-                if ((rc = ensure(out[2 * tier], (size_t)nlist[tier] * 4)) ||
-                    (rc = ensure(out[2 * tier + 1], (size_t)nlist[tier] * 4)))
-                    return rc;
-                VD_HIP(vd::launch_collect(t->d_status.as<int32_t>(), list[tier], nlist[tier],
+                VD_TRY(ensure(out[2 * tier], (size_t)nlist[tier] * 4));
+                VD_TRY(ensure(out[2 * tier + 1], (size_t)nlist[tier] * 4));
+                VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), list[tier], nlist[tier],
                                           out[2 * tier].as<uint32_t>(), out[2 * tier + 1].as<uint32_t>(),
-                                          t->d_counts4.as<uint32_t>() + 4 * tier, t->stream));
+                                          E.d_counts4.as<uint32_t>() + 4 * tier, E.stream));
             }
-            VD_HIP(hipMemcpyAsync(counts, t->d_counts4.p, sizeof(counts), hipMemcpyDeviceToHost, t->stream));
-            VD_HIP(hipStreamSynchronize(t->stream));
+            VD_HIP(hipMemcpyAsync(counts, E.d_counts4.p, sizeof(counts), hipMemcpyDeviceToHost, E.stream));
+            VD_HIP(hipStreamSynchronize(E.stream));
             bool shell = false;
             for (int tier = 0; tier < NT; ++tier) {
                 if (counts[4 * tier + 3]) return fail(VD_ECAPACITY, "a cell has a face with 256 or more vertices");
@@ -365,9 +510,13 @@ int calculate(vd_tess *t) {
             if (arena_full) break;
             if (counts[4 * (NT - 1)]) return fail(VD_ECAPACITY, "a cell exceeds the largest polytope tier");
             if (shell) {
-                if (t->shell.complete) return fail(VD_EDEVICE, "shell walk past a complete table");
-                if ((rc = upload_shell(t, t->shell_k * 2))) return rc;
-                t->stats[2] += 1;
+                bool all_complete = true;
+                for (const Shape &s : E.shapes) all_complete &= s.sp.complete;
+                if (all_complete) return fail(VD_EDEVICE, "shell walk past a complete table");
+                E.shell_k *= 2;
+                VD_TRY(engine_shells(E));
+                VD_TRY(upload_desc(E));
+                E.stats[2] += 1;
             }
             // next list of tier k: its own shell failures + the capacity failures of tier k-1
             DevBuf next[NT];
@@ -377,15 +526,15 @@ int calculate(vd_tess *t) {
                 const uint32_t nc = tier > 0 ? counts[4 * (tier - 1)] : 0u;
                 nnext[tier] = ns + nc;
                 if (!nnext[tier]) continue;
-                if ((rc = ensure(next[tier], (size_t)nnext[tier] * 4))) return rc;
+                VD_TRY(ensure(next[tier], (size_t)nnext[tier] * 4));
                 if (ns)
                     VD_HIP(hipMemcpyAsync(next[tier].p, out[2 * tier + 1].p, ns * 4, hipMemcpyDeviceToDevice,
-                                          t->stream));
+                                          E.stream));
                 if (nc)
                     VD_HIP(hipMemcpyAsync(next[tier].as<uint32_t>() + ns, out[2 * (tier - 1)].p, nc * 4,
-                                          hipMemcpyDeviceToDevice, t->stream));
+                                          hipMemcpyDeviceToDevice, E.stream));
             }
-            VD_HIP(hipStreamSynchronize(t->stream));
+            VD_HIP(hipStreamSynchronize(E.stream));
             for (int tier = 0; tier < NT; ++tier) {
                 release(L[tier]);
                 L[tier] = next[tier];
@@ -397,22 +546,121 @@ int calculate(vd_tess *t) {
         }
         for (DevBuf &b : L) release(b);
         if (!arena_full) break;
-        for (int k = 0; k < 3; ++k) t->cap[k] *= 2;
+        for (int k = 0; k < 3; ++k) E.cap[k] *= 2;
     }
-    VD_HIP(hipStreamSynchronize(t->stream));
-    t->calculated = true;
-    t->stats[3] = (double)t->shell.e.size();
-    t->stats[5] = std::chrono::duration<double, std::micro>(clock::now() - t0).count();
+    VD_HIP(hipStreamSynchronize(E.stream));
+    E.calculated = true;
+    size_t entries = 0;
+    for (const Shape &s : E.shapes) entries += s.sp.e.size();
+    E.stats[3] = (double)entries;
+    E.stats[5] = std::chrono::duration<double, std::micro>(clock::now() - t0).count();
     return VD_OK;
 }
 
-int need_results(const vd_tess *t, size_t n) {
-    if (!t->calculated) return fail(VD_ESTATE, "calculate() has not run since the generators changed");
-    if (n < t->ncells) return fail(VD_EINVAL, "output buffer smaller than count_cells()");
+// relax() of every member (tessellation.rs:482-495): new input points, then set_generators.
+int engine_relax(Engine &E) {
+    if (!E.calculated) return fail(VD_ESTATE, "relax() needs calculate() first");
+    const uint64_t n = E.ngen;
+    VD_TRY(ensure(E.d_raw, (n + 1) * 24));
+    VD_HIP(vd::launch_relax(E.d_gen.as<double>(), E.d_cen.as<double>(), E.d_nvert.as<uint32_t>(), n,
+                            E.d_raw.as<double>(), E.stream));
+    for (Member &m : E.mem) m.nraw = m.ngen;
+    engine_layout(E);
+    return engine_prepare(E);
+}
+
+const Member *member(const Engine &E, size_t t) {
+    if (t >= E.mem.size()) {
+        fail(VD_ERANGE, "member index out of range");
+        return nullptr;
+    }
+    return &E.mem[t];
+}
+
+int need_results(const Engine &E) {
+    if (!E.calculated) return fail(VD_ESTATE, "calculate() has not run since the generators changed");
+    return VD_OK;
+}
+
+int copy_slice(const DevBuf &src, size_t elem, const Member &m, void *out, size_t n) {
+    if (n < m.ngen) return fail(VD_EINVAL, "output buffer smaller than the cell count");
+    if (m.ngen)
+        VD_HIP(hipMemcpy(out, src.as<char>() + m.gen_off * elem, m.ngen * elem, hipMemcpyDeviceToHost));
+    return VD_OK;
+}
+
+int get_neighbors(const Engine &E, const Member &m, uint64_t *offsets, size_t n, int32_t *ids, size_t cap) {
+    VD_TRY(need_results(E));
+    if (!offsets) return fail(VD_EINVAL, "NULL offsets");
+    if (n < m.ngen) return fail(VD_EINVAL, "offsets buffer smaller than the cell count");
+    const size_t nc = m.ngen;
+    std::vector<uint32_t> nf(nc);
+    if (nc) VD_HIP(hipMemcpy(nf.data(), E.d_nface.as<uint32_t>() + m.gen_off, nc * 4, hipMemcpyDeviceToHost));
+    offsets[0] = 0;
+    for (size_t i = 0; i < nc; ++i) offsets[i + 1] = offsets[i] + nf[i];
+    if (!ids) return VD_OK;
+    if (cap < offsets[nc]) return fail(VD_EINVAL, "ids buffer too small");
+    std::vector<uint64_t> off(3 * nc);
+    if (nc) VD_HIP(hipMemcpy(off.data(), E.d_cell_off.as<uint64_t>() + 3 * m.gen_off, nc * 24, hipMemcpyDeviceToHost));
+    uint64_t top = 0;
+    VD_HIP(hipMemcpy(&top, E.d_top.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost));
+    std::vector<int32_t> arena(top);
+    if (top) VD_HIP(hipMemcpy(arena.data(), E.d_fnb.p, top * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nc; ++i)
+        if (nf[i]) std::memcpy(ids + offsets[i], arena.data() + off[3 * i + 1], nf[i] * 4);
+    return VD_OK;
+}
+
+int get_cell(const Engine &E, const Member &m, size_t i, uint64_t sizes[3], double *vertices,
+             uint32_t *face_counts, uint32_t *face_indices, int32_t *face_neighbors) {
+    if (!sizes) return fail(VD_EINVAL, "NULL sizes");
+    VD_TRY(need_results(E));
+    if (i >= m.ngen) return fail(VD_ERANGE, "cell index out of range");
+    if (!E.keep_cells) return fail(VD_ESTATE, "cells are not kept (VD_OPT_KEEP_CELLS = 0)");
+    const size_t g = m.gen_off + i;
+    uint32_t nv = 0, nf = 0;
+    uint64_t off[3];
+    VD_HIP(hipMemcpy(&nv, E.d_nvert.as<uint32_t>() + g, 4, hipMemcpyDeviceToHost));
+    VD_HIP(hipMemcpy(&nf, E.d_nface.as<uint32_t>() + g, 4, hipMemcpyDeviceToHost));
+    VD_HIP(hipMemcpy(off, E.d_cell_off.as<uint64_t>() + 3 * g, 24, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> fc(nf);
+    if (nf) VD_HIP(hipMemcpy(fc.data(), E.d_fcnt.as<uint8_t>() + off[1], nf, hipMemcpyDeviceToHost));
+    uint64_t ns = 0;
+    for (uint8_t c : fc) ns += c;
+    sizes[0] = nv;
+    sizes[1] = nf;
+    sizes[2] = ns;
+    if (vertices && nv) VD_HIP(hipMemcpy(vertices, E.d_vtx.as<double>() + 3 * off[0], nv * 24, hipMemcpyDeviceToHost));
+    if (face_counts) for (uint32_t k = 0; k < nf; ++k) face_counts[k] = fc[k];
+    if (face_neighbors && nf)
+        VD_HIP(hipMemcpy(face_neighbors, E.d_fnb.as<int32_t>() + off[1], nf * 4, hipMemcpyDeviceToHost));
+    if (face_indices && ns) {
+        std::vector<uint16_t> fi(ns);
+        VD_HIP(hipMemcpy(fi.data(), E.d_fidx.as<uint16_t>() + off[2], ns * 2, hipMemcpyDeviceToHost));
+        for (uint64_t k = 0; k < ns; ++k) face_indices[k] = fi[k];
+    }
+    return VD_OK;
+}
+
+int check_grid(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny, uint32_t nz) {
+    if (!bmin || !bmax || nx == 0 || ny == 0 || nz == 0) return fail(VD_EINVAL, "invalid grid");
+    if (nx > 16384 || ny > 16384 || nz > 16384) return fail(VD_EINVAL, "grid resolution above 16384");
+    if ((uint64_t)nx * ny * nz >= 0x7fffffffull) return fail(VD_EINVAL, "too many bins");
     return VD_OK;
 }
 
 }  // namespace
+
+struct vd_tess {
+    Engine E;
+};
+
+struct vd_batch {
+    Engine E;
+    std::vector<double> staged;   // host points of every member, in member order
+    bool uploaded = false;
+    double per_bin = 2.0;
+};
 
 extern "C" {
 
@@ -421,15 +669,11 @@ const char *vd_last_error(void) { return g_err.c_str(); }
 int vd_shell_prefix(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny, uint32_t nz,
                     uint32_t kmax, int32_t *dxyz, double *keys, size_t cap, size_t *count,
                     double *next_key) {
-    if (!bmin || !bmax || nx == 0 || ny == 0 || nz == 0) return fail(VD_EINVAL, "invalid grid");
+    VD_TRY(check_grid(bmin, bmax, nx, ny, nz));
     const int64_t n[3] = {nx, ny, nz};
-    double cs[3];
-    for (int a = 0; a < 3; ++a) {
-        const double s = (double)n[a] / (bmax[a] - bmin[a]);
-        cs[a] = 1.0 / s;
-    }
+    const vd::GridGeom g = make_grid(bmin, bmax, n);
     ShellPrefix sp;
-    build_shell(n, cs, kmax, sp);
+    build_shell(n, g.cs, kmax, sp);
     const size_t m = std::min(cap, sp.e.size());
     for (size_t k = 0; k < m; ++k) {
         if (dxyz) {
@@ -444,39 +688,29 @@ int vd_shell_prefix(const double bmin[3], const double bmax[3], uint32_t nx, uin
     return VD_OK;
 }
 
+int vd_grid_for(const double bmin[3], const double bmax[3], size_t n, double per_bin, uint32_t out[3]) {
+    if (!bmin || !bmax || !out || !(per_bin > 0.0)) return fail(VD_EINVAL, "invalid argument");
+    int64_t g[3];
+    grid_for(bmin, bmax, n, per_bin, g);
+    for (int a = 0; a < 3; ++a) out[a] = (uint32_t)g[a];
+    return VD_OK;
+}
+
+// ---- vd_tess: one Tessellation -----------------------------------------------------------
 int vd_tess_create(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny, uint32_t nz,
                    vd_tess **out) {
     if (!out) return fail(VD_EINVAL, "out is NULL");
     *out = nullptr;
-    if (!bmin || !bmax || nx == 0 || ny == 0 || nz == 0) return fail(VD_EINVAL, "invalid grid");
-    if (nx > 16384 || ny > 16384 || nz > 16384) return fail(VD_EINVAL, "grid resolution above 16384");
-    const uint64_t nb = (uint64_t)nx * ny * nz;
-    if (nb >= 0xffffffffull) return fail(VD_EINVAL, "too many bins");
+    VD_TRY(check_grid(bmin, bmax, nx, ny, nz));
     vd_tess *t = new vd_tess();
-    for (int a = 0; a < 3; ++a) { t->lo[a] = bmin[a]; t->hi[a] = bmax[a]; }
-    t->n[0] = nx; t->n[1] = ny; t->n[2] = nz;
-    t->nbins = nb;
-    for (int a = 0; a < 3; ++a) {
-        t->grid.lo[a] = bmin[a];
-        t->grid.s[a] = (double)t->n[a] / (bmax[a] - bmin[a]);   // algo_3d_grid.rs:48-50
-        t->grid.cs[a] = 1.0 / t->grid.s[a];                     // :52-54, :161-163
-        t->grid.lim[a] = (double)t->n[a] - 1e-5;                // :85-87
-        t->grid.n[a] = (int32_t)t->n[a];
-    }
-    hipError_t e = hipGetDevice(&t->device);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&t->cus, hipDeviceAttributeMultiprocessorCount, t->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&t->ev0);
-    if (e == hipSuccess) e = hipEventCreate(&t->ev1);
-    for (int k = 0; k < kTiers && e == hipSuccess; ++k) e = vd::cells_occupancy(k, &t->occ[k]);
-    if (e != hipSuccess) {
-        vd_tess_destroy(t);
-        return fail(VD_EDEVICE, std::string("device setup: ") + hipGetErrorString(e));
-    }
-    for (int k = 0; k < kTiers; ++k) t->occ[k] = std::max(1, t->occ[k]);
-    int rc = ensure(t->d_gen, 16);
-    if (!rc) rc = rebin(t);
-    if (!rc) rc = hipStreamSynchronize(t->stream) == hipSuccess ? VD_OK : fail(VD_EDEVICE, "sync");
+    Member m;
+    for (int a = 0; a < 3; ++a) { m.lo[a] = bmin[a]; m.hi[a] = bmax[a]; }
+    m.n[0] = nx; m.n[1] = ny; m.n[2] = nz;
+    m.grid = make_grid(m.lo, m.hi, m.n);
+    t->E.mem.push_back(m);
+    int rc = engine_init(t->E);
+    if (!rc) rc = ensure(t->E.d_raw, 24);
+    if (!rc) { engine_layout(t->E); rc = engine_prepare(t->E); }
     if (rc) { vd_tess_destroy(t); return rc; }
     *out = t;
     return VD_OK;
@@ -484,180 +718,137 @@ int vd_tess_create(const double bmin[3], const double bmax[3], uint32_t nx, uint
 
 void vd_tess_destroy(vd_tess *t) {
     if (!t) return;
-    if (t->stream) (void)hipStreamSynchronize(t->stream);
-    DevBuf *bufs[] = {&t->d_shell, &t->d_gen, &t->d_raw, &t->d_flags, &t->d_pos, &t->d_scan, &t->d_gen_bin,
-                      &t->d_counts, &t->d_bin_start, &t->d_fill, &t->d_items, &t->d_spos, &t->d_vol,
-                      &t->d_cen, &t->d_nface, &t->d_nvert, &t->d_status, &t->d_cell_off, &t->d_vtx,
-                      &t->d_fcnt, &t->d_fnb, &t->d_fidx, &t->d_top, &t->d_queue, &t->d_counts4};
-    for (DevBuf *b : bufs) release(*b);
-    if (t->ev0) (void)hipEventDestroy(t->ev0);
-    if (t->ev1) (void)hipEventDestroy(t->ev1);
-    if (t->stream) (void)hipStreamDestroy(t->stream);
+    engine_free(t->E);
     delete t;
+}
+
+static int tess_add_wall(vd_tess *t, const vd_wall &w) {
+    vd::WallDev d;
+    VD_TRY(make_wall(w, d));
+    Engine &E = t->E;
+    Member &m = E.mem[0];
+    if ((int)m.walls.size() >= vd::kMaxWalls) return fail(VD_EINVAL, "too many walls (max 8)");
+    m.walls.push_back(d);
+    // prune_outside_generators (tessellation.rs:175-194): the current generators are filtered
+    // again by every wall (re-binning an unchanged set gives the same bins)
+    const uint64_t n = E.ngen;
+    if (n == 0) return upload_desc(E);
+    DevBuf keep;
+    VD_TRY(ensure(keep, n * 24));
+    VD_HIP(hipMemcpyAsync(keep.p, E.d_gen.p, n * 24, hipMemcpyDeviceToDevice, E.stream));
+    int rc = ensure(E.d_raw, (n + 1) * 24);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(E.d_raw.p, keep.p, n * 24, hipMemcpyDeviceToDevice, E.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
+        rc = e == hipSuccess ? VD_OK : fail(VD_EDEVICE, hipGetErrorString(e));
+    }
+    release(keep);
+    if (rc) return rc;
+    m.nraw = n;
+    engine_layout(E);
+    return engine_prepare(E);
 }
 
 int vd_tess_add_wall_plane(vd_tess *t, int32_t id, const double point[3], const double normal[3]) {
     if (!t || !point || !normal) return fail(VD_EINVAL, "NULL argument");
-    vd::WallDev w{};
+    vd_wall w{};
     w.id = id;
     w.kind = 0;
-    // PlaneGeometry::new, wall_3d.rs:23-27
-    const double len = std::sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
-    if (len == 0.0) { w.b[0] = 0.0; w.b[1] = 0.0; w.b[2] = 1.0; }
-    else { w.b[0] = normal[0] / len; w.b[1] = normal[1] / len; w.b[2] = normal[2] / len; }
-    for (int k = 0; k < 3; ++k) w.a[k] = point[k];
-    return add_wall(t, w);
+    for (int k = 0; k < 3; ++k) { w.a[k] = point[k]; w.b[k] = normal[k]; }
+    return tess_add_wall(t, w);
 }
 
 int vd_tess_add_wall_sphere(vd_tess *t, int32_t id, const double center[3], double radius) {
     if (!t || !center) return fail(VD_EINVAL, "NULL argument");
-    vd::WallDev w{};
+    vd_wall w{};
     w.id = id;
     w.kind = 1;
     w.r = radius;
     for (int k = 0; k < 3; ++k) w.a[k] = center[k];
-    return add_wall(t, w);
+    return tess_add_wall(t, w);
 }
 
 int vd_tess_clear_walls(vd_tess *t) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
-    t->walls.clear();
-    t->calculated = false;
-    return VD_OK;
+    t->E.mem[0].walls.clear();
+    t->E.calculated = false;
+    return upload_desc(t->E);
 }
 
 int vd_tess_set_generators(vd_tess *t, const double *xyz, size_t n) {
     if (!t || (!xyz && n)) return fail(VD_EINVAL, "NULL argument");
     if (n >= 0x7fffffffull) return fail(VD_EINVAL, "too many generators");
-    int rc = ensure(t->d_raw, n * 24);
-    if (rc) return rc;
-    if (n) VD_HIP(hipMemcpyAsync(t->d_raw.p, xyz, n * 24, hipMemcpyHostToDevice, t->stream));
-    if ((rc = filter_into_gen(t, t->d_raw.as<double>(), n))) return rc;
-    if ((rc = rebin(t))) return rc;
-    VD_HIP(hipStreamSynchronize(t->stream));
-    return VD_OK;
+    Engine &E = t->E;
+    VD_TRY(ensure(E.d_raw, (n + 1) * 24));
+    if (n) VD_HIP(hipMemcpyAsync(E.d_raw.p, xyz, n * 24, hipMemcpyHostToDevice, E.stream));
+    E.mem[0].nraw = n;
+    engine_layout(E);
+    return engine_prepare(E);
 }
 
 int vd_tess_set_generators_device(vd_tess *t, const double *d_xyz, size_t n) {
     if (!t || (!d_xyz && n)) return fail(VD_EINVAL, "NULL argument");
     if (n >= 0x7fffffffull) return fail(VD_EINVAL, "too many generators");
-    int rc;
-    if ((rc = filter_into_gen(t, d_xyz, n))) return rc;
-    if ((rc = rebin(t))) return rc;
-    VD_HIP(hipStreamSynchronize(t->stream));
-    return VD_OK;
+    Engine &E = t->E;
+    VD_TRY(ensure(E.d_raw, (n + 1) * 24));
+    if (n) VD_HIP(hipMemcpyAsync(E.d_raw.p, d_xyz, n * 24, hipMemcpyDeviceToDevice, E.stream));
+    E.mem[0].nraw = n;
+    engine_layout(E);
+    return engine_prepare(E);
 }
 
 int vd_tess_calculate(vd_tess *t) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
-    return calculate(t);
+    return engine_calculate(t->E);
 }
 
 int vd_tess_relax(vd_tess *t) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
-    if (!t->calculated) return fail(VD_ESTATE, "relax() needs calculate() first");
-    const size_t n = t->ngen;
-    int rc = ensure(t->d_raw, n * 24);
-    if (rc) return rc;
-    VD_HIP(vd::launch_relax(t->d_gen.as<double>(), t->d_cen.as<double>(), t->d_nvert.as<uint32_t>(),
-                            (uint32_t)n, t->d_raw.as<double>(), t->stream));
-    if ((rc = filter_into_gen(t, t->d_raw.as<double>(), n))) return rc;
-    if ((rc = rebin(t))) return rc;
-    VD_HIP(hipStreamSynchronize(t->stream));
-    return VD_OK;
+    return engine_relax(t->E);
 }
 
-size_t vd_tess_count_generators(const vd_tess *t) { return t ? t->ngen : 0; }
-size_t vd_tess_count_cells(const vd_tess *t) { return (t && t->calculated) ? t->ncells : 0; }
+size_t vd_tess_count_generators(const vd_tess *t) { return t ? t->E.ngen : 0; }
+size_t vd_tess_count_cells(const vd_tess *t) { return (t && t->E.calculated) ? t->E.ngen : 0; }
 
 int vd_tess_get_generators(const vd_tess *t, double *out, size_t n) {
-    if (!t || (!out && t->ngen)) return fail(VD_EINVAL, "NULL argument");
-    if (n < t->ngen) return fail(VD_EINVAL, "output buffer smaller than count_generators()");
-    if (t->ngen) VD_HIP(hipMemcpy(out, t->d_gen.p, t->ngen * 24, hipMemcpyDeviceToHost));
-    return VD_OK;
+    if (!t || (!out && t->E.ngen)) return fail(VD_EINVAL, "NULL argument");
+    return copy_slice(t->E.d_gen, 24, t->E.mem[0], out, n);
 }
 
 int vd_tess_get_volumes(const vd_tess *t, double *out, size_t n) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
-    int rc = need_results(t, n);
-    if (rc) return rc;
-    if (t->ncells) VD_HIP(hipMemcpy(out, t->d_vol.p, t->ncells * 8, hipMemcpyDeviceToHost));
-    return VD_OK;
+    VD_TRY(need_results(t->E));
+    return copy_slice(t->E.d_vol, 8, t->E.mem[0], out, n);
 }
 
 int vd_tess_get_centroids(const vd_tess *t, double *out3, size_t n) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
-    int rc = need_results(t, n);
-    if (rc) return rc;
-    if (t->ncells) VD_HIP(hipMemcpy(out3, t->d_cen.p, t->ncells * 24, hipMemcpyDeviceToHost));
-    return VD_OK;
+    VD_TRY(need_results(t->E));
+    return copy_slice(t->E.d_cen, 24, t->E.mem[0], out3, n);
 }
 
 int vd_tess_get_face_counts(const vd_tess *t, uint32_t *out, size_t n) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
-    int rc = need_results(t, n);
-    if (rc) return rc;
-    if (t->ncells) VD_HIP(hipMemcpy(out, t->d_nface.p, t->ncells * 4, hipMemcpyDeviceToHost));
-    return VD_OK;
+    VD_TRY(need_results(t->E));
+    return copy_slice(t->E.d_nface, 4, t->E.mem[0], out, n);
 }
 
 int vd_tess_get_neighbors(const vd_tess *t, uint64_t *offsets, size_t n, int32_t *ids, size_t cap) {
-    if (!t || !offsets) return fail(VD_EINVAL, "NULL argument");
-    int rc = need_results(t, n);
-    if (rc) return rc;
-    const size_t nc = t->ncells;
-    std::vector<uint32_t> nf(nc);
-    if (nc) VD_HIP(hipMemcpy(nf.data(), t->d_nface.p, nc * 4, hipMemcpyDeviceToHost));
-    offsets[0] = 0;
-    for (size_t i = 0; i < nc; ++i) offsets[i + 1] = offsets[i] + nf[i];
-    if (!ids) return VD_OK;
-    if (cap < offsets[nc]) return fail(VD_EINVAL, "ids buffer too small");
-    std::vector<uint64_t> off(3 * nc);
-    if (nc) VD_HIP(hipMemcpy(off.data(), t->d_cell_off.p, nc * 24, hipMemcpyDeviceToHost));
-    uint64_t top = 0;
-    VD_HIP(hipMemcpy(&top, t->d_top.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost));
-    std::vector<int32_t> arena(top);
-    if (top) VD_HIP(hipMemcpy(arena.data(), t->d_fnb.p, top * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < nc; ++i)
-        if (nf[i]) std::memcpy(ids + offsets[i], arena.data() + off[3 * i + 1], nf[i] * 4);
-    return VD_OK;
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    return get_neighbors(t->E, t->E.mem[0], offsets, n, ids, cap);
 }
 
 int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vertices, uint32_t *face_counts,
                      uint32_t *face_indices, int32_t *face_neighbors) {
-    if (!t || !sizes) return fail(VD_EINVAL, "NULL argument");
-    if (!t->calculated) return fail(VD_ESTATE, "calculate() has not run since the generators changed");
-    if (i >= t->ncells) return fail(VD_ERANGE, "cell index out of range");
-    if (!t->keep_cells) return fail(VD_ESTATE, "cells are not kept (VD_OPT_KEEP_CELLS = 0)");
-    uint32_t nv = 0, nf = 0;
-    uint64_t off[3];
-    VD_HIP(hipMemcpy(&nv, t->d_nvert.as<uint32_t>() + i, 4, hipMemcpyDeviceToHost));
-    VD_HIP(hipMemcpy(&nf, t->d_nface.as<uint32_t>() + i, 4, hipMemcpyDeviceToHost));
-    VD_HIP(hipMemcpy(off, t->d_cell_off.as<uint64_t>() + 3 * i, 24, hipMemcpyDeviceToHost));
-    std::vector<uint8_t> fc(nf);
-    if (nf) VD_HIP(hipMemcpy(fc.data(), t->d_fcnt.as<uint8_t>() + off[1], nf, hipMemcpyDeviceToHost));
-    uint64_t ns = 0;
-    for (uint8_t c : fc) ns += c;
-    sizes[0] = nv;
-    sizes[1] = nf;
-    sizes[2] = ns;
-    if (vertices && nv) VD_HIP(hipMemcpy(vertices, t->d_vtx.as<double>() + 3 * off[0], nv * 24, hipMemcpyDeviceToHost));
-    if (face_counts) for (uint32_t k = 0; k < nf; ++k) face_counts[k] = fc[k];
-    if (face_neighbors && nf)
-        VD_HIP(hipMemcpy(face_neighbors, t->d_fnb.as<int32_t>() + off[1], nf * 4, hipMemcpyDeviceToHost));
-    if (face_indices && ns) {
-        std::vector<uint16_t> fi(ns);
-        VD_HIP(hipMemcpy(fi.data(), t->d_fidx.as<uint16_t>() + off[2], ns * 2, hipMemcpyDeviceToHost));
-        for (uint64_t k = 0; k < ns; ++k) face_indices[k] = fi[k];
-    }
-    return VD_OK;
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    return get_cell(t->E, t->E.mem[0], i, sizes, vertices, face_counts, face_indices, face_neighbors);
 }
 
 int vd_tess_set_option(vd_tess *t, int key, int64_t value) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
     if (key == VD_OPT_KEEP_CELLS) {
-        t->keep_cells = value ? 1 : 0;
-        t->calculated = false;
+        t->E.keep_cells = value ? 1 : 0;
+        t->E.calculated = false;
         return VD_OK;
     }
     return fail(VD_EINVAL, "unknown option");
@@ -665,7 +856,201 @@ int vd_tess_set_option(vd_tess *t, int key, int64_t value) {
 
 int vd_tess_get_stats(const vd_tess *t, double out[8]) {
     if (!t || !out) return fail(VD_EINVAL, "NULL argument");
-    for (int k = 0; k < 8; ++k) out[k] = t->stats[k];
+    for (int k = 0; k < 8; ++k) out[k] = t->E.stats[k];
+    return VD_OK;
+}
+
+// ---- vd_batch: many tessellations in one launch -------------------------------------------
+int vd_batch_create(vd_batch **out) {
+    if (!out) return fail(VD_EINVAL, "out is NULL");
+    vd_batch *b = new vd_batch();
+    b->E.keep_cells = 0;
+    int rc = engine_init(b->E);
+    if (rc) { engine_free(b->E); delete b; return rc; }
+    *out = b;
+    return VD_OK;
+}
+
+void vd_batch_destroy(vd_batch *b) {
+    if (!b) return;
+    engine_free(b->E);
+    delete b;
+}
+
+int vd_batch_clear(vd_batch *b) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    b->E.mem.clear();
+    b->E.shapes.clear();
+    b->E.prepared = b->E.calculated = false;
+    b->staged.clear();
+    b->uploaded = false;
+    return VD_OK;
+}
+
+int vd_batch_add(vd_batch *b, const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny, uint32_t nz,
+                 const vd_wall *walls, int nwalls, const double *xyz, size_t n, uint32_t *index) {
+    if (!b || !bmin || !bmax || (!xyz && n) || (!walls && nwalls > 0)) return fail(VD_EINVAL, "NULL argument");
+    if (nwalls < 0 || nwalls > vd::kMaxWalls) return fail(VD_EINVAL, "0 to 8 walls per member");
+    Member m;
+    for (int a = 0; a < 3; ++a) { m.lo[a] = bmin[a]; m.hi[a] = bmax[a]; }
+    if (nx == 0 && ny == 0 && nz == 0) {
+        grid_for(bmin, bmax, n, b->per_bin, m.n);
+        m.auto_grid = true;
+    } else {
+        VD_TRY(check_grid(bmin, bmax, nx, ny, nz));
+        m.n[0] = nx; m.n[1] = ny; m.n[2] = nz;
+    }
+    m.grid = make_grid(m.lo, m.hi, m.n);
+    for (int k = 0; k < nwalls; ++k) {
+        vd::WallDev d;
+        VD_TRY(make_wall(walls[k], d));
+        m.walls.push_back(d);
+    }
+    m.nraw = n;
+    b->E.mem.push_back(m);
+    b->staged.insert(b->staged.end(), xyz, xyz + 3 * n);
+    b->uploaded = false;
+    b->E.shapes.clear();
+    b->E.prepared = b->E.calculated = false;
+    if (index) *index = (uint32_t)(b->E.mem.size() - 1);
+    return VD_OK;
+}
+
+int vd_batch_upload(vd_batch *b) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    Engine &E = b->E;
+    for (size_t t = 0; t < E.mem.size(); ++t) E.mem[t].nraw = E.mem[t].nraw;
+    engine_layout(E);
+    if (E.nraw * 3 != b->staged.size()) return fail(VD_ESTATE, "staged points and members disagree");
+    VD_TRY(ensure(E.d_raw, (E.nraw + 1) * 24));
+    if (E.nraw)
+        VD_HIP(hipMemcpyAsync(E.d_raw.p, b->staged.data(), E.nraw * 24, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    b->uploaded = true;
+    return VD_OK;
+}
+
+int vd_batch_calculate(vd_batch *b) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    if (!b->uploaded) VD_TRY(vd_batch_upload(b));
+    engine_layout(b->E);
+    VD_TRY(engine_prepare(b->E));
+    return engine_calculate(b->E);
+}
+
+int vd_batch_relax(vd_batch *b) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    VD_TRY(engine_relax(b->E));
+    // the relaxed points are now the device input of every member; the host copy is stale
+    b->staged.clear();
+    b->uploaded = true;
+    return VD_OK;
+}
+
+size_t vd_batch_count(const vd_batch *b) { return b ? b->E.mem.size() : 0; }
+
+int vd_batch_member_info(const vd_batch *b, uint32_t t, uint64_t info[5]) {
+    if (!b || !info) return fail(VD_EINVAL, "NULL argument");
+    const Member *m = member(b->E, t);
+    if (!m) return VD_ERANGE;
+    info[0] = b->E.prepared ? m->ngen : m->nraw;
+    info[1] = (uint64_t)m->n[0];
+    info[2] = (uint64_t)m->n[1];
+    info[3] = (uint64_t)m->n[2];
+    info[4] = m->gen_off;
+    return VD_OK;
+}
+
+int vd_batch_get_volumes(const vd_batch *b, uint32_t t, double *out, size_t n) {
+    if (!b || !out) return fail(VD_EINVAL, "NULL argument");
+    VD_TRY(need_results(b->E));
+    const Member *m = member(b->E, t);
+    if (!m) return VD_ERANGE;
+    return copy_slice(b->E.d_vol, 8, *m, out, n);
+}
+
+int vd_batch_get_centroids(const vd_batch *b, uint32_t t, double *out3, size_t n) {
+    if (!b || !out3) return fail(VD_EINVAL, "NULL argument");
+    VD_TRY(need_results(b->E));
+    const Member *m = member(b->E, t);
+    if (!m) return VD_ERANGE;
+    return copy_slice(b->E.d_cen, 24, *m, out3, n);
+}
+
+int vd_batch_get_face_counts(const vd_batch *b, uint32_t t, uint32_t *out, size_t n) {
+    if (!b || !out) return fail(VD_EINVAL, "NULL argument");
+    VD_TRY(need_results(b->E));
+    const Member *m = member(b->E, t);
+    if (!m) return VD_ERANGE;
+    return copy_slice(b->E.d_nface, 4, *m, out, n);
+}
+
+int vd_batch_get_generators(const vd_batch *b, uint32_t t, double *out, size_t n) {
+    if (!b || !out) return fail(VD_EINVAL, "NULL argument");
+    if (!b->E.prepared) return fail(VD_ESTATE, "the batch has not been calculated");
+    const Member *m = member(b->E, t);
+    if (!m) return VD_ERANGE;
+    return copy_slice(b->E.d_gen, 24, *m, out, n);
+}
+
+int vd_batch_get_neighbors(const vd_batch *b, uint32_t t, uint64_t *offsets, size_t n, int32_t *ids, size_t cap) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    const Member *m = member(b->E, t);
+    if (!m) return VD_ERANGE;
+    return get_neighbors(b->E, *m, offsets, n, ids, cap);
+}
+
+int vd_batch_get_cell(const vd_batch *b, uint32_t t, size_t i, uint64_t sizes[3], double *vertices,
+                      uint32_t *face_counts, uint32_t *face_indices, int32_t *face_neighbors) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    const Member *m = member(b->E, t);
+    if (!m) return VD_ERANGE;
+    return get_cell(b->E, *m, i, sizes, vertices, face_counts, face_indices, face_neighbors);
+}
+
+int vd_batch_set_option(vd_batch *b, int key, int64_t value) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    if (key == VD_BOPT_KEEP_CELLS) {
+        b->E.keep_cells = value ? 1 : 0;
+        b->E.calculated = false;
+        return VD_OK;
+    }
+    if (key == VD_BOPT_PER_BIN) {
+        if (value <= 0) return fail(VD_EINVAL, "generators per bin must be positive");
+        b->per_bin = (double)value / 1000.0;
+        return VD_OK;
+    }
+    return fail(VD_EINVAL, "unknown option");
+}
+
+int vd_batch_get_stats(const vd_batch *b, double out[8]) {
+    if (!b || !out) return fail(VD_EINVAL, "NULL argument");
+    for (int k = 0; k < 8; ++k) out[k] = b->E.stats[k];
+    return VD_OK;
+}
+
+int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *best) {
+    if (!b || !per_bin || ncand <= 0) return fail(VD_EINVAL, "invalid argument");
+    Engine &E = b->E;
+    if (!b->uploaded) VD_TRY(vd_batch_upload(b));
+    double best_t = std::numeric_limits<double>::infinity(), best_r = b->per_bin;
+    for (int pass = 0; pass <= ncand; ++pass) {
+        const double r = pass < ncand ? per_bin[pass] : best_r;   // the last pass applies the winner
+        if (!(r > 0.0)) return fail(VD_EINVAL, "generators per bin must be positive");
+        for (Member &m : E.mem)
+            if (m.auto_grid) {
+                grid_for(m.lo, m.hi, m.nraw, r, m.n);
+                m.grid = make_grid(m.lo, m.hi, m.n);
+            }
+        E.shapes.clear();
+        if (pass == ncand) break;
+        VD_TRY(vd_batch_calculate(b));
+        const double us = E.stats[5];
+        if (us < best_t) { best_t = us; best_r = r; }
+    }
+    b->per_bin = best_r;
+    E.prepared = E.calculated = false;
+    if (best) *best = best_r;
     return VD_OK;
 }
 

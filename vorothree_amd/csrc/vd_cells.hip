@@ -794,16 +794,23 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
 // One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports. The
 // walls and the neighbour candidates feed a single clip site (one inlined copy of the clip).
 template <class C>
-__device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
+__device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     const int l = lane_id();
+    const TessDesc &D = A.desc[A.gen_tess[i]];   // this cell's tessellation (wave-uniform)
     const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
+    const uint32_t li = (uint32_t)(i - D.gen_off);   // index within its tessellation
+    const GridGeom &G = D.grid;
+    const ShellEntry *shell = A.shell + D.shell_off;
+    const uint32_t nshell = D.nshell;
+    const double shell_next = D.shell_next_d2;
+    const uint32_t *bin_start = A.bin_start + D.bin_off;
+    const int nwalls = D.nwalls;
     Poly P;
-    poly_box<C>(m, P, A.box_lo, A.box_hi);
+    poly_box<C>(m, P, D.box_lo, D.box_hi);
     VReg r;
     vreg_load(m, P, r);
     int status = kCellOk;
     double r2 = 0.0;
-    const GridGeom &G = A.grid;
 
     // walk state (algo_3d_grid.rs:145-202)
     int ix = 0, iy = 0, iz = 0;
@@ -833,8 +840,8 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
         int32_t nid = 0;
         bool withgen = false;
         int cl = -1;
-        if (wi < A.nwalls) {
-            const WallDev &W = A.walls[wi];
+        if (wi < nwalls) {
+            const WallDev &W = D.walls[wi];
             if (W.kind == 0) {  // wall_3d.rs:38-42
                 pl = Plane{W.a[0], W.a[1], W.a[2], -W.b[0], -W.b[1], -W.b[2]};
             } else {            // wall_3d.rs:80-100
@@ -865,20 +872,20 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
             if (!win) {
                 if (load) {
                     const uint32_t e = e0 + (uint32_t)l;
-                    bend = e >= A.nshell;
+                    bend = e >= nshell;
                     binr = false;
                     bc = 0;
                     bdist = 0.0;
                     if (!bend) {
-                        const ShellEntry se = A.shell[e];
+                        const ShellEntry se = shell[e];
                         bd2 = se.d2;
                         const int bx = ix + se.dx, by = iy + se.dy, bz = iz + se.dz;
                         binr = bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2];
                         if (binr) {
                             const uint32_t b = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] +
                                                (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
-                            bs = A.bin_start[b];
-                            bc = A.bin_start[b + 1] - bs;
+                            bs = bin_start[b];
+                            bc = bin_start[b + 1] - bs;
                             // algo_3d_grid.rs:182-190
                             double ex = se.dx > 0 ? ((double)se.dx - rx) * G.cs[0]
                                       : se.dx < 0 ? ((double)(-(se.dx + 1)) + rx) * G.cs[0] : 0.0;
@@ -890,7 +897,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                             bdist = ex * ex + ey * ey + ez * ez;
                         }
                     } else {
-                        bd2 = (e == A.nshell) ? A.shell_next_d2 : DBL_MAX;
+                        bd2 = (e == nshell) ? shell_next : DBL_MAX;
                     }
                     load = false;
                 }
@@ -934,7 +941,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
             // next candidate of the window that passes the distance test (tessellation.rs:458)
             const double lim = 4.0 * r2;
             const uint64_t above = last < 0 ? ~0ull : (last >= 63 ? 0ull : ~((2ull << last) - 1ull));
-            const uint64_t pm = wvm & above & ballot(wj != i && !(wdsq > lim));
+            const uint64_t pm = wvm & above & ballot(wj != li && !(wdsq > lim));
             if (pm == 0) {
                 // window exhausted: find where the walk continues
                 win = false;
@@ -959,7 +966,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                     // the walk stops at entry kbc: a real break, or the end of the shell prefix
                     if (rl((int)bend, kbc)) {
                         const uint32_t e = e0 + (uint32_t)kbc;
-                        if (!(e == A.nshell && A.shell_next_d2 > lim)) status = kCellShell;
+                        if (!(e == nshell && shell_next > lim)) status = kCellShell;
                     }
                     break;
                 }
@@ -1051,7 +1058,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
                 A.cell_off[3 * (size_t)i + 2] = oi;
             }
         }
-    } else if (A.fnb_only) {
+    } else {
         // reduced mode keeps the neighbour ids (face order) only
         uint64_t of = 0;
         if (l == 0) of = atomicAdd((unsigned long long *)&A.arena_top[1], (unsigned long long)P.F);
@@ -1068,7 +1075,7 @@ __device__ void process_cell(Mem<C> &m, const TessArgs &A, uint32_t i) {
 
 template <class C>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C::WAVES, 8)))
-void k_cells(TessArgs A) {
+void k_cells(CellArgs A) {
     __shared__ Mem<C> m;
     for (;;) {
         uint32_t q = 0;
@@ -1100,7 +1107,7 @@ size_t tier_lds_bytes(int tier) {
     return tier == 0 ? sizeof(Mem<Tier0>) : tier == 1 ? sizeof(Mem<Tier1>) : sizeof(Mem<Tier2>);
 }
 
-hipError_t launch_cells(int tier, const TessArgs &a, int grid_waves, hipStream_t st) {
+hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st) {
     if (a.ncells == 0) return hipSuccess;
     if (tier == 0) k_cells<Tier0><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 1) k_cells<Tier1><<<grid_waves, 64, 0, st>>>(a);

@@ -1,5 +1,6 @@
 // Device-side generator handling for Algorithm3DGrid: wall admission filter, stable compaction,
-// stable counting-sort binning, bin-ordered position copy, and the Lloyd (relax) update.
+// stable counting-sort binning, bin-ordered position copy, and the Lloyd (relax) update. Every
+// kernel handles a set of tessellations at once (member lookup through TessDesc / gen_tess).
 //
 // Reference: src/tessellation.rs:43-67 (set_generators), :175-200 (add_wall / prune),
 // src/algorithm/algo_3d_grid.rs:98-130 (get_bin_index, set_generators), src/tessellation.rs:482-495
@@ -23,26 +24,48 @@ __device__ inline bool wall_contains(const WallDev &w, double x, double y, doubl
     return (dx * dx + dy * dy + dz * dz) <= w.r * w.r;
 }
 
-struct WallPack { WallDev w[kMaxWalls]; };
+__device__ inline uint64_t gid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 
-__global__ void k_wall_flags(const double *__restrict__ xyz, uint32_t n, WallPack walls, int nw,
-                             uint32_t *__restrict__ flags) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
-    uint32_t in = 1;
-    for (int k = 0; k < nw; ++k)
-        if (!wall_contains(walls.w[k], x, y, z)) { in = 0; break; }
-    flags[i] = in;
+// Member of every input point: binary search over the member offsets (nmem + 1 entries).
+__global__ void k_member_of(const uint64_t *__restrict__ offs, uint32_t nmem, uint32_t *__restrict__ tess,
+                            uint64_t n) {
+    const uint64_t g = gid();
+    if (g >= n) return;
+    uint32_t lo = 0, hi = nmem;   // offs[lo] <= g < offs[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (offs[mid] <= g) lo = mid; else hi = mid;
+    }
+    tess[g] = lo;
 }
 
-// Keeps the flagged points in input order (the indices are compacted, tessellation.rs:58-60).
-__global__ void k_compact3(const double *__restrict__ xyz, const uint32_t *__restrict__ flags,
-                           const uint32_t *__restrict__ pos, uint32_t n, double *__restrict__ out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flags[i]) return;
-    size_t d = 3 * (size_t)pos[i], s = 3 * (size_t)i;
-    out[d] = xyz[s]; out[d + 1] = xyz[s + 1]; out[d + 2] = xyz[s + 2];
+__global__ void k_wall_flags(const double *__restrict__ xyz, uint64_t n, const TessDesc *__restrict__ desc,
+                             const uint32_t *__restrict__ tess, uint32_t *__restrict__ flags) {
+    const uint64_t g = gid();
+    if (g >= n) return;
+    const TessDesc &D = desc[tess[g]];
+    const double x = xyz[3 * g], y = xyz[3 * g + 1], z = xyz[3 * g + 2];
+    uint32_t in = 1;
+    for (int k = 0; k < D.nwalls; ++k)
+        if (!wall_contains(D.walls[k], x, y, z)) { in = 0; break; }
+    flags[g] = in;
+}
+
+// Keeps the flagged points in input order (indices are compacted, tessellation.rs:58-60).
+__global__ void k_compact(const double *__restrict__ xyz, const uint32_t *__restrict__ tess_in,
+                          const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos, uint64_t n,
+                          double *__restrict__ out, uint32_t *__restrict__ tess_out) {
+    const uint64_t g = gid();
+    if (g >= n || !flags[g]) return;
+    const uint64_t d = pos[g];
+    out[3 * d] = xyz[3 * g]; out[3 * d + 1] = xyz[3 * g + 1]; out[3 * d + 2] = xyz[3 * g + 2];
+    tess_out[d] = tess_in[g];
+}
+
+__global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint64_t *__restrict__ idx, uint32_t n,
+                             uint32_t *__restrict__ out) {
+    const uint64_t k = gid();
+    if (k < n) out[k] = src[idx[k]];
 }
 
 // get_bin_index per axis (algo_3d_grid.rs:102-104): clamp((x - min) * s, 0, n - 1e-5) as usize.
@@ -55,70 +78,73 @@ __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
     return (uint32_t)v;
 }
 
-__global__ void k_bin_count(const double *__restrict__ xyz, uint32_t n, GridGeom g,
-                            uint32_t *__restrict__ gen_bin, uint32_t *__restrict__ counts) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    size_t o = 3 * (size_t)i;
-    uint32_t ix = axis_bin(xyz[o], g.lo[0], g.s[0], g.lim[0]);
-    uint32_t iy = axis_bin(xyz[o + 1], g.lo[1], g.s[1], g.lim[1]);
-    uint32_t iz = axis_bin(xyz[o + 2], g.lo[2], g.s[2], g.lim[2]);
-    uint32_t b = ix + iy * (uint32_t)g.n[0] + iz * (uint32_t)g.n[0] * (uint32_t)g.n[1];
-    gen_bin[i] = b;
+__global__ void k_bin_count(const double *__restrict__ xyz, uint64_t n, const TessDesc *__restrict__ desc,
+                            const uint32_t *__restrict__ tess, uint32_t *__restrict__ gen_bin,
+                            uint32_t *__restrict__ counts) {
+    const uint64_t g = gid();
+    if (g >= n) return;
+    const TessDesc &D = desc[tess[g]];
+    const GridGeom &G = D.grid;
+    const uint32_t ix = axis_bin(xyz[3 * g], G.lo[0], G.s[0], G.lim[0]);
+    const uint32_t iy = axis_bin(xyz[3 * g + 1], G.lo[1], G.s[1], G.lim[1]);
+    const uint32_t iz = axis_bin(xyz[3 * g + 2], G.lo[2], G.s[2], G.lim[2]);
+    const uint32_t b = (uint32_t)D.bin_off + ix + iy * (uint32_t)G.n[0] + iz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
+    gen_bin[g] = b;
     atomicAdd(&counts[b], 1u);
 }
 
-__global__ void k_bin_scatter(const uint32_t *__restrict__ gen_bin, uint32_t n,
+__global__ void k_bin_scatter(const uint32_t *__restrict__ gen_bin, uint64_t n,
                               const uint32_t *__restrict__ bin_start, uint32_t *__restrict__ fill,
                               uint32_t *__restrict__ items) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t b = gen_bin[i];
-    uint32_t k = atomicAdd(&fill[b], 1u);
-    items[bin_start[b] + k] = i;
+    const uint64_t g = gid();
+    if (g >= n) return;
+    const uint32_t b = gen_bin[g];
+    const uint32_t k = atomicAdd(&fill[b], 1u);
+    items[bin_start[b] + k] = (uint32_t)g;
 }
 
 // The atomic scatter is unordered; each bin list must be ascending (the reference pushes in
 // index order, algo_3d_grid.rs:121-129). Bins hold a handful of items, so a per-bin insertion
 // sort by one thread is the cheap way to restore the order.
-__global__ void k_bin_sort(const uint32_t *__restrict__ bin_start, uint32_t nbins,
+__global__ void k_bin_sort(const uint32_t *__restrict__ bin_start, uint64_t nbins,
                            uint32_t *__restrict__ items) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t b = gid();
     if (b >= nbins) return;
-    uint32_t lo = bin_start[b], hi = bin_start[b + 1];
+    const uint32_t lo = bin_start[b], hi = bin_start[b + 1];
     for (uint32_t k = lo + 1; k < hi; ++k) {
-        uint32_t v = items[k];
+        const uint32_t v = items[k];
         uint32_t j = k;
         while (j > lo && items[j - 1] > v) { items[j] = items[j - 1]; --j; }
         items[j] = v;
     }
 }
 
-__global__ void k_fill_spos(const double *__restrict__ xyz, const uint32_t *__restrict__ items,
-                            uint32_t n, double4 *__restrict__ spos) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// Bin-ordered copy of the generators; the 4th slot carries the member-local index.
+__global__ void k_fill_spos(const double *__restrict__ xyz, const uint32_t *__restrict__ items, uint64_t n,
+                            const TessDesc *__restrict__ desc, const uint32_t *__restrict__ tess,
+                            double4 *__restrict__ spos) {
+    const uint64_t k = gid();
     if (k >= n) return;
-    uint32_t i = items[k];
-    size_t o = 3 * (size_t)i;
+    const uint32_t g = items[k];
+    const uint64_t local = g - desc[tess[g]].gen_off;
     double4 r;
-    r.x = xyz[o]; r.y = xyz[o + 1]; r.z = xyz[o + 2];
-    r.w = __longlong_as_double((long long)i);
+    r.x = xyz[3 * (uint64_t)g]; r.y = xyz[3 * (uint64_t)g + 1]; r.z = xyz[3 * (uint64_t)g + 2];
+    r.w = __longlong_as_double((long long)local);
     spos[k] = r;
 }
 
 // Tessellation::relax (tessellation.rs:482-495): the centroid of each non-empty cell, the old
 // position for an empty one.
 __global__ void k_relax(const double *__restrict__ gen, const double *__restrict__ cen,
-                        const uint32_t *__restrict__ nvert, uint32_t n, double *__restrict__ out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                        const uint32_t *__restrict__ nvert, uint64_t n, double *__restrict__ out) {
+    const uint64_t i = gid();
     if (i >= n) return;
     const double *src = nvert[i] == 0 ? gen : cen;
-    size_t o = 3 * (size_t)i;
-    out[o] = src[o]; out[o + 1] = src[o + 1]; out[o + 2] = src[o + 2];
+    out[3 * i] = src[3 * i]; out[3 * i + 1] = src[3 * i + 1]; out[3 * i + 2] = src[3 * i + 2];
 }
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = gid();
     if (i < n) p[i] = v;
 }
 
@@ -127,10 +153,10 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
 __global__ void k_collect(const int32_t *__restrict__ status, const uint32_t *__restrict__ list,
                           uint32_t n, uint32_t *__restrict__ out_cap, uint32_t *__restrict__ out_shell,
                           uint32_t *__restrict__ counts) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t k = gid();
     if (k >= n) return;
-    uint32_t i = list[k];
-    int32_t s = status[i];
+    const uint32_t i = list[k];
+    const int32_t s = status[i];
     if (s == kCellOk) return;
     if (s == kCellCapacity) out_cap[atomicAdd(&counts[0], 1u)] = i;
     else if (s == kCellShell) out_shell[atomicAdd(&counts[1], 1u)] = i;
@@ -215,67 +241,76 @@ hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_wall_flags(const double *xyz, uint32_t n, const WallDev *walls, int nwalls,
-                             uint32_t *flags, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    WallPack p{};
-    for (int k = 0; k < nwalls && k < kMaxWalls; ++k) p.w[k] = walls[k];
-    k_wall_flags<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, p, nwalls, flags);
-    return hipGetLastError();
-}
-
-hipError_t launch_compact3(const double *xyz, const uint32_t *flags, const uint32_t *pos,
-                           uint32_t n, double *out, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    k_compact3<<<blocks_for(n), kBlock, 0, st>>>(xyz, flags, pos, n, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_bin_count(const double *xyz, uint32_t n, GridGeom g, uint32_t *gen_bin,
-                            uint32_t *counts, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    k_bin_count<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, g, gen_bin, counts);
-    return hipGetLastError();
-}
-
-hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint32_t n, const uint32_t *bin_start,
-                              uint32_t *fill, uint32_t *items, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    k_bin_scatter<<<blocks_for(n), kBlock, 0, st>>>(gen_bin, n, bin_start, fill, items);
-    return hipGetLastError();
-}
-
-hipError_t launch_bin_sort(const uint32_t *bin_start, uint32_t nbins, uint32_t *items,
-                           hipStream_t st) {
-    if (nbins == 0) return hipSuccess;
-    k_bin_sort<<<blocks_for(nbins), kBlock, 0, st>>>(bin_start, nbins, items);
-    return hipGetLastError();
-}
-
-hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint32_t n, double4 *spos,
-                            hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    k_fill_spos<<<blocks_for(n), kBlock, 0, st>>>(xyz, items, n, spos);
-    return hipGetLastError();
-}
-
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_fill_u32<<<blocks_for(n), kBlock, 0, st>>>(p, v, n);
     return hipGetLastError();
 }
 
-hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n,
-                          uint32_t *out_cap, uint32_t *out_shell, uint32_t *counts, hipStream_t st) {
+hipError_t launch_member_of(const uint64_t *offs, uint32_t nmem, uint32_t *tess, uint64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    k_collect<<<blocks_for(n), kBlock, 0, st>>>(status, list, n, out_cap, out_shell, counts);
+    k_member_of<<<blocks_for(n), kBlock, 0, st>>>(offs, nmem, tess, n);
     return hipGetLastError();
 }
 
-hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint32_t n,
-                        double *out, hipStream_t st) {
+hipError_t launch_wall_flags(const double *xyz, uint64_t n, const TessDesc *desc, const uint32_t *tess,
+                             uint32_t *flags, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_wall_flags<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, desc, tess, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const double *xyz, const uint32_t *tess_in, const uint32_t *flags, const uint32_t *pos,
+                          uint64_t n, double *out, uint32_t *tess_out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_compact<<<blocks_for(n), kBlock, 0, st>>>(xyz, tess_in, flags, pos, n, out, tess_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_u32(const uint32_t *src, const uint64_t *idx, uint32_t n, uint32_t *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_gather_u32<<<blocks_for(n), kBlock, 0, st>>>(src, idx, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_count(const double *xyz, uint64_t n, const TessDesc *desc, const uint32_t *tess,
+                            uint32_t *gen_bin, uint32_t *counts, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_bin_count<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, desc, tess, gen_bin, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint64_t n, const uint32_t *bin_start, uint32_t *fill,
+                              uint32_t *items, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_bin_scatter<<<blocks_for(n), kBlock, 0, st>>>(gen_bin, n, bin_start, fill, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_sort(const uint32_t *bin_start, uint64_t nbins, uint32_t *items, hipStream_t st) {
+    if (nbins == 0) return hipSuccess;
+    k_bin_sort<<<blocks_for(nbins), kBlock, 0, st>>>(bin_start, nbins, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n, const TessDesc *desc,
+                            const uint32_t *tess, double4 *spos, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_fill_spos<<<blocks_for(n), kBlock, 0, st>>>(xyz, items, n, desc, tess, spos);
+    return hipGetLastError();
+}
+
+hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint64_t n, double *out,
+                        hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_relax<<<blocks_for(n), kBlock, 0, st>>>(gen, cen, nvert, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n, uint32_t *out_cap,
+                          uint32_t *out_shell, uint32_t *counts, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_collect<<<blocks_for(n), kBlock, 0, st>>>(status, list, n, out_cap, out_shell, counts);
     return hipGetLastError();
 }
 

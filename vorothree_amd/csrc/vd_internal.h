@@ -1,5 +1,9 @@
 // Internal interface between the C-ABI host code (vd_api.cpp) and the gfx950 kernels (*.hip).
 // Not installed; the public interface is include/vd.h.
+//
+// Every launch works on a set of tessellations ("members"). A single Tessellation is a set of one.
+// Generators, bins and outputs of all members are concatenated; a TessDesc per member holds its
+// geometry, walls and offsets, and gen_tess maps each generator to its member.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,33 +39,44 @@ struct GridGeom {
     int32_t n[3];
 };
 
-// Everything one tessellation's cell kernel reads and writes. Device pointers.
-struct TessArgs {
+// One tessellation of a launch.
+struct TessDesc {
     GridGeom grid;
-    const double *gen;            // generators, xyz AoS, index order
-    const double4 *spos;          // generators in bin order: x, y, z, index (bit pattern of u64)
-    const uint32_t *bin_start;    // nbins + 1
-    const ShellEntry *shell;      // stably sorted prefix of the shell order
-    uint32_t nshell;
-    double shell_next_d2;         // key of the first entry not in the prefix (+inf if complete)
-    int32_t nwalls;
-    WallDev walls[kMaxWalls];
     double box_lo[3], box_hi[3];
-    uint32_t ngen;
+    uint64_t raw_off;       // first input point of this member (before the wall filter)
+    uint64_t gen_off;       // first generator (after the wall filter) = first output slot
+    uint64_t bin_off;       // first bin of this member in the concatenated bin arrays
+    uint32_t nraw, ngen;
+    uint32_t shell_off;     // its shell prefix in the concatenated shell array
+    uint32_t nshell;
+    double shell_next_d2;   // key of the first entry not in the prefix (+inf if complete)
+    int32_t nwalls;
+    int32_t pad;
+    WallDev walls[kMaxWalls];
+};
 
-    // work list: cell (generator) indices to compute
+// Everything one cell launch reads and writes. Device pointers.
+struct CellArgs {
+    const TessDesc *desc;
+    const uint32_t *gen_tess;     // member of each generator
+    const double *gen;            // generators, xyz AoS, members concatenated
+    const double4 *spos;          // generators in bin order: x, y, z, member-local index (u64 bits)
+    const uint32_t *bin_start;    // concatenated bins + 1 (positions in spos)
+    const ShellEntry *shell;      // concatenated shell prefixes
+
+    // work list: generator indices (global) to compute
     const uint32_t *cells;
     uint32_t ncells;
     uint32_t *queue;              // atomic work counter (zeroed before launch)
 
-    // reduced outputs, indexed by generator index
+    // reduced outputs, indexed by global generator index
     double *vol;
     double *cen;                  // 3 per cell
     uint32_t *nface;
     uint32_t *nvert;              // vertex count (0 = empty cell)
-    int32_t *status;              // 0 ok, >0 needs a retry (1 capacity, 2 shell prefix), <0 error
+    int32_t *status;              // 0 ok, >0 needs a retry, <0 error
 
-    // full cell export (optional: vtx == nullptr disables)
+    // full cell export (optional: vtx == nullptr keeps only the neighbour ids in fnb)
     double *vtx;                  // arena of vertices, 3 doubles each
     uint8_t *fcnt;                // arena of face vertex counts
     int32_t *fnb;                 // arena of face neighbour ids (same indexing as fcnt)
@@ -69,7 +84,6 @@ struct TessArgs {
     uint64_t *arena_top;          // [0] vertices, [1] faces, [2] face indices
     uint64_t arena_cap[3];
     uint64_t *cell_off;           // 3 per cell: vertex, face, index offsets
-    int32_t fnb_only;             // without vtx: still export the neighbour ids (fnb arena)
 };
 
 // Status codes written per cell.
@@ -81,32 +95,36 @@ enum : int32_t {
     kCellFace255 = -1,     // a face reached 256 vertices (the reference's u8 count would wrap)
 };
 
-// Launchers (vd_grid.hip, vd_cells.hip).
-hipError_t launch_wall_flags(const double *xyz, uint32_t n, const WallDev *walls, int nwalls,
-                             uint32_t *flags, hipStream_t st);
+// ---- generator pipeline (vd_grid.hip) ------------------------------------------------------
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
+hipError_t launch_member_of(const uint64_t *offs, uint32_t nmem, uint32_t *tess, uint64_t n,
+                            hipStream_t st);   // tess[g] = member t with offs[t] <= g < offs[t+1]
+hipError_t launch_wall_flags(const double *xyz, uint64_t n, const TessDesc *desc,
+                             const uint32_t *tess, uint32_t *flags, hipStream_t st);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp,
                            size_t tmp_words, hipStream_t st);
 size_t scan_tmp_words(uint32_t n);
-hipError_t launch_compact3(const double *xyz, const uint32_t *flags, const uint32_t *pos,
-                           uint32_t n, double *out, hipStream_t st);
-hipError_t launch_bin_count(const double *xyz, uint32_t n, GridGeom g, uint32_t *gen_bin,
-                            uint32_t *counts, hipStream_t st);
-hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint32_t n, const uint32_t *bin_start,
+hipError_t launch_compact(const double *xyz, const uint32_t *tess_in, const uint32_t *flags,
+                          const uint32_t *pos, uint64_t n, double *out, uint32_t *tess_out,
+                          hipStream_t st);
+hipError_t launch_gather_u32(const uint32_t *src, const uint64_t *idx, uint32_t n, uint32_t *out,
+                             hipStream_t st);
+hipError_t launch_bin_count(const double *xyz, uint64_t n, const TessDesc *desc, const uint32_t *tess,
+                            uint32_t *gen_bin, uint32_t *counts, hipStream_t st);
+hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint64_t n, const uint32_t *bin_start,
                               uint32_t *fill, uint32_t *items, hipStream_t st);
-hipError_t launch_bin_sort(const uint32_t *bin_start, uint32_t nbins, uint32_t *items,
-                           hipStream_t st);
-hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint32_t n, double4 *spos,
-                            hipStream_t st);
-hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint32_t n,
+hipError_t launch_bin_sort(const uint32_t *bin_start, uint64_t nbins, uint32_t *items, hipStream_t st);
+hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n, const TessDesc *desc,
+                            const uint32_t *tess, double4 *spos, hipStream_t st);
+hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint64_t n,
                         double *out, hipStream_t st);
-
-hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n,
                           uint32_t *out_cap, uint32_t *out_shell, uint32_t *counts, hipStream_t st);
 
+// ---- cell kernel (vd_cells.hip) ------------------------------------------------------------
 // tier: 0 = register fast path with a small LDS polytope (many waves per CU), 1 = generic path
 // with a medium LDS polytope, 2 = generic path with the largest LDS polytope (one wave per CU)
-hipError_t launch_cells(int tier, const TessArgs &a, int grid_waves, hipStream_t st);
+hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st);
 hipError_t cells_occupancy(int tier, int *blocks_per_cu);
 int tier_count();
 int tier_max_vertices(int tier);

@@ -328,3 +328,141 @@ def shell_prefix(bounds: BoundingBox, nx: int, ny: int, nz: int, kmax: int):
     check(lib().vd_shell_prefix(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, kmax, _ptr(dxyz),
                                 _ptr(keys), m, ctypes.byref(count), ctypes.byref(nxt)))
     return dxyz, keys, nxt.value
+
+
+def grid_for(bounds: BoundingBox, n: int, per_bin: float):
+    """The grid rule of the batch autotuner (vd_grid_for)."""
+    out = np.zeros(3, dtype=np.uint32)
+    check(lib().vd_grid_for(_d3(bounds.min), _d3(bounds.max), int(n), float(per_bin), _ptr(out)))
+    return tuple(int(x) for x in out)
+
+
+def _vd_wall(wall: Wall):
+    w = _lib.VdWall()
+    w.id = wall.id()
+    g = wall.geometry
+    if isinstance(g, PlaneGeometry):
+        w.kind = 0
+        for k in range(3):
+            w.a[k] = g.point[k]
+            w.b[k] = g.normal_raw[k]
+    else:
+        w.kind = 1
+        for k in range(3):
+            w.a[k] = g.center[k]
+        w.r = g.radius
+    return w
+
+
+class Batch:
+    """Many independent tessellations computed in one persistent launch (vd_batch_* in vd.h).
+
+    ``add(bounds, grid, walls, points)`` adds one ``Tessellation::<3, Cell3DFaces,
+    Algorithm3DGrid>`` (grid ``None`` lets the batch's generators-per-bin rule pick it);
+    ``calculate()`` runs every member; results are read per member.
+    """
+
+    def __init__(self, keep_cells: bool = False, per_bin: float = 2.0):
+        h = ctypes.c_void_p()
+        check(lib().vd_batch_create(ctypes.byref(h)))
+        self._h = h
+        if keep_cells:
+            check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_KEEP_CELLS, 1))
+        check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_PER_BIN, int(round(per_bin * 1000))))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().vd_batch_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, bounds: BoundingBox, grid, walls, points) -> int:
+        a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1))
+        nx, ny, nz = (0, 0, 0) if grid is None else (int(g) for g in grid)
+        ws = (_lib.VdWall * max(1, len(walls)))()
+        for k, w in enumerate(walls):
+            ws[k] = _vd_wall(w)
+        idx = ctypes.c_uint32(0)
+        check(lib().vd_batch_add(self._h, _d3(bounds.min), _d3(bounds.max), nx, ny, nz, ws, len(walls),
+                                 _ptr(a), a.size // 3, ctypes.byref(idx)))
+        return int(idx.value)
+
+    def upload(self):
+        check(lib().vd_batch_upload(self._h))
+
+    def calculate(self):
+        check(lib().vd_batch_calculate(self._h))
+
+    def relax(self):
+        check(lib().vd_batch_relax(self._h))
+
+    def autotune(self, candidates):
+        c = np.ascontiguousarray(np.asarray(candidates, dtype=np.float64))
+        best = ctypes.c_double(0.0)
+        check(lib().vd_batch_autotune(self._h, _ptr(c), c.size, ctypes.byref(best)))
+        return best.value
+
+    def __len__(self):
+        return int(lib().vd_batch_count(self._h))
+
+    def info(self, t: int):
+        out = np.zeros(5, dtype=np.uint64)
+        check(lib().vd_batch_member_info(self._h, int(t), _ptr(out)))
+        return {"cells": int(out[0]), "grid": (int(out[1]), int(out[2]), int(out[3])), "offset": int(out[4])}
+
+    def volumes(self, t: int) -> np.ndarray:
+        n = self.info(t)["cells"]
+        out = np.empty(n, dtype=np.float64)
+        check(lib().vd_batch_get_volumes(self._h, int(t), _ptr(out), n))
+        return out
+
+    def centroids(self, t: int) -> np.ndarray:
+        n = self.info(t)["cells"]
+        out = np.empty((n, 3), dtype=np.float64)
+        check(lib().vd_batch_get_centroids(self._h, int(t), _ptr(out), n))
+        return out
+
+    def face_counts(self, t: int) -> np.ndarray:
+        n = self.info(t)["cells"]
+        out = np.empty(n, dtype=np.uint32)
+        check(lib().vd_batch_get_face_counts(self._h, int(t), _ptr(out), n))
+        return out
+
+    def generators(self, t: int) -> np.ndarray:
+        n = self.info(t)["cells"]
+        out = np.empty(3 * n, dtype=np.float64)
+        check(lib().vd_batch_get_generators(self._h, int(t), _ptr(out), n))
+        return out
+
+    def neighbors(self, t: int):
+        n = self.info(t)["cells"]
+        off = np.empty(n + 1, dtype=np.uint64)
+        check(lib().vd_batch_get_neighbors(self._h, int(t), _ptr(off), n, None, 0))
+        ids = np.empty(int(off[-1]), dtype=np.int32)
+        check(lib().vd_batch_get_neighbors(self._h, int(t), _ptr(off), n, _ptr(ids), ids.size))
+        return off, ids
+
+    def get_cell(self, t: int, i: int):
+        sizes = np.zeros(3, dtype=np.uint64)
+        rc = lib().vd_batch_get_cell(self._h, int(t), int(i), _ptr(sizes), None, None, None, None)
+        if rc == _lib.VD_ERANGE:
+            return None
+        check(rc)
+        nv, nf, ni = (int(x) for x in sizes)
+        v = np.empty(3 * nv, dtype=np.float64)
+        fc = np.empty(nf, dtype=np.uint32)
+        fi = np.empty(ni, dtype=np.uint32)
+        fn = np.empty(nf, dtype=np.int32)
+        check(lib().vd_batch_get_cell(self._h, int(t), int(i), _ptr(sizes), _ptr(v), _ptr(fc), _ptr(fi), _ptr(fn)))
+        return Cell3DFaces(int(i), v, fc, fi, fn)
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.float64)
+        check(lib().vd_batch_get_stats(self._h, _ptr(out)))
+        return out
