@@ -43,7 +43,79 @@ def parse():
     p.add_argument("--reduced", action="store_true", help="keep reductions only, not full cells")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters)")
+    p.add_argument("--no-batched", action="store_true", help="skip the batched vs plain comparison")
+    p.add_argument("--batch-count", type=int, default=4096)
+    p.add_argument("--batch-steps", type=int, default=3)
     return p.parse_args()
+
+
+def batched_vs_plain(count: int, steps: int) -> dict:
+    """Mixed-size workload: ``count`` tessellations with log-uniform generator counts in
+    [64, 65536], boxes with edges U[0.5, 2], 30% sphere walls, 20% plane walls, grids at about
+    2 generators per bin. Batched: all members in one vd_batch_calculate (wall filter, binning and
+    one persistent cell launch for all). Plain: one Tessellation per member, each running its own
+    set_generators + calculate launch sequence. Both start from points already in device memory."""
+    from vorothree_amd import (Algorithm3DGrid, Batch, BoundingBox, PlaneGeometry, SphereGeometry,
+                               Tessellation, Wall, _hip)
+    from vorothree_amd import workloads as W
+
+    ws = W.config4(count=count)
+
+    def walls_of(w):
+        out = []
+        for wl in w.walls:
+            g = SphereGeometry(wl[2], wl[3]) if wl[0] == "sphere" else PlaneGeometry(wl[2], wl[3])
+            out.append(Wall(wl[1], g))
+        return out
+
+    b = Batch()
+    for w in ws:
+        b.add(BoundingBox(w.lo, w.hi), w.grid, walls_of(w), w.points)
+    b.upload()
+    b.calculate()                       # warmup (also builds the shell prefixes)
+    _hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.calculate()
+    _hip.synchronize()
+    tb = (time.perf_counter() - t0) / steps
+    cells = int(sum(b.info(t)["cells"] for t in range(len(b))))
+    kern_b = float(b.stats()[4]) * 1e-6
+    b.close()
+
+    allpts = np.concatenate([w.points for w in ws], axis=0)
+    offs = np.concatenate([[0], np.cumsum([len(w.points) for w in ws])])
+    dev = _hip.DeviceArray(allpts)
+    ts = []
+    for w in ws:
+        bb = BoundingBox(w.lo, w.hi)
+        t = Tessellation(bb, Algorithm3DGrid(*w.grid, bb), keep_cells=False)
+        for wl in walls_of(w):
+            t.add_wall(wl)
+        ts.append(t)
+
+    def plain_step():
+        for k, t in enumerate(ts):
+            t.set_generators_device(dev.ptr.value + 24 * int(offs[k]), len(ws[k].points))
+            t.calculate()
+
+    plain_step()
+    _hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain_step()
+    _hip.synchronize()
+    tp = (time.perf_counter() - t0) / steps
+    cells_p = int(sum(t.count_cells() for t in ts))
+    for t in ts:
+        t.close()
+    dev.close()
+    return {"workload": f"{count} tessellations, generators log-uniform in [64, 65536], "
+                        "boxes U[0.5,2]^3, 30% sphere / 20% plane walls, ~2 generators per bin",
+            "generators": int(offs[-1]), "cells": cells, "cells_plain": cells_p,
+            "batched_cells_per_s": cells / tb, "batched_ms": tb * 1e3, "batched_kernel_ms": kern_b * 1e3,
+            "plain_cells_per_s": cells_p / tp, "plain_ms": tp * 1e3,
+            "speedup_batched_vs_plain": tp / tb, "steps": steps}
 
 
 def copy_probe(mib: int) -> None:
@@ -159,6 +231,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not a.no_batched:
+        out["batched"] = batched_vs_plain(a.batch_count, a.batch_steps)
     print(json.dumps(out), flush=True)
 
 
