@@ -174,6 +174,39 @@ int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *bes
  * per_bin generators per bin, at least 1 per axis. */
 int vd_grid_for(const double bmin[3], const double bmax[3], size_t n, double per_bin, uint32_t out[3]);
 
+/* ---- Serving (no counterpart in the reference): continuous batching of tessellation jobs and
+ * Lloyd relaxation jobs on one device ---------------------------------------------------------- */
+
+typedef struct vd_server vd_server;
+
+/* hbm_budget: device bytes the admitted jobs may hold together; max_slots: jobs in flight. */
+int vd_server_create(size_t hbm_budget, int max_slots, vd_server **out);
+void vd_server_destroy(vd_server *s);
+/* Queues one job: a tessellation as vd_batch_add describes it, run for `iterations` calculate()
+ * calls with a relax() between consecutive ones (iterations = 1: a plain tessellation). The points
+ * are copied. A job larger than the whole budget is refused with VD_ENOMEM (its id is still
+ * returned and reports state 3). */
+int vd_server_submit(vd_server *s, const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
+                     uint32_t nz, const vd_wall *walls, int nwalls, const double *xyz, size_t n,
+                     uint32_t iterations, uint64_t *job);
+/* One step: admit queued jobs in arrival order while slots and budget allow, run one batched
+ * calculate over every admitted job, finish the jobs whose last iteration this was and relax the
+ * others on the device. *finished receives the number of jobs finished by this step. */
+int vd_server_step(vd_server *s, uint32_t *finished);
+/* counts[0] queued jobs, [1] admitted jobs, [2] budget bytes in use. */
+int vd_server_pending(const vd_server *s, uint64_t counts[3]);
+/* state: 0 queued, 1 running, 2 finished, 3 refused; info: cells of the last calculate,
+ * calculate() runs done, cells over all runs; times: submit, admit, finish (seconds, steady clock). */
+int vd_server_job_info(const vd_server *s, uint64_t job, int32_t *state, uint64_t info[3], double times[3]);
+/* Results of a finished job's last calculate(): volumes, centroids (3 per cell) and the generators
+ * it ran on (3 per cell); NULL pointers are skipped. */
+int vd_server_result(const vd_server *s, uint64_t job, double *volumes, double *centroids,
+                     double *generators, size_t n);
+/* Forgets a finished or refused job. */
+int vd_server_release(vd_server *s, uint64_t job);
+/* [0] steps, [1] cells computed, [2] budget bytes in use, [3] budget. */
+int vd_server_stats(const vd_server *s, double out[4]);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char *vd_last_error(void);
 

@@ -1,0 +1,96 @@
+"""Serving (vd_server_*): continuous batching of tessellation and Lloyd jobs.
+
+* a job's results are bit-identical alone and in a mix with other jobs that arrive while it runs;
+* a Lloyd job equals the oracle's calculate/relax loop (tessellation.rs:482-495) bit for bit;
+* admission respects the HBM budget; a job larger than the whole budget is refused.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits
+from oracle import OracleTess
+from vorothree_amd import BoundingBox, Server, SphereGeometry, Wall
+from vorothree_amd import _lib
+from vorothree_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+UNIT = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+
+
+def job_points(seed, n, clustered=False):
+    g = W.PCG32(seed)
+    if clustered:
+        return W.clustered_points(g, n, 8, 0.05, 1.0)
+    return g.uniform(n, (0, 0, 0), (1, 1, 1))
+
+
+def run_until_done(s, jobs, max_steps=500):
+    for _ in range(max_steps):
+        if all(s.info(j)["state"] == Server.FINISHED for j in jobs):
+            return
+        s.step()
+    raise AssertionError("jobs did not finish")
+
+
+def test_job_alone_equals_in_mix():
+    pts = job_points(101, 6000)
+    sphere = [Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.45))]
+    alone = Server(hbm_budget=1 << 30)
+    ja = alone.submit(UNIT, (14, 14, 14), sphere, pts, iterations=3)
+    run_until_done(alone, [ja])
+    va, ca, ga = alone.result(ja)
+
+    mix = Server(hbm_budget=1 << 30)
+    others = [mix.submit(UNIT, None, [], job_points(200 + k, 3000 + 1500 * k, clustered=k % 2 == 1),
+                         iterations=1 + (k % 3)) for k in range(3)]
+    mix.step()                                   # the others are already running ...
+    j = mix.submit(UNIT, (14, 14, 14), sphere, pts, iterations=3)   # ... when this one arrives
+    others += [mix.submit(UNIT, None, [], job_points(300 + k, 2000 + 700 * k), iterations=2) for k in range(2)]
+    run_until_done(mix, others + [j])
+    vm, cm, gm = mix.result(j)
+    assert np.array_equal(bits(va), bits(vm))
+    assert np.array_equal(bits(ca), bits(cm))
+    assert np.array_equal(bits(ga), bits(gm))
+    assert mix.stats()["steps"] >= 3
+
+
+def test_lloyd_job_matches_oracle():
+    pts = job_points(111, 5000)
+    s = Server(hbm_budget=1 << 30)
+    j = s.submit(UNIT, (13, 13, 13), [], pts, iterations=5)
+    run_until_done(s, [j])
+    vol, cen, gen = s.result(j)
+    o = OracleTess((0, 0, 0), (1, 1, 1), 13, 13, 13)
+    o.set_generators(pts)
+    o.calculate(threads=8)
+    for _ in range(4):
+        o.relax()
+        o.calculate(threads=8)
+    ov, oc, _, _ = o.reduced()
+    assert np.array_equal(bits(gen.reshape(-1)), bits(o.generators()))
+    assert np.array_equal(bits(vol), bits(ov))
+    assert np.array_equal(bits(cen), bits(oc))
+    info = s.info(j)
+    assert info["runs"] == 5 and info["total_cells"] == 5 * 5000
+
+
+def test_budget_admission_and_refusal():
+    small = [job_points(120 + k, 4000) for k in range(3)]
+    s = Server(hbm_budget=1_500_000)      # one 4000-point job (about 1.2 MB) at a time
+    jobs = [s.submit(UNIT, (12, 12, 12), [], p, iterations=2) for p in small]
+    s.step()
+    p = s.pending()
+    assert p["running"] == 1 and p["queued"] == 2
+    run_until_done(s, jobs)
+    for j in jobs:
+        assert s.info(j)["state"] == Server.FINISHED
+    assert s.pending()["bytes"] == 0
+    with pytest.raises(_lib.VdError) as e:
+        s.submit(UNIT, None, [], job_points(130, 200_000))
+    assert e.value.code == _lib.VD_ENOMEM
+    # a refused job never blocks the queue
+    j = s.submit(UNIT, (10, 10, 10), [], small[0])
+    run_until_done(s, [j])
+    for j0 in jobs:
+        s.release(j0)

@@ -10,6 +10,7 @@ from .tessellation import (  # noqa: F401
     BoundingBox,
     Cell3DFaces,
     PlaneGeometry,
+    Server,
     SphereGeometry,
     Tessellation,
     Wall,

@@ -73,6 +73,15 @@ SIGNATURES = {
     "vd_batch_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_batch_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "vd_grid_for": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_double, _vp]),
+    "vd_server_create": (ctypes.c_int, [_sz, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "vd_server_destroy": (None, [_vp]),
+    "vd_server_submit": (ctypes.c_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp, ctypes.c_int, _vp, _sz, _u32, _vp]),
+    "vd_server_step": (ctypes.c_int, [_vp, _vp]),
+    "vd_server_pending": (ctypes.c_int, [_vp, _vp]),
+    "vd_server_job_info": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _vp, _vp]),
+    "vd_server_result": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _vp, _vp, _sz]),
+    "vd_server_release": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "vd_server_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_last_error": (ctypes.c_char_p, []),
 }
 

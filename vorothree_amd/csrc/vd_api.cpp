@@ -13,8 +13,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -191,9 +194,12 @@ struct Shape {   // one shell prefix per distinct grid geometry
     uint32_t off = 0;
 };
 
+using ShapeKey = std::tuple<int64_t, int64_t, int64_t, uint64_t, uint64_t, uint64_t, uint32_t>;
+
 struct Engine {
     std::vector<Member> mem;
     std::vector<Shape> shapes;
+    std::map<ShapeKey, ShellPrefix> shell_cache;   // prefixes built so far, by grid and k
     uint32_t shell_k = 7;
     int device = 0, cus = 1;
     int occ[kTiers] = {1, 1, 1};
@@ -204,7 +210,7 @@ struct Engine {
     DevBuf d_desc, d_offs, d_raw, d_raw_tess, d_flags, d_pos, d_scan, d_gat;
     DevBuf d_gen, d_gen_tess, d_gen_bin, d_counts, d_bin_start, d_fill, d_items, d_spos, d_shell;
     DevBuf d_vol, d_cen, d_nface, d_nvert, d_status, d_cell_off;
-    DevBuf d_vtx, d_fcnt, d_fnb, d_fidx, d_top, d_queue, d_counts4;
+    DevBuf d_vtx, d_fcnt, d_fnb, d_fidx, d_top, d_queue, d_counts4, d_next;
     uint64_t nraw = 0, ngen = 0, nbins = 0;
     bool prepared = false, calculated = false;
     uint64_t cap[3] = {0, 0, 0};
@@ -230,7 +236,7 @@ void engine_free(Engine &E) {
                       &E.d_gat, &E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start,
                       &E.d_fill, &E.d_items, &E.d_spos, &E.d_shell, &E.d_vol, &E.d_cen, &E.d_nface,
                       &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
-                      &E.d_top, &E.d_queue, &E.d_counts4};
+                      &E.d_top, &E.d_queue, &E.d_counts4, &E.d_next};
     for (DevBuf *b : bufs) release(*b);
     if (E.ev0) (void)hipEventDestroy(E.ev0);
     if (E.ev1) (void)hipEventDestroy(E.ev1);
@@ -266,7 +272,16 @@ int engine_shells(Engine &E) {
     }
     size_t total = 0;
     for (Shape &s : E.shapes) {
-        build_shell(s.n, s.cs, E.shell_k, s.sp);
+        uint64_t c[3];
+        std::memcpy(c, s.cs, sizeof(c));
+        const ShapeKey key{s.n[0], s.n[1], s.n[2], c[0], c[1], c[2], E.shell_k};
+        auto it = E.shell_cache.find(key);
+        if (it == E.shell_cache.end()) {
+            if (E.shell_cache.size() > 65536) E.shell_cache.clear();
+            it = E.shell_cache.emplace(key, ShellPrefix()).first;
+            build_shell(s.n, s.cs, E.shell_k, it->second);
+        }
+        s.sp = it->second;
         s.off = (uint32_t)total;
         total += s.sp.e.size();
     }
@@ -662,6 +677,48 @@ struct vd_batch {
     double per_bin = 2.0;
 };
 
+
+// ---- serving: continuous batching of tessellation and Lloyd jobs --------------------------
+namespace {
+
+struct Job {
+    uint64_t id = 0;
+    Member m;                      // geometry, grid and walls
+    uint32_t iters = 1;            // calculate() runs; a relax() between consecutive ones
+    uint32_t done = 0;
+    std::vector<double> host;      // input points until admission
+    DevBuf pts;                    // current input points on the device (after admission)
+    uint64_t npts = 0;
+    size_t bytes = 0;              // HBM charged to the budget while admitted
+    int state = 0;                 // 0 pending, 1 active, 2 finished, 3 rejected
+    uint64_t cells = 0;            // cells of the last calculate
+    uint64_t total_cells = 0;      // cells over all its calculate() runs
+    std::vector<double> vol, cen, gen;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+// Device bytes one generator occupies across the engine's buffers (input, generators, bins,
+// bin-ordered copy, outputs, neighbour arena) plus the job's own copy, with headroom.
+constexpr size_t kBytesPerPoint = 24 + 24 + 24 + 4 + 4 + 4 + 4 + 32 + 8 + 24 + 4 + 4 + 4 + 24 + 24 * 4 + 16;
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct vd_server {
+    Engine E;
+    size_t budget = 0, used = 0;
+    int max_slots = 64;
+    uint64_t next_id = 1;
+    std::map<uint64_t, Job *> jobs;
+    std::deque<Job *> pending;
+    std::vector<Job *> active;
+    double per_bin = 2.0;
+    uint64_t steps = 0, cells = 0;
+};
+
 extern "C" {
 
 const char *vd_last_error(void) { return g_err.c_str(); }
@@ -1051,6 +1108,211 @@ int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *bes
     b->per_bin = best_r;
     E.prepared = E.calculated = false;
     if (best) *best = best_r;
+    return VD_OK;
+}
+
+// ---- vd_server -----------------------------------------------------------------------------
+int vd_server_create(size_t hbm_budget, int max_slots, vd_server **out) {
+    if (!out || hbm_budget == 0 || max_slots <= 0) return fail(VD_EINVAL, "invalid argument");
+    vd_server *S = new vd_server();
+    S->budget = hbm_budget;
+    S->max_slots = max_slots;
+    S->E.keep_cells = 0;
+    int rc = engine_init(S->E);
+    if (rc) { engine_free(S->E); delete S; return rc; }
+    *out = S;
+    return VD_OK;
+}
+
+void vd_server_destroy(vd_server *S) {
+    if (!S) return;
+    for (auto &kv : S->jobs) { release(kv.second->pts); delete kv.second; }
+    engine_free(S->E);
+    delete S;
+}
+
+int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
+                     uint32_t nz, const vd_wall *walls, int nwalls, const double *xyz, size_t n,
+                     uint32_t iterations, uint64_t *job) {
+    if (!S || !bmin || !bmax || (!xyz && n) || (!walls && nwalls > 0) || !job) return fail(VD_EINVAL, "NULL argument");
+    if (nwalls < 0 || nwalls > vd::kMaxWalls) return fail(VD_EINVAL, "0 to 8 walls per job");
+    if (iterations == 0) return fail(VD_EINVAL, "a job runs at least one calculate()");
+    Job *J = new Job();
+    for (int a = 0; a < 3; ++a) { J->m.lo[a] = bmin[a]; J->m.hi[a] = bmax[a]; }
+    int rc = VD_OK;
+    if (nx == 0 && ny == 0 && nz == 0) {
+        grid_for(bmin, bmax, n, S->per_bin, J->m.n);
+        J->m.auto_grid = true;
+    } else {
+        rc = check_grid(bmin, bmax, nx, ny, nz);
+        J->m.n[0] = nx; J->m.n[1] = ny; J->m.n[2] = nz;
+    }
+    for (int k = 0; k < nwalls && rc == VD_OK; ++k) {
+        vd::WallDev d;
+        rc = make_wall(walls[k], d);
+        J->m.walls.push_back(d);
+    }
+    if (rc) { delete J; return rc; }
+    J->m.grid = make_grid(J->m.lo, J->m.hi, J->m.n);
+    J->iters = iterations;
+    J->host.assign(xyz, xyz + 3 * n);
+    J->npts = n;
+    J->bytes = n * kBytesPerPoint + (uint64_t)J->m.n[0] * J->m.n[1] * J->m.n[2] * 12 + 4096;
+    J->t_submit = now_s();
+    J->id = S->next_id++;
+    if (J->bytes > S->budget) {
+        J->state = 3;   // can never be admitted: refuse it now instead of blocking the queue
+        S->jobs[J->id] = J;
+        *job = J->id;
+        return fail(VD_ENOMEM, "job needs more device memory than the server's HBM budget");
+    }
+    S->jobs[J->id] = J;
+    S->pending.push_back(J);
+    *job = J->id;
+    return VD_OK;
+}
+
+// One engine step: admit pending jobs (FIFO, while slots and the HBM budget allow), run one
+// batched calculate over every active job, finish the jobs whose last iteration this was, and
+// relax the others in place (their points stay on the device).
+int vd_server_step(vd_server *S, uint32_t *finished) {
+    if (!S) return fail(VD_EINVAL, "NULL handle");
+    if (finished) *finished = 0;
+    Engine &E = S->E;
+    while (!S->pending.empty() && (int)S->active.size() < S->max_slots &&
+           S->used + S->pending.front()->bytes <= S->budget) {
+        Job *J = S->pending.front();
+        S->pending.pop_front();
+        VD_TRY(ensure(J->pts, (J->npts + 1) * 24));
+        if (J->npts)
+            VD_HIP(hipMemcpyAsync(J->pts.p, J->host.data(), J->npts * 24, hipMemcpyHostToDevice, E.stream));
+        J->host.clear();
+        J->host.shrink_to_fit();
+        J->state = 1;
+        J->t_admit = now_s();
+        S->used += J->bytes;
+        S->active.push_back(J);
+    }
+    if (S->active.empty()) return VD_OK;
+    // members = active jobs, inputs gathered from the jobs' device buffers
+    E.mem.clear();
+    for (Job *J : S->active) {
+        Member m = J->m;
+        m.nraw = J->npts;
+        E.mem.push_back(m);
+    }
+    E.shapes.clear();
+    engine_layout(E);
+    VD_TRY(ensure(E.d_raw, (E.nraw + 1) * 24));
+    for (size_t t = 0; t < S->active.size(); ++t) {
+        const Job *J = S->active[t];
+        if (J->npts)
+            VD_HIP(hipMemcpyAsync(E.d_raw.as<double>() + 3 * E.mem[t].raw_off, J->pts.p, J->npts * 24,
+                                  hipMemcpyDeviceToDevice, E.stream));
+    }
+    VD_TRY(engine_prepare(E));
+    VD_TRY(engine_calculate(E));
+    S->steps += 1;
+    S->cells += E.ngen;
+    // relaxed points of every member (used by the jobs that continue)
+    bool any_more = false;
+    for (const Job *J : S->active) any_more |= J->done + 1 < J->iters;
+    if (any_more) {
+        VD_TRY(ensure(E.d_next, (E.ngen + 1) * 24));
+        VD_HIP(vd::launch_relax(E.d_gen.as<double>(), E.d_cen.as<double>(), E.d_nvert.as<uint32_t>(), E.ngen,
+                                E.d_next.as<double>(), E.stream));
+    }
+    std::vector<Job *> keep;
+    const double t = now_s();
+    for (size_t k = 0; k < S->active.size(); ++k) {
+        Job *J = S->active[k];
+        const Member &m = E.mem[k];
+        J->done += 1;
+        J->cells = m.ngen;
+        J->total_cells += m.ngen;
+        if (J->done >= J->iters) {
+            J->vol.resize(m.ngen);
+            J->cen.resize(3 * m.ngen);
+            J->gen.resize(3 * m.ngen);
+            if (m.ngen) {
+                VD_HIP(hipMemcpyAsync(J->vol.data(), E.d_vol.as<double>() + m.gen_off, m.ngen * 8,
+                                      hipMemcpyDeviceToHost, E.stream));
+                VD_HIP(hipMemcpyAsync(J->cen.data(), E.d_cen.as<double>() + 3 * m.gen_off, m.ngen * 24,
+                                      hipMemcpyDeviceToHost, E.stream));
+                VD_HIP(hipMemcpyAsync(J->gen.data(), E.d_gen.as<double>() + 3 * m.gen_off, m.ngen * 24,
+                                      hipMemcpyDeviceToHost, E.stream));
+            }
+            J->state = 2;
+            release(J->pts);
+            S->used -= J->bytes;
+            if (finished) *finished += 1;
+        } else {
+            // tessellation.rs:482-495: the job's next input is its relaxed generators
+            if (m.ngen)
+                VD_HIP(hipMemcpyAsync(J->pts.p, E.d_next.as<double>() + 3 * m.gen_off,
+                                      m.ngen * 24, hipMemcpyDeviceToDevice, E.stream));
+            J->npts = m.ngen;
+            keep.push_back(J);
+        }
+    }
+    VD_HIP(hipStreamSynchronize(E.stream));
+    for (Job *J : S->active)
+        if (J->state == 2) J->t_done = t;
+    S->active.swap(keep);
+    return VD_OK;
+}
+
+int vd_server_pending(const vd_server *S, uint64_t counts[3]) {
+    if (!S || !counts) return fail(VD_EINVAL, "NULL argument");
+    counts[0] = S->pending.size();
+    counts[1] = S->active.size();
+    counts[2] = S->used;
+    return VD_OK;
+}
+
+int vd_server_job_info(const vd_server *S, uint64_t job, int32_t *state, uint64_t info[3], double times[3]) {
+    if (!S) return fail(VD_EINVAL, "NULL handle");
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return fail(VD_ERANGE, "unknown job");
+    const Job *J = it->second;
+    if (state) *state = J->state;
+    if (info) { info[0] = J->cells; info[1] = J->done; info[2] = J->total_cells; }
+    if (times) { times[0] = J->t_submit; times[1] = J->t_admit; times[2] = J->t_done; }
+    return VD_OK;
+}
+
+int vd_server_result(const vd_server *S, uint64_t job, double *volumes, double *centroids, double *generators,
+                     size_t n) {
+    if (!S) return fail(VD_EINVAL, "NULL handle");
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return fail(VD_ERANGE, "unknown job");
+    const Job *J = it->second;
+    if (J->state != 2) return fail(VD_ESTATE, "job not finished");
+    if (n < J->cells) return fail(VD_EINVAL, "output buffer smaller than the cell count");
+    if (volumes) std::memcpy(volumes, J->vol.data(), J->cells * 8);
+    if (centroids) std::memcpy(centroids, J->cen.data(), J->cells * 24);
+    if (generators) std::memcpy(generators, J->gen.data(), J->cells * 24);
+    return VD_OK;
+}
+
+int vd_server_release(vd_server *S, uint64_t job) {
+    if (!S) return fail(VD_EINVAL, "NULL handle");
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return fail(VD_ERANGE, "unknown job");
+    Job *J = it->second;
+    if (J->state == 0 || J->state == 1) return fail(VD_ESTATE, "job still queued or running");
+    release(J->pts);
+    delete J;
+    S->jobs.erase(it);
+    return VD_OK;
+}
+
+int vd_server_stats(const vd_server *S, double out[4]) {
+    if (!S || !out) return fail(VD_EINVAL, "NULL argument");
+    out[0] = (double)S->steps;
+    out[1] = (double)S->cells;
+    out[2] = (double)S->used;
+    out[3] = (double)S->budget;
     return VD_OK;
 }
 

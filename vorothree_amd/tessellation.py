@@ -466,3 +466,75 @@ class Batch:
         out = np.zeros(8, dtype=np.float64)
         check(lib().vd_batch_get_stats(self._h, _ptr(out)))
         return out
+
+
+class Server:
+    """Continuous-batching server (vd_server_* in vd.h): tessellation and Lloyd jobs admitted by
+    HBM budget, every step one batched launch over all admitted jobs, Lloyd jobs relaxed on the
+    device between their iterations."""
+
+    QUEUED, RUNNING, FINISHED, REFUSED = 0, 1, 2, 3
+
+    def __init__(self, hbm_budget: int, max_slots: int = 64):
+        h = ctypes.c_void_p()
+        check(lib().vd_server_create(int(hbm_budget), int(max_slots), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().vd_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, bounds: BoundingBox, grid, walls, points, iterations: int = 1) -> int:
+        """Queue a job; raises VdError (VD_ENOMEM) for a job larger than the whole budget."""
+        a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1))
+        nx, ny, nz = (0, 0, 0) if grid is None else (int(g) for g in grid)
+        ws = (_lib.VdWall * max(1, len(walls)))()
+        for k, w in enumerate(walls):
+            ws[k] = _vd_wall(w)
+        job = ctypes.c_uint64(0)
+        check(lib().vd_server_submit(self._h, _d3(bounds.min), _d3(bounds.max), nx, ny, nz, ws, len(walls),
+                                     _ptr(a), a.size // 3, int(iterations), ctypes.byref(job)))
+        return int(job.value)
+
+    def step(self) -> int:
+        fin = ctypes.c_uint32(0)
+        check(lib().vd_server_step(self._h, ctypes.byref(fin)))
+        return int(fin.value)
+
+    def pending(self):
+        c = np.zeros(3, dtype=np.uint64)
+        check(lib().vd_server_pending(self._h, _ptr(c)))
+        return {"queued": int(c[0]), "running": int(c[1]), "bytes": int(c[2])}
+
+    def info(self, job: int):
+        st = ctypes.c_int32(0)
+        info = np.zeros(3, dtype=np.uint64)
+        times = np.zeros(3, dtype=np.float64)
+        check(lib().vd_server_job_info(self._h, int(job), ctypes.byref(st), _ptr(info), _ptr(times)))
+        return {"state": int(st.value), "cells": int(info[0]), "runs": int(info[1]),
+                "total_cells": int(info[2]), "submit": float(times[0]), "admit": float(times[1]),
+                "finish": float(times[2])}
+
+    def result(self, job: int):
+        n = self.info(job)["cells"]
+        vol = np.empty(n, dtype=np.float64)
+        cen = np.empty((n, 3), dtype=np.float64)
+        gen = np.empty((n, 3), dtype=np.float64)
+        check(lib().vd_server_result(self._h, int(job), _ptr(vol), _ptr(cen), _ptr(gen), n))
+        return vol, cen, gen
+
+    def release(self, job: int):
+        check(lib().vd_server_release(self._h, int(job)))
+
+    def stats(self):
+        out = np.zeros(4, dtype=np.float64)
+        check(lib().vd_server_stats(self._h, _ptr(out)))
+        return {"steps": int(out[0]), "cells": int(out[1]), "bytes": int(out[2]), "budget": int(out[3])}
