@@ -103,8 +103,8 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
                                neighbour ids only, the output of Tessellation::map */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
 
-/* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cells
- * retried in the large tier, [2] shell-prefix extensions, [3] entries of the shell prefix,
+/* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cell runs
+ * in the larger tiers, [2] shell-prefix extensions, [3] entries of the shell prefixes,
  * [4] kernel time in microseconds of the first cell launch, [5] total calculate time (us). */
 int vd_tess_get_stats(const vd_tess *t, double out[8]);
 
@@ -116,6 +116,13 @@ int vd_tess_get_stats(const vd_tess *t, double out[8]);
 int vd_shell_prefix(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
                     uint32_t nz, uint32_t kmax, int32_t *dxyz, double *keys, size_t cap,
                     size_t *count, double *next_key);
+
+/* The same prefix built the way calculate() builds it: on the device (key per offset, stable
+ * radix sort by key). No counterpart in the reference; a diagnostic that lets tests compare the
+ * device construction with vd_shell_prefix entry by entry. */
+int vd_shell_prefix_device(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
+                           uint32_t nz, uint32_t kmax, int32_t *dxyz, double *keys, size_t cap,
+                           size_t *count, double *next_key);
 
 /* ---- Batched tessellations (no counterpart in the reference: many independent
  * Tessellation::<3, Cell3DFaces, Algorithm3DGrid> computed in one persistent launch) ---------- */
@@ -174,6 +181,15 @@ int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *bes
  * per_bin generators per bin, at least 1 per axis. */
 int vd_grid_for(const double bmin[3], const double bmax[3], size_t n, double per_bin, uint32_t out[3]);
 
+/* The grid rule for uneven input (the server's choice for jobs submitted without a grid): as
+ * vd_grid_for, with per_bin divided by how much denser the average generator's neighbourhood is
+ * than the mean (sum of c (c - 1) over a coarse histogram of the points), that ratio clamped to
+ * [1, 64], or to [1, 2] when over a quarter of the coarse bins are empty (cells next to empty
+ * space walk across it). Uniform input gets about the vd_grid_for grid; clustered input with a
+ * background a much finer one. */
+int vd_grid_for_points(const double bmin[3], const double bmax[3], const double *xyz, size_t n,
+                       double per_bin, uint32_t out[3]);
+
 /* ---- Serving (no counterpart in the reference): continuous batching of tessellation jobs and
  * Lloyd relaxation jobs on one device ---------------------------------------------------------- */
 
@@ -206,6 +222,9 @@ int vd_server_result(const vd_server *s, uint64_t job, double *volumes, double *
 int vd_server_release(vd_server *s, uint64_t job);
 /* [0] steps, [1] cells computed, [2] budget bytes in use, [3] budget. */
 int vd_server_stats(const vd_server *s, double out[4]);
+#define VD_SOPT_PER_BIN 1   /* generators-per-bin target of vd_grid_for_points for jobs submitted
+                               without a grid, x1000 (default 2000); applies to later submits */
+int vd_server_set_option(vd_server *s, int key, int64_t value);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *vd_last_error(void);

@@ -71,3 +71,26 @@ def test_shell_prefix_grows_to_the_full_table():
     b = BoundingBox((0, 0, 0), (1, 1, 1))
     sizes = [len(shell_prefix(b, 8, 8, 8, k)[1]) for k in (2, 4, 8, 16)]
     assert sizes == sorted(sizes) and sizes[-1] == 17 ** 3
+
+
+def test_grid_for_points_rule():
+    from vorothree_amd import grid_for, grid_for_points
+    from vorothree_amd import workloads as W
+    b = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    uni = W.PCG32(91).uniform(200_000, (0, 0, 0), (1, 1, 1))
+    g_u = grid_for_points(b, uni, 2.0)
+    g_ref = grid_for(b, len(uni), 2.0)
+    assert all(abs(a - r) <= 1 for a, r in zip(g_u, g_ref))           # uniform: the uniform rule
+    # clusters over a uniform background (no empty coarse bins): much finer bins, ratio <= 64
+    bg = W.clustered_points(W.PCG32(92), 200_000, 64, 0.02, 0.8)
+    g_c = grid_for_points(b, bg, 2.0)
+    assert g_c[0] > 2 * g_ref[0]
+    assert g_c[0] <= round(g_ref[0] * 64 ** (1 / 3)) + 1
+    # clusters in empty space: the ratio is capped at 2
+    void = W.clustered_points(W.PCG32(94), 200_000, 8, 0.05, 1.0)
+    assert grid_for_points(b, void, 2.0) == grid_for(b, len(void), 1.0)
+    small = uni[:40]
+    assert grid_for_points(b, small, 2.0) == grid_for(b, 40, 2.0)      # too few points to measure
+    box = BoundingBox((0.0, 0.0, 0.0), (2.0, 1.0, 0.5))
+    g_a = grid_for_points(box, W.PCG32(93).uniform(100_000, (0, 0, 0), (2.0, 1.0, 0.5)), 2.0)
+    assert g_a[0] > g_a[1] > g_a[2]                                    # proportional to the edges

@@ -125,3 +125,27 @@ def test_autotune_picks_a_candidate():
     o.calculate(threads=8)
     vol, _, _, _ = o.reduced()
     assert np.array_equal(bits(b.volumes(t)), bits(vol))
+
+
+def test_sparse_member_extends_only_its_prefix():
+    # one member needs a long shell walk (40 generators in a 48^3 grid), the others do not: the
+    # longer prefix is built for that geometry only and every member still equals the oracle
+    sparse = W.Workload("sparse", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (48, 48, 48),
+                        W.PCG32(61).uniform(40, (0, 0, 0), (1, 1, 1)))
+    dense = [W.Workload(f"dense-{k}", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (12, 12, 12),
+                        W.PCG32(62 + k).uniform(3000, (0, 0, 0), (1, 1, 1))) for k in range(2)]
+    ws = [dense[0], sparse, dense[1]]
+    b = Batch()
+    for w in ws:
+        b.add(BoundingBox(w.lo, w.hi), w.grid, walls_of(w), w.points)
+    b.calculate()
+    assert b.stats()[2] > 0, "the shell prefix was never extended"
+    for t, w in enumerate(ws):
+        o = oracle_of(w, w.grid)
+        o.calculate(threads=8)
+        vol, cen, nf, _ = o.reduced()
+        assert np.array_equal(bits(b.volumes(t)), bits(vol)), t
+        assert np.array_equal(bits(b.centroids(t)), bits(cen)), t
+        off, ids = b.neighbors(t)
+        for i in range(0, len(vol), 11):
+            assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), (t, i)

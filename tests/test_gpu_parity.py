@@ -12,7 +12,7 @@ import pytest
 from conftest import GOLDEN, assert_same_cell, bits, read_generators, read_voropp_vertices
 from oracle import OracleTess
 from vorothree_amd import (Algorithm3DGrid, BoundingBox, PlaneGeometry, SphereGeometry,
-                           Tessellation, Wall, WALL_ID_MAX)
+                           Tessellation, Wall, WALL_ID_MAX, shell_prefix)
 from vorothree_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -215,3 +215,20 @@ def test_sparse_grid_extends_shell_prefix():
     vol = check_all_cells(g, o)
     assert g.stats()[2] > 0, "the shell prefix was never extended"
     assert abs(vol.sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("lo,hi,grid,k", [
+    ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (16, 16, 16), 7),
+    ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (20, 20, 20), 64),       # the whole table
+    ((0.0, 0.0, 0.0), (2.0, 0.7, 1.3), (13, 5, 9), 9),          # anisotropic bins
+    ((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), (40, 40, 40), 28),
+])
+def test_device_shell_prefix_equals_host(lo, hi, grid, k):
+    # calculate() builds its prefixes on the device (keys + stable radix sort); the host build is
+    # checked against the oracle's stable order in the CPU suite
+    b = BoundingBox(lo, hi)
+    d_host, k_host, n_host = shell_prefix(b, *grid, k)
+    d_dev, k_dev, n_dev = shell_prefix(b, *grid, k, device=True)
+    assert np.array_equal(d_host, d_dev)
+    assert np.array_equal(bits(k_host), bits(k_dev))
+    assert bits(np.array([n_host])) == bits(np.array([n_dev]))

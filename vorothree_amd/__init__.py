@@ -16,5 +16,6 @@ from .tessellation import (  # noqa: F401
     Wall,
     box_side,
     grid_for,
+    grid_for_points,
     shell_prefix,
 )

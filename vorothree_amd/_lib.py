@@ -54,6 +54,7 @@ SIGNATURES = {
     "vd_tess_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
     "vd_tess_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_shell_prefix": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _sz, _vp, _vp]),
+    "vd_shell_prefix_device": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _sz, _vp, _vp]),
     "vd_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
     "vd_batch_destroy": (None, [_vp]),
     "vd_batch_clear": (ctypes.c_int, [_vp]),
@@ -73,6 +74,7 @@ SIGNATURES = {
     "vd_batch_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_batch_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "vd_grid_for": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_double, _vp]),
+    "vd_grid_for_points": (ctypes.c_int, [_vp, _vp, _vp, _sz, ctypes.c_double, _vp]),
     "vd_server_create": (ctypes.c_int, [_sz, ctypes.c_int, ctypes.POINTER(_vp)]),
     "vd_server_destroy": (None, [_vp]),
     "vd_server_submit": (ctypes.c_int, [_vp, _vp, _vp, _u32, _u32, _u32, _vp, ctypes.c_int, _vp, _sz, _u32, _vp]),
@@ -82,6 +84,7 @@ SIGNATURES = {
     "vd_server_result": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _vp, _vp, _sz]),
     "vd_server_release": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "vd_server_stats": (ctypes.c_int, [_vp, _vp]),
+    "vd_server_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
     "vd_last_error": (ctypes.c_char_p, []),
 }
 

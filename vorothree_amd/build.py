@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvoronoid_hip.so")
-SOURCES = ["vd_api.cpp", "vd_grid.hip", "vd_cells.hip"]
+SOURCES = ["vd_api.cpp", "vd_grid.hip", "vd_shell.hip", "vd_cells.hip"]
 ARCH = os.environ.get("VD_OFFLOAD_ARCH", "gfx950")
 
 

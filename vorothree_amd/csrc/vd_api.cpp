@@ -22,7 +22,7 @@
 
 namespace {
 
-constexpr int kTiers = 3;   // polytope capacity tiers of the cell kernel (vd_cells.hip)
+constexpr int kTiers = 4;   // polytope capacity tiers of the cell kernel (vd_cells.hip)
 
 thread_local std::string g_err;
 
@@ -75,62 +75,71 @@ void release(DevBuf &b) {
 // ------------------------------------------------------------------------------------------
 // Shell order prefix (Algorithm3DGrid::new, algo_3d_grid.rs:47-95, with a stable sort).
 // ------------------------------------------------------------------------------------------
+struct ShellEntryHost {
+    double d2;
+    int16_t dx, dy, dz;
+};
+
 struct ShellPrefix {
-    std::vector<vd::ShellEntry> e;
+    std::vector<ShellEntryHost> e;
     double next = std::numeric_limits<double>::infinity();
     bool complete = false;
 };
 
-inline double axis_term(int64_t d, double cs) {  // algo_3d_grid.rs:61-66
-    if (d > 0) return (double)(d - 1) * cs;
-    if (d < 0) return (double)(-d - 1) * cs;
-    return 0.0;
-}
 
-// All entries with key <= T, T = ((kmax - 1) * max cs)^2, in stable order; `next` is the smallest
-// key outside the prefix.
-void build_shell(const int64_t n[3], const double cs[3], uint32_t kmax, ShellPrefix &out) {
+// The enumeration box of a prefix: all entries with key <= T, T = ((kmax - 1) * max cs)^2, lie in
+// [-K, K] per axis; `outside` is the smallest key of an offset beyond the box (+inf if the box
+// covers the whole table).
+struct ShellBox {
+    int64_t K[3];
+    double T;
+    double outside;
+};
+
+ShellBox shell_box(const int64_t n[3], const double cs[3], uint32_t kmax) {
+    ShellBox b;
     const double csm = std::max(cs[0], std::max(cs[1], cs[2]));
     const double tm = (double)(kmax > 0 ? kmax - 1 : 0) * csm;
-    const double T = tm * tm;
-    int64_t K[3];
+    b.T = tm * tm;
+    b.outside = std::numeric_limits<double>::infinity();
     for (int a = 0; a < 3; ++a) {
         int64_t k = 1;
         while (k < n[a]) {
-            const double m = axis_term(k + 1, cs[a]);
-            if (m * m > T) break;
+            const double m = vd::shell_axis(k + 1, cs[a]);
+            if (m * m > b.T) break;
             ++k;
         }
-        K[a] = std::min<int64_t>(k, n[a]);
+        b.K[a] = std::min<int64_t>(k, n[a]);
+        // the smallest key beyond the box has one axis at K+1 and the others at 0
+        if (b.K[a] < n[a]) {
+            const double m = vd::shell_axis(b.K[a] + 1, cs[a]);
+            b.outside = std::min(b.outside, m * m);
+        }
     }
+    return b;
+}
+
+// Host construction of the same prefix (used by vd_shell_prefix; the engine builds on the device).
+void build_shell(const int64_t n[3], const double cs[3], uint32_t kmax, ShellPrefix &out) {
+    const ShellBox b = shell_box(n, cs, kmax);
+    const int64_t *K = b.K;
     struct Tmp { double d2; uint64_t gen; int16_t dx, dy, dz; };
     std::vector<Tmp> all;
     all.reserve((size_t)(2 * K[0] + 1) * (2 * K[1] + 1) * (2 * K[2] + 1));
-    double next = std::numeric_limits<double>::infinity();
-    const uint64_t wx = 2 * (uint64_t)n[0] + 1, wy = 2 * (uint64_t)n[1] + 1;
+    double next = b.outside;
+    uint64_t g = 0;
     for (int64_t z = -K[2]; z <= K[2]; ++z)
         for (int64_t y = -K[1]; y <= K[1]; ++y)
-            for (int64_t x = -K[0]; x <= K[0]; ++x) {
-                const double mx = axis_term(x, cs[0]), my = axis_term(y, cs[1]), mz = axis_term(z, cs[2]);
-                const double d2 = mx * mx + my * my + mz * mz;
-                if (d2 <= T) {
-                    const uint64_t g = ((uint64_t)(z + n[2]) * wy + (uint64_t)(y + n[1])) * wx + (uint64_t)(x + n[0]);
-                    all.push_back({d2, g, (int16_t)x, (int16_t)y, (int16_t)z});
-                } else {
-                    next = std::min(next, d2);
-                }
+            for (int64_t x = -K[0]; x <= K[0]; ++x, ++g) {
+                const double d2 = vd::shell_key((int)x, (int)y, (int)z, cs);
+                if (d2 <= b.T) all.push_back({d2, g, (int16_t)x, (int16_t)y, (int16_t)z});
+                else next = std::min(next, d2);
             }
-    // entries beyond the enumerated box: the smallest has one axis at K+1 and the others at 0
-    for (int a = 0; a < 3; ++a)
-        if (K[a] < n[a]) {
-            const double m = axis_term(K[a] + 1, cs[a]);
-            next = std::min(next, m * m);
-        }
     std::sort(all.begin(), all.end(), [](const Tmp &p, const Tmp &q) {
         return p.d2 < q.d2 || (p.d2 == q.d2 && p.gen < q.gen);
     });
     out.e.resize(all.size());
-    for (size_t k = 0; k < all.size(); ++k) out.e[k] = vd::ShellEntry{all[k].d2, all[k].dx, all[k].dy, all[k].dz, 0};
+    for (size_t k = 0; k < all.size(); ++k) out.e[k] = ShellEntryHost{all[k].d2, all[k].dx, all[k].dy, all[k].dz};
     out.next = next;
     out.complete = std::isinf(next);
 }
@@ -153,6 +162,49 @@ void grid_for(const double lo[3], const double hi[3], size_t n, double per_bin, 
     for (int a = 0; a < 3; ++a) { e[a] = hi[a] - lo[a]; vol *= e[a]; }
     const double h = std::cbrt(vol * per_bin / (double)std::max<size_t>(n, 1));
     for (int a = 0; a < 3; ++a) out[a] = std::min<int64_t>(16384, std::max<int64_t>(1, (int64_t)std::llround(e[a] / h)));
+}
+
+// Grid for uneven input: bins sized so that the generators per bin seen by the average generator
+// is about per_bin. That density is the mean over generators of the other generators per unit
+// volume in its coarse bin, sum(c (c - 1)) / n / v over a histogram of about 8 generators per bin
+// (at most 64 bins per axis): about n / volume for uniform input. The ratio to n / volume is
+// clamped to [1, kDensityCap], and to [1, kVoidCap] when more than kVoidFrac of the coarse bins
+// are empty: cells next to empty space reach across it, and the walk over its bins grows with
+// the cube of the bin count per axis.
+constexpr double kDensityCap = 64.0;
+constexpr double kVoidCap = 2.0;
+constexpr double kVoidFrac = 0.25;
+
+double density_ratio(const double lo[3], const double hi[3], const double *xyz, size_t n) {
+    if (n < 64) return 1.0;
+    int64_t cg[3];
+    grid_for(lo, hi, n, 8.0, cg);
+    double e[3];
+    for (int a = 0; a < 3; ++a) { cg[a] = std::min<int64_t>(cg[a], 64); e[a] = hi[a] - lo[a]; }
+    std::vector<uint32_t> cnt((size_t)(cg[0] * cg[1] * cg[2]), 0u);
+    for (size_t i = 0; i < n; ++i) {
+        int64_t b[3];
+        for (int a = 0; a < 3; ++a) {
+            const double u = (xyz[3 * i + a] - lo[a]) / e[a] * (double)cg[a];
+            b[a] = u >= 0.0 ? std::min<int64_t>((int64_t)u, cg[a] - 1) : 0;
+        }
+        cnt[(size_t)((b[2] * cg[1] + b[1]) * cg[0] + b[0])] += 1u;
+    }
+    double s2 = 0.0;
+    size_t empty = 0;
+    for (uint32_t c : cnt) {
+        s2 += (double)c * (double)(c > 0 ? c - 1 : 0);
+        empty += c == 0;
+    }
+    // (s2 / n / v_bin) / (n / volume) with v_bin = volume / bins
+    const double ratio = s2 * (double)cnt.size() / ((double)n * (double)n);
+    const double cap = (double)empty > kVoidFrac * (double)cnt.size() ? kVoidCap : kDensityCap;
+    return std::min(cap, std::max(1.0, ratio));
+}
+
+void density_grid(const double lo[3], const double hi[3], const double *xyz, size_t n, double per_bin,
+                  int64_t out[3]) {
+    grid_for(lo, hi, n, per_bin / density_ratio(lo, hi, xyz, n), out);
 }
 
 int make_wall(const vd_wall &w, vd::WallDev &out) {
@@ -187,28 +239,42 @@ struct Member {
     uint64_t raw_off = 0, nraw = 0, gen_off = 0, ngen = 0, bin_off = 0, nbins = 0;
 };
 
-struct Shape {   // one shell prefix per distinct grid geometry
-    int64_t n[3];
-    double cs[3];
-    ShellPrefix sp;
-    uint32_t off = 0;
+// A shell order prefix resident in device memory, one per distinct grid geometry. Its radius
+// parameter k starts small and doubles only for geometries whose cells walked past the prefix.
+struct ShellTable {
+    DevBuf buf;                 // ShellCode[count]
+    vd::ShellPack pack{};
+    uint32_t k = 0;
+    uint32_t count = 0;
+    double next = std::numeric_limits<double>::infinity();
+    bool complete = false;
+    uint64_t epoch = 0;         // last engine pass that used it
 };
 
-using ShapeKey = std::tuple<int64_t, int64_t, int64_t, uint64_t, uint64_t, uint64_t, uint32_t>;
+using GeomKey = std::tuple<int64_t, int64_t, int64_t, uint64_t, uint64_t, uint64_t>;
+
+struct Shape {   // one distinct grid geometry of the current members
+    int64_t n[3];
+    double cs[3];
+    ShellTable *tab = nullptr;
+};
+
+constexpr uint32_t kShellK0 = 7;                       // initial prefix radius (bins)
+constexpr size_t kShellCacheBytes = size_t(4) << 30;   // cached tables beyond this are evicted (LRU)
 
 struct Engine {
     std::vector<Member> mem;
     std::vector<Shape> shapes;
-    std::map<ShapeKey, ShellPrefix> shell_cache;   // prefixes built so far, by grid and k
-    uint32_t shell_k = 7;
+    std::map<GeomKey, ShellTable> tables;   // device shell prefixes by grid geometry
+    uint64_t epoch = 0;
     int device = 0, cus = 1;
-    int occ[kTiers] = {1, 1, 1};
+    int occ[kTiers] = {1, 1, 1, 1};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int keep_cells = 1;
 
     DevBuf d_desc, d_offs, d_raw, d_raw_tess, d_flags, d_pos, d_scan, d_gat;
-    DevBuf d_gen, d_gen_tess, d_gen_bin, d_counts, d_bin_start, d_fill, d_items, d_spos, d_shell;
+    DevBuf d_gen, d_gen_tess, d_gen_bin, d_counts, d_bin_start, d_fill, d_items, d_spos, d_shell_tmp, d_mflags;
     DevBuf d_vol, d_cen, d_nface, d_nvert, d_status, d_cell_off;
     DevBuf d_vtx, d_fcnt, d_fnb, d_fidx, d_top, d_queue, d_counts4, d_next;
     uint64_t nraw = 0, ngen = 0, nbins = 0;
@@ -234,10 +300,13 @@ void engine_free(Engine &E) {
     if (E.stream) (void)hipStreamSynchronize(E.stream);
     DevBuf *bufs[] = {&E.d_desc, &E.d_offs, &E.d_raw, &E.d_raw_tess, &E.d_flags, &E.d_pos, &E.d_scan,
                       &E.d_gat, &E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start,
-                      &E.d_fill, &E.d_items, &E.d_spos, &E.d_shell, &E.d_vol, &E.d_cen, &E.d_nface,
+                      &E.d_fill, &E.d_items, &E.d_spos, &E.d_shell_tmp, &E.d_mflags, &E.d_vol, &E.d_cen, &E.d_nface,
                       &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
                       &E.d_top, &E.d_queue, &E.d_counts4, &E.d_next};
     for (DevBuf *b : bufs) release(*b);
+    for (auto &kv : E.tables) release(kv.second.buf);
+    E.tables.clear();
+    E.shapes.clear();
     if (E.ev0) (void)hipEventDestroy(E.ev0);
     if (E.ev1) (void)hipEventDestroy(E.ev1);
     if (E.stream) (void)hipStreamDestroy(E.stream);
@@ -252,9 +321,53 @@ int scan(Engine &E, const uint32_t *in, uint32_t *out, uint32_t n) {
     return VD_OK;
 }
 
-// Builds (with the current shell_k) the shell prefix of every distinct grid.
+// Builds the prefix with radius parameter k of one geometry on the device (algo_3d_grid.rs:47-95).
+int bits_for(uint64_t v) {   // bits to hold 0 .. v - 1
+    int b = 0;
+    while (b < 32 && (1ull << b) < v) ++b;
+    return b;
+}
+
+int build_table(Engine &E, const int64_t n[3], const double cs[3], uint32_t k, ShellTable &t) {
+    const ShellBox b = shell_box(n, cs, k);
+    const uint64_t w = (uint64_t)(2 * b.K[0] + 1) * (uint64_t)(2 * b.K[1] + 1) * (uint64_t)(2 * b.K[2] + 1);
+    const int bx = bits_for(2 * b.K[0] + 1), by = bits_for(2 * b.K[1] + 1), bz = bits_for(2 * b.K[2] + 1);
+    if (w >= 0x7fffffffull || bx + by + bz > 32)
+        return fail(VD_ECAPACITY, "shell order prefix too large (grid too sparse for the walk radius)");
+    vd::ShellPack pack{};
+    for (int a = 0; a < 3; ++a) pack.k[a] = (int16_t)b.K[a];
+    pack.sy = (uint8_t)bx;
+    pack.sz = (uint8_t)(bx + by);
+    const size_t sb = vd::shell_scratch_bytes((uint32_t)w);
+    VD_TRY(ensure(E.d_shell_tmp, sb));
+    VD_HIP(vd::shell_count(pack, cs, b.T, (uint32_t)w, E.d_shell_tmp.p, E.stream));
+    uint32_t count = 0;
+    VD_HIP(hipMemcpyAsync(&count, vd::shell_counter(E.d_shell_tmp.p, (uint32_t)w), 4, hipMemcpyDeviceToHost,
+                          E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    release(t.buf);
+    const size_t body = ((size_t)std::max<uint32_t>(count, 1) * sizeof(vd::ShellCode) + 15) & ~(size_t)15;
+    VD_TRY(ensure(t.buf, body + 16));
+    uint64_t *d_next = reinterpret_cast<uint64_t *>(t.buf.as<char>() + body);
+    VD_HIP(vd::shell_finish(pack, cs, (uint32_t)w, count, E.d_shell_tmp.p, sb, t.buf.as<vd::ShellCode>(), d_next,
+                            E.stream));
+    uint64_t nb = 0;
+    VD_HIP(hipMemcpyAsync(&nb, d_next, 8, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    double next;
+    std::memcpy(&next, &nb, 8);
+    t.pack = pack;
+    t.k = k;
+    t.count = count;
+    t.next = std::min(next, b.outside);
+    t.complete = std::isinf(t.next);
+    return VD_OK;
+}
+
+// Groups the members by grid geometry and makes sure every geometry has a device prefix.
 int engine_shells(Engine &E) {
     E.shapes.clear();
+    E.epoch += 1;
     for (Member &m : E.mem) {
         int found = -1;
         for (size_t k = 0; k < E.shapes.size(); ++k) {
@@ -270,29 +383,27 @@ int engine_shells(Engine &E) {
         }
         m.shape = found;
     }
-    size_t total = 0;
     for (Shape &s : E.shapes) {
         uint64_t c[3];
         std::memcpy(c, s.cs, sizeof(c));
-        const ShapeKey key{s.n[0], s.n[1], s.n[2], c[0], c[1], c[2], E.shell_k};
-        auto it = E.shell_cache.find(key);
-        if (it == E.shell_cache.end()) {
-            if (E.shell_cache.size() > 65536) E.shell_cache.clear();
-            it = E.shell_cache.emplace(key, ShellPrefix()).first;
-            build_shell(s.n, s.cs, E.shell_k, it->second);
-        }
-        s.sp = it->second;
-        s.off = (uint32_t)total;
-        total += s.sp.e.size();
+        ShellTable &t = E.tables[GeomKey{s.n[0], s.n[1], s.n[2], c[0], c[1], c[2]}];
+        t.epoch = E.epoch;
+        if (!t.k) VD_TRY(build_table(E, s.n, s.cs, kShellK0, t));
+        s.tab = &t;
     }
-    std::vector<vd::ShellEntry> all;
-    all.reserve(total);
-    for (const Shape &s : E.shapes) all.insert(all.end(), s.sp.e.begin(), s.sp.e.end());
-    VD_TRY(ensure(E.d_shell, std::max<size_t>(total, 1) * sizeof(vd::ShellEntry)));
-    if (total)
-        VD_HIP(hipMemcpyAsync(E.d_shell.p, all.data(), total * sizeof(vd::ShellEntry), hipMemcpyHostToDevice,
-                              E.stream));
-    VD_HIP(hipStreamSynchronize(E.stream));
+    // least recently used tables of other geometries go first when the cache grows too large
+    size_t bytes = 0;
+    for (auto &kv : E.tables) bytes += kv.second.buf.bytes;
+    while (bytes > kShellCacheBytes) {
+        auto victim = E.tables.end();
+        for (auto it = E.tables.begin(); it != E.tables.end(); ++it)
+            if (it->second.epoch != E.epoch && (victim == E.tables.end() || it->second.epoch < victim->second.epoch))
+                victim = it;
+        if (victim == E.tables.end()) break;
+        bytes -= victim->second.buf.bytes;
+        release(victim->second.buf);
+        E.tables.erase(victim);
+    }
     return VD_OK;
 }
 
@@ -310,10 +421,11 @@ int upload_desc(Engine &E) {
         d.bin_off = m.bin_off;
         d.nraw = (uint32_t)m.nraw;
         d.ngen = (uint32_t)m.ngen;
-        const Shape &s = E.shapes[m.shape];
-        d.shell_off = s.off;
-        d.nshell = (uint32_t)s.sp.e.size();
-        d.shell_next_d2 = s.sp.next;
+        const ShellTable &tab = *E.shapes[m.shape].tab;
+        d.shell = tab.buf.as<vd::ShellCode>();
+        d.pack = tab.pack;
+        d.nshell = tab.count;
+        d.shell_next_d2 = tab.next;
         d.nwalls = (int32_t)m.walls.size();
         for (size_t k = 0; k < m.walls.size(); ++k) d.walls[k] = m.walls[k];
     }
@@ -422,7 +534,6 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.gen = E.d_gen.as<double>();
     a.spos = E.d_spos.as<double4>();
     a.bin_start = E.d_bin_start.as<uint32_t>();
-    a.shell = E.d_shell.as<vd::ShellEntry>();
     a.cells = d_list;
     a.ncells = nlist;
     a.queue = E.d_queue.as<uint32_t>();
@@ -525,11 +636,26 @@ int engine_calculate(Engine &E) {
             if (arena_full) break;
             if (counts[4 * (NT - 1)]) return fail(VD_ECAPACITY, "a cell exceeds the largest polytope tier");
             if (shell) {
-                bool all_complete = true;
-                for (const Shape &s : E.shapes) all_complete &= s.sp.complete;
-                if (all_complete) return fail(VD_EDEVICE, "shell walk past a complete table");
-                E.shell_k *= 2;
-                VD_TRY(engine_shells(E));
+                // the geometries of the cells that walked past their prefix get a longer one
+                const size_t M = E.mem.size();
+                VD_TRY(ensure(E.d_mflags, (M + 1) * 4));
+                VD_HIP(vd::launch_fill_u32(E.d_mflags.as<uint32_t>(), 0u, M + 1, E.stream));
+                for (int tier = 0; tier < NT; ++tier)
+                    VD_HIP(vd::launch_mark_members(out[2 * tier + 1].as<uint32_t>(), counts[4 * tier + 1],
+                                                   E.d_gen_tess.as<uint32_t>(), E.d_mflags.as<uint32_t>(),
+                                                   E.stream));
+                std::vector<uint32_t> flags(M);
+                VD_HIP(hipMemcpyAsync(flags.data(), E.d_mflags.p, M * 4, hipMemcpyDeviceToHost, E.stream));
+                VD_HIP(hipStreamSynchronize(E.stream));
+                std::vector<char> grow(E.shapes.size(), 0);
+                for (size_t t = 0; t < M; ++t)
+                    if (flags[t]) grow[E.mem[t].shape] = 1;
+                for (size_t k = 0; k < E.shapes.size(); ++k) {
+                    if (!grow[k]) continue;
+                    Shape &s = E.shapes[k];
+                    if (s.tab->complete) return fail(VD_EDEVICE, "shell walk past a complete table");
+                    VD_TRY(build_table(E, s.n, s.cs, s.tab->k * 2, *s.tab));
+                }
                 VD_TRY(upload_desc(E));
                 E.stats[2] += 1;
             }
@@ -566,7 +692,7 @@ int engine_calculate(Engine &E) {
     VD_HIP(hipStreamSynchronize(E.stream));
     E.calculated = true;
     size_t entries = 0;
-    for (const Shape &s : E.shapes) entries += s.sp.e.size();
+    for (const Shape &s : E.shapes) entries += s.tab->count;
     E.stats[3] = (double)entries;
     E.stats[5] = std::chrono::duration<double, std::micro>(clock::now() - t0).count();
     return VD_OK;
@@ -742,6 +868,48 @@ int vd_shell_prefix(const double bmin[3], const double bmax[3], uint32_t nx, uin
     }
     if (count) *count = sp.e.size();
     if (next_key) *next_key = sp.next;
+    return VD_OK;
+}
+
+int vd_shell_prefix_device(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny, uint32_t nz,
+                           uint32_t kmax, int32_t *dxyz, double *keys, size_t cap, size_t *count,
+                           double *next_key) {
+    VD_TRY(check_grid(bmin, bmax, nx, ny, nz));
+    const int64_t n[3] = {nx, ny, nz};
+    const vd::GridGeom g = make_grid(bmin, bmax, n);
+    Engine E;   // only its stream (the null stream) and scratch buffer are used
+    ShellTable t;
+    int rc = build_table(E, n, g.cs, kmax, t);
+    std::vector<vd::ShellCode> e(t.count);
+    if (rc == VD_OK && t.count) {
+        hipError_t h = hipMemcpy(e.data(), t.buf.p, (size_t)t.count * sizeof(vd::ShellCode), hipMemcpyDeviceToHost);
+        if (h != hipSuccess) rc = fail(VD_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(h));
+    }
+    release(t.buf);
+    release(E.d_shell_tmp);
+    if (rc != VD_OK) return rc;
+    const size_t m = std::min(cap, e.size());
+    for (size_t k = 0; k < m; ++k) {
+        int dx, dy, dz;
+        vd::shell_unpack(e[k], t.pack, dx, dy, dz);
+        if (dxyz) {
+            dxyz[3 * k] = dx;
+            dxyz[3 * k + 1] = dy;
+            dxyz[3 * k + 2] = dz;
+        }
+        if (keys) keys[k] = vd::shell_key(dx, dy, dz, g.cs);
+    }
+    if (count) *count = e.size();
+    if (next_key) *next_key = t.next;
+    return VD_OK;
+}
+
+int vd_grid_for_points(const double bmin[3], const double bmax[3], const double *xyz, size_t n, double per_bin,
+                       uint32_t out[3]) {
+    if (!bmin || !bmax || !out || (!xyz && n) || !(per_bin > 0.0)) return fail(VD_EINVAL, "invalid argument");
+    int64_t g[3];
+    density_grid(bmin, bmax, xyz, n, per_bin, g);
+    for (int a = 0; a < 3; ++a) out[a] = (uint32_t)g[a];
     return VD_OK;
 }
 
@@ -1141,7 +1309,7 @@ int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], u
     for (int a = 0; a < 3; ++a) { J->m.lo[a] = bmin[a]; J->m.hi[a] = bmax[a]; }
     int rc = VD_OK;
     if (nx == 0 && ny == 0 && nz == 0) {
-        grid_for(bmin, bmax, n, S->per_bin, J->m.n);
+        density_grid(bmin, bmax, xyz, n, S->per_bin, J->m.n);
         J->m.auto_grid = true;
     } else {
         rc = check_grid(bmin, bmax, nx, ny, nz);
@@ -1305,6 +1473,16 @@ int vd_server_release(vd_server *S, uint64_t job) {
     delete J;
     S->jobs.erase(it);
     return VD_OK;
+}
+
+int vd_server_set_option(vd_server *S, int key, int64_t value) {
+    if (!S) return fail(VD_EINVAL, "NULL handle");
+    if (key == VD_SOPT_PER_BIN) {
+        if (value <= 0) return fail(VD_EINVAL, "per_bin must be positive");
+        S->per_bin = (double)value / 1000.0;
+        return VD_OK;
+    }
+    return fail(VD_EINVAL, "unknown option");
 }
 
 int vd_server_stats(const vd_server *S, double out[4]) {

@@ -176,8 +176,9 @@ enum ClipRc : int { kNoChange = 0, kModified = 1, kOverflow = -1, kFace255 = -2,
 
 struct Plane { double px, py, pz, nx, ny, nz; };
 
-// The polytope state of one wave; all members are wave-uniform.
-struct Poly { int cur, V, F, S; };
+// The polytope state of one wave; all members are wave-uniform. After a clip that changed the
+// polytope, vertices [0, nk) are bit-exact copies of the kept vertices and [nk, V) are new.
+struct Poly { int cur, V, F, S, nk; };
 
 // Lane v's copy of vertex v (valid while V <= 64).
 struct VReg { double x, y, z; bool ok; };
@@ -220,7 +221,7 @@ __device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi
     }
     if (l < 7) m.fstart[0][l] = (uint16_t)(4 * l);
     if (l < 6) m.fnb[0][l] = -(int32_t)((0x214365u >> (4 * l)) & 0xf);  // -5 -6 -3 -4 -1 -2
-    P.cur = 0; P.V = 8; P.F = 6; P.S = 24;
+    P.cur = 0; P.V = 8; P.F = 6; P.S = 24; P.nk = 0;
     __syncthreads();
 }
 
@@ -264,7 +265,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     }
     if (gtm == 0) return kNoChange;     // all inside (:317-319)
     if (ltm == 0) {                     // all outside (:320-326)
-        P.V = P.F = P.S = 0;
+        P.V = P.F = P.S = P.nk = 0;
         r2 = 0.0;
         return kModified;
     }
@@ -465,6 +466,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     P.V = nk + nint;
     P.F = Fn;
     P.S = Spre;
+    P.nk = nk;
     r2 = withgen ? wave_max(md2) : 0.0;
     return kModified;
 }
@@ -488,7 +490,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
     if (gtm == 0) return kNoChange;
     const uint64_t ltm = ballot(va && d < -kTol);
     if (ltm == 0) {
-        P.V = P.F = P.S = 0;
+        P.V = P.F = P.S = P.nk = 0;
         r2 = 0.0;
         r.ok = true;
         return kModified;
@@ -692,6 +694,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
     P.V = nk + nint;
     P.F = Fn;
     P.S = Spre;
+    P.nk = nk;
     r2 = withgen ? wave_max(md2) : 0.0;
     vreg_load(m, P, r);
     return kModified;
@@ -791,6 +794,93 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
     }
 }
 
+// Candidate pre-test: lane w tests its candidate's bisector plane (p = g + d / 2, n = d, exactly
+// the plane the clip builds) against vertices [v0, v1) of the current polytope, with the clip's
+// own expression for the signed distance. A candidate with no vertex beyond kTol would leave the
+// polytope unchanged (kNoChange), so the walk may skip its clip. Kept vertices of a later cut are
+// bit-exact copies, so after a cut only the new vertices [nk, V) need testing.
+template <class C>
+__device__ inline uint64_t pretest(Mem<C> &m, const Poly &P, bool act, double d0, double d1, double d2,
+                                   double gx, double gy, double gz, int v0, int v1) {
+    bool cut = false;
+    if (act) {
+        const double px = gx + d0 * 0.5, py = gy + d1 * 0.5, pz = gz + d2 * 0.5;
+        const double *vx = m.vx[P.cur], *vy = m.vy[P.cur], *vz = m.vz[P.cur];
+        for (int v = v0; v < v1; ++v) {
+            const double d = (vx[v] - px) * d0 + (vy[v] - py) * d1 + (vz[v] - pz) * d2;
+            cut |= d > kTol;
+        }
+    }
+    return ballot(cut);
+}
+
+// Consecutive no-change clips after which the fast tier switches the pre-test on for the cell.
+#ifndef VD_PRETEST_AFTER
+#define VD_PRETEST_AFTER 32
+#endif
+constexpr int kPretestAfter = VD_PRETEST_AFTER;
+
+// Whether the fast tier also uses the skip-ahead probe (the generic tiers always do).
+#ifndef VD_PROBE_FAST
+#define VD_PROBE_FAST 0
+#endif
+
+// Target bin of shell offset (dx, dy, dz) for a generator in bin (ix, iy, iz) at relative position
+// (rx, ry, rz): false if outside the grid, else its index and the squared distance from the
+// generator to the bin (algo_3d_grid.rs:175-190).
+__device__ inline bool entry_bin(int dx, int dy, int dz, const GridGeom &G, int ix, int iy, int iz, double rx,
+                                 double ry, double rz, uint32_t &b, double &bdist) {
+    const int bx = ix + dx, by = iy + dy, bz = iz + dz;
+    if (!(bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2])) return false;
+    b = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] + (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
+    double ex = dx > 0 ? ((double)dx - rx) * G.cs[0] : dx < 0 ? ((double)(-(dx + 1)) + rx) * G.cs[0] : 0.0;
+    double ey = dy > 0 ? ((double)dy - ry) * G.cs[1] : dy < 0 ? ((double)(-(dy + 1)) + ry) * G.cs[1] : 0.0;
+    double ez = dz > 0 ? ((double)dz - rz) * G.cs[2] : dz < 0 ? ((double)(-(dz + 1)) + rz) * G.cs[2] : 0.0;
+    ex = fmax(ex, 0.0); ey = fmax(ey, 0.0); ez = fmax(ez, 0.0);
+    bdist = ex * ex + ey * ey + ez * ez;
+    return true;
+}
+
+// Skip-ahead over a stretch of the walk with no candidates (cells that reach into empty space):
+// probes kProbe batches of 64 entries from e0 with independent loads and returns how many whole
+// batches are inert under the limit lim = 4 r^2: no entry ends the walk (key > lim or end of the
+// prefix) and no entry has a non-empty in-grid bin within the distance test. Skipping them is
+// exact: the walk would visit them without a clip, so the radius and the order are unchanged.
+constexpr int kProbe = 8;
+
+__device__ inline int probe_inert(const ShellCode *shell, uint32_t nshell, const ShellPack &pk, uint32_t e0,
+                                  const GridGeom &G, const uint32_t *bin_start, int ix, int iy, int iz,
+                                  double rx, double ry, double rz, double lim) {
+    const int l = lane_id();
+    ShellCode sc[kProbe];
+#pragma unroll
+    for (int j = 0; j < kProbe; ++j) {
+        const uint32_t e = e0 + 64u * (uint32_t)j + (uint32_t)l;
+        sc[j] = shell[e < nshell ? e : 0u];
+    }
+    uint32_t lo[kProbe], hi[kProbe];
+    bool live[kProbe];
+#pragma unroll
+    for (int j = 0; j < kProbe; ++j) {
+        const uint32_t e = e0 + 64u * (uint32_t)j + (uint32_t)l;
+        int dx, dy, dz;
+        shell_unpack(sc[j], pk, dx, dy, dz);
+        uint32_t b = 0;
+        double bdist = 0.0;
+        live[j] = e >= nshell || shell_key(dx, dy, dz, G.cs) > lim;
+        lo[j] = hi[j] = 0u;
+        if (!live[j] && entry_bin(dx, dy, dz, G, ix, iy, iz, rx, ry, rz, b, bdist) && bdist <= lim) {
+            lo[j] = bin_start[b];
+            hi[j] = bin_start[b + 1];
+        }
+    }
+    int first = kProbe;
+#pragma unroll
+    for (int j = kProbe - 1; j >= 0; --j)
+        if (ballot(live[j] || hi[j] > lo[j])) first = j;
+    return first;
+}
+
 // One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports. The
 // walls and the neighbour candidates feed a single clip site (one inlined copy of the clip).
 template <class C>
@@ -800,7 +890,8 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
     const uint32_t li = (uint32_t)(i - D.gen_off);   // index within its tessellation
     const GridGeom &G = D.grid;
-    const ShellEntry *shell = A.shell + D.shell_off;
+    const ShellCode *shell = D.shell;
+    const ShellPack pk = D.pack;
     const uint32_t nshell = D.nshell;
     const double shell_next = D.shell_next_d2;
     const uint32_t *bin_start = A.bin_start + D.bin_off;
@@ -831,6 +922,10 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     uint32_t wj = 0xffffffffu;
     double wd0 = 0.0, wd1 = 0.0, wd2 = 0.0, wdsq = 0.0;
     uint64_t wvm = 0;       // window lanes whose entry is still live
+    int inert = 0;          // consecutive whole batches that held no candidate
+    bool pre = !C::FAST;    // pre-test the window's candidates (always in the generic tiers)
+    int noc = 0;            // consecutive candidate clips without a change
+    uint64_t wcm = ~0ull;   // window lanes whose candidate may cut (pre-test), all without it
 
     int wi = 0;             // next wall (tessellation.rs:429-436), then the neighbour walk
     bool walking = false;
@@ -871,30 +966,30 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
             }
             if (!win) {
                 if (load) {
+                    if ((!C::FAST || VD_PROBE_FAST) && inert >= 2) {
+                        // a run of batches without candidates: skip inert batches in bulk
+                        const double plim = 4.0 * r2;
+                        for (;;) {
+                            const int f = probe_inert(shell, nshell, pk, e0, G, bin_start, ix, iy, iz, rx, ry,
+                                                      rz, plim);
+                            e0 += 64u * (uint32_t)f;
+                            if (f < kProbe) break;
+                        }
+                    }
                     const uint32_t e = e0 + (uint32_t)l;
                     bend = e >= nshell;
                     binr = false;
                     bc = 0;
                     bdist = 0.0;
                     if (!bend) {
-                        const ShellEntry se = shell[e];
-                        bd2 = se.d2;
-                        const int bx = ix + se.dx, by = iy + se.dy, bz = iz + se.dz;
-                        binr = bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2];
+                        int dx, dy, dz;
+                        shell_unpack(shell[e], pk, dx, dy, dz);
+                        bd2 = shell_key(dx, dy, dz, G.cs);
+                        uint32_t b = 0;
+                        binr = entry_bin(dx, dy, dz, G, ix, iy, iz, rx, ry, rz, b, bdist);
                         if (binr) {
-                            const uint32_t b = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] +
-                                               (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
                             bs = bin_start[b];
                             bc = bin_start[b + 1] - bs;
-                            // algo_3d_grid.rs:182-190
-                            double ex = se.dx > 0 ? ((double)se.dx - rx) * G.cs[0]
-                                      : se.dx < 0 ? ((double)(-(se.dx + 1)) + rx) * G.cs[0] : 0.0;
-                            double ey = se.dy > 0 ? ((double)se.dy - ry) * G.cs[1]
-                                      : se.dy < 0 ? ((double)(-(se.dy + 1)) + ry) * G.cs[1] : 0.0;
-                            double ez = se.dz > 0 ? ((double)se.dz - rz) * G.cs[2]
-                                      : se.dz < 0 ? ((double)(-(se.dz + 1)) + rz) * G.cs[2] : 0.0;
-                            ex = fmax(ex, 0.0); ey = fmax(ey, 0.0); ez = fmax(ez, 0.0);
-                            bdist = ex * ex + ey * ey + ez * ez;
                         }
                     } else {
                         bd2 = (e == nshell) ? shell_next : DBL_MAX;
@@ -913,6 +1008,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
                 const uint32_t inc = wave_incl_scan(cnt);
                 const uint32_t pos = inc - cnt;
                 wtot = rlu(inc, 63);
+                inert = (wtot == 0u && kbc == 64 && k0 == 0) ? inert + 1 : 0;
                 const int nwin = wtot < 64u ? (int)wtot : 64;
                 if (take) {
                     for (uint32_t t = 0; t < cnt && pos + t < 64u; ++t) {
@@ -934,6 +1030,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
                     wdsq = wd0 * wd0 + wd1 * wd1 + wd2 * wd2;   // tessellation.rs:447-456
                 }
                 wvm = ballot(wa);
+                wcm = pre ? pretest<C>(m, P, wa && wj != li, wd0, wd1, wd2, gx, gy, gz, 0, P.V) : ~0ull;
                 last = -1;
                 win = true;
                 __syncthreads();
@@ -941,7 +1038,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
             // next candidate of the window that passes the distance test (tessellation.rs:458)
             const double lim = 4.0 * r2;
             const uint64_t above = last < 0 ? ~0ull : (last >= 63 ? 0ull : ~((2ull << last) - 1ull));
-            const uint64_t pm = wvm & above & ballot(wj != li && !(wdsq > lim));
+            const uint64_t pm = wvm & wcm & above & ballot(wj != li && !(wdsq > lim));
             if (pm == 0) {
                 // window exhausted: find where the walk continues
                 win = false;
@@ -992,7 +1089,11 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
             if (P.V == 0) break;        // tessellation.rs:433-435
             continue;
         }
-        if (rc == kNoChange) continue;
+        if (rc == kNoChange) {
+            if (++noc >= kPretestAfter) pre = true;   // takes effect at the next window
+            continue;
+        }
+        noc = 0;
         // a cut (tessellation.rs:462-469)
         if (P.V == 0) nr2 = 0.0;
         const uint32_t koc = rlu(wko, cl);
@@ -1007,6 +1108,11 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
             continue;
         }
         // the radius shrank or stayed: re-filter the rest of the window with it
+        if (pre && wcm != ~0ull) {
+            // candidates cleared by the pre-test are tested against the new vertices only
+            const uint64_t todo = wvm & ~wcm & (cl >= 63 ? 0ull : ~((2ull << cl) - 1ull));
+            if (todo) wcm |= pretest<C>(m, P, (todo >> l) & 1ull, wd0, wd1, wd2, gx, gy, gz, P.nk, P.V);
+        }
         const double lim = 4.0 * r2;
         kbc = ffs64(ballot((l > kc) && (bend || bd2 > lim)));
         const int wk = (int)(wko & 0xffu);
@@ -1086,32 +1192,35 @@ void k_cells(CellArgs A) {
     }
 }
 
-// Tier 0: register fast path, small LDS (many waves per CU). Tier 1: generic path for cells that
-// outgrow tier 0. Tier 2: generic path with the largest LDS polytope (one wave per CU).
+// Tier 0: register fast path, small LDS (many waves per CU). Tiers 1-3: generic path for cells
+// that outgrow the tier before, with 128, 256 and 1024 vertices (the last one wave per CU).
 #ifndef VD_TIER0_WAVES
 #define VD_TIER0_WAVES 4
 #endif
 using Tier0 = Cap<64, 192, 48, 64, true, 107, VD_TIER0_WAVES>;
-using Tier1 = Cap<256, 768, 128, 384, false, 255, 1>;
-using Tier2 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
+using Tier1 = Cap<128, 384, 72, 192, false, 127, 2>;
+using Tier2 = Cap<256, 768, 128, 384, false, 255, 1>;
+using Tier3 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
 
 }  // namespace
 
-int tier_count() { return 3; }
+int tier_count() { return 4; }
 
 int tier_max_vertices(int tier) {
-    return tier == 0 ? Tier0::VM : tier == 1 ? Tier1::VM : Tier2::VM;
+    return tier == 0 ? Tier0::VM : tier == 1 ? Tier1::VM : tier == 2 ? Tier2::VM : Tier3::VM;
 }
 
 size_t tier_lds_bytes(int tier) {
-    return tier == 0 ? sizeof(Mem<Tier0>) : tier == 1 ? sizeof(Mem<Tier1>) : sizeof(Mem<Tier2>);
+    return tier == 0 ? sizeof(Mem<Tier0>) : tier == 1 ? sizeof(Mem<Tier1>)
+         : tier == 2 ? sizeof(Mem<Tier2>) : sizeof(Mem<Tier3>);
 }
 
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st) {
     if (a.ncells == 0) return hipSuccess;
     if (tier == 0) k_cells<Tier0><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 1) k_cells<Tier1><<<grid_waves, 64, 0, st>>>(a);
-    else k_cells<Tier2><<<grid_waves, 64, 0, st>>>(a);
+    else if (tier == 2) k_cells<Tier2><<<grid_waves, 64, 0, st>>>(a);
+    else k_cells<Tier3><<<grid_waves, 64, 0, st>>>(a);
     return hipGetLastError();
 }
 
@@ -1120,7 +1229,9 @@ hipError_t cells_occupancy(int tier, int *blocks_per_cu) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0>, 64, 0);
     if (tier == 1)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1>, 64, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2>, 64, 0);
+    if (tier == 2)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier3>, 64, 0);
 }
 
 }  // namespace vd

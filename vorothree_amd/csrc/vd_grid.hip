@@ -148,6 +148,13 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
     if (i < n) p[i] = v;
 }
 
+// Members of the listed cells (for the per-geometry shell extension after a launch).
+__global__ void k_mark_members(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ gen_tess,
+                               uint32_t *__restrict__ flags) {
+    const uint64_t g = gid();
+    if (g < n) flags[gen_tess[list[g]]] = 1u;
+}
+
 // Sorts the listed cells by outcome of a cell launch: capacity overflow -> out_cap, shell prefix
 // too short -> out_shell; counts[0..3] = capacity, shell, arena-full, hard errors.
 __global__ void k_collect(const int32_t *__restrict__ status, const uint32_t *__restrict__ list,
@@ -304,6 +311,13 @@ hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nv
                         hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_relax<<<blocks_for(n), kBlock, 0, st>>>(gen, cen, nvert, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, uint32_t *flags,
+                               hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_mark_members<<<blocks_for(n), kBlock, 0, st>>>(list, n, gen_tess, flags);
     return hipGetLastError();
 }
 

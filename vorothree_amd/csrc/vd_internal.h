@@ -12,12 +12,34 @@
 
 namespace vd {
 
-// One entry of the bin shell order of Algorithm3DGrid (src/algorithm/algo_3d_grid.rs:56-76):
-// offset (dx, dy, dz) and its minimum squared distance. 16 bytes, loaded 64 at a time.
-struct alignas(16) ShellEntry {
-    double d2;
-    int16_t dx, dy, dz, pad;
+// One entry of the bin shell order of Algorithm3DGrid (src/algorithm/algo_3d_grid.rs:56-76): the
+// offset (dx, dy, dz) packed in 32 bits as (dx + K0) | (dy + K1) << sy | (dz + K2) << sz with the
+// prefix's enumeration box [-K, K]. Its key (the squared minimum bin distance) is recomputed from
+// the offset with the expression the table was sorted by.
+using ShellCode = uint32_t;
+
+struct ShellPack {
+    int16_t k[3];     // K per axis
+    uint8_t sy, sz;   // bit positions of dy and dz
 };
+
+__host__ __device__ inline void shell_unpack(ShellCode c, const ShellPack &p, int &dx, int &dy, int &dz) {
+    dx = (int)(c & ((1u << p.sy) - 1u)) - p.k[0];
+    dy = (int)((c >> p.sy) & ((1u << (p.sz - p.sy)) - 1u)) - p.k[1];
+    dz = (int)(c >> p.sz) - p.k[2];
+}
+
+// algo_3d_grid.rs:61-66 (one axis of get_min_dist_sq) and the key of an offset
+__host__ __device__ inline double shell_axis(int64_t d, double cs) {
+    if (d > 0) return (double)(d - 1) * cs;
+    if (d < 0) return (double)(-d - 1) * cs;
+    return 0.0;
+}
+
+__host__ __device__ inline double shell_key(int dx, int dy, int dz, const double cs[3]) {
+    const double mx = shell_axis(dx, cs[0]), my = shell_axis(dy, cs[1]), mz = shell_axis(dz, cs[2]);
+    return mx * mx + my * my + mz * mz;
+}
 
 // A wall as the kernels see it (src/wall/wall_3d.rs:8-101). kind 0 = plane, 1 = sphere.
 struct WallDev {
@@ -47,11 +69,12 @@ struct TessDesc {
     uint64_t gen_off;       // first generator (after the wall filter) = first output slot
     uint64_t bin_off;       // first bin of this member in the concatenated bin arrays
     uint32_t nraw, ngen;
-    uint32_t shell_off;     // its shell prefix in the concatenated shell array
+    const ShellCode *shell;   // shell order prefix of this member's grid geometry
     uint32_t nshell;
-    double shell_next_d2;   // key of the first entry not in the prefix (+inf if complete)
     int32_t nwalls;
-    int32_t pad;
+    double shell_next_d2;   // key of the first entry not in the prefix (+inf if complete)
+    ShellPack pack;
+    uint8_t pad[8];
     WallDev walls[kMaxWalls];
 };
 
@@ -62,7 +85,6 @@ struct CellArgs {
     const double *gen;            // generators, xyz AoS, members concatenated
     const double4 *spos;          // generators in bin order: x, y, z, member-local index (u64 bits)
     const uint32_t *bin_start;    // concatenated bins + 1 (positions in spos)
-    const ShellEntry *shell;      // concatenated shell prefixes
 
     // work list: generator indices (global) to compute
     const uint32_t *cells;
@@ -118,12 +140,26 @@ hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n
                             const uint32_t *tess, double4 *spos, hipStream_t st);
 hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint64_t n,
                         double *out, hipStream_t st);
+hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, uint32_t *flags,
+                               hipStream_t st);   // flags[gen_tess[list[i]]] = 1
 hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n,
                           uint32_t *out_cap, uint32_t *out_shell, uint32_t *counts, hipStream_t st);
 
+// ---- shell order prefix (vd_shell.hip) ------------------------------------------------------
+// Offsets of the box [-K, K] (w = prod(2K + 1) of them) with key <= T, sorted stably by key.
+// shell_count: keys of the box and the number kept (read it from shell_counter after the stream
+// syncs); shell_finish: sort and write the `count` kept entries to out (packed with pack),
+// next[0] = bits of the first key after them (+inf if none). scratch holds shell_scratch_bytes(w).
+size_t shell_scratch_bytes(uint32_t w);
+hipError_t shell_count(const ShellPack &pack, const double cs[3], double T, uint32_t w, void *scratch,
+                       hipStream_t st);
+const uint32_t *shell_counter(const void *scratch, uint32_t w);
+hipError_t shell_finish(const ShellPack &pack, const double cs[3], uint32_t w, uint32_t count, void *scratch,
+                        size_t scratch_bytes, ShellCode *out, uint64_t *next, hipStream_t st);
+
 // ---- cell kernel (vd_cells.hip) ------------------------------------------------------------
-// tier: 0 = register fast path with a small LDS polytope (many waves per CU), 1 = generic path
-// with a medium LDS polytope, 2 = generic path with the largest LDS polytope (one wave per CU)
+// tier: 0 = register fast path with a small LDS polytope (many waves per CU); 1, 2, 3 = generic
+// path with LDS polytopes of 128, 256 and 1024 vertices (the last one wave per CU)
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st);
 hipError_t cells_occupancy(int tier, int *blocks_per_cu);
 int tier_count();

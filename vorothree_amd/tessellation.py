@@ -316,18 +316,29 @@ class Tessellation:
         return out
 
 
-def shell_prefix(bounds: BoundingBox, nx: int, ny: int, nz: int, kmax: int):
-    """Host-side stable shell-order prefix (see vd_shell_prefix in include/vd.h)."""
+def shell_prefix(bounds: BoundingBox, nx: int, ny: int, nz: int, kmax: int, device: bool = False):
+    """Stable shell-order prefix (vd_shell_prefix in include/vd.h), built on the host, or with
+    ``device=True`` on the GPU the way calculate() builds it (vd_shell_prefix_device)."""
+    fn = lib().vd_shell_prefix_device if device else lib().vd_shell_prefix
     count = ctypes.c_size_t(0)
     nxt = ctypes.c_double(0.0)
-    check(lib().vd_shell_prefix(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, kmax, None, None, 0,
-                                ctypes.byref(count), ctypes.byref(nxt)))
+    check(fn(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, kmax, None, None, 0,
+             ctypes.byref(count), ctypes.byref(nxt)))
     m = count.value
     dxyz = np.empty((m, 3), dtype=np.int32)
     keys = np.empty(m, dtype=np.float64)
-    check(lib().vd_shell_prefix(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, kmax, _ptr(dxyz),
-                                _ptr(keys), m, ctypes.byref(count), ctypes.byref(nxt)))
+    check(fn(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, kmax, _ptr(dxyz),
+             _ptr(keys), m, ctypes.byref(count), ctypes.byref(nxt)))
     return dxyz, keys, nxt.value
+
+
+def grid_for_points(bounds: BoundingBox, points, per_bin: float):
+    """The grid rule for uneven input (vd_grid_for_points): per_bin generators per bin as seen by
+    the average generator."""
+    a = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1))
+    out = np.zeros(3, dtype=np.uint32)
+    check(lib().vd_grid_for_points(_d3(bounds.min), _d3(bounds.max), _ptr(a), a.size // 3, float(per_bin), _ptr(out)))
+    return tuple(int(x) for x in out)
 
 
 def grid_for(bounds: BoundingBox, n: int, per_bin: float):
@@ -475,10 +486,12 @@ class Server:
 
     QUEUED, RUNNING, FINISHED, REFUSED = 0, 1, 2, 3
 
-    def __init__(self, hbm_budget: int, max_slots: int = 64):
+    def __init__(self, hbm_budget: int, max_slots: int = 64, per_bin: float | None = None):
         h = ctypes.c_void_p()
         check(lib().vd_server_create(int(hbm_budget), int(max_slots), ctypes.byref(h)))
         self._h = h
+        if per_bin is not None:
+            check(lib().vd_server_set_option(h, 1, int(round(per_bin * 1000))))
 
     def close(self):
         h = getattr(self, "_h", None)

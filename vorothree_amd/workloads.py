@@ -201,3 +201,35 @@ def config4(count: int = 4096, seed: int = 4, nmin: int = 64, nmax: int = 65536,
         pts = g.uniform(n, lo, hi)
         out.append(Workload(f"batch-{t}", lo, hi, grid_for(n, lo, hi, per_bin), pts, walls))
     return out
+
+
+@dataclass
+class Request:
+    """One request of the served workload."""
+    arrival: float          # seconds after the stream starts
+    points: np.ndarray
+    clustered: bool
+    iterations: int         # 1 = plain tessellation, else Lloyd iterations
+
+
+def config5(count: int = 100, rate: float = 2000.0, seed: int = 5, nmin: int = 1000,
+            nmax: int = 2_000_000) -> list:
+    """Served workload: Poisson arrivals at ``rate`` requests/s (inter-arrival -ln(1-u)/rate),
+    generator counts log-uniform in [nmin, nmax] in the unit cube, 50% uniform / 50% from 8
+    Gaussian clusters (sigma 0.05, out-of-box draws re-drawn), 30% Lloyd jobs of U{10..50}
+    iterations. Only the first ``count`` requests of the stream are generated."""
+    g = PCG32(seed)
+    out = []
+    t = 0.0
+    ratio = math.log(nmax / nmin)
+    for _ in range(count):
+        u = g.u01(4)
+        t += -math.log(1.0 - u[0]) / rate
+        n = min(max(int(math.floor(nmin * math.exp(u[1] * ratio))), nmin), nmax)
+        clustered = u[2] < 0.5
+        iters = 1
+        if u[3] < 0.3:
+            iters = 10 + int(g.u01(1)[0] * 41)
+        pts = clustered_points(g, n, 8, 0.05, 1.0) if clustered else g.uniform(n, (0, 0, 0), (1, 1, 1))
+        out.append(Request(t, pts, clustered, iters))
+    return out
