@@ -50,19 +50,20 @@ def parse():
     p.add_argument("--serve-requests", type=int, default=100)
     p.add_argument("--serve-rate", type=float, default=2000.0)
     p.add_argument("--serve-budget-gb", type=float, default=64.0)
+    p.add_argument("--serve-sweep-requests", type=int, default=30)
+    p.add_argument("--serve-sweep-rates", type=str, default="5,50,500")
+    p.add_argument("--no-clustered", action="store_true", help="skip the 4M clustered workload")
+    p.add_argument("--clustered-n", type=int, default=4_000_000)
     return p.parse_args()
 
 
-def served(count: int, rate: float, budget_gb: float) -> dict:
-    """Served workload: the first ``count`` requests of a Poisson stream (rate req/s) of
-    tessellations, 1e3..2e6 generators, 50% clustered, 30% Lloyd jobs of 10-50 iterations. A
-    real-time loop submits every request when its arrival time has passed and steps the server
-    (continuous batching); latency = finish - arrival on the monotonic clock."""
+def serve_run(reqs, rate: float, budget_gb: float) -> dict:
+    """One real-time run of the server over ``reqs``: every request is submitted once its arrival
+    time has passed, the server is stepped (continuous batching) while work is pending;
+    latency = finish - arrival on the monotonic clock."""
     from vorothree_amd import BoundingBox, Server
     from vorothree_amd import workloads as W
 
-    reqs = W.config5(count=count, rate=rate)
-    print(f"serve: {len(reqs)} requests generated", file=sys.stderr, flush=True)
     unit = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
     s = Server(hbm_budget=int(budget_gb * (1 << 30)), max_slots=64)
     # warm the kernels and allocator once with a small job (not counted)
@@ -78,7 +79,8 @@ def served(count: int, rate: float, budget_gb: float) -> dict:
     while done < len(reqs):
         now = time.monotonic() - t0
         if now - last > 10.0:
-            print(f"serve: t={now:.1f} s, {done} of {len(reqs)} finished", file=sys.stderr, flush=True)
+            print(f"serve {rate:g} req/s: t={now:.1f} s, {done} of {len(reqs)} finished", file=sys.stderr,
+                  flush=True)
             last = now
         while nxt < len(reqs) and reqs[nxt].arrival <= now:
             r = reqs[nxt]
@@ -90,7 +92,6 @@ def served(count: int, rate: float, budget_gb: float) -> dict:
         elif nxt < len(reqs):
             time.sleep(max(0.0, reqs[nxt].arrival - (time.monotonic() - t0)))
     wall = time.monotonic() - t0
-    print(f"serve: done in {wall:.2f} s", file=sys.stderr, flush=True)
     lat = []
     cells = 0
     for r, j in zip(reqs, ids):
@@ -100,15 +101,89 @@ def served(count: int, rate: float, budget_gb: float) -> dict:
     lat = np.asarray(lat)
     st = s.stats()
     s.close()
-    return {"workload": f"first {count} requests of a Poisson stream at {rate:g} req/s: 1e3..2e6 "
-                        "generators (log-uniform) in the unit cube, 50% uniform / 50% 8-cluster "
-                        "sigma 0.05, 30% Lloyd jobs of 10-50 iterations; HBM budget "
-                        f"{budget_gb:g} GiB, 64 slots",
-            "requests": count, "lloyd_jobs": int(sum(r.iterations > 1 for r in reqs)),
-            "cells": int(cells), "wall_s": wall, "cells_per_s": cells / wall,
-            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
-            "latency_max_s": float(lat.max()), "server_steps": st["steps"],
-            "note": "the stream offers far more work than one GPU serves, so latency is mostly queueing"}
+    print(f"serve {rate:g} req/s: done in {wall:.2f} s", file=sys.stderr, flush=True)
+    return {"rate_req_s": rate, "requests": len(reqs), "cells": int(cells), "wall_s": wall,
+            "cells_per_s": cells / wall, "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)), "latency_max_s": float(lat.max()),
+            "server_steps": st["steps"]}
+
+
+def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_rates) -> dict:
+    """Served workload: the first ``count`` requests of a Poisson stream (rate req/s) of
+    tessellations, 1e3..2e6 generators, 50% clustered, 30% Lloyd jobs of 10-50 iterations, jobs
+    without a grid (the server picks it with vd_grid_for_points). Then a rate sweep over the first
+    ``sweep_count`` requests of the same stream (same generators, arrival times rescaled)."""
+    import dataclasses
+
+    from vorothree_amd import workloads as W
+
+    reqs = W.config5(count=max(count, sweep_count), rate=rate)
+    print(f"serve: {len(reqs)} requests generated", file=sys.stderr, flush=True)
+    main_run = serve_run(reqs[:count], rate, budget_gb)
+    sweep = []
+    for r in sweep_rates:
+        scaled = [dataclasses.replace(q, arrival=q.arrival * rate / r) for q in reqs[:sweep_count]]
+        sweep.append(serve_run(scaled, r, budget_gb))
+    out = {"workload": f"first {count} requests of a Poisson stream at {rate:g} req/s: 1e3..2e6 "
+                       "generators (log-uniform) in the unit cube, 50% uniform / 50% 8-cluster "
+                       "sigma 0.05, 30% Lloyd jobs of 10-50 iterations; grid per job by "
+                       f"vd_grid_for_points at 2 per bin; HBM budget {budget_gb:g} GiB, 64 slots",
+           "lloyd_jobs": int(sum(q.iterations > 1 for q in reqs[:count])),
+           "note": "the offered load is far above one GPU's capacity at 2000 req/s, so its latency "
+                   "is mostly queueing; the sweep lowers the rate on the first requests"}
+    out.update(main_run)
+    out["sweep"] = sweep
+    return out
+
+
+def clustered(n: int, targets, steps: int) -> dict:
+    """Uneven density: n generators, 80% from 64 Gaussian clusters (sigma 0.02) + 20% uniform, in
+    the unit cube, no walls. The grid is tuned over generators-per-bin targets of the density rule
+    (vd_grid_for_points); the best is timed over ``steps`` calculate() calls."""
+    from vorothree_amd import Algorithm3DGrid, BoundingBox, Tessellation, grid_for_points, _hip
+    from vorothree_amd import workloads as W
+
+    w = W.config3(n=n)
+    b = BoundingBox(w.lo, w.hi)
+    dpts = _hip.DeviceArray(w.points)
+    _hip.synchronize()
+    tried = []
+    best = None
+    for tgt in targets:
+        grid = grid_for_points(b, w.points, tgt)
+        t = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=False)
+        t.set_generators_device(dpts.ptr.value, n)
+        t.calculate()                              # builds the shell prefix
+        _hip.synchronize()
+        c0 = time.perf_counter()
+        t.calculate()
+        dt = time.perf_counter() - c0
+        tried.append({"per_bin_target": tgt, "grid": list(grid), "calc_ms": dt * 1e3})
+        if best is None or dt < best[0]:
+            best = (dt, tgt, grid)
+        t.close()
+    _, tgt, grid = best
+    t = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=False)
+    t.set_generators_device(dpts.ptr.value, n)
+    t.calculate()
+    _hip.synchronize()
+    c0 = time.perf_counter()
+    for _ in range(steps):
+        t.set_generators_device(dpts.ptr.value, n)
+        t.calculate()
+    _hip.synchronize()
+    dt = (time.perf_counter() - c0) / steps
+    st = t.stats()
+    cells = t.count_cells()
+    out = {"workload": f"unit cube, {n} f64 generators: 80% from 64 Gaussian clusters (sigma 0.02, "
+                       "re-drawn inside the box) + 20% uniform, seed 3, no walls",
+           "grid": list(grid), "per_bin_target": tgt, "tuning": tried, "cells": cells,
+           "ms_per_step": dt * 1e3, "cells_per_s": cells / dt,
+           "path": "set_generators(device) + calculate(), reduced outputs",
+           "cells_first_tier": int(st[0]), "cell_runs_larger_tiers": int(st[1]),
+           "volume_sum": float(np.sum(t.volumes()))}
+    t.close()
+    return out
 
 
 def batched_vs_plain(count: int, steps: int) -> dict:
@@ -247,25 +322,38 @@ def main():
         cdt = time.perf_counter() - c0
         _, _, _, work = o.reduced()
         flops_per_cell = float(work[:, 0].mean())
-        cpu = {"value": o.count_cells() / cdt, "unit": "cells/s", "cores": a.cpu_threads,
+        ncpu = o.count_cells()
+        sub = np.arange(0, ncpu, max(1, ncpu // 50_000), dtype=np.uint32)
+        c0 = time.perf_counter()
+        o.calculate(subset=sub, threads=1)
+        sdt = time.perf_counter() - c0
+        cpu = {"value": ncpu / cdt, "unit": "cells/s", "cores": a.cpu_threads,
                "kind": "port",
-               "sample": f"the whole workload: all {o.count_cells()} cells of the timed "
+               "sample": f"the whole workload: all {ncpu} cells of the timed "
                          f"tessellation, one calculate() on {a.cpu_threads} threads "
-                         f"(std::thread, dynamic chunks of 64 cells); host nproc={os.cpu_count()}"}
+                         f"(std::thread, dynamic chunks of 64 cells); host nproc={os.cpu_count()}",
+               "single_thread": {"value": len(sub) / sdt, "unit": "cells/s", "cores": 1,
+                                 "sample": f"every {max(1, ncpu // 50_000)}th cell ({len(sub)} cells)"}}
         del o
 
-    # algorithmic HBM bytes of the launch: generator + bin-list read, cell + reductions written
-    nv = nf = ns = 0
-    if not a.reduced:
-        fc = t.face_counts()
-        nf = int(fc.sum())
+    # algorithmic HBM bytes of the launch: per cell the generator, its bin item and bin offset read
+    # (24 + 4 + 4), volume, centroid, face count, vertex count, status and arena offsets written
+    # (8 + 24 + 4 + 4 + 4 + 24); the cell arena: 24 per vertex, 2 per face index, 1 + 4 per face
+    nf = int(t.face_counts().sum())
+    nv, ns = int(st[6]), int(st[7])
     vol = t.volumes()
-    bytes_launch = cells * (24 + 8 + 8 + 24 + 4 + 4 + 24) + nf * 5
+    bytes_launch = cells * (24 + 4 + 4 + 8 + 24 + 4 + 4 + 4 + 24) + nv * 24 + ns * 2 + nf * 5
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "k_cells_uniform1M_sphere_traffic.json")
+    if os.path.exists(tfile) and a.n == 1_000_000 and a.grid == 100 and not a.reduced:
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roof = None
     if flops_per_cell is not None:
         achieved = cells * flops_per_cell / kern_s / 1e12
         roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                 "kernel": "k_cells<Tier0>", "kernel_ms": kern_s * 1e3,
                 "flops_per_cell": flops_per_cell,
                 "hbm": {"achieved": bytes_launch / kern_s / 1e9, "peak": HBM_PEAK_GBS,
@@ -295,8 +383,11 @@ def main():
     }
     if not a.no_batched:
         out["batched"] = batched_vs_plain(a.batch_count, a.batch_steps)
+    if not a.no_clustered:
+        out["clustered"] = clustered(a.clustered_n, [1.0, 2.0, 4.0], 3)
     if not a.no_serve:
-        out["serve"] = served(a.serve_requests, a.serve_rate, a.serve_budget_gb)
+        rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
+        out["serve"] = served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates)
     print(json.dumps(out), flush=True)
 
 

@@ -105,7 +105,8 @@ int vd_tess_set_option(vd_tess *t, int key, int64_t value);
 
 /* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cell runs
  * in the larger tiers, [2] shell-prefix extensions, [3] entries of the shell prefixes,
- * [4] kernel time in microseconds of the first cell launch, [5] total calculate time (us). */
+ * [4] kernel time in microseconds of the first cell launch, [5] total calculate time (us),
+ * [6] vertices and [7] face-index entries written to the cell arena (full-cell mode, else 0). */
 int vd_tess_get_stats(const vd_tess *t, double out[8]);
 
 /* Host-only (no device needed): the stable bin shell order of Algorithm3DGrid restricted to the

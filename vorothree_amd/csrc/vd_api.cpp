@@ -689,8 +689,12 @@ int engine_calculate(Engine &E) {
         if (!arena_full) break;
         for (int k = 0; k < 3; ++k) E.cap[k] *= 2;
     }
+    uint64_t top[3] = {0, 0, 0};
+    VD_HIP(hipMemcpyAsync(top, E.d_top.p, sizeof(top), hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
     E.calculated = true;
+    E.stats[6] = E.keep_cells ? (double)top[0] : 0.0;
+    E.stats[7] = E.keep_cells ? (double)top[2] : 0.0;
     size_t entries = 0;
     for (const Shape &s : E.shapes) entries += s.tab->count;
     E.stats[3] = (double)entries;
