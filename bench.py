@@ -191,7 +191,8 @@ def batched_vs_plain(count: int, steps: int) -> dict:
     [64, 65536], boxes with edges U[0.5, 2], 30% sphere walls, 20% plane walls, grids at about
     2 generators per bin. Batched: all members in one vd_batch_calculate (wall filter, binning and
     one persistent cell launch for all). Plain: one Tessellation per member, each running its own
-    set_generators + calculate launch sequence. Both start from points already in device memory."""
+    set_generators + calculate launch sequence, eagerly; the same with graph replay of repeated
+    sequences is reported beside it. All start from points already in device memory."""
     from vorothree_amd import (Algorithm3DGrid, Batch, BoundingBox, PlaneGeometry, SphereGeometry,
                                Tessellation, Wall, _hip)
     from vorothree_amd import workloads as W
@@ -218,41 +219,54 @@ def batched_vs_plain(count: int, steps: int) -> dict:
     tb = (time.perf_counter() - t0) / steps
     cells = int(sum(b.info(t)["cells"] for t in range(len(b))))
     kern_b = float(b.stats()[4]) * 1e-6
+    replays_b = int(b.stats()[8])
     b.close()
 
     allpts = np.concatenate([w.points for w in ws], axis=0)
     offs = np.concatenate([[0], np.cumsum([len(w.points) for w in ws])])
     dev = _hip.DeviceArray(allpts)
-    ts = []
-    for w in ws:
-        bb = BoundingBox(w.lo, w.hi)
-        t = Tessellation(bb, Algorithm3DGrid(*w.grid, bb), keep_cells=False)
-        for wl in walls_of(w):
-            t.add_wall(wl)
-        ts.append(t)
 
-    def plain_step():
-        for k, t in enumerate(ts):
-            t.set_generators_device(dev.ptr.value + 24 * int(offs[k]), len(ws[k].points))
-            t.calculate()
+    def plain(graphs):
+        ts = []
+        for w in ws:
+            bb = BoundingBox(w.lo, w.hi)
+            t = Tessellation(bb, Algorithm3DGrid(*w.grid, bb), keep_cells=False, graphs=graphs)
+            for wl in walls_of(w):
+                t.add_wall(wl)
+            ts.append(t)
 
-    plain_step()
-    _hip.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        plain_step()
-    _hip.synchronize()
-    tp = (time.perf_counter() - t0) / steps
-    cells_p = int(sum(t.count_cells() for t in ts))
-    for t in ts:
-        t.close()
+        def plain_step():
+            for k, t in enumerate(ts):
+                t.set_generators_device(dev.ptr.value + 24 * int(offs[k]), len(ws[k].points))
+                t.calculate()
+
+        for _ in range(2 if graphs else 1):
+            plain_step()
+        _hip.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plain_step()
+        _hip.synchronize()
+        tp = (time.perf_counter() - t0) / steps
+        cells_p = int(sum(t.count_cells() for t in ts))
+        for t in ts:
+            t.close()
+        return tp, cells_p
+
+    # each tessellation of the workload is computed once, so the plain path launches eagerly; the
+    # rerun of the same tessellations every step would let them replay graphs, reported apart
+    tp, cells_p = plain(False)
+    tpg, _ = plain(True)
     dev.close()
     return {"workload": f"{count} tessellations, generators log-uniform in [64, 65536], "
                         "boxes U[0.5,2]^3, 30% sphere / 20% plane walls, ~2 generators per bin",
             "generators": int(offs[-1]), "cells": cells, "cells_plain": cells_p,
-            "batched_cells_per_s": cells / tb, "batched_ms": tb * 1e3, "batched_kernel_ms": kern_b * 1e3,
+            "batched_cells_per_s": cells / tb, "batched_ms": tb * 1e3, "batched_first_round_ms": kern_b * 1e3,
+            "batched_graph_replays": replays_b,
             "plain_cells_per_s": cells_p / tp, "plain_ms": tp * 1e3,
-            "speedup_batched_vs_plain": tp / tb, "steps": steps}
+            "speedup_batched_vs_plain": tp / tb,
+            "plain_graph_replay_cells_per_s": cells_p / tpg,
+            "speedup_batched_vs_plain_graph_replay": tpg / tb, "steps": steps}
 
 
 def copy_probe(mib: int) -> None:
@@ -284,7 +298,8 @@ def main():
 
     w = W.config2(n=a.n, grid=a.grid)
     b = BoundingBox(w.lo, w.hi)
-    t = Tessellation(b, Algorithm3DGrid(*w.grid, b), keep_cells=not a.reduced)
+    # eager launches, so the kernel time of the tier-0 launch is measured on its own
+    t = Tessellation(b, Algorithm3DGrid(*w.grid, b), keep_cells=not a.reduced, graphs=False)
     t.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5)))
     dpts = _hip.DeviceArray(w.points)
     _hip.synchronize()
@@ -308,6 +323,24 @@ def main():
     ms = 1000.0 * dt / a.steps
     value = cells / (dt / a.steps)
     kern_s = float(np.mean(kern_us)) * 1e-6
+
+    # the same steps with the first round replayed from a captured graph (end-to-end rate only)
+    tg = Tessellation(b, Algorithm3DGrid(*w.grid, b), keep_cells=not a.reduced, graphs=True)
+    tg.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5)))
+    for _ in range(max(2, a.warmup)):
+        tg.set_generators_device(dpts.ptr.value, a.n)
+        tg.calculate()
+    _hip.synchronize()
+    g0 = time.perf_counter()
+    for _ in range(a.steps):
+        tg.set_generators_device(dpts.ptr.value, a.n)
+        tg.calculate()
+    _hip.synchronize()
+    gdt = time.perf_counter() - g0
+    graph_rate = tg.count_cells() / (gdt / a.steps)
+    graph_replays = int(tg.stats()[8])
+    assert np.array_equal(tg.volumes().view(np.uint64), t.volumes().view(np.uint64))
+    tg.close()
 
     # algorithmic work of one cell launch, counted by the oracle over the same cells
     cpu = None
@@ -377,6 +410,7 @@ def main():
                    "generators": a.n, "cells": cells, "grid": list(w.grid),
                    "keep_cells": not a.reduced, "path": "set_generators(device) + calculate()",
                    "cells_retried_large_tier": st[1], "shell_prefix_entries": st[3],
+                   "graph_replay_cells_per_s": graph_rate, "graph_replays": graph_replays,
                    "volume_sum": float(vol.sum())},
         "roofline": roof,
         "cpu_baseline": cpu,

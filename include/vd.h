@@ -101,13 +101,19 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
 /* Options (no counterpart in the reference; they change what is kept, never the results). */
 #define VD_OPT_KEEP_CELLS 1 /* 1 (default): keep every full cell for get_cell; 0: reductions and
                                neighbour ids only, the output of Tessellation::map */
+#define VD_OPT_GRAPHS 2     /* 1 (default): when calculate() repeats the launch sequence of the
+                               previous call (same sizes, grids, walls and buffers; the generators
+                               may differ), its first round (binning, the fast-tier cell launch,
+                               the retry collect) is captured once into a hipGraph and replayed;
+                               0: always launch eagerly. Results are identical either way. */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
 
 /* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cell runs
  * in the larger tiers, [2] shell-prefix extensions, [3] entries of the shell prefixes,
- * [4] kernel time in microseconds of the first cell launch, [5] total calculate time (us),
- * [6] vertices and [7] face-index entries written to the cell arena (full-cell mode, else 0). */
-int vd_tess_get_stats(const vd_tess *t, double out[8]);
+ * [4] time in microseconds of the first cell launch (of the whole replayed first round when a
+ * graph ran), [5] total calculate time (us), [6] vertices and [7] face-index entries written to
+ * the cell arena (full-cell mode, else 0), [8] graph replays and [9] graph captures so far. */
+int vd_tess_get_stats(const vd_tess *t, double out[10]);
 
 /* Host-only (no device needed): the stable bin shell order of Algorithm3DGrid restricted to the
  * entries whose key is <= ((kmax - 1) * max cell size)^2 (algo_3d_grid.rs:47-95 with a stable
@@ -172,8 +178,9 @@ int vd_batch_get_cell(const vd_batch *b, uint32_t t, size_t i, uint64_t sizes[3]
                       uint32_t *face_counts, uint32_t *face_indices, int32_t *face_neighbors);
 #define VD_BOPT_KEEP_CELLS 1 /* as VD_OPT_KEEP_CELLS; default 0 for batches */
 #define VD_BOPT_PER_BIN 2    /* generators-per-bin target of vd_batch_add's grid choice, x1000 */
+#define VD_BOPT_GRAPHS 3     /* as VD_OPT_GRAPHS (default 1) */
 int vd_batch_set_option(vd_batch *b, int key, int64_t value);
-int vd_batch_get_stats(const vd_batch *b, double out[8]);
+int vd_batch_get_stats(const vd_batch *b, double out[10]);
 /* Times vd_batch_calculate for each candidate generators-per-bin target (grids re-derived with
  * vd_grid_for for the members added with a zero grid), keeps the fastest, and returns it. */
 int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *best);

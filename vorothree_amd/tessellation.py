@@ -191,7 +191,8 @@ class Cell3DFaces:
 class Tessellation:
     """``Tessellation::<3, Cell3DFaces, Algorithm3DGrid>`` on the GPU."""
 
-    def __init__(self, bounds: BoundingBox, algorithm: Algorithm3DGrid, keep_cells: bool = True):
+    def __init__(self, bounds: BoundingBox, algorithm: Algorithm3DGrid, keep_cells: bool = True,
+                 graphs: bool = True):
         self.bounds = bounds
         self.algorithm = algorithm
         self._walls = []
@@ -201,6 +202,8 @@ class Tessellation:
         self._h = h
         if not keep_cells:
             check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_KEEP_CELLS, 0))
+        if not graphs:
+            check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_GRAPHS, 0))
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -244,6 +247,10 @@ class Tessellation:
 
     def set_keep_cells(self, keep: bool) -> None:
         check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_KEEP_CELLS, 1 if keep else 0))
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay repeated first-round launch sequences from a captured hipGraph (VD_OPT_GRAPHS)."""
+        check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_GRAPHS, 1 if on else 0))
 
     # -- queries -------------------------------------------------------------------------------
     def count_generators(self) -> int:
@@ -311,7 +318,7 @@ class Tessellation:
         return [f(self.get_cell(i)) for i in range(self.count_cells())]
 
     def stats(self) -> np.ndarray:
-        out = np.zeros(8, dtype=np.float64)
+        out = np.zeros(10, dtype=np.float64)
         check(lib().vd_tess_get_stats(self._h, _ptr(out)))
         return out
 
@@ -373,12 +380,14 @@ class Batch:
     ``calculate()`` runs every member; results are read per member.
     """
 
-    def __init__(self, keep_cells: bool = False, per_bin: float = 2.0):
+    def __init__(self, keep_cells: bool = False, per_bin: float = 2.0, graphs: bool = True):
         h = ctypes.c_void_p()
         check(lib().vd_batch_create(ctypes.byref(h)))
         self._h = h
         if keep_cells:
             check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_KEEP_CELLS, 1))
+        if not graphs:
+            check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_GRAPHS, 0))
         check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_PER_BIN, int(round(per_bin * 1000))))
 
     def close(self):
@@ -474,7 +483,7 @@ class Batch:
         return Cell3DFaces(int(i), v, fc, fi, fn)
 
     def stats(self) -> np.ndarray:
-        out = np.zeros(8, dtype=np.float64)
+        out = np.zeros(10, dtype=np.float64)
         check(lib().vd_batch_get_stats(self._h, _ptr(out)))
         return out
 
