@@ -32,6 +32,40 @@ extern "C" {
 
 #define VD_WALL_ID_MAX (-1000) /* src/wall/mod.rs:8 */
 
+/* A wall (Wall::new(id, geometry), src/wall/mod.rs:24-31) for vd_tess_add_wall, batch members and
+ * server jobs. id <= VD_WALL_ID_MAX. Per kind (src/wall/wall_3d.rs):
+ *   VD_WALL_PLANE      PlaneGeometry::new(a, b)                     :23-27  (b normalised)
+ *   VD_WALL_SPHERE     SphereGeometry::new(a, r)                    :67-69
+ *   VD_WALL_CYLINDER   CylinderGeometry::new(a, b, r)               :124-128 (b normalised)
+ *   VD_WALL_CONE       ConeGeometry::new(a, b, r = half-angle)      :194-198 (b normalised)
+ *   VD_WALL_TORUS      TorusGeometry::new(a, b, r = major, r2 = minor) :285-289 (b normalised)
+ *   VD_WALL_POLYHEDRON ConvexPolyhedronGeometry { planes }          :462-491; pts = npts planes,
+ *                      6 doubles each (point xyz, outward normal xyz), used as given
+ *   VD_WALL_TUBE       the tube around a sampled curve shared by TrefoilKnotGeometry (:370-456,
+ *                      always closed), CubicBezierGeometry (:718-815) and CatmullRomGeometry
+ *                      (:821-961): pts = npts samples (xyz), r = tube radius, closed = 1 when the
+ *                      last sample joins the first. The samples are the geometry's own `samples`.
+ * pts is read during the call that receives the wall; the library keeps a copy. */
+#define VD_WALL_PLANE 0
+#define VD_WALL_SPHERE 1
+#define VD_WALL_CYLINDER 2
+#define VD_WALL_CONE 3
+#define VD_WALL_TORUS 4
+#define VD_WALL_POLYHEDRON 5
+#define VD_WALL_TUBE 6
+#define VD_WALL_MAX_POINTS 65536 /* planes of a polyhedron or samples of a tube */
+typedef struct vd_wall {
+    int32_t id;
+    int32_t kind;
+    double a[3];
+    double b[3];
+    double r;
+    double r2;
+    const double *pts;
+    uint32_t npts;
+    int32_t closed;
+} vd_wall;
+
 typedef struct vd_tess vd_tess;
 
 /* Tessellation::new(bounds, Algorithm3DGrid::new(nx, ny, nz, &bounds))
@@ -51,6 +85,8 @@ int vd_tess_add_wall_plane(vd_tess *t, int32_t id, const double point[3], const 
 /* add_wall(Wall::new(id, Box::new(SphereGeometry::new(center, radius))))
  *   src/tessellation.rs:197-200, src/wall/wall_3d.rs:66-100. */
 int vd_tess_add_wall_sphere(vd_tess *t, int32_t id, const double center[3], double radius);
+/* add_wall for any wall kind of vd_wall (the same pruning as the two calls above). */
+int vd_tess_add_wall(vd_tess *t, const vd_wall *wall);
 
 /* clear_walls(), src/tessellation.rs:203-205 (generators are kept as they are). */
 int vd_tess_clear_walls(vd_tess *t);
@@ -135,16 +171,6 @@ int vd_shell_prefix_device(const double bmin[3], const double bmax[3], uint32_t 
  * Tessellation::<3, Cell3DFaces, Algorithm3DGrid> computed in one persistent launch) ---------- */
 
 typedef struct vd_batch vd_batch;
-
-/* A wall of a batch member. kind 0: PlaneGeometry::new(a, b) (b is normalised as
- * src/wall/wall_3d.rs:23-27 does); kind 1: SphereGeometry::new(a, r). id <= VD_WALL_ID_MAX. */
-typedef struct vd_wall {
-    int32_t id;
-    int32_t kind;
-    double a[3];
-    double b[3];
-    double r;
-} vd_wall;
 
 int vd_batch_create(vd_batch **out);
 void vd_batch_destroy(vd_batch *b);

@@ -39,6 +39,8 @@ def _load():
         "or_tess_free": (None, [_vp]),
         "or_tess_add_wall_plane": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
         "or_tess_add_wall_sphere": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_double]),
+        "or_tess_add_wall": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                            ctypes.c_double, _dp, ctypes.c_uint32, ctypes.c_int]),
         "or_tess_set_generators": (None, [_vp, _dp, ctypes.c_size_t]),
         "or_tess_count_generators": (ctypes.c_size_t, [_vp]),
         "or_tess_get_generators": (None, [_vp, _dp]),
@@ -130,6 +132,19 @@ class OracleTess:
     def add_wall_sphere(self, wid, center, radius):
         if self._lib.or_tess_add_wall_sphere(self._h, int(wid), _d3(center), float(radius)) != 0:
             raise ValueError("Wall ID must be <= -1000")
+
+    def add_wall(self, wid, kind, a=(0.0, 0.0, 0.0), b=(0.0, 0.0, 0.0), r=0.0, r2=0.0, pts=None,
+                 closed=False):
+        """Any wall kind (or_tess_add_wall): 2 cylinder (centre, axis, radius), 3 cone (tip, axis,
+        half-angle), 4 torus (centre, axis, major, minor), 5 convex polyhedron (pts: planes as
+        point + outward normal rows), 6 tube (pts: samples, r: tube radius, closed)."""
+        p = np.ascontiguousarray(np.zeros(1) if pts is None else np.asarray(pts, dtype=np.float64).reshape(-1))
+        per = 6 if kind == 5 else 3
+        npts = 0 if pts is None else p.size // per
+        rc = self._lib.or_tess_add_wall(self._h, int(wid), int(kind), _d3(a), _d3(b), float(r), float(r2),
+                                        p.ctypes.data_as(_dp), npts, 1 if closed else 0)
+        if rc != 0:
+            raise ValueError("Wall ID must be <= -1000" if rc == -1 else "unknown wall kind")
 
     def set_generators(self, xyz):
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))

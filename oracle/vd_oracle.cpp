@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -293,32 +294,182 @@ uint64_t poly_triangles(const Poly &c) {
 // ---------------------------------------------------------------------------------------------
 struct Wall {
     int32_t id;
-    int kind;       // 0 plane, 1 sphere
-    double a[3];    // plane point / sphere centre
-    double b[3];    // plane unit normal (pointing into the valid region)
-    double r;       // sphere radius
+    int kind;       // 0 plane, 1 sphere, 2 cylinder, 3 cone, 4 torus, 5 convex polyhedron, 6 tube
+    double a[3];    // plane point / sphere, cylinder, torus centre / cone tip
+    double b[3];    // plane unit normal (pointing into the valid region) / unit axis
+    double r;       // sphere, cylinder, tube radius / torus major radius / cone half-angle
+    double r2;      // torus minor radius
+    std::vector<double> pts;   // polyhedron planes (point, outward normal) / tube samples (xyz)
+    bool closed;               // tube: the last sample joins the first
 };
 
-bool wall_contains(const Wall &w, const double q[3]) {
-    double dx = q[0] - w.a[0], dy = q[1] - w.a[1], dz = q[2] - w.a[2];
-    if (w.kind == 0) return (dx * w.b[0] + dy * w.b[1] + dz * w.b[2]) >= 0.0;  // wall_3d.rs:31-36
-    return (dx * dx + dy * dy + dz * dz) <= w.r * w.r;                          // :73-78
+// get_closest_point of the sampled-curve walls: TrefoilKnotGeometry (wall_3d.rs:408-438, always
+// closed), CubicBezierGeometry (:762-797) and CatmullRomGeometry (:908-943) share it.
+void tube_closest(const Wall &w, const double q[3], double c[3]) {
+    const size_t n = w.pts.size() / 3;
+    c[0] = c[1] = c[2] = 0.0;
+    if (n == 0) return;                             // :763-765 (Bezier, Catmull-Rom)
+    double min_dist_sq = DBL_MAX;                   // f64::MAX
+    for (int k = 0; k < 3; ++k) c[k] = w.pts[k];    // samples[0]
+    const size_t limit = w.closed ? n : n - 1;
+    for (size_t i = 0; i < limit; ++i) {
+        const double *p0 = &w.pts[3 * i], *p1 = &w.pts[3 * ((i + 1) % n)];
+        const double v[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const double ww[3] = {q[0] - p0[0], q[1] - p0[1], q[2] - p0[2]};
+        const double c1 = ww[0] * v[0] + ww[1] * v[1] + ww[2] * v[2];
+        const double c2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        double t = 0.0;
+        if (!(c2 <= 0.0)) {
+            t = c1 / c2;
+            if (t < 0.0) t = 0.0;   // f64::clamp(0.0, 1.0)
+            if (t > 1.0) t = 1.0;
+        }
+        const double proj[3] = {p0[0] + v[0] * t, p0[1] + v[1] * t, p0[2] + v[2] * t};
+        const double dx = q[0] - proj[0], dy = q[1] - proj[1], dz = q[2] - proj[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < min_dist_sq) {
+            min_dist_sq = d2;
+            for (int k = 0; k < 3; ++k) c[k] = proj[k];
+        }
+    }
 }
 
-// Emits the cut plane (point, outward normal). Returns false when no plane is emitted.
-bool wall_cut(const Wall &w, const double g[3], double p[3], double n[3], Work *wk) {
-    if (w.kind == 0) {  // wall_3d.rs:38-42
-        for (int k = 0; k < 3; ++k) { p[k] = w.a[k]; n[k] = -w.b[k]; }
-        return true;
+// WallGeometry::contains, src/wall/wall_3d.rs.
+bool wall_contains(const Wall &w, const double q[3]) {
+    double dx = q[0] - w.a[0], dy = q[1] - w.a[1], dz = q[2] - w.a[2];
+    switch (w.kind) {
+    case 0: return (dx * w.b[0] + dy * w.b[1] + dz * w.b[2]) >= 0.0;     // :31-36
+    case 1: return (dx * dx + dy * dy + dz * dz) <= w.r * w.r;           // :73-78
+    case 2: {                                                            // :132-143
+        double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+        double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
+        return (px * px + py * py + pz * pz) <= w.r * w.r;
     }
-    double dx = g[0] - w.a[0], dy = g[1] - w.a[1], dz = g[2] - w.a[2];  // :80-100
-    double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
-    if (dist == 0.0) return false;
-    double sc = w.r / dist;
-    p[0] = w.a[0] + dx * sc; p[1] = w.a[1] + dy * sc; p[2] = w.a[2] + dz * sc;
-    n[0] = dx / dist; n[1] = dy / dist; n[2] = dz / dist;
-    if (wk) wk->flops += 19;
+    case 3: {                                                            // :202-214
+        double h = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+        double px = dx - h * w.b[0], py = dy - h * w.b[1], pz = dz - h * w.b[2];
+        double r = std::sqrt(px * px + py * py + pz * pz);
+        return h >= 0.0 && r <= h * std::tan(w.r);
+    }
+    case 4: {                                                            // :293-309
+        double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+        double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
+        double dist_perp = std::sqrt(px * px + py * py + pz * pz);
+        double t = dist_perp - w.r;
+        double dist_tube = std::sqrt(t * t + dot * dot);                 // powi(2) = square
+        return dist_tube <= w.r2;
+    }
+    case 5:                                                              // :691-701
+        for (size_t k = 0; k + 6 <= w.pts.size(); k += 6) {
+            double ex = q[0] - w.pts[k], ey = q[1] - w.pts[k + 1], ez = q[2] - w.pts[k + 2];
+            if (ex * w.pts[k + 3] + ey * w.pts[k + 4] + ez * w.pts[k + 5] > 0.0) return false;
+        }
+        return true;
+    case 6: {                                                            // :442-446, 801-805, 947-951
+        double c[3];
+        tube_closest(w, q, c);
+        double ex = q[0] - c[0], ey = q[1] - c[1], ez = q[2] - c[2];
+        return ex * ex + ey * ey + ez * ez <= w.r * w.r;
+    }
+    }
     return true;
+}
+
+// WallGeometry::cut: every plane (point, outward normal) the wall emits for generator g, in order.
+template <class Emit>
+void wall_cut(const Wall &w, const double g[3], Emit emit, Work *wk) {
+    double dx = g[0] - w.a[0], dy = g[1] - w.a[1], dz = g[2] - w.a[2];
+    switch (w.kind) {
+    case 0: {  // :38-42
+        double p[3] = {w.a[0], w.a[1], w.a[2]}, n[3] = {-w.b[0], -w.b[1], -w.b[2]};
+        emit(p, n);
+        return;
+    }
+    case 1: {  // :80-100
+        double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (dist == 0.0) return;
+        double sc = w.r / dist;
+        double p[3] = {w.a[0] + dx * sc, w.a[1] + dy * sc, w.a[2] + dz * sc};
+        double n[3] = {dx / dist, dy / dist, dz / dist};
+        if (wk) wk->flops += 19;
+        emit(p, n);
+        return;
+    }
+    case 2: {  // :145-170
+        double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+        double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
+        double dist = std::sqrt(px * px + py * py + pz * pz);
+        if (dist == 0.0) return;
+        double sc = w.r / dist;
+        double p[3] = {w.a[0] + dot * w.b[0] + px * sc, w.a[1] + dot * w.b[1] + py * sc,
+                       w.a[2] + dot * w.b[2] + pz * sc};
+        double n[3] = {px / dist, py / dist, pz / dist};
+        emit(p, n);
+        return;
+    }
+    case 3: {  // :216-258
+        double h = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+        double px = dx - h * w.b[0], py = dy - h * w.b[1], pz = dz - h * w.b[2];
+        double r = std::sqrt(px * px + py * py + pz * pz);
+        if (r == 0.0) return;
+        double rx = px / r, ry = py / r, rz = pz / r;
+        double cos_a = std::cos(w.r), sin_a = std::sin(w.r);
+        double dist = r * cos_a - h * sin_a;
+        double p2d_r = r - dist * cos_a;
+        double p2d_h = h + dist * sin_a;
+        if (p2d_h < 0.0) {
+            double dist_tip = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist_tip == 0.0) return;
+            double p[3] = {w.a[0], w.a[1], w.a[2]}, n[3] = {dx / dist_tip, dy / dist_tip, dz / dist_tip};
+            emit(p, n);
+            return;
+        }
+        double p[3] = {w.a[0] + p2d_h * w.b[0] + p2d_r * rx, w.a[1] + p2d_h * w.b[1] + p2d_r * ry,
+                       w.a[2] + p2d_h * w.b[2] + p2d_r * rz};
+        double n[3] = {cos_a * rx - sin_a * w.b[0], cos_a * ry - sin_a * w.b[1], cos_a * rz - sin_a * w.b[2]};
+        emit(p, n);
+        return;
+    }
+    case 4: {  // :311-363
+        double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+        double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
+        double dist_perp = std::sqrt(px * px + py * py + pz * pz);
+        double ux, uy, uz;
+        if (dist_perp < 1e-9) {
+            double tx = 1.0, ty = 0.0, tz = 0.0;
+            if (std::fabs(w.b[0]) > 0.9) { tx = 0.0; ty = 1.0; }
+            double t_dot = tx * w.b[0] + ty * w.b[1] + tz * w.b[2];
+            double ax = tx - t_dot * w.b[0], ay = ty - t_dot * w.b[1], az = tz - t_dot * w.b[2];
+            double len = std::sqrt(ax * ax + ay * ay + az * az);
+            if (len == 0.0) return;
+            ux = ax / len; uy = ay / len; uz = az / len;
+        } else {
+            ux = px / dist_perp; uy = py / dist_perp; uz = pz / dist_perp;
+        }
+        double cx = w.a[0] + ux * w.r, cy = w.a[1] + uy * w.r, cz = w.a[2] + uz * w.r;
+        double vx = g[0] - cx, vy = g[1] - cy, vz = g[2] - cz;
+        double dist_c = std::sqrt(vx * vx + vy * vy + vz * vz);
+        if (dist_c == 0.0) return;
+        double n[3] = {vx / dist_c, vy / dist_c, vz / dist_c};
+        double p[3] = {cx + n[0] * w.r2, cy + n[1] * w.r2, cz + n[2] * w.r2};
+        emit(p, n);
+        return;
+    }
+    case 5:    // :703-707
+        for (size_t k = 0; k + 6 <= w.pts.size(); k += 6) emit(&w.pts[k], &w.pts[k + 3]);
+        return;
+    case 6: {  // :448-456, 807-815, 953-961
+        double c[3];
+        tube_closest(w, g, c);
+        double ex = g[0] - c[0], ey = g[1] - c[1], ez = g[2] - c[2];
+        double dist = std::sqrt(ex * ex + ey * ey + ez * ez);
+        if (dist == 0.0) return;
+        double n[3] = {ex / dist, ey / dist, ez / dist};
+        double p[3] = {c[0] + n[0] * w.r, c[1] + n[1] * w.r, c[2] + n[2] * w.r};
+        emit(p, n);
+        return;
+    }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -481,12 +632,11 @@ void compute_cell(const Tess &T, size_t i, Poly &cell, ClipScratch &s, Work *w, 
     const double *g = &T.gen[3 * i];
     poly_box(cell, T.lo, T.hi);
     for (const Wall &wl : T.walls) {  // :429-436
-        double p[3], n[3];
-        if (wall_cut(wl, g, p, n, w)) {
+        wall_cut(wl, g, [&](const double *p, const double *n) {
             if (w) w->clips++;
             auto r = poly_clip(cell, p, n, wl.id, s, nullptr, w);
             if (w && r.first) w->cuts++;
-        }
+        }, w);
         if (cell.empty()) return;
     }
     double r2 = poly_max_radius_sq(cell, g);  // :438
@@ -588,6 +738,28 @@ int or_tess_add_wall_sphere(void *t, int32_t id, const double c[3], double r) {
     Wall w{};
     w.id = id; w.kind = 1; w.r = r;
     for (int k = 0; k < 3; ++k) w.a[k] = c[k];
+    tess_add_wall(*(Tess *)t, w);
+    return 0;
+}
+
+// Any wall kind: 2 CylinderGeometry::new(a, b, r) (wall_3d.rs:124-128), 3 ConeGeometry::new(a, b,
+// angle r) (:194-198), 4 TorusGeometry::new(a, b, r, r2) (:285-289), 5 ConvexPolyhedronGeometry
+// with npts planes in pts (point, outward normal; :476-491), 6 a sampled-curve tube with npts
+// samples in pts, radius r, closed as given; 0 and 1 as the two calls above.
+int or_tess_add_wall(void *t, int32_t id, int kind, const double a[3], const double b[3], double r, double r2,
+                     const double *pts, uint32_t npts, int closed) {
+    if (id > -1000) return -1;
+    if (kind < 0 || kind > 6) return -2;
+    Wall w{};
+    w.id = id; w.kind = kind; w.r = r; w.r2 = r2; w.closed = closed != 0;
+    for (int k = 0; k < 3; ++k) w.a[k] = a[k];
+    if (kind == 0 || kind == 2 || kind == 3 || kind == 4) {
+        double len = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        if (len == 0.0) { w.b[0] = 0.0; w.b[1] = 0.0; w.b[2] = 1.0; }
+        else { w.b[0] = b[0] / len; w.b[1] = b[1] / len; w.b[2] = b[2] / len; }
+    }
+    if (kind == 5) w.pts.assign(pts, pts + 6 * (size_t)npts);
+    if (kind == 6) w.pts.assign(pts, pts + 3 * (size_t)npts);
     tess_add_wall(*(Tess *)t, w);
     return 0;
 }

@@ -63,3 +63,24 @@ def assert_same_cell(gpu_cell, ora_cell, i):
     assert np.array_equal(gpu_cell.face_neighbors(), ora_cell.face_neighbors), f"cell {i}: neighbours"
     assert np.array_equal(gpu_cell.face_indices(), ora_cell.face_indices), f"cell {i}: face indices"
     assert np.array_equal(bits(gpu_cell.vertices()), bits(ora_cell.vertices)), f"cell {i}: vertices"
+
+
+def oracle_wall(o, wall):
+    """Adds a vorothree_amd Wall to an OracleTess with the same geometry parameters."""
+    g = wall.geometry
+    name = type(g).__name__
+    wid = wall.id()
+    if name == "PlaneGeometry":
+        o.add_wall_plane(wid, g.point, g.normal_raw)
+    elif name == "SphereGeometry":
+        o.add_wall_sphere(wid, g.center, g.radius)
+    elif name == "CylinderGeometry":
+        o.add_wall(wid, 2, g.center, g.axis_raw, g.radius)
+    elif name == "ConeGeometry":
+        o.add_wall(wid, 3, g.tip, g.axis_raw, g.angle)
+    elif name == "TorusGeometry":
+        o.add_wall(wid, 4, g.center, g.axis_raw, g.major_radius, g.minor_radius)
+    elif name == "ConvexPolyhedronGeometry":
+        o.add_wall(wid, 5, pts=g.planes)
+    else:
+        o.add_wall(wid, 6, r=g.tube_radius, pts=g.samples, closed=g.closed)

@@ -8,11 +8,18 @@ from .tessellation import (  # noqa: F401
     Algorithm3DGrid,
     Batch,
     BoundingBox,
+    CatmullRomGeometry,
     Cell3DFaces,
+    ConeGeometry,
+    ConvexPolyhedronGeometry,
+    CubicBezierGeometry,
+    CylinderGeometry,
     PlaneGeometry,
     Server,
     SphereGeometry,
     Tessellation,
+    TorusGeometry,
+    TrefoilKnotGeometry,
     Wall,
     box_side,
     grid_for,

@@ -27,7 +27,12 @@ VD_BOPT_GRAPHS = 3
 class VdWall(ctypes.Structure):
     """struct vd_wall of include/vd.h."""
     _fields_ = [("id", ctypes.c_int32), ("kind", ctypes.c_int32), ("a", ctypes.c_double * 3),
-                ("b", ctypes.c_double * 3), ("r", ctypes.c_double)]
+                ("b", ctypes.c_double * 3), ("r", ctypes.c_double), ("r2", ctypes.c_double),
+                ("pts", ctypes.c_void_p), ("npts", ctypes.c_uint32), ("closed", ctypes.c_int32)]
+
+
+VD_WALL_PLANE, VD_WALL_SPHERE, VD_WALL_CYLINDER, VD_WALL_CONE, VD_WALL_TORUS = 0, 1, 2, 3, 4
+VD_WALL_POLYHEDRON, VD_WALL_TUBE = 5, 6
 
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -40,6 +45,7 @@ SIGNATURES = {
     "vd_tess_destroy": (None, [_vp]),
     "vd_tess_add_wall_plane": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "vd_tess_add_wall_sphere": (ctypes.c_int, [_vp, _i32, _vp, ctypes.c_double]),
+    "vd_tess_add_wall": (ctypes.c_int, [_vp, ctypes.POINTER(VdWall)]),
     "vd_tess_clear_walls": (ctypes.c_int, [_vp]),
     "vd_tess_set_generators": (ctypes.c_int, [_vp, _vp, _sz]),
     "vd_tess_set_generators_device": (ctypes.c_int, [_vp, _vp, _sz]),

@@ -207,21 +207,55 @@ void density_grid(const double lo[3], const double hi[3], const double *xyz, siz
     grid_for(lo, hi, n, per_bin / density_ratio(lo, hi, xyz, n), out);
 }
 
-int make_wall(const vd_wall &w, vd::WallDev &out) {
+// Normalises an axis as the geometry constructors do (wall_3d.rs:23-27, 124-128, 194-198, 285-289).
+void unit_axis(const double v[3], double out[3]) {
+    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len == 0.0) { out[0] = 0.0; out[1] = 0.0; out[2] = 1.0; }
+    else { out[0] = v[0] / len; out[1] = v[1] / len; out[2] = v[2] / len; }
+}
+
+// vd_wall -> device wall; the polyhedron planes or tube samples go to data (copied).
+int make_wall(const vd_wall &w, vd::WallDev &out, std::vector<double> &data) {
     if (w.id > VD_WALL_ID_MAX) return fail(VD_EINVAL, "Wall ID must be <= -1000");   // wall/mod.rs:25-27
     out = vd::WallDev{};
+    data.clear();
     out.id = w.id;
     out.kind = w.kind;
     for (int k = 0; k < 3; ++k) out.a[k] = w.a[k];
-    if (w.kind == 0) {
-        // PlaneGeometry::new, wall_3d.rs:23-27
-        const double len = std::sqrt(w.b[0] * w.b[0] + w.b[1] * w.b[1] + w.b[2] * w.b[2]);
-        if (len == 0.0) { out.b[0] = 0.0; out.b[1] = 0.0; out.b[2] = 1.0; }
-        else { out.b[0] = w.b[0] / len; out.b[1] = w.b[1] / len; out.b[2] = w.b[2] / len; }
-    } else if (w.kind == 1) {
+    switch (w.kind) {
+    case VD_WALL_PLANE:     // PlaneGeometry::new, wall_3d.rs:23-27
+    case VD_WALL_CYLINDER:  // CylinderGeometry::new, :124-128
+        unit_axis(w.b, out.b);
         out.r = w.r;
-    } else {
-        return fail(VD_EINVAL, "unknown wall kind (0 plane, 1 sphere)");
+        break;
+    case VD_WALL_SPHERE:
+        out.r = w.r;
+        break;
+    case VD_WALL_CONE:      // ConeGeometry::new :194-198; tan (:212), cos and sin (:232-233) of the angle
+        unit_axis(w.b, out.b);
+        out.r = w.r;
+        out.cs = std::cos(w.r);
+        out.sn = std::sin(w.r);
+        out.tn = std::tan(w.r);
+        break;
+    case VD_WALL_TORUS:     // TorusGeometry::new, :285-289
+        unit_axis(w.b, out.b);
+        out.r = w.r;
+        out.r2 = w.r2;
+        break;
+    case VD_WALL_POLYHEDRON:
+    case VD_WALL_TUBE: {
+        const size_t per = w.kind == VD_WALL_POLYHEDRON ? 6 : 3;
+        if (w.npts > VD_WALL_MAX_POINTS) return fail(VD_EINVAL, "too many wall planes or samples");
+        if (w.npts && !w.pts) return fail(VD_EINVAL, "NULL wall points");
+        data.assign(w.pts, w.pts + per * w.npts);
+        out.npts = w.npts;
+        out.r = w.r;
+        out.closed = w.closed ? 1 : 0;
+        break;
+    }
+    default:
+        return fail(VD_EINVAL, "unknown wall kind");
     }
     return VD_OK;
 }
@@ -235,6 +269,7 @@ struct Member {
     bool auto_grid = false;
     vd::GridGeom grid{};
     std::vector<vd::WallDev> walls;
+    std::vector<std::vector<double>> wdata;   // per wall: polyhedron planes or tube samples
     int shape = -1;
     uint64_t raw_off = 0, nraw = 0, gen_off = 0, ngen = 0, bin_off = 0, nbins = 0;
 };
@@ -278,6 +313,9 @@ struct Engine {
     DevBuf d_vol, d_cen, d_nface, d_nvert, d_status, d_cell_off;
     DevBuf d_vtx, d_fcnt, d_fnb, d_fidx, d_top, d_queue, d_counts4, d_next;
     DevBuf d_out0, d_out1;             // first-round retry lists of tier 0
+    DevBuf d_wdata;                    // wall planes and samples of every member
+    DevBuf d_tube;                     // per generator: planes of its member's tube walls
+    uint32_t ntube = 0;                // tube-wall slots per generator
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
     uint64_t nraw = 0, ngen = 0, nbins = 0;
     bool prepared = false, calculated = false;
@@ -316,6 +354,8 @@ void engine_free(Engine &E) {
     for (DevBuf *b : bufs) release(*b);
     release(E.d_out0);
     release(E.d_out1);
+    release(E.d_wdata);
+    release(E.d_tube);
     if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
     E.gexec = nullptr;
     if (E.h_counts) (void)hipHostFree(E.h_counts);
@@ -426,6 +466,26 @@ int engine_shells(Engine &E) {
 int upload_desc(Engine &E) {
     const size_t M = E.mem.size();
     if (E.shapes.empty() && M) VD_TRY(engine_shells(E));
+    // polyhedron planes and tube samples of every member's walls, in one device array
+    size_t words = 0;
+    for (const Member &m : E.mem)
+        for (const std::vector<double> &d : m.wdata) words += d.size();
+    if (words) {
+        std::vector<double> all;
+        all.reserve(words);
+        for (const Member &m : E.mem)
+            for (const std::vector<double> &d : m.wdata) all.insert(all.end(), d.begin(), d.end());
+        VD_TRY(ensure(E.d_wdata, words * 8));
+        VD_HIP(hipMemcpyAsync(E.d_wdata.p, all.data(), words * 8, hipMemcpyHostToDevice, E.stream));
+        VD_HIP(hipStreamSynchronize(E.stream));
+    }
+    size_t woff = 0;
+    E.ntube = 0;
+    for (const Member &m : E.mem) {
+        uint32_t nt = 0;
+        for (const vd::WallDev &w : m.walls) nt += w.kind == VD_WALL_TUBE;
+        E.ntube = std::max(E.ntube, nt);
+    }
     E.desc.assign(M, vd::TessDesc{});
     for (size_t t = 0; t < M; ++t) {
         const Member &m = E.mem[t];
@@ -443,7 +503,11 @@ int upload_desc(Engine &E) {
         d.nshell = tab.count;
         d.shell_next_d2 = tab.next;
         d.nwalls = (int32_t)m.walls.size();
-        for (size_t k = 0; k < m.walls.size(); ++k) d.walls[k] = m.walls[k];
+        for (size_t k = 0; k < m.walls.size(); ++k) {
+            d.walls[k] = m.walls[k];
+            d.walls[k].pts = m.wdata[k].empty() ? nullptr : E.d_wdata.as<double>() + woff;
+            woff += m.wdata[k].size();
+        }
     }
     VD_TRY(ensure(E.d_desc, std::max<size_t>(M, 1) * sizeof(vd::TessDesc)));
     if (M)
@@ -529,6 +593,7 @@ int engine_prepare(Engine &E) {
     VD_TRY(ensure(E.d_items, (ng + 1) * 4));
     VD_TRY(ensure(E.d_spos, (ng + 1) * 32));
     VD_TRY(ensure(E.d_scan, vd::scan_tmp_words((uint32_t)(nb + 1)) * 4));
+    if (E.ntube) VD_TRY(ensure(E.d_tube, (ng + 1) * E.ntube * 7 * sizeof(double)));
     VD_HIP(hipStreamSynchronize(E.stream));
     E.binned = false;
     E.prepared = true;
@@ -549,6 +614,9 @@ int enqueue_binning(Engine &E) {
     VD_HIP(vd::launch_bin_sort(E.d_bin_start.as<uint32_t>(), nb, E.d_items.as<uint32_t>(), E.stream));
     VD_HIP(vd::launch_fill_spos(E.d_gen.as<double>(), E.d_items.as<uint32_t>(), ng, E.d_desc.as<vd::TessDesc>(),
                                 E.d_gen_tess.as<uint32_t>(), E.d_spos.as<double4>(), E.stream));
+    if (E.ntube)
+        VD_HIP(vd::launch_tube_planes(E.d_gen.as<double>(), ng, E.d_desc.as<vd::TessDesc>(),
+                                      E.d_gen_tess.as<uint32_t>(), E.ntube, E.d_tube.as<double>(), E.stream));
     return VD_OK;
 }
 
@@ -559,6 +627,8 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.gen = E.d_gen.as<double>();
     a.spos = E.d_spos.as<double4>();
     a.bin_start = E.d_bin_start.as<uint32_t>();
+    a.tube_planes = E.d_tube.as<double>();
+    a.ntube = E.ntube;
     a.cells = d_list;
     a.ncells = nlist;
     a.queue = E.d_queue.as<uint32_t>();
@@ -606,11 +676,12 @@ int enqueue_round0(Engine &E, uint32_t n, float *ms) {
 // options, buffer addresses and the member descriptors. Equal keys give identical launches.
 std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
-                               E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.mem.size()};
+                               E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.mem.size(), E.ntube};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
-                            &E.d_top, &E.d_queue, &E.d_counts4, &E.d_out0, &E.d_out1};
+                            &E.d_top, &E.d_queue, &E.d_counts4, &E.d_out0, &E.d_out1, &E.d_tube,
+                            &E.d_wdata};
     for (const DevBuf *b : bufs) k.push_back(reinterpret_cast<uint64_t>(b->p));
     k.push_back(reinterpret_cast<uint64_t>(E.h_counts));
     const size_t words = E.desc.size() * sizeof(vd::TessDesc) / 8;
@@ -1067,11 +1138,13 @@ void vd_tess_destroy(vd_tess *t) {
 
 static int tess_add_wall(vd_tess *t, const vd_wall &w) {
     vd::WallDev d;
-    VD_TRY(make_wall(w, d));
+    std::vector<double> data;
+    VD_TRY(make_wall(w, d, data));
     Engine &E = t->E;
     Member &m = E.mem[0];
     if ((int)m.walls.size() >= vd::kMaxWalls) return fail(VD_EINVAL, "too many walls (max 8)");
     m.walls.push_back(d);
+    m.wdata.push_back(std::move(data));
     // prune_outside_generators (tessellation.rs:175-194): the current generators are filtered
     // again by every wall (re-binning an unchanged set gives the same bins)
     const uint64_t n = E.ngen;
@@ -1111,9 +1184,15 @@ int vd_tess_add_wall_sphere(vd_tess *t, int32_t id, const double center[3], doub
     return tess_add_wall(t, w);
 }
 
+int vd_tess_add_wall(vd_tess *t, const vd_wall *wall) {
+    if (!t || !wall) return fail(VD_EINVAL, "NULL argument");
+    return tess_add_wall(t, *wall);
+}
+
 int vd_tess_clear_walls(vd_tess *t) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
     t->E.mem[0].walls.clear();
+    t->E.mem[0].wdata.clear();
     t->E.calculated = false;
     return upload_desc(t->E);
 }
@@ -1256,8 +1335,10 @@ int vd_batch_add(vd_batch *b, const double bmin[3], const double bmax[3], uint32
     m.grid = make_grid(m.lo, m.hi, m.n);
     for (int k = 0; k < nwalls; ++k) {
         vd::WallDev d;
-        VD_TRY(make_wall(walls[k], d));
+        std::vector<double> data;
+        VD_TRY(make_wall(walls[k], d, data));
         m.walls.push_back(d);
+        m.wdata.push_back(std::move(data));
     }
     m.nraw = n;
     b->E.mem.push_back(m);
@@ -1449,8 +1530,10 @@ int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], u
     }
     for (int k = 0; k < nwalls && rc == VD_OK; ++k) {
         vd::WallDev d;
-        rc = make_wall(walls[k], d);
+        std::vector<double> data;
+        rc = make_wall(walls[k], d, data);
         J->m.walls.push_back(d);
+        J->m.wdata.push_back(std::move(data));
     }
     if (rc) { delete J; return rc; }
     J->m.grid = make_grid(J->m.lo, J->m.hi, J->m.n);

@@ -40,7 +40,7 @@
 // (break test and bin test per entry, distance test per candidate) instead of being rebuilt; if a
 // cut ever grows the radius (possible by rounding), the window is rebuilt from the position after
 // the cut, because entries dropped earlier might then pass.
-#include "vd_internal.h"
+#include "vd_walls.h"
 
 #include <cfloat>
 
@@ -927,34 +927,44 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     int noc = 0;            // consecutive candidate clips without a change
     uint64_t wcm = ~0ull;   // window lanes whose candidate may cut (pre-test), all without it
 
-    int wi = 0;             // next wall (tessellation.rs:429-436), then the neighbour walk
     bool walking = false;
+
+    // walls (tessellation.rs:429-436): every plane each wall's cut emits, in order; the cell is
+    // done once a wall has left it empty
+    uint32_t tk = 0;        // tube walls seen so far (their planes come from a pre-pass)
+    for (int wi = 0; wi < nwalls; ++wi) {
+        const WallDev &W = D.walls[wi];
+        for (uint32_t wp = 0;; ++wp) {
+            double pp[3], pn[3];
+            bool has;
+            if (W.kind == kWallTube) {
+                const double *tp = A.tube_planes + 7 * ((size_t)i * A.ntube + tk);
+                has = wp == 0 && tp[6] != 0.0;
+                for (int k = 0; k < 3; ++k) { pp[k] = tp[k]; pn[k] = tp[3 + k]; }
+            } else {
+                has = wall_plane(W, wp, gx, gy, gz, pp, pn);
+            }
+            if (!has) break;
+            double wr2 = 0.0;
+            const int rc = clip_any<C>(m, P, r, Plane{pp[0], pp[1], pp[2], pn[0], pn[1], pn[2]}, W.id, false,
+                                       gx, gy, gz, wr2);
+            if (rc < 0) {
+                status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
+                break;
+            }
+        }
+        if (W.kind == kWallTube) ++tk;
+        if (status != kCellOk || P.V == 0) break;
+    }
 
     for (;;) {
         Plane pl;
         int32_t nid = 0;
         bool withgen = false;
         int cl = -1;
-        if (wi < nwalls) {
-            const WallDev &W = D.walls[wi];
-            if (W.kind == 0) {  // wall_3d.rs:38-42
-                pl = Plane{W.a[0], W.a[1], W.a[2], -W.b[0], -W.b[1], -W.b[2]};
-            } else {            // wall_3d.rs:80-100
-                const double dx = gx - W.a[0], dy = gy - W.a[1], dz = gz - W.a[2];
-                const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-                if (dist == 0.0) {
-                    ++wi;           // no plane from this wall
-                    if (P.V == 0) break;
-                    continue;
-                }
-                const double sc = W.r / dist;
-                pl = Plane{W.a[0] + dx * sc, W.a[1] + dy * sc, W.a[2] + dz * sc,
-                           dx / dist, dy / dist, dz / dist};
-            }
-            nid = W.id;
-        } else {
+        {
             if (!walking) {
-                if (P.V == 0) break;
+                if (P.V == 0 || status != kCellOk) break;
                 r2 = poly_max_radius_sq<C>(m, P, gx, gy, gz);   // tessellation.rs:438
                 ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
                 iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
@@ -1083,11 +1093,6 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
         if (rc < 0) {
             status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
             break;
-        }
-        if (cl < 0) {                   // a wall plane
-            ++wi;
-            if (P.V == 0) break;        // tessellation.rs:433-435
-            continue;
         }
         if (rc == kNoChange) {
             if (++noc >= kPretestAfter) pre = true;   // takes effect at the next window

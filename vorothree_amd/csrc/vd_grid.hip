@@ -6,7 +6,7 @@
 // src/algorithm/algo_3d_grid.rs:98-130 (get_bin_index, set_generators), src/tessellation.rs:482-495
 // (relax). All HBM-bound integer/byte work: one thread per element, coalesced where the access
 // pattern allows (generators and flags), 256-thread blocks.
-#include "vd_internal.h"
+#include "vd_walls.h"
 
 namespace vd {
 namespace {
@@ -15,13 +15,6 @@ constexpr int kBlock = 256;
 
 inline unsigned blocks_for(uint64_t n, int per = kBlock) {
     return (unsigned)((n + per - 1) / per);
-}
-
-// Wall::contains for every wall in insertion order (wall_3d.rs:31-36 plane, :73-78 sphere).
-__device__ inline bool wall_contains(const WallDev &w, double x, double y, double z) {
-    double dx = x - w.a[0], dy = y - w.a[1], dz = z - w.a[2];
-    if (w.kind == 0) return (dx * w.b[0] + dy * w.b[1] + dz * w.b[2]) >= 0.0;
-    return (dx * dx + dy * dy + dz * dz) <= w.r * w.r;
 }
 
 __device__ inline uint64_t gid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
@@ -49,6 +42,30 @@ __global__ void k_wall_flags(const double *__restrict__ xyz, uint64_t n, const T
     for (int k = 0; k < D.nwalls; ++k)
         if (!wall_contains(D.walls[k], x, y, z)) { in = 0; break; }
     flags[g] = in;
+}
+
+// The cut plane of every tube wall for every generator (get_closest_point + cut,
+// wall_3d.rs:408-456, 762-815, 908-961): slot k of generator g holds the plane of its member's
+// k-th tube wall as point, normal and a flag (1: a plane is emitted, 0: g lies on the curve).
+__global__ void k_tube_planes(const double *__restrict__ gen, uint64_t n, const TessDesc *__restrict__ desc,
+                              const uint32_t *__restrict__ tess, uint32_t ntube, double *__restrict__ out) {
+    const uint64_t g = gid();
+    if (g >= n) return;
+    const TessDesc &D = desc[tess[g]];
+    const double x = gen[3 * g], y = gen[3 * g + 1], z = gen[3 * g + 2];
+    uint32_t k = 0;
+    for (int w = 0; w < D.nwalls && k < ntube; ++w) {
+        const WallDev &W = D.walls[w];
+        if (W.kind != kWallTube) continue;
+        double cx, cy, cz, p[3] = {0.0, 0.0, 0.0}, nn[3] = {0.0, 0.0, 0.0};
+        tube_closest(W, x, y, z, cx, cy, cz);
+        const bool has = tube_plane(W, x, y, z, cx, cy, cz, p, nn);
+        double *o = out + 7 * (g * ntube + k);
+        o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+        o[3] = nn[0]; o[4] = nn[1]; o[5] = nn[2];
+        o[6] = has ? 1.0 : 0.0;
+        ++k;
+    }
 }
 
 // Keeps the flagged points in input order (indices are compacted, tessellation.rs:58-60).
@@ -304,6 +321,13 @@ hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n
                             const uint32_t *tess, double4 *spos, hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_fill_spos<<<blocks_for(n), kBlock, 0, st>>>(xyz, items, n, desc, tess, spos);
+    return hipGetLastError();
+}
+
+hipError_t launch_tube_planes(const double *gen, uint64_t n, const TessDesc *desc, const uint32_t *tess,
+                              uint32_t ntube, double *out, hipStream_t st) {
+    if (!n || !ntube) return hipSuccess;
+    k_tube_planes<<<blocks_for(n), kBlock, 0, st>>>(gen, n, desc, tess, ntube, out);
     return hipGetLastError();
 }
 

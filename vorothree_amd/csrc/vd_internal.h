@@ -41,13 +41,19 @@ __host__ __device__ inline double shell_key(int dx, int dy, int dz, const double
     return mx * mx + my * my + mz * mz;
 }
 
-// A wall as the kernels see it (src/wall/wall_3d.rs:8-101). kind 0 = plane, 1 = sphere.
+// A wall as the kernels see it (src/wall/wall_3d.rs). kind: 0 plane, 1 sphere, 2 cylinder, 3 cone,
+// 4 torus, 5 convex polyhedron, 6 tube around a sampled curve (the vd_wall kinds of vd.h).
 struct WallDev {
     int32_t id;
     int32_t kind;
-    double a[3];   // plane point / sphere centre
-    double b[3];   // plane unit normal pointing into the valid region
-    double r;      // sphere radius
+    double a[3];          // plane point / sphere, cylinder, torus centre / cone tip
+    double b[3];          // plane unit normal into the valid region / unit axis
+    double r;             // sphere, cylinder, tube radius / torus major radius / cone half-angle
+    double r2;            // torus minor radius
+    double cs, sn, tn;    // cone: cos, sin and tan of the half-angle, evaluated on the host
+    const double *pts;    // polyhedron planes (point, outward normal; 6 each) / tube samples (xyz)
+    uint32_t npts;
+    int32_t closed;       // tube: the last sample joins the first
 };
 
 constexpr int kMaxWalls = 8;
@@ -85,6 +91,8 @@ struct CellArgs {
     const double *gen;            // generators, xyz AoS, members concatenated
     const double4 *spos;          // generators in bin order: x, y, z, member-local index (u64 bits)
     const uint32_t *bin_start;    // concatenated bins + 1 (positions in spos)
+    const double *tube_planes;    // per generator and tube wall: plane point, normal, emitted flag
+    uint32_t ntube;               // tube-wall slots per generator (0: no member has one)
 
     // work list: generator indices (global) to compute
     const uint32_t *cells;
@@ -138,6 +146,8 @@ hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint64_t n, const uint32_
 hipError_t launch_bin_sort(const uint32_t *bin_start, uint64_t nbins, uint32_t *items, hipStream_t st);
 hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n, const TessDesc *desc,
                             const uint32_t *tess, double4 *spos, hipStream_t st);
+hipError_t launch_tube_planes(const double *gen, uint64_t n, const TessDesc *desc, const uint32_t *tess,
+                              uint32_t ntube, double *out, hipStream_t st);   // 7 doubles per slot
 hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint64_t n,
                         double *out, hipStream_t st);
 hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, uint32_t *flags,

@@ -71,24 +71,244 @@ class Algorithm3DGrid:
         return idx[0] + idx[1] * n[0] + idx[2] * n[0] * n[1]
 
 
+def _f3(v):
+    return [float(x) for x in v]
+
+
 class PlaneGeometry:
+    """PlaneGeometry::new (src/wall/wall_3d.rs:23-27); the library normalises the normal."""
+    kind = _lib.VD_WALL_PLANE
+
     def __init__(self, point, normal):
-        self.point = [float(x) for x in point]
-        self.normal_raw = [float(x) for x in normal]
+        self.point = _f3(point)
+        self.normal_raw = _f3(normal)
+
+    def _fill(self, w):
+        w.a[:] = self.point
+        w.b[:] = self.normal_raw
 
 
 class SphereGeometry:
+    """SphereGeometry::new (wall_3d.rs:67-69)."""
+    kind = _lib.VD_WALL_SPHERE
+
     def __init__(self, center, radius):
-        self.center = [float(x) for x in center]
+        self.center = _f3(center)
         self.radius = float(radius)
+
+    def _fill(self, w):
+        w.a[:] = self.center
+        w.r = self.radius
+
+
+class CylinderGeometry:
+    """CylinderGeometry::new (wall_3d.rs:124-128): infinite cylinder, valid region inside."""
+    kind = _lib.VD_WALL_CYLINDER
+
+    def __init__(self, center, axis, radius):
+        self.center, self.axis_raw, self.radius = _f3(center), _f3(axis), float(radius)
+
+    def _fill(self, w):
+        w.a[:] = self.center
+        w.b[:] = self.axis_raw
+        w.r = self.radius
+
+
+class ConeGeometry:
+    """ConeGeometry::new (wall_3d.rs:194-198): infinite cone from `tip` along `axis` with
+    half-angle `angle` (radians), valid region inside."""
+    kind = _lib.VD_WALL_CONE
+
+    def __init__(self, tip, axis, angle):
+        self.tip, self.axis_raw, self.angle = _f3(tip), _f3(axis), float(angle)
+
+    def _fill(self, w):
+        w.a[:] = self.tip
+        w.b[:] = self.axis_raw
+        w.r = self.angle
+
+
+class TorusGeometry:
+    """TorusGeometry::new (wall_3d.rs:285-289): valid region inside the tube."""
+    kind = _lib.VD_WALL_TORUS
+
+    def __init__(self, center, axis, major_radius, minor_radius):
+        self.center, self.axis_raw = _f3(center), _f3(axis)
+        self.major_radius, self.minor_radius = float(major_radius), float(minor_radius)
+
+    def _fill(self, w):
+        w.a[:] = self.center
+        w.b[:] = self.axis_raw
+        w.r = self.major_radius
+        w.r2 = self.minor_radius
+
+
+class ConvexPolyhedronGeometry:
+    """ConvexPolyhedronGeometry::new(points, normals) (wall_3d.rs:476-491): planes (point,
+    outward normal), used as given."""
+    kind = _lib.VD_WALL_POLYHEDRON
+
+    def __init__(self, points, normals):
+        p = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+        n = np.asarray(normals, dtype=np.float64).reshape(-1, 3)
+        if p.shape != n.shape:
+            raise ValueError("Points and normals must have same length and be multiple of 3")
+        self.planes = np.ascontiguousarray(np.concatenate([p, n], axis=1))
+
+    @classmethod
+    def new_tetrahedron(cls, center, radius):
+        """wall_3d.rs:499-529: faces at the inradius R / 3 with normals (+-1, +-1, +-1)/sqrt(3)."""
+        dist = float(radius) / 3.0
+        pts, nrm = [], []
+        for n in ([-1.0, -1.0, -1.0], [-1.0, 1.0, 1.0], [1.0, -1.0, 1.0], [1.0, 1.0, -1.0]):
+            ln = math.sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])
+            u = [n[0] / ln, n[1] / ln, n[2] / ln]
+            nrm.append(u)
+            pts.append([center[0] + u[0] * dist, center[1] + u[1] * dist, center[2] + u[2] * dist])
+        return cls(pts, nrm)
+
+    @classmethod
+    def new_hexahedron(cls, center, radius):
+        """wall_3d.rs:534-560: axis-aligned cube with inradius R / sqrt(3)."""
+        dist = float(radius) / math.sqrt(3.0)
+        pts, nrm = [], []
+        for n in ([1.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, -1.0, 0.0],
+                  [0.0, 0.0, 1.0], [0.0, 0.0, -1.0]):
+            nrm.append(n)
+            pts.append([center[0] + n[0] * dist, center[1] + n[1] * dist, center[2] + n[2] * dist])
+        return cls(pts, nrm)
+
+    def _fill(self, w):
+        w.pts = self.planes.ctypes.data
+        w.npts = len(self.planes)
+
+
+class _Tube:
+    """Tube of radius tube_radius around a sampled curve (the closest point on the sample
+    polyline defines contains and cut: wall_3d.rs:408-456, 762-815, 908-961)."""
+    kind = _lib.VD_WALL_TUBE
+
+    def _fill(self, w):
+        w.pts = self.samples.ctypes.data if len(self.samples) else None
+        w.npts = len(self.samples)
+        w.r = self.tube_radius
+        w.closed = 1 if self.closed else 0
+
+
+class TrefoilKnotGeometry(_Tube):
+    """TrefoilKnotGeometry::new (wall_3d.rs:390-407); the polyline is closed."""
+
+    def __init__(self, center, scale, tube_radius, resolution):
+        c = _f3(center)
+        s = []
+        for i in range(int(resolution)):
+            t = (i / int(resolution)) * math.tau
+            x = math.sin(t) + 2.0 * math.sin(2.0 * t)
+            y = math.cos(t) - 2.0 * math.cos(2.0 * t)
+            z = -math.sin(3.0 * t)
+            s.append([c[0] + x * scale, c[1] + y * scale, c[2] + z * scale])
+        self.samples = np.ascontiguousarray(np.asarray(s, dtype=np.float64).reshape(-1, 3))
+        self.tube_radius, self.closed = float(tube_radius), True
+
+
+class CubicBezierGeometry(_Tube):
+    """CubicBezierGeometry::new (wall_3d.rs:739-760): resolution + 1 samples of the curve."""
+
+    def __init__(self, p0, p1, p2, p3, tube_radius, resolution, closed):
+        p0, p1, p2, p3 = _f3(p0), _f3(p1), _f3(p2), _f3(p3)
+        s = []
+        for i in range(int(resolution) + 1):
+            t = i / int(resolution)
+            t2 = t * t
+            t3 = t2 * t
+            mt = 1.0 - t
+            mt2 = mt * mt
+            mt3 = mt2 * mt
+            s.append([mt3 * p0[k] + 3.0 * mt2 * t * p1[k] + 3.0 * mt * t2 * p2[k] + t3 * p3[k] for k in range(3)])
+        self.samples = np.ascontiguousarray(np.asarray(s, dtype=np.float64).reshape(-1, 3))
+        self.tube_radius, self.closed = float(tube_radius), bool(closed)
+
+
+def _rem(a: int, b: int) -> int:
+    """Rust's % on signed integers (the sign of the dividend)."""
+    r = abs(a) % b
+    return -r if a < 0 else r
+
+
+class CatmullRomGeometry(_Tube):
+    """CatmullRomGeometry::new (wall_3d.rs:831-906, 963-995): centripetal Catmull-Rom samples."""
+
+    def __init__(self, points, tube_radius, resolution, closed):
+        pts = [_f3(p) for p in points]
+        s = []
+        if len(pts) >= 2:
+            for i in range(int(resolution) + 1):
+                s.append(self._point(i / int(resolution), pts, bool(closed)))
+        self.samples = np.ascontiguousarray(np.asarray(s, dtype=np.float64).reshape(-1, 3))
+        self.tube_radius, self.closed = float(tube_radius), bool(closed)
+
+    @staticmethod
+    def _point(t, points, closed):
+        l = len(points)
+        p = (l - (0.0 if closed else 1.0)) * t
+        ip = math.floor(p)
+        weight = p - ip
+        if closed:
+            if ip <= 0:
+                ip += (abs(ip) // l + 1) * l
+        elif weight == 0.0 and ip == l - 1:
+            ip = l - 2
+        if closed or ip > 0:
+            p0 = points[(_rem(ip - 1, l) + l) % l]
+        else:
+            p0 = [points[0][k] - (points[1][k] - points[0][k]) for k in range(3)]
+        p1 = points[ip % l]
+        p2 = points[(ip + 1) % l]
+        if closed or ip + 2 < l:
+            p3 = points[(ip + 2) % l]
+        else:
+            last, prev = points[l - 1], points[l - 2]
+            p3 = [last[k] - (prev[k] - last[k]) for k in range(3)]
+
+        def dsq(a, b):
+            dx, dy, dz = a[0] - b[0], a[1] - b[1], a[2] - b[2]
+            return dx * dx + dy * dy + dz * dz
+
+        dt0 = math.pow(dsq(p0, p1), 0.25)
+        dt1 = math.pow(dsq(p1, p2), 0.25)
+        dt2 = math.pow(dsq(p2, p3), 0.25)
+        if dt1 < 1e-4:
+            dt1 = 1.0
+        if dt0 < 1e-4:
+            dt0 = dt1
+        if dt2 < 1e-4:
+            dt2 = dt1
+        out = []
+        for k in range(3):
+            x0, x1, x2, x3 = p0[k], p1[k], p2[k], p3[k]
+            t1 = (x1 - x0) / dt0 - (x2 - x0) / (dt0 + dt1) + (x2 - x1) / dt1
+            t2 = (x2 - x1) / dt1 - (x3 - x1) / (dt1 + dt2) + (x3 - x2) / dt2
+            t1 *= dt1
+            t2 *= dt1
+            c0, c1 = x1, t1
+            c2 = -3.0 * x1 + 3.0 * x2 - 2.0 * t1 - t2
+            c3 = 2.0 * x1 - 2.0 * x2 + t1 + t2
+            w2 = weight * weight
+            w3 = w2 * weight
+            out.append(c0 + c1 * weight + c2 * w2 + c3 * w3)
+        return out
+
+
+_GEOMETRIES = (PlaneGeometry, SphereGeometry, CylinderGeometry, ConeGeometry, TorusGeometry,
+               ConvexPolyhedronGeometry, _Tube)
 
 
 class Wall:
     def __init__(self, id: int, geometry):  # noqa: A002
         if id > WALL_ID_MAX:
             raise ValueError(f"Wall ID must be <= {WALL_ID_MAX}")
-        if not isinstance(geometry, (PlaneGeometry, SphereGeometry)):
-            raise TypeError("only PlaneGeometry and SphereGeometry walls run on the device")
+        if not isinstance(geometry, _GEOMETRIES):
+            raise TypeError("unsupported wall geometry")
         self._id = int(id)
         self.geometry = geometry
 
@@ -231,8 +451,11 @@ class Tessellation:
         g = wall.geometry
         if isinstance(g, PlaneGeometry):
             check(lib().vd_tess_add_wall_plane(self._h, wall.id(), _d3(g.point), _d3(g.normal_raw)))
-        else:
+        elif isinstance(g, SphereGeometry):
             check(lib().vd_tess_add_wall_sphere(self._h, wall.id(), _d3(g.center), g.radius))
+        else:
+            w = _vd_wall(wall)
+            check(lib().vd_tess_add_wall(self._h, ctypes.byref(w)))
         self._walls.append(wall)
 
     def clear_walls(self) -> None:
@@ -356,19 +579,11 @@ def grid_for(bounds: BoundingBox, n: int, per_bin: float):
 
 
 def _vd_wall(wall: Wall):
+    """struct vd_wall of a Wall; its point arrays stay owned by the geometry object."""
     w = _lib.VdWall()
     w.id = wall.id()
-    g = wall.geometry
-    if isinstance(g, PlaneGeometry):
-        w.kind = 0
-        for k in range(3):
-            w.a[k] = g.point[k]
-            w.b[k] = g.normal_raw[k]
-    else:
-        w.kind = 1
-        for k in range(3):
-            w.a[k] = g.center[k]
-        w.r = g.radius
+    w.kind = wall.geometry.kind
+    wall.geometry._fill(w)
     return w
 
 
