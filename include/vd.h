@@ -105,6 +105,27 @@ int vd_tess_calculate(vd_tess *t);
  * cell keeps its position), then set_generators. Requires calculate() first. */
 int vd_tess_relax(vd_tess *t);
 
+/* calculate_sealed(), src/tessellation.rs:234-318: calculate(), then every cell with faces on a
+ * curved wall (not plane or polyhedron) is clipped, for each of its neighbours in face order and
+ * each curved wall both touch, by that wall's plane for the neighbour's generator, when this
+ * generator is on the plane's valid side ((g - p).n <= 1e-9). The seal log holds one
+ * (cell, neighbour, wall id) triple per clip that changed the cell, in cell order. */
+int vd_tess_calculate_sealed(vd_tess *t);
+
+/* prune_boundaries(), src/tessellation.rs:324-385 (needs full cells): for each cell, neighbour
+ * (first occurrence in face order) and curved wall both touch, the vertices of the cell's faces on
+ * that wall that also lie on its face shared with the neighbour (Cell3DFaces::shared_vertices,
+ * src/cell/cell_3d_faces.rs:536-568). The prune log holds (cell, neighbour, wall id, vertex count)
+ * per non-empty entry, the position log their vertices (xyz). calculate() clears both logs. */
+int vd_tess_prune_boundaries(vd_tess *t);
+
+/* seal_log: up to cap ints (3 per entry); *count receives the total number of ints. */
+int vd_tess_get_seal_log(const vd_tess *t, int32_t *out, size_t cap, size_t *count);
+/* prune_log and prune_pos_log: up to icap ints and pcap doubles; counts[0], counts[1] receive
+ * their totals. */
+int vd_tess_get_prune_log(const vd_tess *t, int32_t *ints, size_t icap, double *pos, size_t pcap,
+                          size_t counts[2]);
+
 /* count_generators() / count_cells(), src/tessellation.rs:498-505. */
 size_t vd_tess_count_generators(const vd_tess *t);
 size_t vd_tess_count_cells(const vd_tess *t);

@@ -50,6 +50,10 @@ def _load():
         "or_tess_calculate": (None, [_vp, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]),
         "or_tess_count_cells": (ctypes.c_size_t, [_vp]),
         "or_tess_relax": (None, [_vp]),
+        "or_tess_calculate_sealed": (None, [_vp, ctypes.c_int]),
+        "or_tess_prune_boundaries": (None, [_vp]),
+        "or_tess_seal_log": (ctypes.c_size_t, [_vp, _vp, ctypes.c_size_t]),
+        "or_tess_prune_log": (None, [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _vp]),
         "or_cell_sizes": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp]),
         "or_cell_get": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]),
         "or_tess_reduced": (None, [_vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]),
@@ -145,6 +149,29 @@ class OracleTess:
                                         p.ctypes.data_as(_dp), npts, 1 if closed else 0)
         if rc != 0:
             raise ValueError("Wall ID must be <= -1000" if rc == -1 else "unknown wall kind")
+
+    def calculate_sealed(self, threads=1):
+        """Tessellation::calculate_sealed (tessellation.rs:234-318)."""
+        self._lib.or_tess_calculate_sealed(self._h, int(threads))
+
+    def prune_boundaries(self):
+        """Tessellation::prune_boundaries (tessellation.rs:324-385)."""
+        self._lib.or_tess_prune_boundaries(self._h)
+
+    def seal_log(self):
+        n = self._lib.or_tess_seal_log(self._h, None, 0)
+        out = np.zeros(n, dtype=np.int32)
+        self._lib.or_tess_seal_log(self._h, out.ctypes.data_as(ctypes.c_void_p), n)
+        return out.reshape(-1, 3)
+
+    def prune_log(self):
+        c = np.zeros(2, dtype=np.uint64)
+        self._lib.or_tess_prune_log(self._h, None, 0, None, 0, c.ctypes.data_as(ctypes.c_void_p))
+        ints = np.zeros(int(c[0]), dtype=np.int32)
+        pos = np.zeros(int(c[1]), dtype=np.float64)
+        self._lib.or_tess_prune_log(self._h, ints.ctypes.data_as(ctypes.c_void_p), ints.size,
+                                    pos.ctypes.data_as(ctypes.c_void_p), pos.size, c.ctypes.data_as(ctypes.c_void_p))
+        return ints, pos.reshape(-1, 3)
 
     def set_generators(self, xyz):
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))

@@ -591,6 +591,8 @@ struct Tess {
     std::vector<Poly> cells;
     std::vector<uint8_t> computed;
     std::vector<Work> work;   // per cell
+    std::vector<int32_t> seal_log, prune_log;   // tessellation.rs:19-21
+    std::vector<double> prune_pos_log;
 };
 
 void tess_set_generators(Tess &T, const double *xyz, size_t n) {  // tessellation.rs:43-67
@@ -670,6 +672,9 @@ void compute_cell(const Tess &T, size_t i, Poly &cell, ClipScratch &s, Work *w, 
 
 void tess_calculate(Tess &T, const int64_t *subset, size_t m, int threads, int brute) {
     const size_t n = T.gen.size() / 3;
+    T.seal_log.clear();   // tessellation.rs:214-216
+    T.prune_log.clear();
+    T.prune_pos_log.clear();
     T.cells.assign(n, Poly());
     T.computed.assign(n, 0);
     T.work.assign(n, Work());
@@ -797,12 +802,139 @@ size_t or_tess_shell_order(void *t, int32_t *dxyz, double *d2, size_t cap) {
     return T.grid.order.size();
 }
 
+// ---------------------------------------------------------------------------------------------
+// Post-passes. Restate src/tessellation.rs:234-385 and Cell3DFaces::shared_vertices
+// (src/cell/cell_3d_faces.rs:536-568).
+// ---------------------------------------------------------------------------------------------
+bool wall_planar(const Wall &w) { return w.kind == 0 || w.kind == 5; }   // wall_3d.rs:44, 709
+
+// Neighbour generators per cell in face order (deduplicated for prune_boundaries) and the ids of
+// the curved walls it touches (tessellation.rs:244-264, 330-350).
+void cell_topology(const Tess &T, bool dedupe, std::vector<std::vector<size_t>> &topo,
+                   std::vector<std::vector<int32_t>> &cw) {
+    const size_t count = T.gen.size() / 3;
+    topo.assign(count, {});
+    cw.assign(count, {});
+    for (size_t i = 0; i < count; ++i) {
+        for (int32_t fn : T.cells[i].fnb) {
+            if (fn >= 0 && (size_t)fn < count) {
+                if (!dedupe || std::find(topo[i].begin(), topo[i].end(), (size_t)fn) == topo[i].end())
+                    topo[i].push_back((size_t)fn);
+            } else {
+                const Wall *w = nullptr;
+                for (const Wall &x : T.walls)
+                    if (x.id == fn) { w = &x; break; }
+                if (w && !wall_planar(*w) && std::find(cw[i].begin(), cw[i].end(), fn) == cw[i].end())
+                    cw[i].push_back(fn);
+            }
+        }
+    }
+}
+
+// calculate_sealed after calculate (tessellation.rs:238-317); cells are independent, so the
+// sequential loop gives what the reference's parallel one gives.
+void tess_seal(Tess &T) {
+    std::vector<std::vector<size_t>> topo;
+    std::vector<std::vector<int32_t>> cw;
+    cell_topology(T, false, topo, cw);
+    T.seal_log.clear();
+    ClipScratch s;
+    auto has = [](const std::vector<int32_t> &v, int32_t x) { return std::find(v.begin(), v.end(), x) != v.end(); };
+    for (size_t i = 0; i < T.cells.size(); ++i) {
+        if (cw[i].empty()) continue;
+        const double *g = &T.gen[3 * i];
+        for (size_t nb : topo[i]) {
+            const double *q = &T.gen[3 * nb];
+            for (const Wall &wall : T.walls) {
+                if (wall_planar(wall) || !has(cw[i], wall.id) || !has(cw[nb], wall.id)) continue;
+                wall_cut(wall, q, [&](const double *p, const double *n) {
+                    double dot = 0.0;
+                    for (int k = 0; k < 3; ++k) dot += (g[k] - p[k]) * n[k];
+                    if (dot <= 1e-9) {
+                        auto r = poly_clip(T.cells[i], p, n, wall.id, s, nullptr, nullptr);
+                        if (r.first) {
+                            T.seal_log.push_back((int32_t)i);
+                            T.seal_log.push_back((int32_t)nb);
+                            T.seal_log.push_back(wall.id);
+                        }
+                    }
+                }, nullptr);
+            }
+        }
+    }
+}
+
+// Cell3DFaces::shared_vertices (cell_3d_faces.rs:536-568).
+std::vector<double> shared_vertices(const Poly &c, int32_t a, int32_t b) {
+    std::vector<size_t> av, bv;
+    size_t off = 0;
+    for (size_t f = 0; f < c.fcnt.size(); ++f) {
+        const size_t cnt = c.fcnt[f];
+        for (size_t j = 0; j < cnt; ++j) {
+            const size_t v = c.fidx[off + j];
+            if (c.fnb[f] == a && std::find(av.begin(), av.end(), v) == av.end()) av.push_back(v);
+            if (c.fnb[f] == b && std::find(bv.begin(), bv.end(), v) == bv.end()) bv.push_back(v);
+        }
+        off += cnt;
+    }
+    std::vector<double> out;
+    for (size_t v : av)
+        if (std::find(bv.begin(), bv.end(), v) != bv.end()) out.insert(out.end(), &c.v[3 * v], &c.v[3 * v] + 3);
+    return out;
+}
+
+// prune_boundaries (tessellation.rs:324-385).
+void tess_prune(Tess &T) {
+    std::vector<std::vector<size_t>> topo;
+    std::vector<std::vector<int32_t>> cw;
+    cell_topology(T, true, topo, cw);
+    T.prune_log.clear();
+    T.prune_pos_log.clear();
+    auto has = [](const std::vector<int32_t> &v, int32_t x) { return std::find(v.begin(), v.end(), x) != v.end(); };
+    for (size_t i = 0; i < T.cells.size(); ++i) {
+        if (cw[i].empty()) continue;
+        for (size_t nb : topo[i])
+            for (const Wall &wall : T.walls) {
+                if (wall_planar(wall) || !has(cw[i], wall.id) || !has(cw[nb], wall.id)) continue;
+                const std::vector<double> sh = shared_vertices(T.cells[i], wall.id, (int32_t)nb);
+                if (sh.empty()) continue;
+                T.prune_log.push_back((int32_t)i);
+                T.prune_log.push_back((int32_t)nb);
+                T.prune_log.push_back(wall.id);
+                T.prune_log.push_back((int32_t)(sh.size() / 3));
+                T.prune_pos_log.insert(T.prune_pos_log.end(), sh.begin(), sh.end());
+            }
+    }
+}
+
 // Compute all cells (subset == NULL) or only the listed cell indices.
 void or_tess_calculate(void *t, const int64_t *subset, size_t m, int threads, int brute) {
     tess_calculate(*(Tess *)t, subset, m, threads, brute);
 }
 
 size_t or_tess_count_cells(void *t) { return ((Tess *)t)->cells.size(); }
+
+void or_tess_calculate_sealed(void *t, int threads) {
+    Tess &T = *(Tess *)t;
+    tess_calculate(T, nullptr, 0, threads, 0);
+    tess_seal(T);
+}
+
+void or_tess_prune_boundaries(void *t) { tess_prune(*(Tess *)t); }
+
+size_t or_tess_seal_log(void *t, int32_t *out, size_t cap) {
+    const Tess &T = *(Tess *)t;
+    for (size_t k = 0; k < std::min(cap, T.seal_log.size()); ++k) out[k] = T.seal_log[k];
+    return T.seal_log.size();
+}
+
+void or_tess_prune_log(void *t, int32_t *ints, size_t icap, double *pos, size_t pcap, uint64_t counts[2]) {
+    const Tess &T = *(Tess *)t;
+    for (size_t k = 0; k < std::min(icap, T.prune_log.size()); ++k) ints[k] = T.prune_log[k];
+    for (size_t k = 0; k < std::min(pcap, T.prune_pos_log.size()); ++k) pos[k] = T.prune_pos_log[k];
+    counts[0] = T.prune_log.size();
+    counts[1] = T.prune_pos_log.size();
+}
 
 // Tessellation::relax, tessellation.rs:482-495 (requires a full calculate first).
 void or_tess_relax(void *t) {

@@ -1,0 +1,91 @@
+"""calculate_sealed and prune_boundaries on the device against the oracle, bit for bit: every
+sealed cell (vertices, faces, neighbours, volume, centroid), the seal log, and the prune log with
+its vertex positions."""
+import numpy as np
+import pytest
+
+from conftest import assert_same_cell, bits, oracle_wall
+from oracle import OracleTess
+from vorothree_amd import (Algorithm3DGrid, BoundingBox, ConvexPolyhedronGeometry, CylinderGeometry,
+                           SphereGeometry, Tessellation, TorusGeometry, TrefoilKnotGeometry, Wall)
+from vorothree_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+UNIT = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+
+CASES = {
+    "sphere": lambda: [SphereGeometry((0.5, 0.5, 0.5), 0.45)],
+    "cylinder": lambda: [CylinderGeometry((0.5, 0.5, 0.0), (0.0, 0.0, 1.0), 0.35)],
+    "torus": lambda: [TorusGeometry((0.5, 0.5, 0.5), (0.2, 0.0, 1.0), 0.27, 0.14)],
+    "trefoil": lambda: [TrefoilKnotGeometry((0.5, 0.5, 0.5), 0.12, 0.08, 100)],
+    "mixed": lambda: [ConvexPolyhedronGeometry.new_hexahedron((0.5, 0.5, 0.5), 0.7),
+                      SphereGeometry((0.5, 0.5, 0.5), 0.48), CylinderGeometry((0.5, 0.5, 0.5), (1.0, 0.0, 0.0), 0.4)],
+}
+
+
+def pair(name, n=9000, grid=17):
+    walls = [Wall(-1000 - k, g) for k, g in enumerate(CASES[name]())]
+    pts = W.PCG32(1000 + len(name)).uniform(n, (0, 0, 0), (1, 1, 1))
+    t = Tessellation(UNIT, Algorithm3DGrid(grid, grid, grid, UNIT))
+    o = OracleTess((0, 0, 0), (1, 1, 1), grid, grid, grid)
+    for w in walls:
+        t.add_wall(w)
+        oracle_wall(o, w)
+    t.set_generators(pts)
+    o.set_generators(pts)
+    return t, o
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_calculate_sealed_bit_exact(name):
+    t, o = pair(name)
+    t.calculate_sealed()
+    o.calculate_sealed(threads=8)
+    n = o.count_cells()
+    vol, cen, nf, _ = o.reduced()
+    assert np.array_equal(bits(t.volumes()), bits(vol))
+    assert np.array_equal(bits(t.centroids()), bits(cen))
+    assert np.array_equal(t.face_counts(), nf)
+    off, ids = t.neighbors()
+    for i in range(n):
+        assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), i
+    for i in range(0, n, 41):
+        assert_same_cell(t.get_cell(i), o.get_cell(i), i)
+    ol = o.seal_log()
+    assert np.array_equal(t.seal_log(), ol)
+    if name != "mixed":
+        assert len(ol) > 0
+
+
+@pytest.mark.parametrize("name", ["sphere", "torus", "trefoil", "mixed"])
+@pytest.mark.parametrize("sealed", [False, True])
+def test_prune_boundaries_bit_exact(name, sealed):
+    t, o = pair(name, n=7000)
+    if sealed:
+        t.calculate_sealed()
+        o.calculate_sealed(threads=8)
+    else:
+        t.calculate()
+        o.calculate(threads=8)
+    t.prune_boundaries()
+    o.prune_boundaries()
+    ti, tp = t.prune_log()
+    oi, op = o.prune_log()
+    assert np.array_equal(ti, oi)
+    assert np.array_equal(bits(tp.reshape(-1)), bits(op.reshape(-1)))
+    assert len(oi) > 0
+    # calculate() clears both logs (tessellation.rs:214-216)
+    t.calculate()
+    assert len(t.prune_log()[0]) == 0 and len(t.seal_log()) == 0
+
+
+def test_prune_needs_full_cells():
+    from vorothree_amd import _lib
+    t = Tessellation(UNIT, Algorithm3DGrid(8, 8, 8, UNIT), keep_cells=False)
+    t.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.4)))
+    t.set_generators(W.PCG32(1010).uniform(2000, (0, 0, 0), (1, 1, 1)))
+    t.calculate()
+    with pytest.raises(_lib.VdError) as e:
+        t.prune_boundaries()
+    assert e.value.code == _lib.VD_ESTATE

@@ -137,3 +137,62 @@ def test_curve_constructors_hit_their_control_points():
     # open curve at t = 1: get_point (wall_3d.rs:853-855) moves to the last segment but keeps
     # weight 0, so the last sample repeats the second-to-last control point
     assert np.array_equal(o.samples[30], cp[2])
+
+
+def sealed_pair(walls, n=12_000, seed=990, grid=18):
+    pts = W.PCG32(seed).uniform(n, (0, 0, 0), (1, 1, 1))
+    o = OracleTess((0, 0, 0), (1, 1, 1), grid, grid, grid)
+    for w in walls:
+        oracle_wall(o, w)
+    o.set_generators(pts)
+    o.calculate(threads=8)
+    v0, _, _, _ = o.reduced()
+    nbs = [o.get_cell(i).face_neighbors for i in range(len(v0))]
+    o.calculate_sealed(threads=8)
+    v1, _, _, _ = o.reduced()
+    return o, v0, v1, nbs
+
+
+def test_sealing_only_shrinks_cells_and_keeps_the_ball():
+    from vorothree_amd import SphereGeometry
+    R = 0.45
+    o, v0, v1, nbs = sealed_pair([Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), R))])
+    ball = 4.0 / 3.0 * math.pi * R ** 3
+    # tangent planes of a convex wall never cut into it: the sealed union still covers the ball
+    assert ball <= v1.sum() < v0.sum()
+    assert np.all(v1 <= v0 + 1e-15)
+    log = o.seal_log()
+    assert len(log) > 0
+    assert set(log[:, 0]) == set(np.nonzero(v1 < v0)[0])
+    for i, j, wid in log[::7]:
+        assert j in nbs[i] and wid == -1000 and -1000 in nbs[i] and -1000 in nbs[j]
+
+
+def test_sealing_ignores_planar_walls():
+    cube = ConvexPolyhedronGeometry.new_hexahedron((0.5, 0.5, 0.5), 0.7)
+    from vorothree_amd import PlaneGeometry
+    walls = [Wall(-1000, cube), Wall(-1001, PlaneGeometry((0.5, 0.5, 0.5), (1.0, 0.2, 0.0)))]
+    o, v0, v1, _ = sealed_pair(walls, seed=991)
+    assert len(o.seal_log()) == 0
+    assert np.array_equal(v0.view(np.uint64), v1.view(np.uint64))
+
+
+def test_prune_vertices_lie_on_both_faces():
+    R = 0.42
+    from vorothree_amd import SphereGeometry
+    o, _, _, _ = sealed_pair([Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), R))], seed=992)
+    o.prune_boundaries()
+    ints, pos = o.prune_log()
+    rows = ints.reshape(-1, 4)
+    assert len(rows) > 0 and rows[:, 3].sum() == len(pos)
+    gen = o.generators().reshape(-1, 3)
+    k = 0
+    for i, j, wid, c in rows:
+        assert wid == -1000 and c >= 1
+        v = pos[k:k + c]
+        k += c
+        # on the bisector of generators i and j, and outside or on the sphere (tangent planes)
+        di = np.sum((v - gen[i]) ** 2, axis=1)
+        dj = np.sum((v - gen[j]) ** 2, axis=1)
+        assert np.all(np.abs(di - dj) < 1e-9)
+        assert np.all(np.linalg.norm(v - 0.5, axis=1) >= R - 1e-9)

@@ -51,6 +51,10 @@ SIGNATURES = {
     "vd_tess_set_generators_device": (ctypes.c_int, [_vp, _vp, _sz]),
     "vd_tess_calculate": (ctypes.c_int, [_vp]),
     "vd_tess_relax": (ctypes.c_int, [_vp]),
+    "vd_tess_calculate_sealed": (ctypes.c_int, [_vp]),
+    "vd_tess_prune_boundaries": (ctypes.c_int, [_vp]),
+    "vd_tess_get_seal_log": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "vd_tess_get_prune_log": (ctypes.c_int, [_vp, _vp, _sz, _vp, _sz, _vp]),
     "vd_tess_count_generators": (_sz, [_vp]),
     "vd_tess_count_cells": (_sz, [_vp]),
     "vd_tess_get_generators": (ctypes.c_int, [_vp, _vp, _sz]),

@@ -315,6 +315,9 @@ struct Engine {
     DevBuf d_out0, d_out1;             // first-round retry lists of tier 0
     DevBuf d_wdata;                    // wall planes and samples of every member
     DevBuf d_tube;                     // per generator: planes of its member's tube walls
+    // post-passes: cell topology of the previous pass, seal log, prune log
+    int seal = 0;                      // cell launches run the seal mode
+    DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
     uint32_t ntube = 0;                // tube-wall slots per generator
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
     uint64_t nraw = 0, ngen = 0, nbins = 0;
@@ -356,6 +359,9 @@ void engine_free(Engine &E) {
     release(E.d_out1);
     release(E.d_wdata);
     release(E.d_tube);
+    DevBuf *post[] = {&E.d_smask, &E.d_scnt, &E.d_soff, &E.d_snb, &E.d_slog, &E.d_slogcnt, &E.d_pcnt, &E.d_poff,
+                      &E.d_pi, &E.d_pp};
+    for (DevBuf *b : post) release(*b);
     if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
     E.gexec = nullptr;
     if (E.h_counts) (void)hipHostFree(E.h_counts);
@@ -629,6 +635,13 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.bin_start = E.d_bin_start.as<uint32_t>();
     a.tube_planes = E.d_tube.as<double>();
     a.ntube = E.ntube;
+    if (E.seal) {
+        a.seal_mask = E.d_smask.as<uint32_t>();
+        a.seal_off = E.d_soff.as<uint32_t>();
+        a.seal_nb = E.d_snb.as<uint32_t>();
+        a.seal_log = E.d_slog.as<int32_t>();
+        a.seal_logcnt = E.d_slogcnt.as<uint32_t>();
+    }
     a.cells = d_list;
     a.ncells = nlist;
     a.queue = E.d_queue.as<uint32_t>();
@@ -648,7 +661,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     int waves = E.cus * E.occ[tier];
     if ((uint32_t)waves > nlist) waves = (int)nlist;
     if (ms) VD_HIP(hipEventRecord(E.ev0, E.stream));
-    VD_HIP(vd::launch_cells(tier, a, waves, E.stream));
+    VD_HIP(vd::launch_cells(tier, a, waves, E.stream, E.seal != 0));
     if (ms) {
         VD_HIP(hipEventRecord(E.ev1, E.stream));
         VD_HIP(hipEventSynchronize(E.ev1));
@@ -676,12 +689,13 @@ int enqueue_round0(Engine &E, uint32_t n, float *ms) {
 // options, buffer addresses and the member descriptors. Equal keys give identical launches.
 std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
-                               E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.mem.size(), E.ntube};
+                               E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.mem.size(), E.ntube,
+                               (uint64_t)E.seal};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
                             &E.d_top, &E.d_queue, &E.d_counts4, &E.d_out0, &E.d_out1, &E.d_tube,
-                            &E.d_wdata};
+                            &E.d_wdata, &E.d_smask, &E.d_soff, &E.d_snb, &E.d_slog, &E.d_slogcnt};
     for (const DevBuf *b : bufs) k.push_back(reinterpret_cast<uint64_t>(b->p));
     k.push_back(reinterpret_cast<uint64_t>(E.h_counts));
     const size_t words = E.desc.size() * sizeof(vd::TessDesc) / 8;
@@ -887,6 +901,98 @@ int engine_calculate(Engine &E) {
     return VD_OK;
 }
 
+// Topology of the last calculate for the post-passes (tessellation.rs:244-264): per cell the mask
+// of its curved walls and, for cells with one, the neighbour list in face order.
+int engine_cell_topology(Engine &E, bool lists) {
+    const uint64_t n = E.ngen;
+    VD_TRY(ensure(E.d_smask, (n + 1) * 4));
+    VD_TRY(ensure(E.d_scnt, (n + 1) * 4));
+    VD_TRY(ensure(E.d_soff, (n + 1) * 4));
+    VD_HIP(vd::launch_fill_u32(E.d_scnt.as<uint32_t>(), 0u, n + 1, E.stream));
+    VD_HIP(vd::launch_fill_u32(E.d_smask.as<uint32_t>(), 0u, n + 1, E.stream));
+    VD_HIP(vd::launch_cell_topology(E.d_desc.as<vd::TessDesc>(), E.d_gen_tess.as<uint32_t>(), n,
+                                    E.d_nface.as<uint32_t>(), E.d_cell_off.as<uint64_t>(), E.d_fnb.as<int32_t>(),
+                                    E.d_smask.as<uint32_t>(), E.d_scnt.as<uint32_t>(), E.stream));
+    if (!lists) return VD_OK;
+    VD_TRY(scan(E, E.d_scnt.as<uint32_t>(), E.d_soff.as<uint32_t>(), (uint32_t)(n + 1)));
+    uint32_t total = 0;
+    VD_HIP(hipMemcpyAsync(&total, E.d_soff.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    VD_TRY(ensure(E.d_snb, ((size_t)total + 1) * 4));
+    VD_TRY(ensure(E.d_slog, ((size_t)total + 1) * 3 * vd::kMaxWalls * 4));
+    VD_TRY(ensure(E.d_slogcnt, (n + 1) * 4));
+    VD_HIP(vd::launch_cell_neighbors(E.d_desc.as<vd::TessDesc>(), E.d_gen_tess.as<uint32_t>(), n,
+                                     E.d_nface.as<uint32_t>(), E.d_cell_off.as<uint64_t>(), E.d_fnb.as<int32_t>(),
+                                     E.d_smask.as<uint32_t>(), E.d_soff.as<uint32_t>(), E.d_snb.as<uint32_t>(),
+                                     E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    return VD_OK;
+}
+
+// calculate_sealed (tessellation.rs:234-318): calculate, the topology of its cells, then a second
+// calculate whose cells take the sealing clips after their walk. log: the entries in cell order.
+int engine_calculate_sealed(Engine &E, std::vector<int32_t> &log) {
+    log.clear();
+    VD_TRY(engine_calculate(E));
+    const uint64_t n = E.ngen;
+    if (n == 0) return VD_OK;
+    VD_TRY(engine_cell_topology(E, true));
+    E.seal = 1;
+    const int rc = engine_calculate(E);
+    E.seal = 0;
+    VD_TRY(rc);
+    std::vector<uint32_t> cnt(n), off(n + 1);
+    VD_HIP(hipMemcpyAsync(cnt.data(), E.d_slogcnt.p, n * 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipMemcpyAsync(off.data(), E.d_soff.p, (n + 1) * 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    std::vector<int32_t> all((size_t)off[n] * 3 * vd::kMaxWalls);
+    if (!all.empty())
+        VD_HIP(hipMemcpyAsync(all.data(), E.d_slog.p, all.size() * 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    for (uint64_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < cnt[i]; ++k)
+            for (int c = 0; c < 3; ++c) log.push_back(all[(size_t)off[i] * 3 * vd::kMaxWalls + 3 * k + c]);
+    return VD_OK;
+}
+
+// prune_boundaries (tessellation.rs:324-385) over the cells of the last calculate; needs full cells.
+int engine_prune(Engine &E, std::vector<int32_t> &log, std::vector<double> &pos) {
+    if (!E.calculated) return fail(VD_ESTATE, "prune_boundaries() needs calculate() first");
+    if (!E.keep_cells) return fail(VD_ESTATE, "prune_boundaries() needs full cells (VD_OPT_KEEP_CELLS)");
+    log.clear();
+    pos.clear();
+    const uint64_t n = E.ngen;
+    if (n == 0) return VD_OK;
+    VD_TRY(engine_cell_topology(E, false));
+    VD_TRY(ensure(E.d_pcnt, (n + 1) * 8));
+    VD_TRY(ensure(E.d_poff, (n + 1) * 8));
+    uint32_t *icnt = E.d_pcnt.as<uint32_t>(), *pcnt = icnt + (n + 1);
+    uint32_t *ioff = E.d_poff.as<uint32_t>(), *poff = ioff + (n + 1);
+    VD_HIP(vd::launch_fill_u32(icnt, 0u, 2 * (n + 1), E.stream));
+    const auto prune = [&](int32_t *oi, double *op) {
+        return vd::launch_prune(E.d_desc.as<vd::TessDesc>(), E.d_gen_tess.as<uint32_t>(), n,
+                                E.d_nface.as<uint32_t>(), E.d_cell_off.as<uint64_t>(), E.d_fnb.as<int32_t>(),
+                                E.d_fcnt.as<uint8_t>(), E.d_fidx.as<uint16_t>(), E.d_vtx.as<double>(),
+                                E.d_smask.as<uint32_t>(), icnt, pcnt, ioff, poff, oi, op, E.stream);
+    };
+    VD_HIP(prune(nullptr, nullptr));
+    VD_TRY(scan(E, icnt, ioff, (uint32_t)(n + 1)));
+    VD_TRY(scan(E, pcnt, poff, (uint32_t)(n + 1)));
+    uint32_t tot[2] = {0, 0};
+    VD_HIP(hipMemcpyAsync(&tot[0], ioff + n, 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipMemcpyAsync(&tot[1], poff + n, 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    VD_TRY(ensure(E.d_pi, ((size_t)tot[0] + 1) * 4));
+    VD_TRY(ensure(E.d_pp, ((size_t)tot[1] + 1) * 8));
+    VD_HIP(prune(E.d_pi.as<int32_t>(), E.d_pp.as<double>()));
+    log.resize(tot[0]);
+    pos.resize(tot[1]);
+    if (tot[0]) VD_HIP(hipMemcpyAsync(log.data(), E.d_pi.p, (size_t)tot[0] * 4, hipMemcpyDeviceToHost, E.stream));
+    if (tot[1]) VD_HIP(hipMemcpyAsync(pos.data(), E.d_pp.p, (size_t)tot[1] * 8, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    return VD_OK;
+}
+
 // relax() of every member (tessellation.rs:482-495): new input points, then set_generators.
 int engine_relax(Engine &E) {
     if (!E.calculated) return fail(VD_ESTATE, "relax() needs calculate() first");
@@ -983,6 +1089,9 @@ int check_grid(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t
 
 struct vd_tess {
     Engine E;
+    std::vector<int32_t> seal_log;     // (cell, neighbour, wall id) per sealing clip
+    std::vector<int32_t> prune_log;    // (cell, neighbour, wall id, vertex count) per entry
+    std::vector<double> prune_pos;     // the vertices of the prune entries, xyz
 };
 
 struct vd_batch {
@@ -1221,7 +1330,38 @@ int vd_tess_set_generators_device(vd_tess *t, const double *d_xyz, size_t n) {
 
 int vd_tess_calculate(vd_tess *t) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
+    t->seal_log.clear();   // tessellation.rs:214-216
+    t->prune_log.clear();
+    t->prune_pos.clear();
     return engine_calculate(t->E);
+}
+
+int vd_tess_calculate_sealed(vd_tess *t) {
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    t->prune_log.clear();
+    t->prune_pos.clear();
+    return engine_calculate_sealed(t->E, t->seal_log);
+}
+
+int vd_tess_prune_boundaries(vd_tess *t) {
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    return engine_prune(t->E, t->prune_log, t->prune_pos);
+}
+
+int vd_tess_get_seal_log(const vd_tess *t, int32_t *out, size_t cap, size_t *count) {
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    if (out) std::memcpy(out, t->seal_log.data(), std::min(cap, t->seal_log.size()) * 4);
+    if (count) *count = t->seal_log.size();
+    return VD_OK;
+}
+
+int vd_tess_get_prune_log(const vd_tess *t, int32_t *ints, size_t icap, double *pos, size_t pcap,
+                          size_t counts[2]) {
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    if (ints) std::memcpy(ints, t->prune_log.data(), std::min(icap, t->prune_log.size()) * 4);
+    if (pos) std::memcpy(pos, t->prune_pos.data(), std::min(pcap, t->prune_pos.size()) * 8);
+    if (counts) { counts[0] = t->prune_log.size(); counts[1] = t->prune_pos.size(); }
+    return VD_OK;
 }
 
 int vd_tess_relax(vd_tess *t) {

@@ -883,7 +883,7 @@ __device__ inline int probe_inert(const ShellCode *shell, uint32_t nshell, const
 
 // One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports. The
 // walls and the neighbour candidates feed a single clip site (one inlined copy of the clip).
-template <class C>
+template <class C, bool SEAL>
 __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     const int l = lane_id();
     const TessDesc &D = A.desc[A.gen_tess[i]];   // this cell's tessellation (wave-uniform)
@@ -1126,6 +1126,65 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     }
     __syncthreads();
 
+    if constexpr (SEAL) {
+        // calculate_sealed (tessellation.rs:266-315): for every neighbour of the previous pass, in
+        // face order, and every curved wall both cells touched, the wall's plane for the
+        // neighbour's generator clips this cell when this generator is on its valid side
+        const uint32_t mi = A.seal_mask[i];
+        uint32_t nlog = 0;
+        if (status == kCellOk && mi) {
+            const uint32_t q0 = A.seal_off[i], q1 = A.seal_off[i + 1];
+            int32_t *lg = A.seal_log + 3 * (size_t)kMaxWalls * q0;
+            for (uint32_t q = q0; q < q1 && status == kCellOk; ++q) {
+                const uint32_t j = A.seal_nb[q];
+                const size_t gj = D.gen_off + j;
+                const uint32_t mj = A.seal_mask[gj];
+                const double nx = A.gen[3 * gj], ny = A.gen[3 * gj + 1], nz = A.gen[3 * gj + 2];
+                uint32_t tk = 0;
+                for (int w = 0; w < nwalls; ++w) {
+                    const WallDev &W = D.walls[w];
+                    const bool tube = W.kind == kWallTube;
+                    const bool use = W.kind != kWallPlane && W.kind != kWallPolyhedron && ((mi >> w) & 1u) &&
+                                     ((mj >> w) & 1u);
+                    double pp[3], pn[3];
+                    bool has = false;
+                    if (use) {
+                        if (tube) {
+                            const double *tp = A.tube_planes + 7 * (gj * A.ntube + tk);
+                            has = tp[6] != 0.0;
+                            for (int k = 0; k < 3; ++k) { pp[k] = tp[k]; pn[k] = tp[3 + k]; }
+                        } else {
+                            has = wall_plane(W, 0u, nx, ny, nz, pp, pn);
+                        }
+                    }
+                    if (tube) ++tk;
+                    if (!has) continue;
+                    double dot = 0.0;
+                    dot += (gx - pp[0]) * pn[0];
+                    dot += (gy - pp[1]) * pn[1];
+                    dot += (gz - pp[2]) * pn[2];
+                    if (!(dot <= 1e-9)) continue;
+                    double wr2 = 0.0;
+                    const int rc = clip_any<C>(m, P, r, Plane{pp[0], pp[1], pp[2], pn[0], pn[1], pn[2]}, W.id,
+                                               false, gx, gy, gz, wr2);
+                    if (rc < 0) {
+                        status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
+                        break;
+                    }
+                    if (rc == kModified) {
+                        if (l == 0) {
+                            lg[3 * nlog] = (int32_t)li;
+                            lg[3 * nlog + 1] = (int32_t)j;
+                            lg[3 * nlog + 2] = W.id;
+                        }
+                        ++nlog;
+                    }
+                }
+            }
+        }
+        if (l == 0) A.seal_logcnt[i] = nlog;
+    }
+
     // outputs
     if (l == 0) A.status[i] = status;
     if (status != kCellOk) { __syncthreads(); return; }
@@ -1184,7 +1243,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     __syncthreads();
 }
 
-template <class C>
+template <class C, bool SEAL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C::WAVES, 8)))
 void k_cells(CellArgs A) {
     __shared__ Mem<C> m;
@@ -1193,7 +1252,7 @@ void k_cells(CellArgs A) {
         if (lane_id() == 0) q = atomicAdd(A.queue, 1u);
         q = first_lane_u32(q);
         if (q >= A.ncells) break;
-        process_cell<C>(m, A, A.cells[q]);
+        process_cell<C, SEAL>(m, A, A.cells[q]);
     }
 }
 
@@ -1220,23 +1279,29 @@ size_t tier_lds_bytes(int tier) {
          : tier == 2 ? sizeof(Mem<Tier2>) : sizeof(Mem<Tier3>);
 }
 
-hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st) {
+template <bool SEAL>
+void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st) {
+    if (tier == 0) k_cells<Tier0, SEAL><<<grid_waves, 64, 0, st>>>(a);
+    else if (tier == 1) k_cells<Tier1, SEAL><<<grid_waves, 64, 0, st>>>(a);
+    else if (tier == 2) k_cells<Tier2, SEAL><<<grid_waves, 64, 0, st>>>(a);
+    else k_cells<Tier3, SEAL><<<grid_waves, 64, 0, st>>>(a);
+}
+
+hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal) {
     if (a.ncells == 0) return hipSuccess;
-    if (tier == 0) k_cells<Tier0><<<grid_waves, 64, 0, st>>>(a);
-    else if (tier == 1) k_cells<Tier1><<<grid_waves, 64, 0, st>>>(a);
-    else if (tier == 2) k_cells<Tier2><<<grid_waves, 64, 0, st>>>(a);
-    else k_cells<Tier3><<<grid_waves, 64, 0, st>>>(a);
+    if (seal) launch_tier<true>(tier, a, grid_waves, st);
+    else launch_tier<false>(tier, a, grid_waves, st);
     return hipGetLastError();
 }
 
 hipError_t cells_occupancy(int tier, int *blocks_per_cu) {
     if (tier == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0>, 64, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0, false>, 64, 0);
     if (tier == 1)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1>, 64, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1, false>, 64, 0);
     if (tier == 2)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2>, 64, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier3>, 64, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2, false>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier3, false>, 64, 0);
 }
 
 }  // namespace vd

@@ -68,6 +68,136 @@ __global__ void k_tube_planes(const double *__restrict__ gen, uint64_t n, const 
     }
 }
 
+// ---- post-passes: calculate_sealed / prune_boundaries (tessellation.rs:234-385) -------------
+
+__device__ inline bool wall_planar(int32_t kind) {   // WallGeometry::is_planar (wall_3d.rs:44, 709)
+    return kind == kWallPlane || kind == kWallPolyhedron;
+}
+
+// The walls whose id is in a cell's curved-wall set: for every face whose neighbour id is not a
+// generator of the tessellation, the first wall with that id decides (walls.iter().find); if it is
+// curved, the id joins the set (tessellation.rs:248-257), and every wall with that id gets its bit.
+__device__ inline uint32_t curved_wall_mask(const TessDesc &D, const int32_t *fl, uint32_t F, uint32_t &nbc) {
+    uint32_t m = 0;
+    nbc = 0;
+    for (uint32_t f = 0; f < F; ++f) {
+        const int32_t fn = fl[f];
+        if (fn >= 0 && (uint32_t)fn < D.ngen) { ++nbc; continue; }
+        int w0 = -1;
+        for (int w = 0; w < D.nwalls; ++w)
+            if (D.walls[w].id == fn) { w0 = w; break; }
+        if (w0 < 0 || wall_planar(D.walls[w0].kind)) continue;
+        for (int w = 0; w < D.nwalls; ++w)
+            if (D.walls[w].id == fn) m |= 1u << w;
+    }
+    return m;
+}
+
+__global__ void k_cell_topology(const TessDesc *__restrict__ desc, const uint32_t *__restrict__ tess, uint64_t n,
+                                const uint32_t *__restrict__ nface, const uint64_t *__restrict__ cell_off,
+                                const int32_t *__restrict__ fnb, uint32_t *__restrict__ mask,
+                                uint32_t *__restrict__ cnt) {
+    const uint64_t g = gid();
+    if (g >= n) return;
+    const TessDesc &D = desc[tess[g]];
+    uint32_t nbc = 0;
+    const uint32_t m = curved_wall_mask(D, fnb + cell_off[3 * g + 1], nface[g], nbc);
+    mask[g] = m;
+    cnt[g] = m ? nbc : 0u;
+}
+
+__global__ void k_cell_neighbors(const TessDesc *__restrict__ desc, const uint32_t *__restrict__ tess, uint64_t n,
+                                 const uint32_t *__restrict__ nface, const uint64_t *__restrict__ cell_off,
+                                 const int32_t *__restrict__ fnb, const uint32_t *__restrict__ mask,
+                                 const uint32_t *__restrict__ off, uint32_t *__restrict__ nb) {
+    const uint64_t g = gid();
+    if (g >= n || !mask[g]) return;
+    const TessDesc &D = desc[tess[g]];
+    const int32_t *fl = fnb + cell_off[3 * g + 1];
+    uint32_t k = off[g];
+    for (uint32_t f = 0; f < nface[g]; ++f) {
+        const int32_t fn = fl[f];
+        if (fn >= 0 && (uint32_t)fn < D.ngen) nb[k++] = (uint32_t)fn;   // face order, as listed
+    }
+}
+
+// prune_boundaries for one cell: neighbours in first-occurrence face order (deduplicated,
+// tessellation.rs:335-339), curved walls in wall order shared with the neighbour, and for each the
+// vertices of the cell's wall faces (first occurrence order) that also lie on a face shared with
+// the neighbour (Cell3DFaces::shared_vertices, cell_3d_faces.rs:536-568).
+__global__ void k_prune(const TessDesc *__restrict__ desc, const uint32_t *__restrict__ tess, uint64_t n,
+                        const uint32_t *__restrict__ nface, const uint64_t *__restrict__ cell_off,
+                        const int32_t *__restrict__ fnb, const uint8_t *__restrict__ fcnt,
+                        const uint16_t *__restrict__ fidx, const double *__restrict__ vtx,
+                        const uint32_t *__restrict__ mask, uint32_t *__restrict__ icnt, uint32_t *__restrict__ pcnt,
+                        const uint32_t *__restrict__ ioff, const uint32_t *__restrict__ poff,
+                        int32_t *__restrict__ out_i, double *__restrict__ out_p) {
+    const uint64_t g = gid();
+    if (g >= n) return;
+    const uint32_t mi = mask[g];
+    uint32_t ni = 0, np = 0;
+    if (mi) {
+        const TessDesc &D = desc[tess[g]];
+        const uint32_t li = (uint32_t)(g - D.gen_off);
+        const uint32_t F = nface[g];
+        const int32_t *fl = fnb + cell_off[3 * g + 1];
+        const uint8_t *fc = fcnt + cell_off[3 * g + 1];
+        const uint16_t *fi = fidx + cell_off[3 * g + 2];
+        const double *vv = vtx + 3 * cell_off[3 * g];
+        for (uint32_t f = 0; f < F; ++f) {
+            const int32_t j = fl[f];
+            if (!(j >= 0 && (uint32_t)j < D.ngen)) continue;
+            bool seen = false;
+            for (uint32_t e = 0; e < f && !seen; ++e) seen = fl[e] == j;
+            if (seen) continue;
+            const uint32_t mj = mask[D.gen_off + (uint32_t)j];
+            for (int w = 0; w < D.nwalls; ++w) {
+                const WallDev &W = D.walls[w];
+                if (wall_planar(W.kind) || !((mi >> w) & 1u) || !((mj >> w) & 1u)) continue;
+                // shared_vertices(W.id, j)
+                uint32_t c = 0;
+                uint32_t s = 0;
+                for (uint32_t fa = 0; fa < F; s += fc[fa], ++fa) {
+                    if (fl[fa] != W.id) continue;
+                    for (uint32_t q = 0; q < fc[fa]; ++q) {
+                        const uint16_t v = fi[s + q];
+                        // first occurrence among the wall faces' slots so far
+                        bool dup = false;
+                        uint32_t s2 = 0;
+                        for (uint32_t fb = 0; fb <= fa && !dup; s2 += fc[fb], ++fb) {
+                            if (fl[fb] != W.id) continue;
+                            const uint32_t lim = fb == fa ? q : fc[fb];
+                            for (uint32_t r = 0; r < lim && !dup; ++r) dup = fi[s2 + r] == v;
+                        }
+                        if (dup) continue;
+                        bool onb = false;
+                        uint32_t s3 = 0;
+                        for (uint32_t fb = 0; fb < F && !onb; s3 += fc[fb], ++fb) {
+                            if (fl[fb] != j) continue;
+                            for (uint32_t r = 0; r < fc[fb] && !onb; ++r) onb = fi[s3 + r] == v;
+                        }
+                        if (!onb) continue;
+                        if (out_p) {
+                            double *o = out_p + poff[g] + np + 3 * c;
+                            o[0] = vv[3 * v]; o[1] = vv[3 * v + 1]; o[2] = vv[3 * v + 2];
+                        }
+                        ++c;
+                    }
+                }
+                if (c) {
+                    if (out_i) {
+                        int32_t *o = out_i + ioff[g] + ni;
+                        o[0] = (int32_t)li; o[1] = j; o[2] = W.id; o[3] = (int32_t)c;
+                    }
+                    ni += 4;
+                    np += 3 * c;
+                }
+            }
+        }
+    }
+    if (!out_i) { icnt[g] = ni; pcnt[g] = np; }
+}
+
 // Keeps the flagged points in input order (indices are compacted, tessellation.rs:58-60).
 __global__ void k_compact(const double *__restrict__ xyz, const uint32_t *__restrict__ tess_in,
                           const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos, uint64_t n,
@@ -328,6 +458,32 @@ hipError_t launch_tube_planes(const double *gen, uint64_t n, const TessDesc *des
                               uint32_t ntube, double *out, hipStream_t st) {
     if (!n || !ntube) return hipSuccess;
     k_tube_planes<<<blocks_for(n), kBlock, 0, st>>>(gen, n, desc, tess, ntube, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_topology(const TessDesc *desc, const uint32_t *tess, uint64_t n, const uint32_t *nface,
+                                const uint64_t *cell_off, const int32_t *fnb, uint32_t *mask, uint32_t *cnt,
+                                hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_cell_topology<<<blocks_for(n), kBlock, 0, st>>>(desc, tess, n, nface, cell_off, fnb, mask, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_neighbors(const TessDesc *desc, const uint32_t *tess, uint64_t n, const uint32_t *nface,
+                                 const uint64_t *cell_off, const int32_t *fnb, const uint32_t *mask,
+                                 const uint32_t *off, uint32_t *nb, hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_cell_neighbors<<<blocks_for(n), kBlock, 0, st>>>(desc, tess, n, nface, cell_off, fnb, mask, off, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_prune(const TessDesc *desc, const uint32_t *tess, uint64_t n, const uint32_t *nface,
+                        const uint64_t *cell_off, const int32_t *fnb, const uint8_t *fcnt, const uint16_t *fidx,
+                        const double *vtx, const uint32_t *mask, uint32_t *icnt, uint32_t *pcnt,
+                        const uint32_t *ioff, const uint32_t *poff, int32_t *out_i, double *out_p, hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_prune<<<blocks_for(n), kBlock, 0, st>>>(desc, tess, n, nface, cell_off, fnb, fcnt, fidx, vtx, mask, icnt,
+                                              pcnt, ioff, poff, out_i, out_p);
     return hipGetLastError();
 }
 

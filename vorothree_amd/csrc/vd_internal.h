@@ -114,6 +114,14 @@ struct CellArgs {
     uint64_t *arena_top;          // [0] vertices, [1] faces, [2] face indices
     uint64_t arena_cap[3];
     uint64_t *cell_off;           // 3 per cell: vertex, face, index offsets
+
+    // seal mode (calculate_sealed, tessellation.rs:234-318): cell topology of the previous pass
+    const uint32_t *seal_mask;    // per cell: walls (bit = wall index) of its curved-wall set
+    const uint32_t *seal_off;     // per cell: its neighbour list in seal_nb (n + 1 offsets)
+    const uint32_t *seal_nb;      // member-local neighbour indices in face order
+    int32_t *seal_log;            // (cell, neighbour, wall id) per modifying clip, kMaxWalls
+                                  // entries per neighbour from 3 * kMaxWalls * seal_off[i]
+    uint32_t *seal_logcnt;        // per cell: log entries written
 };
 
 // Status codes written per cell.
@@ -148,6 +156,22 @@ hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n
                             const uint32_t *tess, double4 *spos, hipStream_t st);
 hipError_t launch_tube_planes(const double *gen, uint64_t n, const TessDesc *desc, const uint32_t *tess,
                               uint32_t ntube, double *out, hipStream_t st);   // 7 doubles per slot
+// Post-pass topology of the cells of the last launch (tessellation.rs:244-264, 330-350): per cell
+// the wall mask of its curved-wall set and its count of neighbour faces (0 <= id < member count).
+// cnt[i] is 0 for a cell whose mask is empty.
+hipError_t launch_cell_topology(const TessDesc *desc, const uint32_t *tess, uint64_t n, const uint32_t *nface,
+                                const uint64_t *cell_off, const int32_t *fnb, uint32_t *mask, uint32_t *cnt,
+                                hipStream_t st);
+// The neighbour lists of the cells with a non-empty mask, at off[i] (exclusive scan of cnt).
+hipError_t launch_cell_neighbors(const TessDesc *desc, const uint32_t *tess, uint64_t n, const uint32_t *nface,
+                                 const uint64_t *cell_off, const int32_t *fnb, const uint32_t *mask,
+                                 const uint32_t *off, uint32_t *nb, hipStream_t st);
+// prune_boundaries (tessellation.rs:324-385) per cell: count (ints, doubles) when out_i is null,
+// else write the log at (ioff[i], poff[i]).
+hipError_t launch_prune(const TessDesc *desc, const uint32_t *tess, uint64_t n, const uint32_t *nface,
+                        const uint64_t *cell_off, const int32_t *fnb, const uint8_t *fcnt, const uint16_t *fidx,
+                        const double *vtx, const uint32_t *mask, uint32_t *icnt, uint32_t *pcnt,
+                        const uint32_t *ioff, const uint32_t *poff, int32_t *out_i, double *out_p, hipStream_t st);
 hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint64_t n,
                         double *out, hipStream_t st);
 hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, uint32_t *flags,
@@ -170,7 +194,7 @@ hipError_t shell_finish(const ShellPack &pack, const double cs[3], uint32_t w, u
 // ---- cell kernel (vd_cells.hip) ------------------------------------------------------------
 // tier: 0 = register fast path with a small LDS polytope (many waves per CU); 1, 2, 3 = generic
 // path with LDS polytopes of 128, 256 and 1024 vertices (the last one wave per CU)
-hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st);
+hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal = false);
 hipError_t cells_occupancy(int tier, int *blocks_per_cu);
 int tier_count();
 int tier_max_vertices(int tier);

@@ -468,6 +468,32 @@ class Tessellation:
     def relax(self) -> None:
         check(lib().vd_tess_relax(self._h))
 
+    def calculate_sealed(self) -> None:
+        """calculate(), then the curved-wall sealing pass (vd_tess_calculate_sealed)."""
+        check(lib().vd_tess_calculate_sealed(self._h))
+
+    def prune_boundaries(self) -> None:
+        """The boundary-vertex log of the current cells (vd_tess_prune_boundaries)."""
+        check(lib().vd_tess_prune_boundaries(self._h))
+
+    def seal_log(self) -> np.ndarray:
+        """(cell, neighbour, wall id) rows, one per sealing clip that changed a cell."""
+        n = ctypes.c_size_t(0)
+        check(lib().vd_tess_get_seal_log(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.int32)
+        check(lib().vd_tess_get_seal_log(self._h, _ptr(out), out.size, ctypes.byref(n)))
+        return out.reshape(-1, 3)
+
+    def prune_log(self):
+        """(ints, positions): the flat prune log ((cell, neighbour, wall id, count) per entry) and
+        the vertices of the entries (count rows of xyz each)."""
+        c = np.zeros(2, dtype=np.uint64)
+        check(lib().vd_tess_get_prune_log(self._h, None, 0, None, 0, _ptr(c)))
+        ints = np.zeros(int(c[0]), dtype=np.int32)
+        pos = np.zeros(int(c[1]), dtype=np.float64)
+        check(lib().vd_tess_get_prune_log(self._h, _ptr(ints), ints.size, _ptr(pos), pos.size, _ptr(c)))
+        return ints, pos.reshape(-1, 3)
+
     def set_keep_cells(self, keep: bool) -> None:
         check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_KEEP_CELLS, 1 if keep else 0))
 
