@@ -137,6 +137,7 @@ struct Cap {
     static constexpr bool FAST = FAST_; // register fast path (VM <= 64) instead of the generic one
     static constexpr int TM = TM_;      // triangles staged in LDS for the volume/centroid folds
     static constexpr int WAVES = WAVES_;  // occupancy target (waves per SIMD) for the compiler
+    static constexpr int NV = FAST_ ? (VM_ + 63) / 64 : 1;   // register vertices per lane
 };
 
 template <class C>
@@ -180,19 +181,48 @@ struct Plane { double px, py, pz, nx, ny, nz; };
 // polytope, vertices [0, nk) are bit-exact copies of the kept vertices and [nk, V) are new.
 struct Poly { int cur, V, F, S, nk; };
 
-// Lane v's copy of vertex v (valid while V <= 64).
-struct VReg { double x, y, z; bool ok; };
+// Lane l's register copies of vertices l, 64 + l, ... (NV per lane; the fast tiers only).
+template <int NV>
+struct VRegN { double x[NV], y[NV], z[NV]; };
 
 template <class C>
-__device__ inline void vreg_load(Mem<C> &m, const Poly &P, VReg &r) {
-    const int l = lane_id();
-    r.ok = P.V <= 64;
-    if (r.ok && l < P.V) {
-        r.x = m.vx[P.cur][l];
-        r.y = m.vy[P.cur][l];
-        r.z = m.vz[P.cur][l];
+using VReg = VRegN<C::NV>;
+
+template <class C>
+__device__ inline void vreg_load(Mem<C> &m, const Poly &P, VReg<C> &r) {
+    if constexpr (C::FAST) {
+        const int l = lane_id();
+#pragma unroll
+        for (int k = 0; k < C::NV; ++k) {
+            const int v = 64 * k + l;
+            if (v < P.V) {
+                r.x[k] = m.vx[P.cur][v];
+                r.y[k] = m.vy[P.cur][v];
+                r.z[k] = m.vz[P.cur][v];
+            }
+        }
     }
 }
+
+// A vertex set as NV 64-bit ballots (vertex v in word v / 64, bit v % 64).
+template <int NV>
+struct VMask {
+    uint64_t w[NV];
+    __device__ bool test(uint32_t v) const {
+        if constexpr (NV == 1) return (w[0] >> v) & 1ull;
+        else return v < 64 ? ((w[0] >> v) & 1ull) : ((w[1] >> (v - 64)) & 1ull);
+    }
+    __device__ int below(uint32_t v) const {   // members with an index below v
+        if constexpr (NV == 1) return popc(w[0] & ((1ull << v) - 1ull));
+        else return v < 64 ? popc(w[0] & ((1ull << v) - 1ull)) : popc(w[0]) + popc(w[1] & ((1ull << (v - 64)) - 1ull));
+    }
+    __device__ int count() const {
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) c += popc(w[k]);
+        return c;
+    }
+};
 
 template <class C>
 __device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi[3]) {
@@ -472,58 +502,83 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
 }
 
 
-// Fast clip for V <= 64, S <= 192, F <= 64 (the common case): vertex distances from the register
-// copy, slot data of the clip held in registers (three 64-slot chunks), crossings and lid cycle
-// handled with lane shuffles. Same result as clip_generic. Returns kGeneric (with the current
-// polytope untouched) when the crossings or the lid do not fit its register layout.
+// Fast clip for V <= 64 * NV, S <= 192 * NV, F <= 64: vertex distances from the register copies
+// (NV vertices per lane), slot data of the clip held in registers (3 * NV chunks of 64 slots),
+// crossings and lid cycle handled with lane shuffles. Same result as clip_generic. Returns kGeneric
+// (with the current polytope untouched) when the faces, crossings or the lid do not fit its
+// register layout.
 template <class C>
-__device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t nid, bool withgen,
+__device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t nid, bool withgen,
                          double gx, double gy, double gz, double &r2) {
+    constexpr int NV = C::NV;
+    constexpr int NCH = 3 * NV;                 // slot chunks
+    constexpr uint32_t SH = NV == 1 ? 6 : 7;    // bits of a vertex index in the packed slot word
+    constexpr uint32_t VMSK = (1u << SH) - 1u;
     const int l = lane_id();
     const uint64_t lt = lt_mask();
     const int c = P.cur, o = c ^ 1;
-    const bool va = l < P.V;
+    if (NV > 1 && P.F > 64) return kGeneric;
 
     // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362)
-    const double d = va ? (r.x - pl.px) * pl.nx + (r.y - pl.py) * pl.ny + (r.z - pl.pz) * pl.nz : 0.0;
-    const uint64_t gtm = ballot(va && d > kTol);
-    if (gtm == 0) return kNoChange;
-    const uint64_t ltm = ballot(va && d < -kTol);
-    if (ltm == 0) {
+    double d[NV];
+    bool va[NV];
+    VMask<NV> gtm, ltm, inm;
+    bool anyg = false, anyl = false;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        va[k] = 64 * k + l < P.V;
+        d[k] = va[k] ? (r.x[k] - pl.px) * pl.nx + (r.y[k] - pl.py) * pl.ny + (r.z[k] - pl.pz) * pl.nz : 0.0;
+        gtm.w[k] = ballot(va[k] && d[k] > kTol);
+        anyg |= gtm.w[k] != 0;
+    }
+    if (!anyg) return kNoChange;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        ltm.w[k] = ballot(va[k] && d[k] < -kTol);
+        anyl |= ltm.w[k] != 0;
+    }
+    if (!anyl) {
         P.V = P.F = P.S = P.nk = 0;
         r2 = 0.0;
-        r.ok = true;
         return kModified;
     }
-    const uint64_t inm = ballot(va && d <= kTol);
-    const int nk = popc(inm);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) inm.w[k] = ballot(va[k] && d[k] <= kTol);
+    const int nk = inm.count();
     double md2 = 0.0;
-    if (va) m.dist[l] = d;
-    if ((inm >> l) & 1ull) {
-        const int ni = popc(inm & lt);
-        m.vx[o][ni] = r.x; m.vy[o][ni] = r.y; m.vz[o][ni] = r.z;
-        if (withgen) {
-            const double dx = r.x - gx, dy = r.y - gy, dz = r.z - gz;
-            md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
+    int kbase = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if (va[k]) m.dist[64 * k + l] = d[k];
+        if ((inm.w[k] >> l) & 1ull) {
+            const int ni = kbase + popc(inm.w[k] & lt);
+            m.vx[o][ni] = r.x[k]; m.vy[o][ni] = r.y[k]; m.vz[o][ni] = r.z[k];
+            if (withgen) {
+                const double dx = r.x[k] - gx, dy = r.y[k] - gy, dz = r.z[k] - gz;
+                md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
+            }
         }
+        kbase += popc(inm.w[k]);
     }
 
     // 2. slots: output entry counts (prefix in slot order) and plane crossings (:378-452).
-    //    Per chunk and lane two registers: ea = a | b << 6 | face << 12 (a, b < 64, face < 64)
-    //    and eq = output prefix | crossing index << 16.
+    //    Per chunk and lane two registers: ea = a | b << SH | face << 2 SH and
+    //    eq = output prefix | crossing index << 16.
     const int nch = (P.S + 63) >> 6;
-    uint32_t ea[3] = {0u, 0u, 0u};
-    uint32_t eq[3] = {0u, 0u, 0u};
+    uint32_t ea[NCH];
+    uint32_t eq[NCH];
     int K = 0, ptot = 0;
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
+    for (int ch = 0; ch < NCH; ++ch) {
+        ea[ch] = 0u;
+        eq[ch] = 0u;
         if (ch < nch) {
             const int s = 64 * ch + l;
             const bool sa = s < P.S;
             const uint32_t e = sa ? m.sedge[c][s] : 0u;
             const uint32_t a = e & 0xffffu, b = e >> 16;
-            ea[ch] = a | (b << 6) | ((sa ? (uint32_t)m.sface[c][s] : 0u) << 12);
-            const bool ain = (inm >> a) & 1ull, bin = (inm >> b) & 1ull;
+            ea[ch] = a | (b << SH) | ((sa ? (uint32_t)m.sface[c][s] : 0u) << (2 * SH));
+            const bool ain = inm.test(a), bin = inm.test(b);
             const uint64_t b0 = ballot(sa && ain);
             const uint64_t b1 = ballot(sa && !ain && bin);
             const uint32_t pp = (uint32_t)(ptot + popc(b0 & lt) + 2 * popc(b1 & lt));
@@ -608,19 +663,19 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
 
     // 5. emit the entries of every surviving face (:386-451), with each entry's successor slot
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
+    for (int ch = 0; ch < NCH; ++ch) {
         if (ch < nch) {
             const int s = 64 * ch + l;
-            const uint32_t a = ea[ch] & 63u, b = (ea[ch] >> 6) & 63u;
-            const bool ain = (inm >> a) & 1ull, bin = (inm >> b) & 1ull;
+            const uint32_t a = ea[ch] & VMSK, b = (ea[ch] >> SH) & VMSK;
+            const bool ain = inm.test(a), bin = inm.test(b);
             const int xv = (int)lane_get((uint32_t)xid, (int)(eq[ch] >> 16) & 63);
             if (s < P.S && (ain || bin)) {
-                const int f = (int)(ea[ch] >> 12);
+                const int f = (int)(ea[ch] >> (2 * SH));
                 const int nf = m.fnew[f];
                 if (nf >= 0) {
                     const int pos = m.fbase[f] + (int)(eq[ch] & 0xffffu);
                     const int ns = m.fnoff[f], ne = m.nend[f];
-                    const uint16_t rb = (uint16_t)popc(inm & ((1ull << b) - 1ull));
+                    const uint16_t rb = (uint16_t)inm.below(b);
                     m.fidxn[pos] = (ain && bin) ? rb : (uint16_t)xv;
                     m.sface[o][pos] = (uint16_t)nf;
                     m.nextp[pos] = (uint16_t)(pos + 1 == ne ? ns : pos + 1);
@@ -701,7 +756,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t ni
 }
 
 template <class C>
-__device__ inline int clip_any(Mem<C> &m, Poly &P, VReg &r, const Plane pl, int32_t nid, bool withgen,
+__device__ inline int clip_any(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t nid, bool withgen,
                                double gx, double gy, double gz, double &r2) {
     if constexpr (C::FAST) {
         // a polytope that leaves the register layout moves the cell to a generic tier
@@ -898,7 +953,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     const int nwalls = D.nwalls;
     Poly P;
     poly_box<C>(m, P, D.box_lo, D.box_hi);
-    VReg r;
+    VReg<C> r;
     vreg_load(m, P, r);
     int status = kCellOk;
     double r2 = 0.0;
@@ -1256,13 +1311,14 @@ void k_cells(CellArgs A) {
     }
 }
 
-// Tier 0: register fast path, small LDS (many waves per CU). Tiers 1-3: generic path for cells
-// that outgrow the tier before, with 128, 256 and 1024 vertices (the last one wave per CU).
+// Tier 0: register fast path (one vertex per lane), small LDS (many waves per CU). Tier 1: the
+// same path with two vertices per lane (128 vertices). Tiers 2-3: generic path for cells that
+// outgrow the tier before, with 256 and 1024 vertices (the last one wave per CU).
 #ifndef VD_TIER0_WAVES
 #define VD_TIER0_WAVES 4
 #endif
 using Tier0 = Cap<64, 192, 48, 64, true, 107, VD_TIER0_WAVES>;
-using Tier1 = Cap<128, 384, 72, 192, false, 127, 2>;
+using Tier1 = Cap<128, 384, 64, 64, true, 127, 2>;
 using Tier2 = Cap<256, 768, 128, 384, false, 255, 1>;
 using Tier3 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
 

@@ -192,8 +192,9 @@ hipError_t shell_finish(const ShellPack &pack, const double cs[3], uint32_t w, u
                         size_t scratch_bytes, ShellCode *out, uint64_t *next, hipStream_t st);
 
 // ---- cell kernel (vd_cells.hip) ------------------------------------------------------------
-// tier: 0 = register fast path with a small LDS polytope (many waves per CU); 1, 2, 3 = generic
-// path with LDS polytopes of 128, 256 and 1024 vertices (the last one wave per CU)
+// tier: 0 = register fast path with a small LDS polytope (many waves per CU); 1 = the fast path
+// with two vertices per lane (128 vertices); 2, 3 = generic path with LDS polytopes of 256 and 1024
+// vertices (the last one wave per CU)
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal = false);
 hipError_t cells_occupancy(int tier, int *blocks_per_cu);
 int tier_count();
