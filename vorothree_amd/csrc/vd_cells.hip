@@ -51,6 +51,10 @@ constexpr double kTol = 1e-9;
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
 __device__ inline uint64_t lt_mask() { return (1ull << lane_id()) - 1ull; }
+// Set bits of m below this lane (v_mbcnt_lo / v_mbcnt_hi): the prefix count of a ballot.
+__device__ inline int below_lane(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 __device__ inline uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
 __device__ inline int popc(uint64_t x) { return __popcll(x); }
 __device__ inline int ffs64(uint64_t x) { return x ? (int)__builtin_ctzll(x) : 64; }
@@ -262,7 +266,6 @@ template <class C>
 __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, bool withgen,
                             double gx, double gy, double gz, double &r2) {
     const int l = lane_id();
-    const uint64_t lt = lt_mask();
     const int c = P.cur, o = c ^ 1;
 
     // 1. signed distances, early outs, kept-vertex numbering (:303-362)
@@ -283,7 +286,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const bool in = act && d <= kTol;
         const uint64_t im = ballot(in);
         if (in) {
-            const int ni = nk + popc(im & lt);
+            const int ni = nk + below_lane(im);
             m.remap[v] = (uint16_t)ni;
             m.vx[o][ni] = x; m.vy[o][ni] = y; m.vz[o][ni] = z;
             if (withgen) {
@@ -312,12 +315,12 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const bool ain = da <= kTol, bin = db <= kTol;
         const uint64_t b0 = ballot(act && ain);            // in->in, in->out: 1 entry
         const uint64_t b1 = ballot(act && !ain && bin);    // out->in: 2 entries
-        if (act) m.P[s] = (uint16_t)(ptot + popc(b0 & lt) + 2 * popc(b1 & lt));
+        if (act) m.P[s] = (uint16_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
         ptot += popc(b0) + 2 * popc(b1);
         const bool cr = act && (ain != bin);
         const uint64_t cm = ballot(cr);
         if (cr) {
-            const int ci = K + popc(cm & lt);
+            const int ci = K + below_lane(cm);
             if (ci < C::KM) {
                 m.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
                 m.cab[ci] = a | (b << 16);
@@ -339,7 +342,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const int tot = act ? (int)m.P[fe] - (int)m.P[fs] : 0;
         const bool sv = act && tot >= 3;
         const uint64_t smk = ballot(sv);
-        const int nf = Fn + popc(smk & lt);
+        const int nf = Fn + below_lane(smk);
         const uint32_t t = sv ? (uint32_t)tot : 0u;
         const uint32_t inc = wave_incl_scan(t);
         const int off = Spre + (int)(inc - t);
@@ -373,7 +376,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         }
         const bool isf = act && first == ci;
         const uint64_t fm = ballot(isf);
-        const int xid = nk + nint + popc(fm & lt);
+        const int xid = nk + nint + below_lane(fm);
         nint += popc(fm);
         if (act) m.cxid[ci] = (uint16_t)(isf ? xid : (0x8000 | first));
         if (isf && xid < C::VM) {
@@ -414,7 +417,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const bool ain = da <= kTol, bin = db <= kTol;
         const bool cr = act && (ain != bin);
         const uint64_t cm = ballot(cr);
-        const int ci = Kb + popc(cm & lt);
+        const int ci = Kb + below_lane(cm);
         Kb += popc(cm);
         const int f = act ? m.sface[c][s] : 0;
         const int nf = act ? m.fnew[f] : -1;
@@ -448,7 +451,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         if (act) m.sedge[o][p] = ua | (vb << 16);
         const bool sg = act && (int)ua >= nk && (int)vb >= nk;
         const uint64_t gm = ballot(sg);
-        if (sg) m.seg[M + popc(gm & lt)] = vb | (ua << 16);
+        if (sg) m.seg[M + below_lane(gm)] = vb | (ua << 16);
         M += popc(gm);
     }
 
@@ -515,7 +518,6 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     constexpr uint32_t SH = NV == 1 ? 6 : 7;    // bits of a vertex index in the packed slot word
     constexpr uint32_t VMSK = (1u << SH) - 1u;
     const int l = lane_id();
-    const uint64_t lt = lt_mask();
     const int c = P.cur, o = c ^ 1;
     if (NV > 1 && P.F > 64) return kGeneric;
 
@@ -551,7 +553,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     for (int k = 0; k < NV; ++k) {
         if (va[k]) m.dist[64 * k + l] = d[k];
         if ((inm.w[k] >> l) & 1ull) {
-            const int ni = kbase + popc(inm.w[k] & lt);
+            const int ni = kbase + below_lane(inm.w[k]);
             m.vx[o][ni] = r.x[k]; m.vy[o][ni] = r.y[k]; m.vz[o][ni] = r.z[k];
             if (withgen) {
                 const double dx = r.x[k] - gx, dy = r.y[k] - gy, dz = r.z[k] - gz;
@@ -581,12 +583,12 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             const bool ain = inm.test(a), bin = inm.test(b);
             const uint64_t b0 = ballot(sa && ain);
             const uint64_t b1 = ballot(sa && !ain && bin);
-            const uint32_t pp = (uint32_t)(ptot + popc(b0 & lt) + 2 * popc(b1 & lt));
+            const uint32_t pp = (uint32_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
             ptot += popc(b0) + 2 * popc(b1);
             if (sa) m.P[s] = (uint16_t)pp;
             const bool cr = sa && (ain != bin);
             const uint64_t cm = ballot(cr);
-            const int ci = K + popc(cm & lt);
+            const int ci = K + below_lane(cm);
             eq[ch] = pp | ((uint32_t)(ci & 0xffff) << 16);
             if (cr && ci < 64) {
                 m.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
@@ -614,7 +616,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     if (ballot(sv && tot > 255)) return kFace255;
     if (Fn + 1 > C::FM || Spre > C::SM) return kOverflow;
     if (fa) {
-        const int nf = popc(svm & lt);
+        const int nf = below_lane(svm);
         m.fnew[l] = (int16_t)(sv ? nf : -1);
         m.fbase[l] = off - pfs;
         m.fnoff[l] = (uint16_t)off;
@@ -638,7 +640,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     const bool isf = ka && first == l;
     const uint64_t fm = ballot(isf);
     const int nint = popc(fm);
-    const int myx = nk + popc(fm & lt);
+    const int myx = nk + below_lane(fm);
     const int xid = (int)lane_get((uint32_t)myx, first);
     if (nk + nint > C::VM) return kOverflow;
     if (isf) {
@@ -701,7 +703,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         if (pa) m.sedge[o][p] = ua | (vb << 16);
         const bool sg = pa && (int)ua >= nk && (int)vb >= nk;
         const uint64_t gm = ballot(sg);
-        if (sg) m.seg[M + popc(gm & lt)] = vb | (ua << 16);
+        if (sg) m.seg[M + below_lane(gm)] = vb | (ua << 16);
         M += popc(gm);
     }
 
