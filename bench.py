@@ -127,7 +127,7 @@ def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_ra
     out = {"workload": f"first {count} requests of a Poisson stream at {rate:g} req/s: 1e3..2e6 "
                        "generators (log-uniform) in the unit cube, 50% uniform / 50% 8-cluster "
                        "sigma 0.05, 30% Lloyd jobs of 10-50 iterations; grid per job by "
-                       f"vd_grid_for_points at 2 per bin; HBM budget {budget_gb:g} GiB, 64 slots",
+                       f"vd_grid_for_points at its default target (0.35 per bin); HBM budget {budget_gb:g} GiB, 64 slots",
            "lloyd_jobs": int(sum(q.iterations > 1 for q in reqs[:count])),
            "note": "the offered load is far above one GPU's capacity at 2000 req/s, so its latency "
                    "is mostly queueing; the sweep lowers the rate on the first requests"}
@@ -136,10 +136,21 @@ def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_ra
     return out
 
 
+def seen_per_bin(points, grid) -> float:
+    """Generators per bin as seen by the average generator (points in the unit cube): sum over
+    bins of c^2, over n."""
+    g = np.asarray(grid, dtype=np.int64)
+    idx = np.minimum((points * g).astype(np.int64), g - 1)
+    key = (idx[:, 2] * g[1] + idx[:, 1]) * g[0] + idx[:, 0]
+    _, c = np.unique(key, return_counts=True)
+    return float((c.astype(np.float64) ** 2).sum() / len(points))
+
+
 def clustered(n: int, targets, steps: int) -> dict:
     """Uneven density: n generators, 80% from 64 Gaussian clusters (sigma 0.02) + 20% uniform, in
     the unit cube, no walls. The grid is tuned over generators-per-bin targets of the density rule
-    (vd_grid_for_points); the best is timed over ``steps`` calculate() calls."""
+    (vd_grid_for_points); the best is timed over ``steps`` calculate() calls. Each grid also
+    reports the generators per bin seen by the average generator, sum(c^2) / n over its bins."""
     from vorothree_amd import Algorithm3DGrid, BoundingBox, Tessellation, grid_for_points, _hip
     from vorothree_amd import workloads as W
 
@@ -158,7 +169,8 @@ def clustered(n: int, targets, steps: int) -> dict:
         c0 = time.perf_counter()
         t.calculate()
         dt = time.perf_counter() - c0
-        tried.append({"per_bin_target": tgt, "grid": list(grid), "calc_ms": dt * 1e3})
+        tried.append({"per_bin_target": tgt, "grid": list(grid), "seen_per_bin": seen_per_bin(w.points, grid),
+                      "calc_ms": dt * 1e3})
         if best is None or dt < best[0]:
             best = (dt, tgt, grid)
         t.close()
@@ -186,13 +198,18 @@ def clustered(n: int, targets, steps: int) -> dict:
     return out
 
 
-def batched_vs_plain(count: int, steps: int) -> dict:
+BATCH_TUNE = (0.25, 0.35, 0.5, 1.0, 2.0)
+
+
+def batched_vs_plain(count: int, steps: int, tune=None) -> dict:
     """Mixed-size workload: ``count`` tessellations with log-uniform generator counts in
-    [64, 65536], boxes with edges U[0.5, 2], 30% sphere walls, 20% plane walls, grids at about
-    2 generators per bin. Batched: all members in one vd_batch_calculate (wall filter, binning and
-    one persistent cell launch for all). Plain: one Tessellation per member, each running its own
-    set_generators + calculate launch sequence, eagerly; the same with graph replay of repeated
-    sequences is reported beside it. All start from points already in device memory."""
+    [64, 65536], boxes with edges U[0.5, 2], 30% sphere walls, 20% plane walls. Grids: about 2
+    generators per bin (``tune`` None), or the batch autotuner's pick among the generators-per-bin
+    targets in ``tune`` (vd_batch_autotune); the plain path runs the same grids. Batched: all
+    members in one vd_batch_calculate (wall filter, binning and one persistent cell launch for
+    all). Plain: one Tessellation per member, each running its own set_generators + calculate
+    launch sequence, eagerly; the same with graph replay of repeated sequences is reported beside
+    it. All start from points already in device memory."""
     from vorothree_amd import (Algorithm3DGrid, Batch, BoundingBox, PlaneGeometry, SphereGeometry,
                                Tessellation, Wall, _hip)
     from vorothree_amd import workloads as W
@@ -208,8 +225,9 @@ def batched_vs_plain(count: int, steps: int) -> dict:
 
     b = Batch()
     for w in ws:
-        b.add(BoundingBox(w.lo, w.hi), w.grid, walls_of(w), w.points)
+        b.add(BoundingBox(w.lo, w.hi), None if tune else w.grid, walls_of(w), w.points)
     b.upload()
+    best = b.autotune(tune) if tune else None
     b.calculate()                       # warmup (also builds the shell prefixes)
     _hip.synchronize()
     t0 = time.perf_counter()
@@ -218,6 +236,7 @@ def batched_vs_plain(count: int, steps: int) -> dict:
     _hip.synchronize()
     tb = (time.perf_counter() - t0) / steps
     cells = int(sum(b.info(t)["cells"] for t in range(len(b))))
+    grids = [b.info(t)["grid"] for t in range(len(b))]
     kern_b = float(b.stats()[4]) * 1e-6
     replays_b = int(b.stats()[8])
     b.close()
@@ -228,9 +247,9 @@ def batched_vs_plain(count: int, steps: int) -> dict:
 
     def plain(graphs):
         ts = []
-        for w in ws:
+        for w, grid in zip(ws, grids):
             bb = BoundingBox(w.lo, w.hi)
-            t = Tessellation(bb, Algorithm3DGrid(*w.grid, bb), keep_cells=False, graphs=graphs)
+            t = Tessellation(bb, Algorithm3DGrid(*grid, bb), keep_cells=False, graphs=graphs)
             for wl in walls_of(w):
                 t.add_wall(wl)
             ts.append(t)
@@ -258,8 +277,12 @@ def batched_vs_plain(count: int, steps: int) -> dict:
     tp, cells_p = plain(False)
     tpg, _ = plain(True)
     dev.close()
+    bins = sum(int(np.prod(g)) for g in grids)
+    grid_note = (f"autotuned grids (vd_batch_autotune over {list(tune)} generators per bin; picked {best:g})"
+                 if tune else "grids of about 2 generators per bin")
     return {"workload": f"{count} tessellations, generators log-uniform in [64, 65536], "
-                        "boxes U[0.5,2]^3, 30% sphere / 20% plane walls, ~2 generators per bin",
+                        f"boxes U[0.5,2]^3, 30% sphere / 20% plane walls, {grid_note}",
+            "per_bin": best if tune else 2.0, "generators_per_bin": float(offs[-1]) / bins,
             "generators": int(offs[-1]), "cells": cells, "cells_plain": cells_p,
             "batched_cells_per_s": cells / tb, "batched_ms": tb * 1e3, "batched_first_round_ms": kern_b * 1e3,
             "batched_graph_replays": replays_b,
@@ -417,8 +440,9 @@ def main():
     }
     if not a.no_batched:
         out["batched"] = batched_vs_plain(a.batch_count, a.batch_steps)
+        out["batched"]["autotuned"] = batched_vs_plain(a.batch_count, a.batch_steps, BATCH_TUNE)
     if not a.no_clustered:
-        out["clustered"] = clustered(a.clustered_n, [1.0, 2.0, 4.0], 3)
+        out["clustered"] = clustered(a.clustered_n, [0.25, 0.35, 0.7], 3)
     if not a.no_serve:
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
         out["serve"] = served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates)

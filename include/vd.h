@@ -224,7 +224,7 @@ int vd_batch_get_neighbors(const vd_batch *b, uint32_t t, uint64_t *offsets, siz
 int vd_batch_get_cell(const vd_batch *b, uint32_t t, size_t i, uint64_t sizes[3], double *vertices,
                       uint32_t *face_counts, uint32_t *face_indices, int32_t *face_neighbors);
 #define VD_BOPT_KEEP_CELLS 1 /* as VD_OPT_KEEP_CELLS; default 0 for batches */
-#define VD_BOPT_PER_BIN 2    /* generators-per-bin target of vd_batch_add's grid choice, x1000 */
+#define VD_BOPT_PER_BIN 2    /* generators-per-bin target of vd_batch_add's grid choice, x1000 (default 350) */
 #define VD_BOPT_GRAPHS 3     /* as VD_OPT_GRAPHS (default 1) */
 int vd_batch_set_option(vd_batch *b, int key, int64_t value);
 int vd_batch_get_stats(const vd_batch *b, double out[10]);
@@ -237,11 +237,11 @@ int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *bes
 int vd_grid_for(const double bmin[3], const double bmax[3], size_t n, double per_bin, uint32_t out[3]);
 
 /* The grid rule for uneven input (the server's choice for jobs submitted without a grid): as
- * vd_grid_for, with per_bin divided by how much denser the average generator's neighbourhood is
- * than the mean (sum of c (c - 1) over a coarse histogram of the points), that ratio clamped to
- * [1, 64], or to [1, 2] when over a quarter of the coarse bins are empty (cells next to empty
- * space walk across it). Uniform input gets about the vd_grid_for grid; clustered input with a
- * background a much finer one. */
+ * vd_grid_for, with per_bin divided by r^(2/3), where r is how much denser the average
+ * generator's neighbourhood is than the mean (sum of c (c - 1) over a coarse histogram of the
+ * points), clamped to [1, 64]; when over a quarter of the coarse bins are empty, per_bin times 3
+ * instead (cells next to empty space walk across it). Uniform input gets about the vd_grid_for
+ * grid; clustered input with a background a finer one; clusters in empty space a coarser one. */
 int vd_grid_for_points(const double bmin[3], const double bmax[3], const double *xyz, size_t n,
                        double per_bin, uint32_t out[3]);
 
@@ -278,7 +278,9 @@ int vd_server_release(vd_server *s, uint64_t job);
 /* [0] steps, [1] cells computed, [2] budget bytes in use, [3] budget. */
 int vd_server_stats(const vd_server *s, double out[4]);
 #define VD_SOPT_PER_BIN 1   /* generators-per-bin target of vd_grid_for_points for jobs submitted
-                               without a grid, x1000 (default 2000); applies to later submits */
+                               without a grid, x1000 (default 350); applies to later submits. A
+                               Lloyd job (iterations > 1) whose input has large empty regions gets
+                               the plain target instead of three times it: relaxation fills them. */
 int vd_server_set_option(vd_server *s, int key, int64_t value);
 
 /* Message of the calling thread's last failure ("" if none). */

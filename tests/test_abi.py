@@ -73,24 +73,37 @@ def test_shell_prefix_grows_to_the_full_table():
     assert sizes == sorted(sizes) and sizes[-1] == 17 ** 3
 
 
+def coarse_density(points, n_axis):
+    """Density ratio and empty-bin fraction over a histogram of n_axis bins per axis (unit cube)."""
+    idx = np.minimum((points * n_axis).astype(np.int64), n_axis - 1)
+    c = np.bincount((idx[:, 2] * n_axis + idx[:, 1]) * n_axis + idx[:, 0], minlength=n_axis ** 3).astype(float)
+    n = len(points)
+    return (c * np.maximum(c - 1, 0)).sum() * n_axis ** 3 / (n * n), float((c == 0).mean())
+
+
 def test_grid_for_points_rule():
     from vorothree_amd import grid_for, grid_for_points
     from vorothree_amd import workloads as W
     b = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
-    uni = W.PCG32(91).uniform(200_000, (0, 0, 0), (1, 1, 1))
-    g_u = grid_for_points(b, uni, 2.0)
-    g_ref = grid_for(b, len(uni), 2.0)
+    n = 200_000
+    n_axis = grid_for(b, n, 8.0)[0]                                    # about 8 generators per coarse bin
+    uni = W.PCG32(91).uniform(n, (0, 0, 0), (1, 1, 1))
+    g_u = grid_for_points(b, uni, 0.35)
+    g_ref = grid_for(b, n, 0.35)
     assert all(abs(a - r) <= 1 for a, r in zip(g_u, g_ref))           # uniform: the uniform rule
-    # clusters over a uniform background (no empty coarse bins): much finer bins, ratio <= 64
-    bg = W.clustered_points(W.PCG32(92), 200_000, 64, 0.02, 0.8)
-    g_c = grid_for_points(b, bg, 2.0)
-    assert g_c[0] > 2 * g_ref[0]
-    assert g_c[0] <= round(g_ref[0] * 64 ** (1 / 3)) + 1
-    # clusters in empty space: the ratio is capped at 2
-    void = W.clustered_points(W.PCG32(94), 200_000, 8, 0.05, 1.0)
-    assert grid_for_points(b, void, 2.0) == grid_for(b, len(void), 1.0)
+    # clusters over a uniform background (few empty coarse bins): per_bin / ratio^(2/3)
+    bg = W.clustered_points(W.PCG32(92), n, 64, 0.02, 0.8)
+    r, empty = coarse_density(bg, n_axis)
+    assert 10 < r < 64 and empty < 0.25
+    g_c = grid_for_points(b, bg, 0.35)
+    g_exp = grid_for(b, n, 0.35 / r ** (2.0 / 3.0))
+    assert all(abs(a - e) <= 1 for a, e in zip(g_c, g_exp)) and g_c[0] > 2 * g_ref[0]
+    # clusters in empty space: three times the target, a coarser grid than uniform input gets
+    void = W.clustered_points(W.PCG32(94), n, 8, 0.05, 1.0)
+    assert coarse_density(void, n_axis)[1] > 0.25
+    assert grid_for_points(b, void, 0.35) == grid_for(b, n, 1.05)
     small = uni[:40]
-    assert grid_for_points(b, small, 2.0) == grid_for(b, 40, 2.0)      # too few points to measure
+    assert grid_for_points(b, small, 0.35) == grid_for(b, 40, 0.35)    # too few points to measure
     box = BoundingBox((0.0, 0.0, 0.0), (2.0, 1.0, 0.5))
-    g_a = grid_for_points(box, W.PCG32(93).uniform(100_000, (0, 0, 0), (2.0, 1.0, 0.5)), 2.0)
+    g_a = grid_for_points(box, W.PCG32(93).uniform(100_000, (0, 0, 0), (2.0, 1.0, 0.5)), 0.35)
     assert g_a[0] > g_a[1] > g_a[2]                                    # proportional to the edges

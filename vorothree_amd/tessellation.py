@@ -621,7 +621,7 @@ class Batch:
     ``calculate()`` runs every member; results are read per member.
     """
 
-    def __init__(self, keep_cells: bool = False, per_bin: float = 2.0, graphs: bool = True):
+    def __init__(self, keep_cells: bool = False, per_bin: float = 0.35, graphs: bool = True):
         h = ctypes.c_void_p()
         check(lib().vd_batch_create(ctypes.byref(h)))
         self._h = h
