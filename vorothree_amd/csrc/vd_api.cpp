@@ -313,6 +313,17 @@ struct Shape {   // one distinct grid geometry of the current members
 constexpr uint32_t kShellK0 = 7;                       // initial prefix radius (bins)
 constexpr size_t kShellCacheBytes = size_t(4) << 30;   // cached tables beyond this are evicted (LRU)
 
+// Prefix radius k (bins) for the next extension of shape s: twice the current one, or enough for
+// keys up to 4 r2 (the prefix holds every key up to ((k - 1) max cell size)^2), whichever is
+// larger. Capped where the prefix holds the whole grid's diagonal, i.e. is complete.
+uint32_t shell_k_for(const Shape &s, double r2) {
+    const double csm = std::max(s.cs[0], std::max(s.cs[1], s.cs[2]));
+    const int64_t nmax = std::max(s.n[0], std::max(s.n[1], s.n[2]));
+    double k = 2.0 * (double)s.tab->k;
+    if (r2 > 0.0 && csm > 0.0) k = std::max(k, std::floor(2.0 * std::sqrt(r2) / csm) + 3.0);
+    return (uint32_t)std::min<double>(k, std::ceil(std::sqrt(3.0) * (double)nmax) + 3.0);
+}
+
 struct Engine {
     std::vector<Member> mem;
     std::vector<Shape> shapes;
@@ -325,7 +336,8 @@ struct Engine {
     int keep_cells = 1;
 
     DevBuf d_desc, d_offs, d_raw, d_raw_tess, d_flags, d_pos, d_scan, d_gat;
-    DevBuf d_gen, d_gen_tess, d_gen_bin, d_counts, d_bin_start, d_fill, d_items, d_spos, d_shell_tmp, d_mflags;
+    DevBuf d_gen, d_gen_tess, d_gen_bin, d_counts, d_bin_start, d_fill, d_items, d_spos, d_shell_tmp, d_mflags,
+        d_mrmax;
     DevBuf d_vol, d_cen, d_nface, d_nvert, d_status, d_cell_off;
     DevBuf d_vtx, d_fcnt, d_fnb, d_fidx, d_top, d_queue, d_counts4, d_next;
     DevBuf d_out0, d_out1;             // first-round retry lists of tier 0
@@ -367,8 +379,9 @@ void engine_free(Engine &E) {
     if (E.stream) (void)hipStreamSynchronize(E.stream);
     DevBuf *bufs[] = {&E.d_desc, &E.d_offs, &E.d_raw, &E.d_raw_tess, &E.d_flags, &E.d_pos, &E.d_scan,
                       &E.d_gat, &E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start,
-                      &E.d_fill, &E.d_items, &E.d_spos, &E.d_shell_tmp, &E.d_mflags, &E.d_vol, &E.d_cen, &E.d_nface,
-                      &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
+                      &E.d_fill, &E.d_items, &E.d_spos, &E.d_shell_tmp, &E.d_mflags, &E.d_mrmax, &E.d_vol,
+                      &E.d_cen, &E.d_nface, &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt,
+                      &E.d_fnb, &E.d_fidx,
                       &E.d_top, &E.d_queue, &E.d_counts4, &E.d_next};
     for (DevBuf *b : bufs) release(*b);
     release(E.d_out0);
@@ -642,7 +655,7 @@ int enqueue_binning(Engine &E) {
     return VD_OK;
 }
 
-int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms) {
+int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms, bool probe = false) {
     vd::CellArgs a{};
     a.desc = E.d_desc.as<vd::TessDesc>();
     a.gen_tess = E.d_gen_tess.as<uint32_t>();
@@ -677,7 +690,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     int waves = E.cus * E.occ[tier];
     if ((uint32_t)waves > nlist) waves = (int)nlist;
     if (ms) VD_HIP(hipEventRecord(E.ev0, E.stream));
-    VD_HIP(vd::launch_cells(tier, a, waves, E.stream, E.seal != 0));
+    VD_HIP(vd::launch_cells(tier, a, waves, E.stream, E.seal != 0, probe));
     if (ms) {
         VD_HIP(hipEventRecord(E.ev1, E.stream));
         VD_HIP(hipEventSynchronize(E.ev1));
@@ -779,7 +792,7 @@ int engine_calculate(Engine &E) {
     VD_TRY(ensure(E.d_cell_off, n * 24));
     VD_TRY(ensure(E.d_top, 64));
     VD_TRY(ensure(E.d_queue, 64));
-    VD_TRY(ensure(E.d_counts4, 64));
+    VD_TRY(ensure(E.d_counts4, 8 * kTiers * sizeof(uint32_t)));
     if (E.cap[1] < n * 16) {
         E.cap[0] = n * 40 + 4096;
         E.cap[1] = n * 24 + 4096;
@@ -800,95 +813,127 @@ int engine_calculate(Engine &E) {
         // Tier 0 (register fast path, small LDS) over every cell in bin order. A cell that
         // outgrows its tier moves to the next one; a cell whose walk runs past the shell prefix
         // reruns in its tier after the prefixes are extended. Reruns start from scratch and give
-        // the same result.
+        // the same result. Cells that once ran past their prefix reach across empty space: they
+        // rerun in the tiers' skip-ahead instantiation from then on. Each tier's list holds
+        // those cells first (group 1, nsh of them), then the others (group 0).
         constexpr int NT = kTiers;
         DevBuf L[NT];
         const uint32_t *list[NT] = {E.d_items.as<uint32_t>()};
         uint32_t nlist[NT] = {(uint32_t)n};
+        uint32_t nsh[NT] = {};
         bool arena_full = false;
         auto pending = [&]() { for (int k = 0; k < NT; ++k) if (nlist[k]) return true; return false; };
         for (int round = 0; round < 64 && pending(); ++round) {
-            // outcome per tier: counts[4*tier + {0 capacity, 1 shell, 2 arena, 3 error}], and the
-            // per-tier lists of capacity and shell failures
-            uint32_t counts[4 * NT] = {};
-            DevBuf out[2 * NT];
-            const uint32_t *ocap[NT] = {}, *oshell[NT] = {};
+            // outcome per group g and tier: counts[4 * (NT * g + tier) + {0 capacity, 1 shell,
+            // 2 arena, 3 error}], and the lists of capacity and shell failures per group and tier
+            uint32_t counts[8 * NT] = {};
+            DevBuf out[4 * NT];
+            const uint32_t *ocap[2][NT] = {}, *oshell[2][NT] = {};
+            auto cnt = [&](int g, int tier, int what) { return counts[4 * (NT * g + tier) + what]; };
             if (round == 0 && attempt == 0) {
                 float ms = 0.f;
                 VD_TRY(run_round0(E, (uint32_t)n, &ms));
                 E.stats[4] = 1000.0 * ms;
                 E.stats[0] = (double)n;
-                std::memcpy(counts, E.h_counts, sizeof(counts));
-                ocap[0] = E.d_out0.as<uint32_t>();
-                oshell[0] = E.d_out1.as<uint32_t>();
+                std::memcpy(counts, E.h_counts, 4 * NT * sizeof(uint32_t));
+                ocap[0][0] = E.d_out0.as<uint32_t>();
+                oshell[0][0] = E.d_out1.as<uint32_t>();
             } else {
                 if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
                 for (int tier = 0; tier < NT; ++tier) {
                     if (!nlist[tier]) continue;
-                    VD_TRY(run_cells(E, tier, list[tier], nlist[tier], nullptr));
+                    if (nsh[tier]) VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true));
+                    if (nlist[tier] > nsh[tier])
+                        VD_TRY(run_cells(E, tier, list[tier] + nsh[tier], nlist[tier] - nsh[tier], nullptr));
                     if (tier > 0) E.stats[1] += nlist[tier];
                 }
-                VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 4 * NT, E.stream));
+                VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 8 * NT, E.stream));
                 for (int tier = 0; tier < NT; ++tier) {
-                    if (!nlist[tier]) continue;
-                    VD_TRY(ensure(out[2 * tier], (size_t)nlist[tier] * 4));
-                    VD_TRY(ensure(out[2 * tier + 1], (size_t)nlist[tier] * 4));
-                    ocap[tier] = out[2 * tier].as<uint32_t>();
-                    oshell[tier] = out[2 * tier + 1].as<uint32_t>();
-                    VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), list[tier], nlist[tier],
-                                              out[2 * tier].as<uint32_t>(), out[2 * tier + 1].as<uint32_t>(),
-                                              E.d_counts4.as<uint32_t>() + 4 * tier, E.stream));
+                    for (int g = 0; g < 2; ++g) {
+                        const uint32_t *lst = g ? list[tier] : list[tier] + nsh[tier];
+                        const uint32_t m = g ? nsh[tier] : nlist[tier] - nsh[tier];
+                        if (!m) continue;
+                        DevBuf &oc = out[2 * (NT * g + tier)], &os = out[2 * (NT * g + tier) + 1];
+                        VD_TRY(ensure(oc, (size_t)m * 4));
+                        VD_TRY(ensure(os, (size_t)m * 4));
+                        ocap[g][tier] = oc.as<uint32_t>();
+                        oshell[g][tier] = os.as<uint32_t>();
+                        VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), lst, m, oc.as<uint32_t>(),
+                                                  os.as<uint32_t>(), E.d_counts4.as<uint32_t>() + 4 * (NT * g + tier),
+                                                  E.stream));
+                    }
                 }
                 VD_HIP(hipMemcpyAsync(E.h_counts, E.d_counts4.p, sizeof(counts), hipMemcpyDeviceToHost, E.stream));
                 VD_HIP(hipStreamSynchronize(E.stream));
                 std::memcpy(counts, E.h_counts, sizeof(counts));
             }
             bool shell = false;
-            for (int tier = 0; tier < NT; ++tier) {
-                if (counts[4 * tier + 3]) return fail(VD_ECAPACITY, "a cell has a face with 256 or more vertices");
-                if (counts[4 * tier + 2]) arena_full = true;
-                shell |= counts[4 * tier + 1] != 0;
-            }
+            for (int g = 0; g < 2; ++g)
+                for (int tier = 0; tier < NT; ++tier) {
+                    if (cnt(g, tier, 3)) return fail(VD_ECAPACITY, "a cell has a face with 256 or more vertices");
+                    if (cnt(g, tier, 2)) arena_full = true;
+                    shell |= cnt(g, tier, 1) != 0;
+                }
             if (arena_full) break;
-            if (counts[4 * (NT - 1)]) return fail(VD_ECAPACITY, "a cell exceeds the largest polytope tier");
+            if (cnt(0, NT - 1, 0) || cnt(1, NT - 1, 0))
+                return fail(VD_ECAPACITY, "a cell exceeds the largest polytope tier");
             if (shell) {
-                // the geometries of the cells that walked past their prefix get a longer one
+                // The geometries of the cells that walked past their prefix get a longer one: at
+                // least twice as long, and long enough for the largest radius r^2 those cells had
+                // when they stopped (the walk needs every key up to 4 r^2, and r^2 only shrinks as
+                // the walk goes on), so one extension usually covers cells reaching across empty space.
                 const size_t M = E.mem.size();
                 VD_TRY(ensure(E.d_mflags, (M + 1) * 4));
+                VD_TRY(ensure(E.d_mrmax, M * 8));
                 VD_HIP(vd::launch_fill_u32(E.d_mflags.as<uint32_t>(), 0u, M + 1, E.stream));
-                for (int tier = 0; tier < NT; ++tier)
-                    VD_HIP(vd::launch_mark_members(oshell[tier], counts[4 * tier + 1],
-                                                   E.d_gen_tess.as<uint32_t>(), E.d_mflags.as<uint32_t>(),
-                                                   E.stream));
+                VD_HIP(vd::launch_fill_u32(E.d_mrmax.as<uint32_t>(), 0u, 2 * M, E.stream));
+                uint64_t *d_rmax = E.d_mrmax.as<uint64_t>();
+                for (int g = 0; g < 2; ++g)
+                    for (int tier = 0; tier < NT; ++tier)
+                        VD_HIP(vd::launch_mark_members(oshell[g][tier], cnt(g, tier, 1), E.d_gen_tess.as<uint32_t>(),
+                                                       E.d_vol.as<double>(), E.d_mflags.as<uint32_t>(), d_rmax,
+                                                       E.stream));
                 std::vector<uint32_t> flags(M);
+                std::vector<double> rmax(M);
                 VD_HIP(hipMemcpyAsync(flags.data(), E.d_mflags.p, M * 4, hipMemcpyDeviceToHost, E.stream));
+                VD_HIP(hipMemcpyAsync(rmax.data(), d_rmax, M * 8, hipMemcpyDeviceToHost, E.stream));
                 VD_HIP(hipStreamSynchronize(E.stream));
                 std::vector<char> grow(E.shapes.size(), 0);
+                std::vector<double> need(E.shapes.size(), 0.0);
                 for (size_t t = 0; t < M; ++t)
-                    if (flags[t]) grow[E.mem[t].shape] = 1;
+                    if (flags[t]) {
+                        grow[E.mem[t].shape] = 1;
+                        need[E.mem[t].shape] = std::max(need[E.mem[t].shape], rmax[t]);
+                    }
                 for (size_t k = 0; k < E.shapes.size(); ++k) {
                     if (!grow[k]) continue;
                     Shape &s = E.shapes[k];
                     if (s.tab->complete) return fail(VD_EDEVICE, "shell walk past a complete table");
-                    VD_TRY(build_table(E, s.n, s.cs, s.tab->k * 2, *s.tab));
+                    VD_TRY(build_table(E, s.n, s.cs, shell_k_for(s, need[k]), *s.tab));
                 }
                 VD_TRY(upload_desc(E));
                 E.stats[2] += 1;
             }
-            // next list of tier k: its own shell failures + the capacity failures of tier k-1
+            // next list of tier k: group 1 = its own shell failures (both groups) + the capacity
+            // failures of group 1 of tier k-1; group 0 = the capacity failures of group 0 of tier k-1
             DevBuf next[NT];
-            uint32_t nnext[NT] = {};
+            uint32_t nnext[NT] = {}, nnsh[NT] = {};
             for (int tier = 0; tier < NT; ++tier) {
-                const uint32_t ns = counts[4 * tier + 1];
-                const uint32_t nc = tier > 0 ? counts[4 * (tier - 1)] : 0u;
-                nnext[tier] = ns + nc;
+                const uint32_t parts[4] = {cnt(0, tier, 1), cnt(1, tier, 1), tier > 0 ? cnt(1, tier - 1, 0) : 0u,
+                                           tier > 0 ? cnt(0, tier - 1, 0) : 0u};
+                const uint32_t *src[4] = {oshell[0][tier], oshell[1][tier], tier > 0 ? ocap[1][tier - 1] : nullptr,
+                                          tier > 0 ? ocap[0][tier - 1] : nullptr};
+                nnsh[tier] = parts[0] + parts[1] + parts[2];
+                nnext[tier] = nnsh[tier] + parts[3];
                 if (!nnext[tier]) continue;
                 VD_TRY(ensure(next[tier], (size_t)nnext[tier] * 4));
-                if (ns)
-                    VD_HIP(hipMemcpyAsync(next[tier].p, oshell[tier], ns * 4, hipMemcpyDeviceToDevice, E.stream));
-                if (nc)
-                    VD_HIP(hipMemcpyAsync(next[tier].as<uint32_t>() + ns, ocap[tier - 1], nc * 4,
-                                          hipMemcpyDeviceToDevice, E.stream));
+                uint32_t at = 0;
+                for (int q = 0; q < 4; ++q) {
+                    if (parts[q])
+                        VD_HIP(hipMemcpyAsync(next[tier].as<uint32_t>() + at, src[q], parts[q] * 4,
+                                              hipMemcpyDeviceToDevice, E.stream));
+                    at += parts[q];
+                }
             }
             VD_HIP(hipStreamSynchronize(E.stream));
             for (int tier = 0; tier < NT; ++tier) {
@@ -897,6 +942,7 @@ int engine_calculate(Engine &E) {
                 next[tier] = DevBuf();
                 list[tier] = L[tier].as<uint32_t>();
                 nlist[tier] = nnext[tier];
+                nsh[tier] = nnsh[tier];
             }
             for (DevBuf &b : out) release(b);
         }

@@ -142,16 +142,18 @@ struct Cap {
     static constexpr int TM = TM_;      // triangles staged in LDS for the volume/centroid folds
     static constexpr int WAVES = WAVES_;  // occupancy target (waves per SIMD) for the compiler
     static constexpr int NV = FAST_ ? (VM_ + 63) / 64 : 1;   // register vertices per lane
+    // polytope buffers: the fast path rewrites the polytope in place, the generic one ping-pongs
+    static constexpr int NB = FAST_ ? 1 : 2;
 };
 
 template <class C>
 struct Mem {
-    double vx[2][C::VM], vy[2][C::VM], vz[2][C::VM];
-    uint32_t sedge[2][C::SM];
-    int32_t fnb[2][C::FM];
+    double vx[C::NB][C::VM], vy[C::NB][C::VM], vz[C::NB][C::VM];
+    uint32_t sedge[C::NB][C::SM];
+    int32_t fnb[C::NB][C::FM];
     uint32_t winq[64];
-    uint16_t sface[2][C::SM];
-    uint16_t fstart[2][C::FM + 1];
+    uint16_t sface[C::NB][C::SM];
+    uint16_t fstart[C::NB][C::FM + 1];
     uint16_t winent[64];
     uint16_t winoff[64];
     // Scratch of one clip; at the end of a cell the same bytes stage the fold terms.
@@ -518,7 +520,12 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     constexpr uint32_t SH = NV == 1 ? 6 : 7;    // bits of a vertex index in the packed slot word
     constexpr uint32_t VMSK = (1u << SH) - 1u;
     const int l = lane_id();
-    const int c = P.cur, o = c ^ 1;
+    // One buffer: every read of an old entry comes before the write that can replace it (the slot
+    // and face words are in registers after step 2; old vertices are last read in step 4, so the
+    // kept ones are stored after it). A clip that fails part way leaves a mixed polytope, but every
+    // failure moves the cell to the next tier, which starts it again.
+    static_assert(C::NB == 1, "the fast path rewrites the polytope in place");
+    const int c = P.cur, o = c;
     if (NV > 1 && P.F > 64) return kGeneric;
 
     // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362)
@@ -548,19 +555,13 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     for (int k = 0; k < NV; ++k) inm.w[k] = ballot(va[k] && d[k] <= kTol);
     const int nk = inm.count();
     double md2 = 0.0;
-    int kbase = 0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         if (va[k]) m.dist[64 * k + l] = d[k];
-        if ((inm.w[k] >> l) & 1ull) {
-            const int ni = kbase + below_lane(inm.w[k]);
-            m.vx[o][ni] = r.x[k]; m.vy[o][ni] = r.y[k]; m.vz[o][ni] = r.z[k];
-            if (withgen) {
-                const double dx = r.x[k] - gx, dy = r.y[k] - gy, dz = r.z[k] - gz;
-                md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
-            }
+        if (withgen && ((inm.w[k] >> l) & 1ull)) {
+            const double dx = r.x[k] - gx, dy = r.y[k] - gy, dz = r.z[k] - gz;
+            md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
         }
-        kbase += popc(inm.w[k]);
     }
 
     // 2. slots: output entry counts (prefix in slot order) and plane crossings (:378-452).
@@ -643,6 +644,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     const int myx = nk + below_lane(fm);
     const int xid = (int)lane_get((uint32_t)myx, first);
     if (nk + nint > C::VM) return kOverflow;
+    double x = 0.0, y = 0.0, z = 0.0;
     if (isf) {
         const uint32_t ab = m.cab[l];
         const uint32_t a = ab & 0xffffu, b = ab >> 16;
@@ -652,15 +654,26 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         if (t > 1.0) t = 1.0;
         const double ax = m.vx[c][a], ay = m.vy[c][a], az = m.vz[c][a];
         const double bx = m.vx[c][b], by = m.vy[c][b], bz = m.vz[c][b];
-        const double x = ax + t * (bx - ax);
-        const double y = ay + t * (by - ay);
-        const double z = az + t * (bz - az);
-        m.vx[o][myx] = x; m.vy[o][myx] = y; m.vz[o][myx] = z;
+        x = ax + t * (bx - ax);
+        y = ay + t * (by - ay);
+        z = az + t * (bz - az);
         if (withgen) {
             const double dx = x - gx, dy = y - gy, dz = z - gz;
             md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
         }
     }
+    __syncthreads();
+    // the old vertices are read: store the kept ones at [0, nk) and the new ones after them
+    int kbase = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if ((inm.w[k] >> l) & 1ull) {
+            const int ni = kbase + below_lane(inm.w[k]);
+            m.vx[o][ni] = r.x[k]; m.vy[o][ni] = r.y[k]; m.vz[o][ni] = r.z[k];
+        }
+        kbase += popc(inm.w[k]);
+    }
+    if (isf) { m.vx[o][myx] = x; m.vy[o][myx] = y; m.vz[o][myx] = z; }
     __syncthreads();
 
     // 5. emit the entries of every surviving face (:386-451), with each entry's successor slot
@@ -877,11 +890,6 @@ __device__ inline uint64_t pretest(Mem<C> &m, const Poly &P, bool act, double d0
 #endif
 constexpr int kPretestAfter = VD_PRETEST_AFTER;
 
-// Whether the fast tier also uses the skip-ahead probe (the generic tiers always do).
-#ifndef VD_PROBE_FAST
-#define VD_PROBE_FAST 0
-#endif
-
 // Target bin of shell offset (dx, dy, dz) for a generator in bin (ix, iy, iz) at relative position
 // (rx, ry, rz): false if outside the grid, else its index and the squared distance from the
 // generator to the bin (algo_3d_grid.rs:175-190).
@@ -940,7 +948,10 @@ __device__ inline int probe_inert(const ShellCode *shell, uint32_t nshell, const
 
 // One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports. The
 // walls and the neighbour candidates feed a single clip site (one inlined copy of the clip).
-template <class C, bool SEAL>
+// PROBE: skip inert runs of shell batches in bulk (probe_inert). The generic tiers always do; the
+// fast tiers only in the instantiation that reruns cells whose walk ran past the shell prefix,
+// because the probe's registers slow the common walk down.
+template <class C, bool SEAL, bool PROBE>
 __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     const int l = lane_id();
     const TessDesc &D = A.desc[A.gen_tess[i]];   // this cell's tessellation (wave-uniform)
@@ -1033,7 +1044,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
             }
             if (!win) {
                 if (load) {
-                    if ((!C::FAST || VD_PROBE_FAST) && inert >= 2) {
+                    if ((!C::FAST || PROBE) && inert >= 2) {
                         // a run of batches without candidates: skip inert batches in bulk
                         const double plim = 4.0 * r2;
                         for (;;) {
@@ -1242,9 +1253,14 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
         if (l == 0) A.seal_logcnt[i] = nlog;
     }
 
-    // outputs
+    // outputs; a cell that walked past its shell prefix leaves its radius r^2 in the volume slot,
+    // so the prefix can be extended to the length its rerun needs
     if (l == 0) A.status[i] = status;
-    if (status != kCellOk) { __syncthreads(); return; }
+    if (status != kCellOk) {
+        if (status == kCellShell && l == 0) A.vol[i] = r2;
+        __syncthreads();
+        return;
+    }
     double vol, cen[3];
     poly_reduce<C>(m, P, vol, cen);
     if (l == 0) {
@@ -1300,7 +1316,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     __syncthreads();
 }
 
-template <class C, bool SEAL>
+template <class C, bool SEAL, bool PROBE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C::WAVES, 8)))
 void k_cells(CellArgs A) {
     __shared__ Mem<C> m;
@@ -1309,7 +1325,7 @@ void k_cells(CellArgs A) {
         if (lane_id() == 0) q = atomicAdd(A.queue, 1u);
         q = first_lane_u32(q);
         if (q >= A.ncells) break;
-        process_cell<C, SEAL>(m, A, A.cells[q]);
+        process_cell<C, SEAL, PROBE>(m, A, A.cells[q]);
     }
 }
 
@@ -1337,29 +1353,34 @@ size_t tier_lds_bytes(int tier) {
          : tier == 2 ? sizeof(Mem<Tier2>) : sizeof(Mem<Tier3>);
 }
 
-template <bool SEAL>
+template <bool SEAL, bool PROBE>
 void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st) {
-    if (tier == 0) k_cells<Tier0, SEAL><<<grid_waves, 64, 0, st>>>(a);
-    else if (tier == 1) k_cells<Tier1, SEAL><<<grid_waves, 64, 0, st>>>(a);
-    else if (tier == 2) k_cells<Tier2, SEAL><<<grid_waves, 64, 0, st>>>(a);
-    else k_cells<Tier3, SEAL><<<grid_waves, 64, 0, st>>>(a);
+    if (tier == 0) k_cells<Tier0, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+    else if (tier == 1) k_cells<Tier1, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+    else if (tier == 2) k_cells<Tier2, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
+    else k_cells<Tier3, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
 }
 
-hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal) {
+hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal, bool probe) {
     if (a.ncells == 0) return hipSuccess;
-    if (seal) launch_tier<true>(tier, a, grid_waves, st);
-    else launch_tier<false>(tier, a, grid_waves, st);
+    if (seal) {
+        if (probe) launch_tier<true, true>(tier, a, grid_waves, st);
+        else launch_tier<true, false>(tier, a, grid_waves, st);
+    } else {
+        if (probe) launch_tier<false, true>(tier, a, grid_waves, st);
+        else launch_tier<false, false>(tier, a, grid_waves, st);
+    }
     return hipGetLastError();
 }
 
 hipError_t cells_occupancy(int tier, int *blocks_per_cu) {
     if (tier == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0, false>, 64, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0, false, false>, 64, 0);
     if (tier == 1)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1, false>, 64, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1, false, false>, 64, 0);
     if (tier == 2)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2, false>, 64, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier3, false>, 64, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2, false, true>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier3, false, true>, 64, 0);
 }
 
 }  // namespace vd

@@ -295,11 +295,17 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
     if (i < n) p[i] = v;
 }
 
-// Members of the listed cells (for the per-geometry shell extension after a launch).
+// Members of the listed cells and the largest radius r^2 among each member's listed cells (for the
+// per-geometry shell extension after a launch). Non-negative doubles order as their bit patterns.
 __global__ void k_mark_members(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ gen_tess,
-                               uint32_t *__restrict__ flags) {
+                               const double *__restrict__ r2, uint32_t *__restrict__ flags,
+                               uint64_t *__restrict__ rmax) {
     const uint64_t g = gid();
-    if (g < n) flags[gen_tess[list[g]]] = 1u;
+    if (g >= n) return;
+    const uint32_t i = list[g];
+    const uint32_t t = gen_tess[i];
+    flags[t] = 1u;
+    atomicMax((unsigned long long *)&rmax[t], (unsigned long long)__double_as_longlong(r2[i]));
 }
 
 // Sorts the listed cells by outcome of a cell launch: capacity overflow -> out_cap, shell prefix
@@ -494,10 +500,10 @@ hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nv
     return hipGetLastError();
 }
 
-hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, uint32_t *flags,
-                               hipStream_t st) {
+hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, const double *r2,
+                               uint32_t *flags, uint64_t *rmax, hipStream_t st) {
     if (!n) return hipSuccess;
-    k_mark_members<<<blocks_for(n), kBlock, 0, st>>>(list, n, gen_tess, flags);
+    k_mark_members<<<blocks_for(n), kBlock, 0, st>>>(list, n, gen_tess, r2, flags, rmax);
     return hipGetLastError();
 }
 

@@ -174,8 +174,9 @@ hipError_t launch_prune(const TessDesc *desc, const uint32_t *tess, uint64_t n, 
                         const uint32_t *ioff, const uint32_t *poff, int32_t *out_i, double *out_p, hipStream_t st);
 hipError_t launch_relax(const double *gen, const double *cen, const uint32_t *nvert, uint64_t n,
                         double *out, hipStream_t st);
-hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, uint32_t *flags,
-                               hipStream_t st);   // flags[gen_tess[list[i]]] = 1
+// flags[t] = 1 and rmax[t] = max r2[i] (as bits; r2 >= 0) over the listed cells i of member t
+hipError_t launch_mark_members(const uint32_t *list, uint32_t n, const uint32_t *gen_tess, const double *r2,
+                               uint32_t *flags, uint64_t *rmax, hipStream_t st);
 hipError_t launch_collect(const int32_t *status, const uint32_t *list, uint32_t n,
                           uint32_t *out_cap, uint32_t *out_shell, uint32_t *counts, hipStream_t st);
 
@@ -195,7 +196,8 @@ hipError_t shell_finish(const ShellPack &pack, const double cs[3], uint32_t w, u
 // tier: 0 = register fast path with a small LDS polytope (many waves per CU); 1 = the fast path
 // with two vertices per lane (128 vertices); 2, 3 = generic path with LDS polytopes of 256 and 1024
 // vertices (the last one wave per CU)
-hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal = false);
+hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal = false,
+                        bool probe = false);
 hipError_t cells_occupancy(int tier, int *blocks_per_cu);
 int tier_count();
 int tier_max_vertices(int tier);
