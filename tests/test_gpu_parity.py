@@ -217,6 +217,40 @@ def test_sparse_grid_extends_shell_prefix():
     assert abs(vol.sum() - 1.0) < 1e-12
 
 
+def test_voids_extend_once_and_probe_on_every_later_calculate():
+    # 8 clusters in empty space: cells next to the voids walk past the first shell prefix; the
+    # prefix grows once to the length their radius needs and they rerun in the skip-ahead
+    # instantiation; later calculates on the extended prefix use it from the first launch
+    n = 30_000
+    pts = W.clustered_points(W.PCG32(57), n, 8, 0.05, 1.0)
+    grid = W.grid_for(n, (0, 0, 0), (1, 1, 1), 0.35)
+    g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), grid, pts, keep_cells=False)
+    g.calculate()
+    o.calculate(threads=8)
+    st = g.stats()
+    assert st[2] >= 1, "the shell prefix was never extended"
+    assert st[1] > 0, "cells next to the voids run in the larger tier too"
+    vol, cen, nf, _ = o.reduced()
+    assert np.array_equal(bits(g.volumes()), bits(vol))
+    assert np.array_equal(bits(g.centroids()), bits(cen))
+    assert np.array_equal(g.face_counts(), nf)
+    g.calculate()                                   # extended prefix: the first launch probes
+    assert g.stats()[2] == 0
+    assert np.array_equal(bits(g.volumes()), bits(vol))
+    assert np.array_equal(bits(g.centroids()), bits(cen))
+    g.relax()
+    o.relax()
+    g.calculate()
+    o.calculate(threads=8)
+    vol, cen, nf, _ = o.reduced()
+    assert np.array_equal(bits(g.generators()), bits(o.generators()))
+    assert np.array_equal(bits(g.volumes()), bits(vol))
+    assert np.array_equal(bits(g.centroids()), bits(cen))
+    off, ids = g.neighbors()
+    for i in range(0, n, 97):
+        assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), i
+
+
 @pytest.mark.parametrize("lo,hi,grid,k", [
     ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (16, 16, 16), 7),
     ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (20, 20, 20), 64),       # the whole table

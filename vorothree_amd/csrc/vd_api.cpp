@@ -300,7 +300,16 @@ struct ShellTable {
     double next = std::numeric_limits<double>::infinity();
     bool complete = false;
     uint64_t epoch = 0;         // last engine pass that used it
+    double far_frac = 0.0;      // share of a launch's cells that walked past the prefix when it grew
 };
+
+// Shares of a launch's cells walking past the shell prefix between which later launches on that
+// geometry use the skip-ahead instantiation from the start: a few cells per thousand is the
+// signature of empty regions between clusters (those cells cross them); a larger share means
+// many ordinary cells in fine bins (a uniform background under dense clusters), whose walks are
+// short and whose launches the probe's registers slow down.
+constexpr double kFarFracMin = 1.0 / 1024.0;
+constexpr double kFarFracMax = 1.0 / 16.0;
 
 using GeomKey = std::tuple<int64_t, int64_t, int64_t, uint64_t, uint64_t, uint64_t>;
 
@@ -315,13 +324,27 @@ constexpr size_t kShellCacheBytes = size_t(4) << 30;   // cached tables beyond t
 
 // Prefix radius k (bins) for the next extension of shape s: twice the current one, or enough for
 // keys up to 4 r2 (the prefix holds every key up to ((k - 1) max cell size)^2), whichever is
-// larger. Capped where the prefix holds the whole grid's diagonal, i.e. is complete.
+// larger, and no larger than the complete prefix (one spanning the grid's diagonal). The direct
+// jump is limited to an enumeration box of kShellJumpBox offsets: a cell that runs out of prefix
+// before its first cut still has the box-sized radius, which would ask for the whole grid.
+constexpr uint64_t kShellJumpBox = uint64_t(1) << 26;
+
+uint64_t shell_box_size(const Shape &s, uint32_t k) {
+    uint64_t w = 1;
+    for (int a = 0; a < 3; ++a) w *= (uint64_t)(2 * std::min<int64_t>(k, s.n[a]) + 1);
+    return w;
+}
+
 uint32_t shell_k_for(const Shape &s, double r2) {
     const double csm = std::max(s.cs[0], std::max(s.cs[1], s.cs[2]));
     const int64_t nmax = std::max(s.n[0], std::max(s.n[1], s.n[2]));
-    double k = 2.0 * (double)s.tab->k;
-    if (r2 > 0.0 && csm > 0.0) k = std::max(k, std::floor(2.0 * std::sqrt(r2) / csm) + 3.0);
-    return (uint32_t)std::min<double>(k, std::ceil(std::sqrt(3.0) * (double)nmax) + 3.0);
+    const uint32_t twice = 2 * s.tab->k;
+    const uint32_t full = (uint32_t)std::ceil(std::sqrt(3.0) * (double)nmax) + 3;
+    double want = (double)twice;
+    if (r2 > 0.0 && csm > 0.0) want = std::max(want, std::floor(2.0 * std::sqrt(r2) / csm) + 3.0);
+    uint32_t k = (uint32_t)std::min<double>(want, (double)full);
+    while (k > twice && shell_box_size(s, k) > kShellJumpBox) --k;
+    return std::max(k, std::min(twice, full));
 }
 
 struct Engine {
@@ -345,6 +368,7 @@ struct Engine {
     DevBuf d_tube;                     // per generator: planes of its member's tube walls
     // post-passes: cell topology of the previous pass, seal log, prune log
     int seal = 0;                      // cell launches run the seal mode
+    bool probe0 = false;               // the first launch uses the skip-ahead instantiation
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
     uint32_t ntube = 0;                // tube-wall slots per generator
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
@@ -705,7 +729,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
 int enqueue_round0(Engine &E, uint32_t n, float *ms) {
     if (!E.binned) VD_TRY(enqueue_binning(E));
     VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
-    VD_TRY(run_cells(E, 0, E.d_items.as<uint32_t>(), n, ms));
+    VD_TRY(run_cells(E, 0, E.d_items.as<uint32_t>(), n, ms, E.probe0));
     VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 4 * kTiers, E.stream));
     VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), E.d_items.as<uint32_t>(), n, E.d_out0.as<uint32_t>(),
                               E.d_out1.as<uint32_t>(), E.d_counts4.as<uint32_t>(), E.stream));
@@ -719,7 +743,7 @@ int enqueue_round0(Engine &E, uint32_t n, float *ms) {
 std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
                                E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.mem.size(), E.ntube,
-                               (uint64_t)E.seal};
+                               (uint64_t)E.seal, (uint64_t)E.probe0};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
@@ -800,6 +824,11 @@ int engine_calculate(Engine &E) {
     }
     E.calculated = false;
     if (n == 0) { E.calculated = true; return VD_OK; }
+    // a geometry whose prefix grew for a few cells per thousand has cells reaching across empty
+    // space: the first launch skips inert shell batches too (no geometry grows on uniform input)
+    E.probe0 = false;
+    for (const Shape &s : E.shapes)
+        E.probe0 |= s.tab && s.tab->far_frac > kFarFracMin && s.tab->far_frac <= kFarFracMax;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (E.keep_cells) {
             VD_TRY(ensure(E.d_vtx, E.cap[0] * 24));
@@ -820,7 +849,7 @@ int engine_calculate(Engine &E) {
         DevBuf L[NT];
         const uint32_t *list[NT] = {E.d_items.as<uint32_t>()};
         uint32_t nlist[NT] = {(uint32_t)n};
-        uint32_t nsh[NT] = {};
+        uint32_t nsh[NT] = {E.probe0 ? (uint32_t)n : 0u};
         bool arena_full = false;
         auto pending = [&]() { for (int k = 0; k < NT; ++k) if (nlist[k]) return true; return false; };
         for (int round = 0; round < 64 && pending(); ++round) {
@@ -835,9 +864,10 @@ int engine_calculate(Engine &E) {
                 VD_TRY(run_round0(E, (uint32_t)n, &ms));
                 E.stats[4] = 1000.0 * ms;
                 E.stats[0] = (double)n;
-                std::memcpy(counts, E.h_counts, 4 * NT * sizeof(uint32_t));
-                ocap[0][0] = E.d_out0.as<uint32_t>();
-                oshell[0][0] = E.d_out1.as<uint32_t>();
+                const int g0 = E.probe0 ? 1 : 0;   // the group the first launch ran as
+                std::memcpy(counts + 4 * NT * g0, E.h_counts, 4 * NT * sizeof(uint32_t));
+                ocap[g0][0] = E.d_out0.as<uint32_t>();
+                oshell[g0][0] = E.d_out1.as<uint32_t>();
             } else {
                 if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
                 for (int tier = 0; tier < NT; ++tier) {
@@ -905,11 +935,16 @@ int engine_calculate(Engine &E) {
                         grow[E.mem[t].shape] = 1;
                         need[E.mem[t].shape] = std::max(need[E.mem[t].shape], rmax[t]);
                     }
+                uint64_t nfar = 0;
+                for (int g = 0; g < 2; ++g)
+                    for (int tier = 0; tier < NT; ++tier) nfar += cnt(g, tier, 1);
                 for (size_t k = 0; k < E.shapes.size(); ++k) {
                     if (!grow[k]) continue;
                     Shape &s = E.shapes[k];
                     if (s.tab->complete) return fail(VD_EDEVICE, "shell walk past a complete table");
+                    const double frac = std::max(s.tab->far_frac, (double)nfar / (double)n);
                     VD_TRY(build_table(E, s.n, s.cs, shell_k_for(s, need[k]), *s.tab));
+                    s.tab->far_frac = frac;
                 }
                 VD_TRY(upload_desc(E));
                 E.stats[2] += 1;

@@ -890,6 +890,15 @@ __device__ inline uint64_t pretest(Mem<C> &m, const Poly &P, bool act, double d0
 #endif
 constexpr int kPretestAfter = VD_PRETEST_AFTER;
 
+// The pre-test costs one pass over the polytope's V vertices per window; a candidate it does not
+// clear costs a clip, which exits after its distance pass when nothing is cut. So a window of fewer
+// than V / kPretestMinDiv candidates (walks through sparse space gather few per batch of shell
+// entries) is clipped without it.
+#ifndef VD_PRETEST_MIN_DIV
+#define VD_PRETEST_MIN_DIV 4
+#endif
+constexpr int kPretestMinDiv = VD_PRETEST_MIN_DIV;
+
 // Target bin of shell offset (dx, dy, dz) for a generator in bin (ix, iy, iz) at relative position
 // (rx, ry, rz): false if outside the grid, else its index and the squared distance from the
 // generator to the bin (algo_3d_grid.rs:175-190).
@@ -1108,7 +1117,8 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
                     wdsq = wd0 * wd0 + wd1 * wd1 + wd2 * wd2;   // tessellation.rs:447-456
                 }
                 wvm = ballot(wa);
-                wcm = pre ? pretest<C>(m, P, wa && wj != li, wd0, wd1, wd2, gx, gy, gz, 0, P.V) : ~0ull;
+                wcm = (pre && kPretestMinDiv * nwin >= P.V)
+                          ? pretest<C>(m, P, wa && wj != li, wd0, wd1, wd2, gx, gy, gz, 0, P.V) : ~0ull;
                 last = -1;
                 win = true;
                 __syncthreads();
