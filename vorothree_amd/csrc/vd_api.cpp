@@ -301,15 +301,22 @@ struct ShellTable {
     bool complete = false;
     uint64_t epoch = 0;         // last engine pass that used it
     double far_frac = 0.0;      // share of a launch's cells that walked past the prefix when it grew
+    uint32_t count0 = 0;        // the first prefix (radius kShellK0): its entries lead every longer one
+    double next0 = std::numeric_limits<double>::infinity();
 };
 
-// Shares of a launch's cells walking past the shell prefix between which later launches on that
-// geometry use the skip-ahead instantiation from the start: a few cells per thousand is the
-// signature of empty regions between clusters (those cells cross them); a larger share means
-// many ordinary cells in fine bins (a uniform background under dense clusters), whose walks are
-// short and whose launches the probe's registers slow down.
+// Shares of a launch's cells walking past the shell prefix between which a geometry counts as one
+// with empty regions between clusters: a few cells per thousand cross them. Later first launches on
+// such a geometry see only its first prefix (radius kShellK0), so those cells stop early instead
+// of walking the long prefix in tier 0, which they outgrow, and rerun in the skip-ahead
+// instantiation of tier 1 on the whole prefix. A larger share means many ordinary cells in fine
+// bins (a uniform background under dense clusters) whose walks are short: they keep the whole
+// prefix from the start.
 constexpr double kFarFracMin = 1.0 / 1024.0;
 constexpr double kFarFracMax = 1.0 / 16.0;
+constexpr int kFarTier = 1;   // lowest tier in which cells that walked past their prefix rerun
+
+bool far_geometry(const ShellTable &t) { return t.far_frac > kFarFracMin && t.far_frac <= kFarFracMax; }
 
 using GeomKey = std::tuple<int64_t, int64_t, int64_t, uint64_t, uint64_t, uint64_t>;
 
@@ -317,6 +324,7 @@ struct Shape {   // one distinct grid geometry of the current members
     int64_t n[3];
     double cs[3];
     ShellTable *tab = nullptr;
+    bool trunc = false;   // the uploaded descriptors show only its first prefix
 };
 
 constexpr uint32_t kShellK0 = 7;                       // initial prefix radius (bins)
@@ -368,7 +376,7 @@ struct Engine {
     DevBuf d_tube;                     // per generator: planes of its member's tube walls
     // post-passes: cell topology of the previous pass, seal log, prune log
     int seal = 0;                      // cell launches run the seal mode
-    bool probe0 = false;               // the first launch uses the skip-ahead instantiation
+    bool trunc0 = false;               // the first launch sees only the first prefix of far geometries
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
     uint32_t ntube = 0;                // tube-wall slots per generator
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
@@ -476,6 +484,7 @@ int build_table(Engine &E, const int64_t n[3], const double cs[3], uint32_t k, S
     t.count = count;
     t.next = std::min(next, b.outside);
     t.complete = std::isinf(t.next);
+    if (k == kShellK0) { t.count0 = t.count; t.next0 = t.next; }
     return VD_OK;
 }
 
@@ -556,11 +565,13 @@ int upload_desc(Engine &E) {
         d.bin_off = m.bin_off;
         d.nraw = (uint32_t)m.nraw;
         d.ngen = (uint32_t)m.ngen;
-        const ShellTable &tab = *E.shapes[m.shape].tab;
+        Shape &sh = E.shapes[m.shape];
+        const ShellTable &tab = *sh.tab;
+        sh.trunc = E.trunc0 && far_geometry(tab) && tab.k > kShellK0;
         d.shell = tab.buf.as<vd::ShellCode>();
         d.pack = tab.pack;
-        d.nshell = tab.count;
-        d.shell_next_d2 = tab.next;
+        d.nshell = sh.trunc ? tab.count0 : tab.count;
+        d.shell_next_d2 = sh.trunc ? tab.next0 : tab.next;
         d.nwalls = (int32_t)m.walls.size();
         for (size_t k = 0; k < m.walls.size(); ++k) {
             d.walls[k] = m.walls[k];
@@ -729,7 +740,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
 int enqueue_round0(Engine &E, uint32_t n, float *ms) {
     if (!E.binned) VD_TRY(enqueue_binning(E));
     VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
-    VD_TRY(run_cells(E, 0, E.d_items.as<uint32_t>(), n, ms, E.probe0));
+    VD_TRY(run_cells(E, 0, E.d_items.as<uint32_t>(), n, ms));
     VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 4 * kTiers, E.stream));
     VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), E.d_items.as<uint32_t>(), n, E.d_out0.as<uint32_t>(),
                               E.d_out1.as<uint32_t>(), E.d_counts4.as<uint32_t>(), E.stream));
@@ -743,7 +754,7 @@ int enqueue_round0(Engine &E, uint32_t n, float *ms) {
 std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
                                E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.mem.size(), E.ntube,
-                               (uint64_t)E.seal, (uint64_t)E.probe0};
+                               (uint64_t)E.seal};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
@@ -824,11 +835,13 @@ int engine_calculate(Engine &E) {
     }
     E.calculated = false;
     if (n == 0) { E.calculated = true; return VD_OK; }
-    // a geometry whose prefix grew for a few cells per thousand has cells reaching across empty
-    // space: the first launch skips inert shell batches too (no geometry grows on uniform input)
-    E.probe0 = false;
-    for (const Shape &s : E.shapes)
-        E.probe0 |= s.tab && s.tab->far_frac > kFarFracMin && s.tab->far_frac <= kFarFracMax;
+    // geometries with cells crossing empty regions: the first launch sees their first prefix only
+    bool far = false;
+    for (const Shape &s : E.shapes) far |= s.tab && far_geometry(*s.tab) && s.tab->k > kShellK0;
+    if (far || E.trunc0) {
+        E.trunc0 = far;
+        VD_TRY(upload_desc(E));
+    }
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (E.keep_cells) {
             VD_TRY(ensure(E.d_vtx, E.cap[0] * 24));
@@ -849,7 +862,7 @@ int engine_calculate(Engine &E) {
         DevBuf L[NT];
         const uint32_t *list[NT] = {E.d_items.as<uint32_t>()};
         uint32_t nlist[NT] = {(uint32_t)n};
-        uint32_t nsh[NT] = {E.probe0 ? (uint32_t)n : 0u};
+        uint32_t nsh[NT] = {};
         bool arena_full = false;
         auto pending = [&]() { for (int k = 0; k < NT; ++k) if (nlist[k]) return true; return false; };
         for (int round = 0; round < 64 && pending(); ++round) {
@@ -864,10 +877,9 @@ int engine_calculate(Engine &E) {
                 VD_TRY(run_round0(E, (uint32_t)n, &ms));
                 E.stats[4] = 1000.0 * ms;
                 E.stats[0] = (double)n;
-                const int g0 = E.probe0 ? 1 : 0;   // the group the first launch ran as
-                std::memcpy(counts + 4 * NT * g0, E.h_counts, 4 * NT * sizeof(uint32_t));
-                ocap[g0][0] = E.d_out0.as<uint32_t>();
-                oshell[g0][0] = E.d_out1.as<uint32_t>();
+                std::memcpy(counts, E.h_counts, 4 * NT * sizeof(uint32_t));
+                ocap[0][0] = E.d_out0.as<uint32_t>();
+                oshell[0][0] = E.d_out1.as<uint32_t>();
             } else {
                 if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
                 for (int tier = 0; tier < NT; ++tier) {
@@ -938,32 +950,43 @@ int engine_calculate(Engine &E) {
                 uint64_t nfar = 0;
                 for (int g = 0; g < 2; ++g)
                     for (int tier = 0; tier < NT; ++tier) nfar += cnt(g, tier, 1);
+                bool rebuilt = false;
                 for (size_t k = 0; k < E.shapes.size(); ++k) {
                     if (!grow[k]) continue;
                     Shape &s = E.shapes[k];
+                    if (s.trunc) continue;   // its whole prefix is there: the reruns see it
                     if (s.tab->complete) return fail(VD_EDEVICE, "shell walk past a complete table");
                     const double frac = std::max(s.tab->far_frac, (double)nfar / (double)n);
                     VD_TRY(build_table(E, s.n, s.cs, shell_k_for(s, need[k]), *s.tab));
                     s.tab->far_frac = frac;
+                    rebuilt = true;
                 }
+                E.trunc0 = false;
                 VD_TRY(upload_desc(E));
-                E.stats[2] += 1;
+                if (rebuilt) E.stats[2] += 1;
             }
-            // next list of tier k: group 1 = its own shell failures (both groups) + the capacity
-            // failures of group 1 of tier k-1; group 0 = the capacity failures of group 0 of tier k-1
+            // next list of tier k: group 1 = the shell failures that rerun in tier k (both groups;
+            // those of tier 0 rerun in tier 1: cells reaching across empty space mostly outgrow
+            // tier 0, and any tier gives the same cell) + the capacity failures of group 1 of tier
+            // k-1; group 0 = the capacity failures of group 0 of tier k-1
+            auto far_tier = [](int tier) { return tier < kFarTier ? kFarTier : tier; };
             DevBuf next[NT];
             uint32_t nnext[NT] = {}, nnsh[NT] = {};
             for (int tier = 0; tier < NT; ++tier) {
-                const uint32_t parts[4] = {cnt(0, tier, 1), cnt(1, tier, 1), tier > 0 ? cnt(1, tier - 1, 0) : 0u,
-                                           tier > 0 ? cnt(0, tier - 1, 0) : 0u};
-                const uint32_t *src[4] = {oshell[0][tier], oshell[1][tier], tier > 0 ? ocap[1][tier - 1] : nullptr,
-                                          tier > 0 ? ocap[0][tier - 1] : nullptr};
-                nnsh[tier] = parts[0] + parts[1] + parts[2];
-                nnext[tier] = nnsh[tier] + parts[3];
+                uint32_t parts[2 * NT + 2] = {};
+                const uint32_t *src[2 * NT + 2] = {};
+                int np = 0;
+                for (int t = 0; t < NT; ++t)
+                    if (far_tier(t) == tier)
+                        for (int g = 0; g < 2; ++g) { parts[np] = cnt(g, t, 1); src[np++] = oshell[g][t]; }
+                if (tier > 0) { parts[np] = cnt(1, tier - 1, 0); src[np++] = ocap[1][tier - 1]; }
+                for (int q = 0; q < np; ++q) nnsh[tier] += parts[q];
+                if (tier > 0) { parts[np] = cnt(0, tier - 1, 0); src[np++] = ocap[0][tier - 1]; }
+                for (int q = 0; q < np; ++q) nnext[tier] += parts[q];
                 if (!nnext[tier]) continue;
                 VD_TRY(ensure(next[tier], (size_t)nnext[tier] * 4));
                 uint32_t at = 0;
-                for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < np; ++q) {
                     if (parts[q])
                         VD_HIP(hipMemcpyAsync(next[tier].as<uint32_t>() + at, src[q], parts[q] * 4,
                                               hipMemcpyDeviceToDevice, E.stream));
