@@ -260,9 +260,12 @@ void vd_server_destroy(vd_server *s);
 int vd_server_submit(vd_server *s, const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
                      uint32_t nz, const vd_wall *walls, int nwalls, const double *xyz, size_t n,
                      uint32_t iterations, uint64_t *job);
-/* One step: admit queued jobs in arrival order while slots and budget allow, run one batched
- * calculate over every admitted job, finish the jobs whose last iteration this was and relax the
- * others on the device. *finished receives the number of jobs finished by this step. */
+/* One step: admit queued jobs in arrival order while slots and budget allow; collect the large-job
+ * lane's step if it has finished and start its next one (one batched calculate over its jobs, on a
+ * worker thread); then run one batched calculate over the small-job lane (VD_SOPT_SMALL_JOB). A
+ * lane's step finishes the jobs whose last iteration it was and relaxes the others on the device.
+ * *finished receives the number of jobs finished. Call it in a loop while jobs are pending or
+ * running; with no small-job work it waits at most a few milliseconds for the large lane. */
 int vd_server_step(vd_server *s, uint32_t *finished);
 /* counts[0] queued jobs, [1] admitted jobs, [2] budget bytes in use. */
 int vd_server_pending(const vd_server *s, uint64_t counts[3]);
@@ -277,6 +280,10 @@ int vd_server_result(const vd_server *s, uint64_t job, double *volumes, double *
 int vd_server_release(vd_server *s, uint64_t job);
 /* [0] steps, [1] cells computed, [2] budget bytes in use, [3] budget. */
 int vd_server_stats(const vd_server *s, double out[4]);
+#define VD_SOPT_SMALL_JOB 2 /* jobs with at most this many points (default 65536; 0: none) run in a
+                               lane of their own, stepped by vd_server_step itself, while the steps
+                               of larger jobs run on a worker thread and leave 16 CUs to it: small
+                               jobs do not wait for large ones. Applies to later submits. */
 #define VD_SOPT_PER_BIN 1   /* generators-per-bin target of vd_grid_for_points for jobs submitted
                                without a grid, x1000 (default 350); applies to later submits. A
                                Lloyd job (iterations > 1) whose input has large empty regions gets

@@ -1,6 +1,8 @@
 """Serving (vd_server_*): continuous batching of tessellation and Lloyd jobs.
 
-* a job's results are bit-identical alone and in a mix with other jobs that arrive while it runs;
+* a job's results are bit-identical alone and in a mix with other jobs that arrive while it runs,
+  with the jobs in the small-job lane, in the large-job lane (worker thread) or split between them;
+* a small job submitted while a large one runs finishes first and equals the oracle;
 * a Lloyd job equals the oracle's calculate/relax loop (tessellation.rs:482-495) bit for bit;
 * admission respects the HBM budget; a job larger than the whole budget is refused.
 """
@@ -33,7 +35,8 @@ def run_until_done(s, jobs, max_steps=500):
     raise AssertionError("jobs did not finish")
 
 
-def test_job_alone_equals_in_mix():
+@pytest.mark.parametrize("small_job", [None, 4000, 0])    # all small / split lanes / all large
+def test_job_alone_equals_in_mix(small_job):
     pts = job_points(101, 6000)
     sphere = [Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.45))]
     alone = Server(hbm_budget=1 << 30)
@@ -41,7 +44,7 @@ def test_job_alone_equals_in_mix():
     run_until_done(alone, [ja])
     va, ca, ga = alone.result(ja)
 
-    mix = Server(hbm_budget=1 << 30)
+    mix = Server(hbm_budget=1 << 30, small_job=small_job)
     others = [mix.submit(UNIT, None, [], job_points(200 + k, 3000 + 1500 * k, clustered=k % 2 == 1),
                          iterations=1 + (k % 3)) for k in range(3)]
     mix.step()                                   # the others are already running ...
@@ -73,6 +76,25 @@ def test_lloyd_job_matches_oracle():
     assert np.array_equal(bits(cen), bits(oc))
     info = s.info(j)
     assert info["runs"] == 5 and info["total_cells"] == 5 * 5000
+
+
+def test_small_job_runs_beside_a_large_one():
+    s = Server(hbm_budget=8 << 30, small_job=20_000)
+    big = s.submit(UNIT, None, [], job_points(140, 400_000), iterations=6)
+    s.step()                                     # the large job's first step runs on the worker
+    small_pts = job_points(141, 5000)
+    j = s.submit(UNIT, (12, 12, 12), [], small_pts)
+    run_until_done(s, [big, j])
+    assert s.info(j)["finish"] < s.info(big)["finish"]
+    assert s.info(big)["runs"] == 6 and s.info(big)["total_cells"] == 6 * 400_000
+    vol, cen, gen = s.result(j)
+    o = OracleTess((0, 0, 0), (1, 1, 1), 12, 12, 12)
+    o.set_generators(small_pts)
+    o.calculate(threads=8)
+    ov, oc, _, _ = o.reduced()
+    assert np.array_equal(bits(vol), bits(ov))
+    assert np.array_equal(bits(cen), bits(oc))
+    assert np.array_equal(bits(gen.reshape(-1)), bits(o.generators()))
 
 
 def test_budget_admission_and_refusal():

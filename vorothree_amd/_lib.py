@@ -22,6 +22,8 @@ VD_OPT_GRAPHS = 2
 VD_BOPT_KEEP_CELLS = 1
 VD_BOPT_PER_BIN = 2
 VD_BOPT_GRAPHS = 3
+VD_SOPT_PER_BIN = 1
+VD_SOPT_SMALL_JOB = 2
 
 
 class VdWall(ctypes.Structure):

@@ -15,8 +15,10 @@
 #include <cstring>
 #include <deque>
 #include <limits>
+#include <atomic>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -51,12 +53,23 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// While g_defer is set, released device memory goes to that list instead of to hipFree, which waits
+// for the whole device, work that other threads have in flight included.
+thread_local std::vector<DevBuf> *g_defer = nullptr;
+
+void release(DevBuf &b) {
+    if (b.p) {
+        if (g_defer) g_defer->push_back(b);
+        else (void)hipFree(b.p);
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
 int ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return VD_OK;
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
+    release(b);
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -64,12 +77,6 @@ int ensure(DevBuf &b, size_t bytes) {
     }
     b.bytes = bytes;
     return VD_OK;
-}
-
-void release(DevBuf &b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -377,6 +384,7 @@ struct Engine {
     // post-passes: cell topology of the previous pass, seal log, prune log
     int seal = 0;                      // cell launches run the seal mode
     bool trunc0 = false;               // the first launch sees only the first prefix of far geometries
+    int reserve_cus = 0;               // CUs the persistent cell launches leave to other engines
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
     uint32_t ntube = 0;                // tube-wall slots per generator
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
@@ -722,7 +730,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     for (int k = 0; k < 3; ++k) a.arena_cap[k] = E.cap[k];
     a.cell_off = E.d_cell_off.as<uint64_t>();
     VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>(), 0u, 1, E.stream));
-    int waves = E.cus * E.occ[tier];
+    int waves = std::max(1, E.cus - E.reserve_cus) * E.occ[tier];
     if ((uint32_t)waves > nlist) waves = (int)nlist;
     if (ms) VD_HIP(hipEventRecord(E.ev0, E.stream));
     VD_HIP(vd::launch_cells(tier, a, waves, E.stream, E.seal != 0, probe));
@@ -753,8 +761,8 @@ int enqueue_round0(Engine &E, uint32_t n, float *ms) {
 // options, buffer addresses and the member descriptors. Equal keys give identical launches.
 std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
-                               E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.mem.size(), E.ntube,
-                               (uint64_t)E.seal};
+                               E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.reserve_cus,
+                               (uint64_t)E.mem.size(), E.ntube, (uint64_t)E.seal};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
@@ -1239,6 +1247,8 @@ struct Job {
     uint64_t total_cells = 0;      // cells over all its calculate() runs
     std::vector<double> vol, cen, gen;
     double t_submit = 0, t_admit = 0, t_done = 0;
+    int lane = 0;                  // 0 large jobs, 1 small jobs
+    bool uploaded = false;         // pts holds the input (the lane uploads `host` at its next step)
 };
 
 // Device bytes one generator occupies across the engine's buffers (input, generators, bins,
@@ -1251,15 +1261,44 @@ double now_s() {
 
 }  // namespace
 
-struct vd_server {
+// One lane of the server: an engine (with its own stream) and the jobs it runs. The lane of large
+// jobs runs its steps on a worker thread, so that small jobs, stepped by the calling thread on the
+// other lane, do not wait for them; its cell launches leave kReservedCUs to the small lane. While
+// a lane's step runs, only that step touches the lane's engine and jobs; the calling thread applies
+// the outcome (job states, budget) when it collects the step.
+struct Lane {
     Engine E;
+    std::vector<Job *> active;       // jobs of the running or next step
+    std::vector<uint64_t> cells;     // per active job: cells of the step's calculate
+    std::thread worker;
+    std::atomic<bool> running{false};
+    std::atomic<bool> ready{false};
+    int rc = VD_OK;
+    std::string err;
+    uint64_t step_cells = 0;
+};
+
+constexpr uint64_t kSmallJobDefault = 65536;   // jobs with at most this many points use the small lane
+#ifndef VD_RESERVED_CUS
+#define VD_RESERVED_CUS 16
+#endif
+constexpr int kReservedCUs = VD_RESERVED_CUS;
+constexpr size_t kSpareBuffers = 64;
+
+struct vd_server {
+    Lane lanes[2];                 // 0: large jobs (worker thread), 1: small jobs (calling thread)
     size_t budget = 0, used = 0;
     int max_slots = 64;
     uint64_t next_id = 1;
     std::map<uint64_t, Job *> jobs;
     std::deque<Job *> pending;
-    std::vector<Job *> active;
+    std::vector<Job *> admitted0;  // admitted large jobs waiting for the large lane's next step
+    std::vector<DevBuf> spare;     // point buffers of finished jobs, reused by later ones: freeing
+                                   // device memory waits for the whole device, running lanes included
+    std::vector<DevBuf> deferred;  // memory the small lane released while a large step ran
+    size_t nslots = 0;             // admitted, unfinished jobs
     double per_bin = kPerBinDefault;
+    uint64_t small_max = kSmallJobDefault;
     uint64_t steps = 0, cells = 0;
 };
 
@@ -1758,17 +1797,28 @@ int vd_server_create(size_t hbm_budget, int max_slots, vd_server **out) {
     vd_server *S = new vd_server();
     S->budget = hbm_budget;
     S->max_slots = max_slots;
-    S->E.keep_cells = 0;
-    int rc = engine_init(S->E);
-    if (rc) { engine_free(S->E); delete S; return rc; }
+    int rc = VD_OK;
+    for (Lane &L : S->lanes) {
+        L.E.keep_cells = 0;
+        if (!rc) rc = engine_init(L.E);
+    }
+    if (rc) {
+        for (Lane &L : S->lanes) engine_free(L.E);
+        delete S;
+        return rc;
+    }
     *out = S;
     return VD_OK;
 }
 
 void vd_server_destroy(vd_server *S) {
     if (!S) return;
+    for (Lane &L : S->lanes)
+        if (L.worker.joinable()) L.worker.join();
     for (auto &kv : S->jobs) { release(kv.second->pts); delete kv.second; }
-    engine_free(S->E);
+    for (DevBuf &b : S->spare) release(b);
+    for (DevBuf &b : S->deferred) release(b);
+    for (Lane &L : S->lanes) engine_free(L.E);
     delete S;
 }
 
@@ -1800,6 +1850,7 @@ int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], u
     J->iters = iterations;
     J->host.assign(xyz, xyz + 3 * n);
     J->npts = n;
+    J->lane = n <= S->small_max ? 1 : 0;
     J->bytes = n * kBytesPerPoint + (uint64_t)J->m.n[0] * J->m.n[1] * J->m.n[2] * 12 + 4096;
     J->t_submit = now_s();
     J->id = S->next_id++;
@@ -1815,31 +1866,17 @@ int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], u
     return VD_OK;
 }
 
-// One engine step: admit pending jobs (FIFO, while slots and the HBM budget allow), run one
-// batched calculate over every active job, finish the jobs whose last iteration this was, and
-// relax the others in place (their points stay on the device).
-int vd_server_step(vd_server *S, uint32_t *finished) {
-    if (!S) return fail(VD_EINVAL, "NULL handle");
-    if (finished) *finished = 0;
-    Engine &E = S->E;
-    while (!S->pending.empty() && (int)S->active.size() < S->max_slots &&
-           S->used + S->pending.front()->bytes <= S->budget) {
-        Job *J = S->pending.front();
-        S->pending.pop_front();
-        VD_TRY(ensure(J->pts, (J->npts + 1) * 24));
-        if (J->npts)
+// One step of a lane: upload the points of newly admitted jobs, one batched calculate over the
+// lane's jobs, the results of the jobs whose last iteration this was copied to their host buffers,
+// the others' generators relaxed in place (tessellation.rs:482-495; their next input never leaves
+// the device). Touches only the lane's engine and jobs, so it can run on a worker thread.
+int lane_run(Lane &L) {
+    Engine &E = L.E;
+    for (Job *J : L.active)
+        if (!J->uploaded && J->npts)
             VD_HIP(hipMemcpyAsync(J->pts.p, J->host.data(), J->npts * 24, hipMemcpyHostToDevice, E.stream));
-        J->host.clear();
-        J->host.shrink_to_fit();
-        J->state = 1;
-        J->t_admit = now_s();
-        S->used += J->bytes;
-        S->active.push_back(J);
-    }
-    if (S->active.empty()) return VD_OK;
-    // members = active jobs, inputs gathered from the jobs' device buffers
     E.mem.clear();
-    for (Job *J : S->active) {
+    for (Job *J : L.active) {
         Member m = J->m;
         m.nraw = J->npts;
         E.mem.push_back(m);
@@ -1847,33 +1884,27 @@ int vd_server_step(vd_server *S, uint32_t *finished) {
     E.shapes.clear();
     engine_layout(E);
     VD_TRY(ensure(E.d_raw, (E.nraw + 1) * 24));
-    for (size_t t = 0; t < S->active.size(); ++t) {
-        const Job *J = S->active[t];
+    for (size_t t = 0; t < L.active.size(); ++t) {
+        const Job *J = L.active[t];
         if (J->npts)
             VD_HIP(hipMemcpyAsync(E.d_raw.as<double>() + 3 * E.mem[t].raw_off, J->pts.p, J->npts * 24,
                                   hipMemcpyDeviceToDevice, E.stream));
     }
     VD_TRY(engine_prepare(E));
     VD_TRY(engine_calculate(E));
-    S->steps += 1;
-    S->cells += E.ngen;
-    // relaxed points of every member (used by the jobs that continue)
     bool any_more = false;
-    for (const Job *J : S->active) any_more |= J->done + 1 < J->iters;
+    for (const Job *J : L.active) any_more |= J->done + 1 < J->iters;
     if (any_more) {
         VD_TRY(ensure(E.d_next, (E.ngen + 1) * 24));
         VD_HIP(vd::launch_relax(E.d_gen.as<double>(), E.d_cen.as<double>(), E.d_nvert.as<uint32_t>(), E.ngen,
                                 E.d_next.as<double>(), E.stream));
     }
-    std::vector<Job *> keep;
-    const double t = now_s();
-    for (size_t k = 0; k < S->active.size(); ++k) {
-        Job *J = S->active[k];
+    L.cells.assign(L.active.size(), 0);
+    for (size_t k = 0; k < L.active.size(); ++k) {
+        Job *J = L.active[k];
         const Member &m = E.mem[k];
-        J->done += 1;
-        J->cells = m.ngen;
-        J->total_cells += m.ngen;
-        if (J->done >= J->iters) {
+        L.cells[k] = m.ngen;
+        if (J->done + 1 >= J->iters) {
             J->vol.resize(m.ngen);
             J->cen.resize(3 * m.ngen);
             J->gen.resize(3 * m.ngen);
@@ -1885,30 +1916,132 @@ int vd_server_step(vd_server *S, uint32_t *finished) {
                 VD_HIP(hipMemcpyAsync(J->gen.data(), E.d_gen.as<double>() + 3 * m.gen_off, m.ngen * 24,
                                       hipMemcpyDeviceToHost, E.stream));
             }
-            J->state = 2;
-            release(J->pts);
-            S->used -= J->bytes;
-            if (finished) *finished += 1;
-        } else {
-            // tessellation.rs:482-495: the job's next input is its relaxed generators
-            if (m.ngen)
-                VD_HIP(hipMemcpyAsync(J->pts.p, E.d_next.as<double>() + 3 * m.gen_off,
-                                      m.ngen * 24, hipMemcpyDeviceToDevice, E.stream));
-            J->npts = m.ngen;
-            keep.push_back(J);
+        } else if (m.ngen) {
+            VD_HIP(hipMemcpyAsync(J->pts.p, E.d_next.as<double>() + 3 * m.gen_off, m.ngen * 24,
+                                  hipMemcpyDeviceToDevice, E.stream));
         }
     }
     VD_HIP(hipStreamSynchronize(E.stream));
-    for (Job *J : S->active)
-        if (J->state == 2) J->t_done = t;
-    S->active.swap(keep);
+    for (size_t k = 0; k < L.active.size(); ++k) {
+        Job *J = L.active[k];
+        if (!J->uploaded) { J->host.clear(); J->host.shrink_to_fit(); J->uploaded = true; }
+        J->npts = L.cells[k];
+    }
+    L.step_cells = E.ngen;
+    return VD_OK;
+}
+
+// Applies a finished lane step on the calling thread: job states and counters, finished jobs leave
+// the lane and the budget, their point buffers go to the spare list.
+void lane_collect(vd_server *S, Lane &L, uint32_t *finished) {
+    const double t = now_s();
+    std::vector<Job *> keep;
+    for (size_t k = 0; k < L.active.size(); ++k) {
+        Job *J = L.active[k];
+        J->done += 1;
+        J->cells = L.cells[k];
+        J->total_cells += L.cells[k];
+        if (J->done >= J->iters) {
+            J->state = 2;
+            J->t_done = t;
+            S->spare.push_back(J->pts);
+            J->pts = DevBuf();
+            S->used -= J->bytes;
+            S->nslots -= 1;
+            if (finished) *finished += 1;
+        } else {
+            keep.push_back(J);
+        }
+    }
+    L.active.swap(keep);
+    S->steps += 1;
+    S->cells += L.step_cells;
+}
+
+// A device buffer of at least `bytes` for a job's points: the smallest spare one that fits, else a
+// new allocation.
+int take_buffer(vd_server *S, size_t bytes, DevBuf &out) {
+    size_t best = S->spare.size();
+    for (size_t k = 0; k < S->spare.size(); ++k)
+        if (S->spare[k].bytes >= bytes && (best == S->spare.size() || S->spare[k].bytes < S->spare[best].bytes))
+            best = k;
+    if (best < S->spare.size()) {
+        out = S->spare[best];
+        S->spare.erase(S->spare.begin() + (long)best);
+        return VD_OK;
+    }
+    return ensure(out, bytes);
+}
+
+// The large lane's step finished on its worker: join it and apply it.
+int collect_large(vd_server *S, uint32_t *finished) {
+    Lane &L = S->lanes[0];
+    if (!L.running.load() || !L.ready.load()) return VD_OK;
+    L.worker.join();
+    L.running.store(false);
+    L.ready.store(false);
+    if (L.rc) return fail(L.rc, L.err);
+    lane_collect(S, L, finished);
+    return VD_OK;
+}
+
+// One server step: admit pending jobs (FIFO, while slots and the HBM budget allow) to the lane of
+// their size, collect the large lane's step if it finished and start its next one on the worker
+// (new large jobs join there), then run one step of the small lane on the calling thread. With no
+// small work it waits up to a few milliseconds for the large lane, so a polling caller does not spin.
+int vd_server_step(vd_server *S, uint32_t *finished) {
+    if (!S) return fail(VD_EINVAL, "NULL handle");
+    if (finished) *finished = 0;
+    while (!S->pending.empty() && (int)S->nslots < S->max_slots &&
+           S->used + S->pending.front()->bytes <= S->budget) {
+        Job *J = S->pending.front();
+        S->pending.pop_front();
+        VD_TRY(take_buffer(S, (J->npts + 1) * 24, J->pts));
+        J->state = 1;
+        J->t_admit = now_s();
+        S->used += J->bytes;
+        S->nslots += 1;
+        if (J->lane == 1) S->lanes[1].active.push_back(J);
+        else S->admitted0.push_back(J);
+    }
+    VD_TRY(collect_large(S, finished));
+    Lane &L0 = S->lanes[0];
+    if (!L0.running.load()) {
+        // no step in flight: deferred memory and spare buffers beyond kSpareBuffers can be freed
+        for (DevBuf &b : S->deferred) release(b);
+        S->deferred.clear();
+        while (S->spare.size() > kSpareBuffers) { release(S->spare.front()); S->spare.erase(S->spare.begin()); }
+    }
+    if (!L0.running.load() && (!L0.active.empty() || !S->admitted0.empty())) {
+        L0.active.insert(L0.active.end(), S->admitted0.begin(), S->admitted0.end());
+        S->admitted0.clear();
+        L0.E.reserve_cus = S->small_max ? kReservedCUs : 0;
+        L0.running.store(true);
+        L0.ready.store(false);
+        L0.worker = std::thread([&L0]() {
+            L0.rc = lane_run(L0);
+            if (L0.rc) L0.err = vd_last_error();
+            L0.ready.store(true);
+        });
+    }
+    Lane &L1 = S->lanes[1];
+    if (!L1.active.empty()) {
+        g_defer = &S->deferred;        // a running large step must not hold up the small lane
+        const int rc = lane_run(L1);
+        g_defer = nullptr;
+        VD_TRY(rc);
+        lane_collect(S, L1, finished);
+    } else if (L0.running.load()) {
+        for (int k = 0; k < 20 && !L0.ready.load(); ++k) std::this_thread::sleep_for(std::chrono::microseconds(250));
+    }
+    VD_TRY(collect_large(S, finished));
     return VD_OK;
 }
 
 int vd_server_pending(const vd_server *S, uint64_t counts[3]) {
     if (!S || !counts) return fail(VD_EINVAL, "NULL argument");
     counts[0] = S->pending.size();
-    counts[1] = S->active.size();
+    counts[1] = S->nslots;
     counts[2] = S->used;
     return VD_OK;
 }
@@ -1955,6 +2088,11 @@ int vd_server_set_option(vd_server *S, int key, int64_t value) {
     if (key == VD_SOPT_PER_BIN) {
         if (value <= 0) return fail(VD_EINVAL, "per_bin must be positive");
         S->per_bin = (double)value / 1000.0;
+        return VD_OK;
+    }
+    if (key == VD_SOPT_SMALL_JOB) {
+        if (value < 0) return fail(VD_EINVAL, "the small-job limit cannot be negative");
+        S->small_max = (uint64_t)value;
         return VD_OK;
     }
     return fail(VD_EINVAL, "unknown option");

@@ -731,17 +731,21 @@ class Batch:
 
 class Server:
     """Continuous-batching server (vd_server_* in vd.h): tessellation and Lloyd jobs admitted by
-    HBM budget, every step one batched launch over all admitted jobs, Lloyd jobs relaxed on the
-    device between their iterations."""
+    HBM budget, each of two lanes (small jobs, and large ones on a worker thread) running one
+    batched launch over its admitted jobs per step, Lloyd jobs relaxed on the device between their
+    iterations."""
 
     QUEUED, RUNNING, FINISHED, REFUSED = 0, 1, 2, 3
 
-    def __init__(self, hbm_budget: int, max_slots: int = 64, per_bin: float | None = None):
+    def __init__(self, hbm_budget: int, max_slots: int = 64, per_bin: float | None = None,
+                 small_job: int | None = None):
         h = ctypes.c_void_p()
         check(lib().vd_server_create(int(hbm_budget), int(max_slots), ctypes.byref(h)))
         self._h = h
         if per_bin is not None:
-            check(lib().vd_server_set_option(h, 1, int(round(per_bin * 1000))))
+            check(lib().vd_server_set_option(h, _lib.VD_SOPT_PER_BIN, int(round(per_bin * 1000))))
+        if small_job is not None:
+            check(lib().vd_server_set_option(h, _lib.VD_SOPT_SMALL_JOB, int(small_job)))
 
     def close(self):
         h = getattr(self, "_h", None)
