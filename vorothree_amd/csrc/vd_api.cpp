@@ -25,6 +25,7 @@
 namespace {
 
 constexpr int kTiers = 4;   // polytope capacity tiers of the cell kernel (vd_cells.hip)
+constexpr uint64_t kArenaChunk[3] = {512, 256, 1536};   // vertices, faces, face indices per wave chunk
 
 thread_local std::string g_err;
 
@@ -727,11 +728,17 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.fnb = E.d_fnb.as<int32_t>();
     a.fidx = E.d_fidx.as<uint16_t>();
     a.arena_top = E.d_top.as<uint64_t>();
-    for (int k = 0; k < 3; ++k) a.arena_cap[k] = E.cap[k];
     a.cell_off = E.d_cell_off.as<uint64_t>();
     VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>(), 0u, 1, E.stream));
     int waves = std::max(1, E.cus - E.reserve_cus) * E.occ[tier];
     if ((uint32_t)waves > nlist) waves = (int)nlist;
+    // arena chunks per wave: up to kArenaChunk[k] entries, at most an eighth of the capacity over
+    // all waves (the unused tails of the last chunks are holes in the arena)
+    for (int k = 0; k < 3; ++k) {
+        a.arena_cap[k] = E.cap[k];
+        const uint64_t share = E.cap[k] / (8 * (uint64_t)std::max(waves, 1));
+        a.arena_chunk[k] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kArenaChunk[k], share));
+    }
     if (ms) VD_HIP(hipEventRecord(E.ev0, E.stream));
     VD_HIP(vd::launch_cells(tier, a, waves, E.stream, E.seal != 0, probe));
     if (ms) {
@@ -747,7 +754,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
 // counts into pinned memory. Launches and copies only, so the sequence can be captured.
 int enqueue_round0(Engine &E, uint32_t n, float *ms) {
     if (!E.binned) VD_TRY(enqueue_binning(E));
-    VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
+    VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 12, E.stream));
     VD_TRY(run_cells(E, 0, E.d_items.as<uint32_t>(), n, ms));
     VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 4 * kTiers, E.stream));
     VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), E.d_items.as<uint32_t>(), n, E.d_out0.as<uint32_t>(),
@@ -889,7 +896,7 @@ int engine_calculate(Engine &E) {
                 ocap[0][0] = E.d_out0.as<uint32_t>();
                 oshell[0][0] = E.d_out1.as<uint32_t>();
             } else {
-                if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 6, E.stream));
+                if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 12, E.stream));
                 for (int tier = 0; tier < NT; ++tier) {
                     if (!nlist[tier]) continue;
                     if (nsh[tier]) VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true));
@@ -1016,12 +1023,12 @@ int engine_calculate(Engine &E) {
         if (!arena_full) break;
         for (int k = 0; k < 3; ++k) E.cap[k] *= 2;
     }
-    uint64_t top[3] = {0, 0, 0};
+    uint64_t top[6] = {0, 0, 0, 0, 0, 0};
     VD_HIP(hipMemcpyAsync(top, E.d_top.p, sizeof(top), hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
     E.calculated = true;
-    E.stats[6] = E.keep_cells ? (double)top[0] : 0.0;
-    E.stats[7] = E.keep_cells ? (double)top[2] : 0.0;
+    E.stats[6] = E.keep_cells ? (double)top[3] : 0.0;
+    E.stats[7] = E.keep_cells ? (double)top[5] : 0.0;
     size_t entries = 0;
     for (const Shape &s : E.shapes) entries += s.tab->count;
     E.stats[3] = (double)entries;

@@ -50,6 +50,16 @@ namespace {
 constexpr double kTol = 1e-9;
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
+
+// Every cell-kernel workgroup is one wavefront: its lanes execute each LDS instruction together and
+// the LDS unit processes one wave's instructions in order, so a value one lane writes is seen by a
+// later read of another lane without s_barrier. The wavefront-scope fence emits no instruction; it
+// keeps the compiler from moving LDS accesses across the point.
+#ifndef VD_WAVE_SYNC_BARRIER
+__device__ inline void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+#else
+__device__ inline void wave_sync() { __syncthreads(); }
+#endif
 __device__ inline uint64_t lt_mask() { return (1ull << lane_id()) - 1ull; }
 // Set bits of m below this lane (v_mbcnt_lo / v_mbcnt_hi): the prefix count of a ballot.
 __device__ inline int below_lane(uint64_t m) {
@@ -187,6 +197,28 @@ struct Plane { double px, py, pz, nx, ny, nz; };
 // polytope, vertices [0, nk) are bit-exact copies of the kept vertices and [nk, V) are new.
 struct Poly { int cur, V, F, S, nk; };
 
+// A wave's reserved arena ranges [at, end) for vertices, faces and face indices. Cells are carved
+// from them, so the shared arena counters see one atomic per chunk instead of three per cell
+// (with thousands of waves, same-address atomics per cell cost more than the cell's stores).
+struct ArenaChunks { uint64_t at[3] = {0, 0, 0}, end[3] = {0, 0, 0}, used[3] = {0, 0, 0}; };
+
+// Offset of n entries of arena k from the wave's chunk, reserving a new chunk (or a range of its
+// own for a cell larger than a chunk) when the current one cannot hold them. Wave-uniform.
+__device__ inline uint64_t arena_take(const CellArgs &A, ArenaChunks &ac, int k, uint64_t n) {
+    if (ac.at[k] + n > ac.end[k]) {
+        const uint64_t sz = n > A.arena_chunk[k] ? n : (uint64_t)A.arena_chunk[k];
+        uint64_t base = 0;
+        if (lane_id() == 0) base = atomicAdd((unsigned long long *)&A.arena_top[k], (unsigned long long)sz);
+        base = bcast_u64(base);
+        ac.at[k] = base;
+        ac.end[k] = base + sz;
+    }
+    const uint64_t o = ac.at[k];
+    ac.at[k] += n;
+    ac.used[k] += n;
+    return o;
+}
+
 // Lane l's register copies of vertices l, 64 + l, ... (NV per lane; the fast tiers only).
 template <int NV>
 struct VRegN { double x[NV], y[NV], z[NV]; };
@@ -258,7 +290,7 @@ __device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi
     if (l < 7) m.fstart[0][l] = (uint16_t)(4 * l);
     if (l < 6) m.fnb[0][l] = -(int32_t)((0x214365u >> (4 * l)) & 0xf);  // -5 -6 -3 -4 -1 -2
     P.cur = 0; P.V = 8; P.F = 6; P.S = 24; P.nk = 0;
-    __syncthreads();
+    wave_sync();
 }
 
 // Half-space clip keeping (x - p).n <= 1e-9 (cell_3d_faces.rs:288-503). Returns kNoChange,
@@ -304,7 +336,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         r2 = 0.0;
         return kModified;
     }
-    __syncthreads();
+    wave_sync();
 
     // 2. per slot: output entry count and plane crossings (:378-452)
     int K = 0, ptot = 0;
@@ -332,7 +364,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     }
     if (l == 0) m.P[P.S] = (uint16_t)ptot;
     if (K > C::KM || ptot > 0xffff) return kOverflow;
-    __syncthreads();
+    wave_sync();
 
     // 3. per face: surviving faces (>= 3 entries, :454-456) and their new offsets
     int Fn = 0, Spre = 0;
@@ -401,12 +433,12 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         }
     }
     if (nk + nint > C::VM) return kOverflow;
-    __syncthreads();
+    wave_sync();
     for (int ci = l; ci < K; ci += 64) {
         const uint16_t v = m.cxid[ci];
         if (v & 0x8000) m.cxid[ci] = m.cxid[v & 0x7fff];
     }
-    __syncthreads();
+    wave_sync();
 
     // 5. emit the surviving faces' entries (:386-451, :466)
     int Kb = 0;
@@ -439,7 +471,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         }
     }
     if (l == 0) m.fstart[o][Fn] = (uint16_t)Spre;
-    __syncthreads();
+    wave_sync();
 
     // 6. successor of every new slot; lid segments (v, u) for intersection pairs (:459-465)
     int M = 0;
@@ -460,12 +492,12 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     // 7. lid polygon (:471-495): map[first] = second (later segments overwrite), walk from seg[0]
     if (M > 0) {
         for (int t = l; t < nint; t += 64) m.lmap[t] = 0u;
-        __syncthreads();
+        wave_sync();
         for (int t = l; t < M; t += 64) {
             const uint32_t sg = m.seg[t];
             atomicMax(&m.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
         }
-        __syncthreads();
+        wave_sync();
         const uint32_t s0 = m.seg[0];
         const int start = (int)(s0 & 0xffffu);
         int cv = (int)(s0 >> 16);
@@ -481,7 +513,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         }
         if (len >= 3) {
             if (Spre + len > C::SM || len > 255) return len > 255 ? kFace255 : kOverflow;
-            __syncthreads();
+            wave_sync();
             for (int t = l; t < len; t += 64) {
                 const uint32_t u = m.lid[t], w = m.lid[(t + 1) % len];
                 m.sedge[o][Spre + t] = u | (w << 16);
@@ -496,7 +528,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
             Spre += len;
         }
     }
-    __syncthreads();
+    wave_sync();
     P.cur = o;
     P.V = nk + nint;
     P.F = Fn;
@@ -600,7 +632,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     }
     if (K > 64) return kGeneric;
     if (l == 0) m.P[P.S] = (uint16_t)ptot;
-    __syncthreads();
+    wave_sync();
 
     // 3. faces: survivors (>= 3 entries, :454-456) and their new offsets
     const bool fa = l < P.F;
@@ -662,7 +694,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
         }
     }
-    __syncthreads();
+    wave_sync();
     // the old vertices are read: store the kept ones at [0, nk) and the new ones after them
     int kbase = 0;
 #pragma unroll
@@ -674,7 +706,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         kbase += popc(inm.w[k]);
     }
     if (isf) { m.vx[o][myx] = x; m.vy[o][myx] = y; m.vz[o][myx] = z; }
-    __syncthreads();
+    wave_sync();
 
     // 5. emit the entries of every surviving face (:386-451), with each entry's successor slot
 #pragma unroll
@@ -703,7 +735,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             }
         }
     }
-    __syncthreads();
+    wave_sync();
 
     // 6. successor of every new slot; lid segments (v, u) for intersection pairs (:459-465)
     int M = 0;
@@ -723,12 +755,12 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     // 7. lid polygon (:471-495); lane j holds the map entry of new vertex nk + j
     if (M > 0) {
         if (l < nint) m.lmap[l] = 0u;
-        __syncthreads();
+        wave_sync();
         for (int t = l; t < M; t += 64) {
             const uint32_t sg = m.seg[t];
             atomicMax(&m.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
         }
-        __syncthreads();
+        wave_sync();
         const uint32_t mv = (l < nint) ? m.lmap[l] : 0u;
         const uint32_t s0 = m.seg[0];
         const int start = (int)(s0 & 0xffffu);
@@ -759,7 +791,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             Spre += len;
         }
     }
-    __syncthreads();
+    wave_sync();
     P.cur = o;
     P.V = nk + nint;
     P.F = Fn;
@@ -848,12 +880,12 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
         }
     }
     if (staged) {
-        __syncthreads();
+        wave_sync();
         double acc = 0.0;
         if (l < 4)
             for (int t = 0; t < T; ++t) acc += m.tri[l][t];
         tv = rld(acc, 0); sx = rld(acc, 1); sy = rld(acc, 2); sz = rld(acc, 3);
-        __syncthreads();
+        wave_sync();
     }
     vol = fabs(tv / 6.0);
     if (fabs(tv) < 1e-9) {
@@ -961,7 +993,7 @@ __device__ inline int probe_inert(const ShellCode *shell, uint32_t nshell, const
 // fast tiers only in the instantiation that reruns cells whose walk ran past the shell prefix,
 // because the probe's registers slow the common walk down.
 template <class C, bool SEAL, bool PROBE>
-__device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
+__device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChunks &ac) {
     const int l = lane_id();
     const TessDesc &D = A.desc[A.gen_tess[i]];   // this cell's tessellation (wave-uniform)
     const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
@@ -1104,7 +1136,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
                         m.winoff[pos + t] = (uint16_t)(skip + t);
                     }
                 }
-                __syncthreads();
+                wave_sync();
                 const bool wa = l < nwin;
                 const uint32_t wq = wa ? m.winq[l] : 0u;
                 wko = wa ? ((uint32_t)m.winent[l] | ((uint32_t)m.winoff[l] << 8)) : 0u;
@@ -1121,7 +1153,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
                           ? pretest<C>(m, P, wa && wj != li, wd0, wd1, wd2, gx, gy, gz, 0, P.V) : ~0ull;
                 last = -1;
                 win = true;
-                __syncthreads();
+                wave_sync();
             }
             // next candidate of the window that passes the distance test (tessellation.rs:458)
             const double lim = 4.0 * r2;
@@ -1202,7 +1234,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
         const double wbd = lane_get_d(bdist, wk);   // bin distance of the candidate's entry
         wvm &= ballot(l <= cl || wk == kc || (wk < kbc && wbd <= lim));
     }
-    __syncthreads();
+    wave_sync();
 
     if constexpr (SEAL) {
         // calculate_sealed (tessellation.rs:266-315): for every neighbour of the previous pass, in
@@ -1268,7 +1300,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
     if (l == 0) A.status[i] = status;
     if (status != kCellOk) {
         if (status == kCellShell && l == 0) A.vol[i] = r2;
-        __syncthreads();
+        wave_sync();
         return;
     }
     double vol, cen[3];
@@ -1282,15 +1314,9 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
         A.nvert[i] = (uint32_t)P.V;
     }
     if (A.vtx) {
-        uint64_t ov = 0, of = 0, oi = 0;
-        if (l == 0) {
-            ov = atomicAdd((unsigned long long *)&A.arena_top[0], (unsigned long long)P.V);
-            of = atomicAdd((unsigned long long *)&A.arena_top[1], (unsigned long long)P.F);
-            oi = atomicAdd((unsigned long long *)&A.arena_top[2], (unsigned long long)P.S);
-        }
-        ov = bcast_u64(ov);
-        of = bcast_u64(of);
-        oi = bcast_u64(oi);
+        const uint64_t ov = arena_take(A, ac, 0, (uint64_t)P.V);
+        const uint64_t of = arena_take(A, ac, 1, (uint64_t)P.F);
+        const uint64_t oi = arena_take(A, ac, 2, (uint64_t)P.S);
         if (ov + P.V > A.arena_cap[0] || of + P.F > A.arena_cap[1] || oi + P.S > A.arena_cap[2]) {
             if (l == 0) A.status[i] = kCellArena;
         } else {
@@ -1313,9 +1339,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
         }
     } else {
         // reduced mode keeps the neighbour ids (face order) only
-        uint64_t of = 0;
-        if (l == 0) of = atomicAdd((unsigned long long *)&A.arena_top[1], (unsigned long long)P.F);
-        of = bcast_u64(of);
+        const uint64_t of = arena_take(A, ac, 1, (uint64_t)P.F);
         if (of + P.F > A.arena_cap[1]) {
             if (l == 0) A.status[i] = kCellArena;
         } else {
@@ -1323,20 +1347,25 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i) {
             if (l == 0) A.cell_off[3 * (size_t)i + 1] = of;
         }
     }
-    __syncthreads();
+    wave_sync();
 }
 
 template <class C, bool SEAL, bool PROBE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C::WAVES, 8)))
 void k_cells(CellArgs A) {
     __shared__ Mem<C> m;
+    ArenaChunks ac;
     for (;;) {
         uint32_t q = 0;
         if (lane_id() == 0) q = atomicAdd(A.queue, 1u);
         q = first_lane_u32(q);
         if (q >= A.ncells) break;
-        process_cell<C, SEAL, PROBE>(m, A, A.cells[q]);
+        process_cell<C, SEAL, PROBE>(m, A, A.cells[q], ac);
     }
+    // entries the wave's cells occupy (the chunks' unused tails excluded)
+    if (lane_id() == 0)
+        for (int k = 0; k < 3; ++k)
+            if (ac.used[k]) atomicAdd((unsigned long long *)&A.arena_top[3 + k], (unsigned long long)ac.used[k]);
 }
 
 // Tier 0: register fast path (one vertex per lane), small LDS (many waves per CU). Tier 1: the

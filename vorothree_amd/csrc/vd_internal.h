@@ -111,8 +111,9 @@ struct CellArgs {
     uint8_t *fcnt;                // arena of face vertex counts
     int32_t *fnb;                 // arena of face neighbour ids (same indexing as fcnt)
     uint16_t *fidx;               // arena of face vertex indices
-    uint64_t *arena_top;          // [0] vertices, [1] faces, [2] face indices
+    uint64_t *arena_top;          // reserved [0] vertices, [1] faces, [2] face indices; [3..5] used
     uint64_t arena_cap[3];
+    uint32_t arena_chunk[3];      // entries a wave reserves at a time (cells are carved from them)
     uint64_t *cell_off;           // 3 per cell: vertex, face, index offsets
 
     // seal mode (calculate_sealed, tessellation.rs:234-318): cell topology of the previous pass
