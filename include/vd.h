@@ -163,6 +163,9 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
                                may differ), its first round (binning, the fast-tier cell launch,
                                the retry collect) is captured once into a hipGraph and replayed;
                                0: always launch eagerly. Results are identical either way. */
+#define VD_OPT_ARENA 3      /* initial full-cell arena per cell, in vertices (default 40; 3/5 as many
+                               faces and 3 times as many face indices): a launch that fills it
+                               doubles it and runs again, with identical results */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
 
 /* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cell runs

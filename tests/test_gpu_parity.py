@@ -217,6 +217,33 @@ def test_sparse_grid_extends_shell_prefix():
     assert abs(vol.sum() - 1.0) < 1e-12
 
 
+def test_arena_growth_keeps_cells():
+    # an arena of one vertex per cell fills at once: it doubles until the cells fit (the waves'
+    # arena chunks shrink with it), and every cell comes out as with the default arena
+    w = W.config2(n=40_000, grid=34)
+    b = BoundingBox(w.lo, w.hi)
+    ref = Tessellation(b, Algorithm3DGrid(*w.grid, b))
+    small = Tessellation(b, Algorithm3DGrid(*w.grid, b))
+    small.set_arena(1)
+    for t in (ref, small):
+        t.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5)))
+        t.set_generators(w.points)
+        t.calculate()
+    n = ref.count_cells()
+    assert small.count_cells() == n
+    assert np.array_equal(bits(small.volumes()), bits(ref.volumes()))
+    assert small.stats()[6] == ref.stats()[6] and small.stats()[7] == ref.stats()[7]
+    for i in range(0, n, 37):
+        c, r = small.get_cell(i), ref.get_cell(i)
+        assert np.array_equal(bits(c.vertices()), bits(r.vertices()))
+        assert np.array_equal(c.face_counts(), r.face_counts())
+        assert np.array_equal(c.face_indices(), r.face_indices())
+        assert np.array_equal(c.face_neighbors(), r.face_neighbors())
+    off_a, ids_a = small.neighbors()
+    off_b, ids_b = ref.neighbors()
+    assert np.array_equal(off_a, off_b) and np.array_equal(ids_a, ids_b)
+
+
 def test_voids_extend_once_and_probe_on_every_later_calculate():
     # 8 clusters in empty space: cells next to the voids walk past the first shell prefix; the
     # prefix grows once to the length their radius needs and they rerun in the skip-ahead

@@ -386,6 +386,8 @@ struct Engine {
     int seal = 0;                      // cell launches run the seal mode
     bool trunc0 = false;               // the first launch sees only the first prefix of far geometries
     int reserve_cus = 0;               // CUs the persistent cell launches leave to other engines
+    uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
+    uint64_t cap_cells = 0;            // cells the arena capacities were sized for
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
     uint32_t ntube = 0;                // tube-wall slots per generator
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
@@ -843,10 +845,14 @@ int engine_calculate(Engine &E) {
     VD_TRY(ensure(E.d_top, 64));
     VD_TRY(ensure(E.d_queue, 64));
     VD_TRY(ensure(E.d_counts4, 8 * kTiers * sizeof(uint32_t)));
-    if (E.cap[1] < n * 16) {
-        E.cap[0] = n * 40 + 4096;
-        E.cap[1] = n * 24 + 4096;
-        E.cap[2] = n * 120 + 4096;
+    // arena sizes before the first calculate or after more cells: arena_hint vertices, 3/5 of
+    // that many faces and three times as many face indices per cell (a uniform cell has about 27,
+    // 16 and 82); a launch that fills an arena doubles all three and runs again
+    const uint64_t h = E.arena_hint;
+    if (E.cap_cells < n) {
+        const uint64_t want[3] = {n * h + 4096, n * h * 3 / 5 + 4096, n * h * 3 + 4096};
+        for (int k = 0; k < 3; ++k) E.cap[k] = E.cap_cells ? std::max(E.cap[k], want[k]) : want[k];
+        E.cap_cells = n;
     }
     E.calculated = false;
     if (n == 0) { E.calculated = true; return VD_OK; }
@@ -1581,6 +1587,12 @@ int vd_tess_set_option(vd_tess *t, int key, int64_t value) {
     }
     if (key == VD_OPT_GRAPHS) {
         t->E.graphs = value ? 1 : 0;
+        return VD_OK;
+    }
+    if (key == VD_OPT_ARENA) {
+        if (value < 1) return fail(VD_EINVAL, "at least one arena vertex per cell");
+        t->E.arena_hint = (uint64_t)value;
+        t->E.cap_cells = 0;   // resize at the next calculate
         return VD_OK;
     }
     return fail(VD_EINVAL, "unknown option");

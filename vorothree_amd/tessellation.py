@@ -501,6 +501,10 @@ class Tessellation:
         """Replay repeated first-round launch sequences from a captured hipGraph (VD_OPT_GRAPHS)."""
         check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_GRAPHS, 1 if on else 0))
 
+    def set_arena(self, vertices_per_cell: int) -> None:
+        """Initial full-cell arena per cell (VD_OPT_ARENA); a calculate that fills it grows it."""
+        check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_ARENA, int(vertices_per_cell)))
+
     # -- queries -------------------------------------------------------------------------------
     def count_generators(self) -> int:
         return int(lib().vd_tess_count_generators(self._h))
