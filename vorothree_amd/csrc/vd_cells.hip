@@ -173,7 +173,7 @@ struct Mem {
             uint32_t ckey[C::KM];
             uint32_t cab[C::KM];
             uint32_t lmap[C::VM];
-            uint32_t seg[C::SM];
+            uint32_t seg[C::FAST ? C::KM : C::SM];   // lid segments: one per cut face
             int32_t fbase[C::FM];
             uint16_t remap[C::FAST ? 1 : C::VM];   // generic path only
             uint16_t P[C::SM + 1];
@@ -748,9 +748,10 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         if (pa) m.sedge[o][p] = ua | (vb << 16);
         const bool sg = pa && (int)ua >= nk && (int)vb >= nk;
         const uint64_t gm = ballot(sg);
-        if (sg) m.seg[M + below_lane(gm)] = vb | (ua << 16);
+        if (sg && M + below_lane(gm) < C::KM) m.seg[M + below_lane(gm)] = vb | (ua << 16);
         M += popc(gm);
     }
+    if (M > C::KM) return kGeneric;   // a convex face has one such segment: at most K / 2
 
     // 7. lid polygon (:471-495); lane j holds the map entry of new vertex nk + j
     if (M > 0) {
@@ -1372,7 +1373,7 @@ void k_cells(CellArgs A) {
 // same path with two vertices per lane (128 vertices). Tiers 2-3: generic path for cells that
 // outgrow the tier before, with 256 and 1024 vertices (the last one wave per CU).
 #ifndef VD_TIER0_WAVES
-#define VD_TIER0_WAVES 4
+#define VD_TIER0_WAVES 5
 #endif
 using Tier0 = Cap<64, 192, 48, 64, true, 107, VD_TIER0_WAVES>;
 using Tier1 = Cap<128, 384, 64, 64, true, 127, 2>;
