@@ -51,6 +51,15 @@ constexpr double kTol = 1e-9;
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
 
+// The lane index computed again where it is used (v_mbcnt in a volatile block, no memory access):
+// LDS addresses derived from it in once-per-cell code then stay there, instead of being hoisted to
+// the kernel's start where they would hold registers, or scratch, through the whole walk.
+__device__ inline int lane_id_here() {
+    int r;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(r));
+    return r;
+}
+
 // Every cell-kernel workgroup is one wavefront: its lanes execute each LDS instruction together and
 // the LDS unit processes one wave's instructions in order, so a value one lane writes is seen by a
 // later read of another lane without s_barrier. The wavefront-scope fence emits no instruction; it
@@ -229,7 +238,7 @@ using VReg = VRegN<C::NV>;
 template <class C>
 __device__ inline void vreg_load(Mem<C> &m, const Poly &P, VReg<C> &r) {
     if constexpr (C::FAST) {
-        const int l = lane_id();
+        const int l = lane_id_here();
 #pragma unroll
         for (int k = 0; k < C::NV; ++k) {
             const int v = 64 * k + l;
@@ -266,7 +275,7 @@ template <class C>
 __device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi[3]) {
     // cell_3d_faces.rs:37-74: vertex order (lo,lo,lo) (hi,lo,lo) (hi,hi,lo) (lo,hi,lo), then z=hi;
     // faces z-, z+, y-, y+, x-, x+ with the listed index cycles; neighbours -5 -6 -3 -4 -1 -2.
-    const int l = lane_id();
+    const int l = lane_id_here();
     if (l < 8) {
         const int xb = (l == 1 || l == 2 || l == 5 || l == 6);
         const int yb = (l == 2 || l == 3 || l == 6 || l == 7);
@@ -818,7 +827,7 @@ __device__ inline int clip_any(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, i
 template <class C>
 __device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double gy, double gz) {
     double md = 0.0;  // cell_3d_faces.rs:271-286
-    for (int v = lane_id(); v < P.V; v += 64) {
+    for (int v = lane_id_here(); v < P.V; v += 64) {
         const double dx = m.vx[P.cur][v] - gx, dy = m.vy[P.cur][v] - gy, dz = m.vz[P.cur][v] - gz;
         md = gt_max(md, dx * dx + dy * dy + dz * dz);
     }
@@ -831,7 +840,7 @@ __device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double
 // that order. More than TM triangles fall back to a fold by lane reads.
 template <class C>
 __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]) {
-    const int l = lane_id();
+    const int l = lane_id_here();
     const int c = P.cur;
     const int T = P.S - 2 * P.F;
     const bool staged = T <= C::TM;
@@ -1298,15 +1307,16 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
 
     // outputs; a cell that walked past its shell prefix leaves its radius r^2 in the volume slot,
     // so the prefix can be extended to the length its rerun needs
-    if (l == 0) A.status[i] = status;
+    const int lo = lane_id_here();   // once-per-cell code: see lane_id_here
+    if (lo == 0) A.status[i] = status;
     if (status != kCellOk) {
-        if (status == kCellShell && l == 0) A.vol[i] = r2;
+        if (status == kCellShell && lo == 0) A.vol[i] = r2;
         wave_sync();
         return;
     }
     double vol, cen[3];
     poly_reduce<C>(m, P, vol, cen);
-    if (l == 0) {
+    if (lo == 0) {
         A.vol[i] = vol;
         A.cen[3 * (size_t)i] = cen[0];
         A.cen[3 * (size_t)i + 1] = cen[1];
@@ -1319,20 +1329,20 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
         const uint64_t of = arena_take(A, ac, 1, (uint64_t)P.F);
         const uint64_t oi = arena_take(A, ac, 2, (uint64_t)P.S);
         if (ov + P.V > A.arena_cap[0] || of + P.F > A.arena_cap[1] || oi + P.S > A.arena_cap[2]) {
-            if (l == 0) A.status[i] = kCellArena;
+            if (lo == 0) A.status[i] = kCellArena;
         } else {
             const int c = P.cur;
-            for (int v = l; v < P.V; v += 64) {
+            for (int v = lo; v < P.V; v += 64) {
                 A.vtx[3 * (ov + v)] = m.vx[c][v];
                 A.vtx[3 * (ov + v) + 1] = m.vy[c][v];
                 A.vtx[3 * (ov + v) + 2] = m.vz[c][v];
             }
-            for (int f = l; f < P.F; f += 64) {
+            for (int f = lo; f < P.F; f += 64) {
                 A.fcnt[of + f] = (uint8_t)(m.fstart[c][f + 1] - m.fstart[c][f]);
                 A.fnb[of + f] = m.fnb[c][f];
             }
-            for (int s = l; s < P.S; s += 64) A.fidx[oi + s] = (uint16_t)(m.sedge[c][s] & 0xffffu);
-            if (l == 0) {
+            for (int s = lo; s < P.S; s += 64) A.fidx[oi + s] = (uint16_t)(m.sedge[c][s] & 0xffffu);
+            if (lo == 0) {
                 A.cell_off[3 * (size_t)i] = ov;
                 A.cell_off[3 * (size_t)i + 1] = of;
                 A.cell_off[3 * (size_t)i + 2] = oi;
@@ -1342,10 +1352,10 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
         // reduced mode keeps the neighbour ids (face order) only
         const uint64_t of = arena_take(A, ac, 1, (uint64_t)P.F);
         if (of + P.F > A.arena_cap[1]) {
-            if (l == 0) A.status[i] = kCellArena;
+            if (lo == 0) A.status[i] = kCellArena;
         } else {
-            for (int f = l; f < P.F; f += 64) A.fnb[of + f] = m.fnb[P.cur][f];
-            if (l == 0) A.cell_off[3 * (size_t)i + 1] = of;
+            for (int f = lo; f < P.F; f += 64) A.fnb[of + f] = m.fnb[P.cur][f];
+            if (lo == 0) A.cell_off[3 * (size_t)i + 1] = of;
         }
     }
     wave_sync();
