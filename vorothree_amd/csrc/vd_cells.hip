@@ -560,7 +560,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     constexpr int NCH = 3 * NV;                 // slot chunks
     constexpr uint32_t SH = NV == 1 ? 6 : 7;    // bits of a vertex index in the packed slot word
     constexpr uint32_t VMSK = (1u << SH) - 1u;
-    const int l = lane_id();
+    const int l = lane_id_here();
     // One buffer: every read of an old entry comes before the write that can replace it (the slot
     // and face words are in registers after step 2; old vertices are last read in step 4, so the
     // kept ones are stored after it). A clip that fails part way leaves a mixed polytope, but every
@@ -1004,7 +1004,7 @@ __device__ inline int probe_inert(const ShellCode *shell, uint32_t nshell, const
 // because the probe's registers slow the common walk down.
 template <class C, bool SEAL, bool PROBE>
 __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChunks &ac) {
-    const int l = lane_id();
+    const int l = lane_id_here();
     const TessDesc &D = A.desc[A.gen_tess[i]];   // this cell's tessellation (wave-uniform)
     const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
     const uint32_t li = (uint32_t)(i - D.gen_off);   // index within its tessellation
