@@ -371,7 +371,8 @@ struct Engine {
     int device = 0, cus = 1;
     int occ[kTiers] = {1, 1, 1, 1};
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t stream2 = nullptr;     // retry launches of cells crossing empty space, beside the rest
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     int keep_cells = 1;
 
     DevBuf d_desc, d_offs, d_raw, d_raw_tess, d_flags, d_pos, d_scan, d_gat;
@@ -411,6 +412,9 @@ int engine_init(Engine &E) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&E.ev0);
     if (e == hipSuccess) e = hipEventCreate(&E.ev1);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&E.stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&E.ev2, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&E.ev3, hipEventDisableTiming);
     for (int k = 0; k < kTiers && e == hipSuccess; ++k) e = vd::cells_occupancy(k, &E.occ[k]);
     if (e == hipSuccess) e = hipHostMalloc((void **)&E.h_counts, 64 * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) return fail(VD_EDEVICE, std::string("device setup: ") + hipGetErrorString(e));
@@ -441,11 +445,15 @@ void engine_free(Engine &E) {
     for (auto &kv : E.tables) release(kv.second.buf);
     E.tables.clear();
     E.shapes.clear();
+    if (E.stream2) (void)hipStreamSynchronize(E.stream2);
     if (E.ev0) (void)hipEventDestroy(E.ev0);
     if (E.ev1) (void)hipEventDestroy(E.ev1);
+    if (E.ev2) (void)hipEventDestroy(E.ev2);
+    if (E.ev3) (void)hipEventDestroy(E.ev3);
     if (E.stream) (void)hipStreamDestroy(E.stream);
-    E.ev0 = E.ev1 = nullptr;
-    E.stream = nullptr;
+    if (E.stream2) (void)hipStreamDestroy(E.stream2);
+    E.ev0 = E.ev1 = E.ev2 = E.ev3 = nullptr;
+    E.stream = E.stream2 = nullptr;
 }
 
 int scan(Engine &E, const uint32_t *in, uint32_t *out, uint32_t n) {
@@ -701,7 +709,11 @@ int enqueue_binning(Engine &E) {
     return VD_OK;
 }
 
-int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms, bool probe = false) {
+// One cell launch over d_list on stream st (the engine's stream by default), drawing cells from work
+// counter `slot` of d_queue (launches that may run at the same time use different counters).
+int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms, bool probe = false,
+              int slot = 0, hipStream_t st = nullptr) {
+    if (!st) st = E.stream;
     vd::CellArgs a{};
     a.desc = E.d_desc.as<vd::TessDesc>();
     a.gen_tess = E.d_gen_tess.as<uint32_t>();
@@ -719,7 +731,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     }
     a.cells = d_list;
     a.ncells = nlist;
-    a.queue = E.d_queue.as<uint32_t>();
+    a.queue = E.d_queue.as<uint32_t>() + slot;
     a.vol = E.d_vol.as<double>();
     a.cen = E.d_cen.as<double>();
     a.nface = E.d_nface.as<uint32_t>();
@@ -731,7 +743,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.fidx = E.d_fidx.as<uint16_t>();
     a.arena_top = E.d_top.as<uint64_t>();
     a.cell_off = E.d_cell_off.as<uint64_t>();
-    VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>(), 0u, 1, E.stream));
+    VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>() + slot, 0u, 1, st));
     int waves = std::max(1, E.cus - E.reserve_cus) * E.occ[tier];
     if ((uint32_t)waves > nlist) waves = (int)nlist;
     // arena chunks per wave: up to kArenaChunk[k] entries, at most an eighth of the capacity over
@@ -741,10 +753,10 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
         const uint64_t share = E.cap[k] / (8 * (uint64_t)std::max(waves, 1));
         a.arena_chunk[k] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kArenaChunk[k], share));
     }
-    if (ms) VD_HIP(hipEventRecord(E.ev0, E.stream));
-    VD_HIP(vd::launch_cells(tier, a, waves, E.stream, E.seal != 0, probe));
+    if (ms) VD_HIP(hipEventRecord(E.ev0, st));
+    VD_HIP(vd::launch_cells(tier, a, waves, st, E.seal != 0, probe));
     if (ms) {
-        VD_HIP(hipEventRecord(E.ev1, E.stream));
+        VD_HIP(hipEventRecord(E.ev1, st));
         VD_HIP(hipEventSynchronize(E.ev1));
         VD_HIP(hipEventElapsedTime(ms, E.ev0, E.ev1));
     }
@@ -903,13 +915,20 @@ int engine_calculate(Engine &E) {
                 oshell[0][0] = E.d_out1.as<uint32_t>();
             } else {
                 if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 12, E.stream));
+                // group 1 (cells crossing empty space: few, long walks) runs on the second stream,
+                // so the rest of the round fills the CUs its long-running waves leave idle
+                VD_HIP(hipEventRecord(E.ev2, E.stream));
+                VD_HIP(hipStreamWaitEvent(E.stream2, E.ev2, 0));
                 for (int tier = 0; tier < NT; ++tier) {
                     if (!nlist[tier]) continue;
-                    if (nsh[tier]) VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true));
+                    if (nsh[tier]) VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier, E.stream2));
                     if (nlist[tier] > nsh[tier])
-                        VD_TRY(run_cells(E, tier, list[tier] + nsh[tier], nlist[tier] - nsh[tier], nullptr));
+                        VD_TRY(run_cells(E, tier, list[tier] + nsh[tier], nlist[tier] - nsh[tier], nullptr, false,
+                                         1 + NT + tier));
                     if (tier > 0) E.stats[1] += nlist[tier];
                 }
+                VD_HIP(hipEventRecord(E.ev3, E.stream2));
+                VD_HIP(hipStreamWaitEvent(E.stream, E.ev3, 0));
                 VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 8 * NT, E.stream));
                 for (int tier = 0; tier < NT; ++tier) {
                     for (int g = 0; g < 2; ++g) {

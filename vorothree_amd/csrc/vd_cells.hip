@@ -165,6 +165,54 @@ struct Cap {
     static constexpr int NB = FAST_ ? 1 : 2;
 };
 
+// Scratch of one clip. The fast path's fields live in phases: A (steps 1-4: distances, crossing
+// keys, output prefixes), B (steps 3-5: face bookkeeping) and C (steps 5-7: new slot lists, lid
+// segments). A and C are never live together, so they share bytes. The generic path keeps one
+// field per name.
+template <class C, bool FAST = C::FAST>
+struct ClipScratch;
+
+template <class C>
+struct ClipScratch<C, true> {
+    union {
+        struct {
+            double dist[C::VM];
+            uint32_t ckey[C::KM];
+            uint32_t cab[C::KM];
+            uint16_t P[C::SM + 1];
+        };
+        struct {
+            uint16_t fidxn[C::SM];
+            uint16_t nextp[C::SM];
+            uint32_t seg[C::KM];    // lid segments: one per cut face
+            uint32_t lmap[C::KM];   // by new vertex minus the kept count (at most K new vertices)
+        };
+    };
+    int32_t fbase[C::FM];
+    int16_t fnew[C::FM];
+    uint16_t fnoff[C::FM];
+    uint16_t nend[C::FM];
+};
+
+template <class C>
+struct ClipScratch<C, false> {
+    double dist[C::VM];
+    uint32_t ckey[C::KM];
+    uint32_t cab[C::KM];
+    uint32_t lmap[C::VM];
+    uint32_t seg[C::SM];
+    int32_t fbase[C::FM];
+    uint16_t remap[C::VM];
+    uint16_t P[C::SM + 1];
+    uint16_t cxid[C::KM];
+    uint16_t fidxn[C::SM];
+    uint16_t nextp[C::SM];
+    int16_t fnew[C::FM];
+    uint16_t fnoff[C::FM];
+    uint16_t nend[C::FM];
+    uint16_t lid[C::SM];
+};
+
 template <class C>
 struct Mem {
     double vx[C::NB][C::VM], vy[C::NB][C::VM], vz[C::NB][C::VM];
@@ -177,23 +225,7 @@ struct Mem {
     uint16_t winoff[64];
     // Scratch of one clip; at the end of a cell the same bytes stage the fold terms.
     union {
-        struct {
-            double dist[C::VM];
-            uint32_t ckey[C::KM];
-            uint32_t cab[C::KM];
-            uint32_t lmap[C::VM];
-            uint32_t seg[C::FAST ? C::KM : C::SM];   // lid segments: one per cut face
-            int32_t fbase[C::FM];
-            uint16_t remap[C::FAST ? 1 : C::VM];   // generic path only
-            uint16_t P[C::SM + 1];
-            uint16_t cxid[C::FAST ? 1 : C::KM];    // generic path only
-            uint16_t fidxn[C::SM];
-            uint16_t nextp[C::SM];
-            int16_t fnew[C::FM];
-            uint16_t fnoff[C::FM];
-            uint16_t nend[C::FM];
-            uint16_t lid[C::FAST ? 1 : C::SM];     // generic path only
-        };
+        ClipScratch<C> sc;
         double tri[4][C::TM + 1];   // +1 keeps the four rows in different LDS banks
     };
 };
@@ -322,7 +354,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         if (act) {
             x = m.vx[c][v]; y = m.vy[c][v]; z = m.vz[c][v];
             d = (x - pl.px) * pl.nx + (y - pl.py) * pl.ny + (z - pl.pz) * pl.nz;
-            m.dist[v] = d;
+            m.sc.dist[v] = d;
         }
         gtm |= ballot(act && d > kTol);
         ltm |= ballot(act && d < -kTol);
@@ -330,7 +362,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const uint64_t im = ballot(in);
         if (in) {
             const int ni = nk + below_lane(im);
-            m.remap[v] = (uint16_t)ni;
+            m.sc.remap[v] = (uint16_t)ni;
             m.vx[o][ni] = x; m.vy[o][ni] = y; m.vz[o][ni] = z;
             if (withgen) {
                 const double dx = x - gx, dy = y - gy, dz = z - gz;
@@ -354,24 +386,24 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const bool act = s < P.S;
         const uint32_t e = act ? m.sedge[c][s] : 0u;
         const uint32_t a = e & 0xffffu, b = e >> 16;
-        const double da = m.dist[a], db = m.dist[b];
+        const double da = m.sc.dist[a], db = m.sc.dist[b];
         const bool ain = da <= kTol, bin = db <= kTol;
         const uint64_t b0 = ballot(act && ain);            // in->in, in->out: 1 entry
         const uint64_t b1 = ballot(act && !ain && bin);    // out->in: 2 entries
-        if (act) m.P[s] = (uint16_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
+        if (act) m.sc.P[s] = (uint16_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
         ptot += popc(b0) + 2 * popc(b1);
         const bool cr = act && (ain != bin);
         const uint64_t cm = ballot(cr);
         if (cr) {
             const int ci = K + below_lane(cm);
             if (ci < C::KM) {
-                m.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
-                m.cab[ci] = a | (b << 16);
+                m.sc.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
+                m.sc.cab[ci] = a | (b << 16);
             }
         }
         K += popc(cm);
     }
-    if (l == 0) m.P[P.S] = (uint16_t)ptot;
+    if (l == 0) m.sc.P[P.S] = (uint16_t)ptot;
     if (K > C::KM || ptot > 0xffff) return kOverflow;
     wave_sync();
 
@@ -382,14 +414,14 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const int f = base + l;
         const bool act = f < P.F;
         const int fs = act ? m.fstart[c][f] : 0, fe = act ? m.fstart[c][f + 1] : 0;
-        const int tot = act ? (int)m.P[fe] - (int)m.P[fs] : 0;
+        const int tot = act ? (int)m.sc.P[fe] - (int)m.sc.P[fs] : 0;
         const bool sv = act && tot >= 3;
         const uint64_t smk = ballot(sv);
         const int nf = Fn + below_lane(smk);
         const uint32_t t = sv ? (uint32_t)tot : 0u;
         const uint32_t inc = wave_incl_scan(t);
         const int off = Spre + (int)(inc - t);
-        if (act) { m.fnew[f] = (int16_t)(sv ? nf : -1); m.fnoff[f] = (uint16_t)off; }
+        if (act) { m.sc.fnew[f] = (int16_t)(sv ? nf : -1); m.sc.fnoff[f] = (uint16_t)off; }
         if (sv && nf < C::FM) { m.fstart[o][nf] = (uint16_t)off; m.fnb[o][nf] = m.fnb[c][f]; }
         big |= ballot(sv && tot > 255) != 0;
         Fn += popc(smk);
@@ -403,7 +435,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     for (int base = 0; base < K; base += 64) {
         const int ci = base + l;
         const bool act = ci < K;
-        const uint32_t key = act ? m.ckey[ci] : 0xffffffffu;
+        const uint32_t key = act ? m.sc.ckey[ci] : 0xffffffffu;
         int first = ci;
         if (K <= 64) {
             for (int cj = 0; cj < K; ++cj) {
@@ -413,7 +445,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         } else {
             const int lim = min(K, base + 64);
             for (int cj = 0; cj < lim; ++cj) {
-                const uint32_t kj = m.ckey[cj];
+                const uint32_t kj = m.sc.ckey[cj];
                 if (cj < first && kj == key) first = cj;
             }
         }
@@ -421,11 +453,11 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const uint64_t fm = ballot(isf);
         const int xid = nk + nint + below_lane(fm);
         nint += popc(fm);
-        if (act) m.cxid[ci] = (uint16_t)(isf ? xid : (0x8000 | first));
+        if (act) m.sc.cxid[ci] = (uint16_t)(isf ? xid : (0x8000 | first));
         if (isf && xid < C::VM) {
-            const uint32_t ab = m.cab[ci];
+            const uint32_t ab = m.sc.cab[ci];
             const uint32_t a = ab & 0xffffu, b = ab >> 16;
-            const double da = m.dist[a], db = m.dist[b];
+            const double da = m.sc.dist[a], db = m.sc.dist[b];
             double t = da / (da - db);
             if (t < 0.0) t = 0.0;   // f64::clamp(0, 1)
             if (t > 1.0) t = 1.0;
@@ -444,8 +476,8 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     if (nk + nint > C::VM) return kOverflow;
     wave_sync();
     for (int ci = l; ci < K; ci += 64) {
-        const uint16_t v = m.cxid[ci];
-        if (v & 0x8000) m.cxid[ci] = m.cxid[v & 0x7fff];
+        const uint16_t v = m.sc.cxid[ci];
+        if (v & 0x8000) m.sc.cxid[ci] = m.sc.cxid[v & 0x7fff];
     }
     wave_sync();
 
@@ -456,24 +488,24 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const bool act = s < P.S;
         const uint32_t e = act ? m.sedge[c][s] : 0u;
         const uint32_t a = e & 0xffffu, b = e >> 16;
-        const double da = m.dist[a], db = m.dist[b];
+        const double da = m.sc.dist[a], db = m.sc.dist[b];
         const bool ain = da <= kTol, bin = db <= kTol;
         const bool cr = act && (ain != bin);
         const uint64_t cm = ballot(cr);
         const int ci = Kb + below_lane(cm);
         Kb += popc(cm);
         const int f = act ? m.sface[c][s] : 0;
-        const int nf = act ? m.fnew[f] : -1;
+        const int nf = act ? m.sc.fnew[f] : -1;
         if (act && nf >= 0 && (ain || bin)) {
-            const int pos = (int)m.fnoff[f] + ((int)m.P[s] - (int)m.P[m.fstart[c][f]]);
+            const int pos = (int)m.sc.fnoff[f] + ((int)m.sc.P[s] - (int)m.sc.P[m.fstart[c][f]]);
             if (ain && bin) {
-                m.fidxn[pos] = m.remap[b];
+                m.sc.fidxn[pos] = m.sc.remap[b];
                 m.sface[o][pos] = (uint16_t)nf;
             } else {
-                m.fidxn[pos] = m.cxid[ci];
+                m.sc.fidxn[pos] = m.sc.cxid[ci];
                 m.sface[o][pos] = (uint16_t)nf;
                 if (!ain) {
-                    m.fidxn[pos + 1] = m.remap[b];
+                    m.sc.fidxn[pos + 1] = m.sc.remap[b];
                     m.sface[o][pos + 1] = (uint16_t)nf;
                 }
             }
@@ -490,33 +522,33 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         const int fp = act ? m.sface[o][p] : 0;
         const int fs = m.fstart[o][fp], fe = m.fstart[o][fp + 1];
         const int nx = (p + 1 == fe) ? fs : p + 1;
-        const uint32_t ua = act ? m.fidxn[p] : 0u, vb = act ? m.fidxn[nx] : 0u;
+        const uint32_t ua = act ? m.sc.fidxn[p] : 0u, vb = act ? m.sc.fidxn[nx] : 0u;
         if (act) m.sedge[o][p] = ua | (vb << 16);
         const bool sg = act && (int)ua >= nk && (int)vb >= nk;
         const uint64_t gm = ballot(sg);
-        if (sg) m.seg[M + below_lane(gm)] = vb | (ua << 16);
+        if (sg) m.sc.seg[M + below_lane(gm)] = vb | (ua << 16);
         M += popc(gm);
     }
 
     // 7. lid polygon (:471-495): map[first] = second (later segments overwrite), walk from seg[0]
     if (M > 0) {
-        for (int t = l; t < nint; t += 64) m.lmap[t] = 0u;
+        for (int t = l; t < nint; t += 64) m.sc.lmap[t] = 0u;
         wave_sync();
         for (int t = l; t < M; t += 64) {
-            const uint32_t sg = m.seg[t];
-            atomicMax(&m.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
+            const uint32_t sg = m.sc.seg[t];
+            atomicMax(&m.sc.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
         }
         wave_sync();
-        const uint32_t s0 = m.seg[0];
+        const uint32_t s0 = m.sc.seg[0];
         const int start = (int)(s0 & 0xffffu);
         int cv = (int)(s0 >> 16);
-        if (l == 0) m.lid[0] = (uint16_t)start;
+        if (l == 0) m.sc.lid[0] = (uint16_t)start;
         int len = 1;
         while (cv != start && len <= M) {
             if (len >= C::SM) return kOverflow;
-            if (l == 0) m.lid[len] = (uint16_t)cv;
+            if (l == 0) m.sc.lid[len] = (uint16_t)cv;
             ++len;
-            const uint32_t mv = m.lmap[cv - nk];
+            const uint32_t mv = m.sc.lmap[cv - nk];
             cv = mv ? (int)(mv & 0xffffu) : 0xffff;
             if (cv == 0xffff) break;
         }
@@ -524,7 +556,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
             if (Spre + len > C::SM || len > 255) return len > 255 ? kFace255 : kOverflow;
             wave_sync();
             for (int t = l; t < len; t += 64) {
-                const uint32_t u = m.lid[t], w = m.lid[(t + 1) % len];
+                const uint32_t u = m.sc.lid[t], w = m.sc.lid[(t + 1) % len];
                 m.sedge[o][Spre + t] = u | (w << 16);
                 m.sface[o][Spre + t] = (uint16_t)Fn;
             }
@@ -598,7 +630,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     double md2 = 0.0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        if (va[k]) m.dist[64 * k + l] = d[k];
+        if (va[k]) m.sc.dist[64 * k + l] = d[k];
         if (withgen && ((inm.w[k] >> l) & 1ull)) {
             const double dx = r.x[k] - gx, dy = r.y[k] - gy, dz = r.z[k] - gz;
             md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
@@ -627,27 +659,27 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             const uint64_t b1 = ballot(sa && !ain && bin);
             const uint32_t pp = (uint32_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
             ptot += popc(b0) + 2 * popc(b1);
-            if (sa) m.P[s] = (uint16_t)pp;
+            if (sa) m.sc.P[s] = (uint16_t)pp;
             const bool cr = sa && (ain != bin);
             const uint64_t cm = ballot(cr);
             const int ci = K + below_lane(cm);
             eq[ch] = pp | ((uint32_t)(ci & 0xffff) << 16);
             if (cr && ci < 64) {
-                m.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
-                m.cab[ci] = a | (b << 16);
+                m.sc.ckey[ci] = a < b ? ((a << 16) | b) : ((b << 16) | a);
+                m.sc.cab[ci] = a | (b << 16);
             }
             K += popc(cm);
         }
     }
     if (K > 64) return kGeneric;
-    if (l == 0) m.P[P.S] = (uint16_t)ptot;
+    if (l == 0) m.sc.P[P.S] = (uint16_t)ptot;
     wave_sync();
 
     // 3. faces: survivors (>= 3 entries, :454-456) and their new offsets
     const bool fa = l < P.F;
     const int fs = fa ? m.fstart[c][l] : 0, fe = fa ? m.fstart[c][l + 1] : 0;
-    const int pfs = fa ? (int)m.P[fs] : 0;
-    const int tot = fa ? (int)m.P[fe] - pfs : 0;
+    const int pfs = fa ? (int)m.sc.P[fs] : 0;
+    const int tot = fa ? (int)m.sc.P[fe] - pfs : 0;
     const bool sv = fa && tot >= 3;
     const uint64_t svm = ballot(sv);
     int Fn = popc(svm);
@@ -659,10 +691,10 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     if (Fn + 1 > C::FM || Spre > C::SM) return kOverflow;
     if (fa) {
         const int nf = below_lane(svm);
-        m.fnew[l] = (int16_t)(sv ? nf : -1);
-        m.fbase[l] = off - pfs;
-        m.fnoff[l] = (uint16_t)off;
-        m.nend[l] = (uint16_t)(off + tot);
+        m.sc.fnew[l] = (int16_t)(sv ? nf : -1);
+        m.sc.fbase[l] = off - pfs;
+        m.sc.fnoff[l] = (uint16_t)off;
+        m.sc.nend[l] = (uint16_t)(off + tot);
         if (sv) {
             m.fstart[o][nf] = (uint16_t)off;
             m.fnb[o][nf] = m.fnb[c][l];
@@ -673,7 +705,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     // 4. crossings: the first occurrence of an edge key in slot order creates the vertex
     //    (:391-417); new vertices are numbered in that order after the kept ones
     const bool ka = l < K;
-    const uint32_t key = ka ? m.ckey[l] : 0xffffffffu;
+    const uint32_t key = ka ? m.sc.ckey[l] : 0xffffffffu;
     int first = l;
     for (int cj = 0; cj < K; ++cj) {
         const uint32_t kj = rlu(key, cj);
@@ -687,9 +719,9 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     if (nk + nint > C::VM) return kOverflow;
     double x = 0.0, y = 0.0, z = 0.0;
     if (isf) {
-        const uint32_t ab = m.cab[l];
+        const uint32_t ab = m.sc.cab[l];
         const uint32_t a = ab & 0xffffu, b = ab >> 16;
-        const double da = m.dist[a], db = m.dist[b];
+        const double da = m.sc.dist[a], db = m.sc.dist[b];
         double t = da / (da - db);
         if (t < 0.0) t = 0.0;   // f64::clamp(0, 1)
         if (t > 1.0) t = 1.0;
@@ -727,18 +759,18 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             const int xv = (int)lane_get((uint32_t)xid, (int)(eq[ch] >> 16) & 63);
             if (s < P.S && (ain || bin)) {
                 const int f = (int)(ea[ch] >> (2 * SH));
-                const int nf = m.fnew[f];
+                const int nf = m.sc.fnew[f];
                 if (nf >= 0) {
-                    const int pos = m.fbase[f] + (int)(eq[ch] & 0xffffu);
-                    const int ns = m.fnoff[f], ne = m.nend[f];
+                    const int pos = m.sc.fbase[f] + (int)(eq[ch] & 0xffffu);
+                    const int ns = m.sc.fnoff[f], ne = m.sc.nend[f];
                     const uint16_t rb = (uint16_t)inm.below(b);
-                    m.fidxn[pos] = (ain && bin) ? rb : (uint16_t)xv;
+                    m.sc.fidxn[pos] = (ain && bin) ? rb : (uint16_t)xv;
                     m.sface[o][pos] = (uint16_t)nf;
-                    m.nextp[pos] = (uint16_t)(pos + 1 == ne ? ns : pos + 1);
+                    m.sc.nextp[pos] = (uint16_t)(pos + 1 == ne ? ns : pos + 1);
                     if (!ain) {
-                        m.fidxn[pos + 1] = rb;
+                        m.sc.fidxn[pos + 1] = rb;
                         m.sface[o][pos + 1] = (uint16_t)nf;
-                        m.nextp[pos + 1] = (uint16_t)(pos + 2 == ne ? ns : pos + 2);
+                        m.sc.nextp[pos + 1] = (uint16_t)(pos + 2 == ne ? ns : pos + 2);
                     }
                 }
             }
@@ -751,28 +783,28 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     for (int base = 0; base < Spre; base += 64) {
         const int p = base + l;
         const bool pa = p < Spre;
-        const uint32_t ua = pa ? m.fidxn[p] : 0u;
-        const int np = pa ? m.nextp[p] : 0;
-        const uint32_t vb = pa ? m.fidxn[np] : 0u;
+        const uint32_t ua = pa ? m.sc.fidxn[p] : 0u;
+        const int np = pa ? m.sc.nextp[p] : 0;
+        const uint32_t vb = pa ? m.sc.fidxn[np] : 0u;
         if (pa) m.sedge[o][p] = ua | (vb << 16);
         const bool sg = pa && (int)ua >= nk && (int)vb >= nk;
         const uint64_t gm = ballot(sg);
-        if (sg && M + below_lane(gm) < C::KM) m.seg[M + below_lane(gm)] = vb | (ua << 16);
+        if (sg && M + below_lane(gm) < C::KM) m.sc.seg[M + below_lane(gm)] = vb | (ua << 16);
         M += popc(gm);
     }
     if (M > C::KM) return kGeneric;   // a convex face has one such segment: at most K / 2
 
     // 7. lid polygon (:471-495); lane j holds the map entry of new vertex nk + j
     if (M > 0) {
-        if (l < nint) m.lmap[l] = 0u;
+        if (l < nint) m.sc.lmap[l] = 0u;
         wave_sync();
         for (int t = l; t < M; t += 64) {
-            const uint32_t sg = m.seg[t];
-            atomicMax(&m.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
+            const uint32_t sg = m.sc.seg[t];
+            atomicMax(&m.sc.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
         }
         wave_sync();
-        const uint32_t mv = (l < nint) ? m.lmap[l] : 0u;
-        const uint32_t s0 = m.seg[0];
+        const uint32_t mv = (l < nint) ? m.sc.lmap[l] : 0u;
+        const uint32_t s0 = m.sc.seg[0];
         const int start = (int)(s0 & 0xffffu);
         int cv = (int)(s0 >> 16);
         uint32_t lidv = (uint32_t)start;
@@ -1386,7 +1418,7 @@ void k_cells(CellArgs A) {
 #define VD_TIER0_WAVES 5
 #endif
 using Tier0 = Cap<64, 192, 48, 64, true, 107, VD_TIER0_WAVES>;
-using Tier1 = Cap<128, 384, 64, 64, true, 127, 2>;
+using Tier1 = Cap<128, 384, 64, 64, true, 111, 4>;
 using Tier2 = Cap<256, 768, 128, 384, false, 255, 1>;
 using Tier3 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
 
