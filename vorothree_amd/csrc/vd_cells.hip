@@ -638,7 +638,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     }
 
     // 2. slots: output entry counts (prefix in slot order) and plane crossings (:378-452).
-    //    Per chunk and lane two registers: ea = a | b << SH | face << 2 SH and
+    //    Per chunk and lane two registers: ea = a | b << SH | face << 2 SH | a kept << 30 | b kept << 31 and
     //    eq = output prefix | crossing index << 16.
     const int nch = (P.S + 63) >> 6;
     uint32_t ea[NCH];
@@ -653,8 +653,9 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             const bool sa = s < P.S;
             const uint32_t e = sa ? m.sedge[c][s] : 0u;
             const uint32_t a = e & 0xffffu, b = e >> 16;
-            ea[ch] = a | (b << SH) | ((sa ? (uint32_t)m.sface[c][s] : 0u) << (2 * SH));
             const bool ain = inm.test(a), bin = inm.test(b);
+            ea[ch] = a | (b << SH) | ((sa ? (uint32_t)m.sface[c][s] : 0u) << (2 * SH)) | ((uint32_t)ain << 30) |
+                     ((uint32_t)bin << 31);
             const uint64_t b0 = ballot(sa && ain);
             const uint64_t b1 = ballot(sa && !ain && bin);
             const uint32_t pp = (uint32_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
@@ -754,11 +755,11 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     for (int ch = 0; ch < NCH; ++ch) {
         if (ch < nch) {
             const int s = 64 * ch + l;
-            const uint32_t a = ea[ch] & VMSK, b = (ea[ch] >> SH) & VMSK;
-            const bool ain = inm.test(a), bin = inm.test(b);
+            const uint32_t b = (ea[ch] >> SH) & VMSK;
+            const bool ain = (ea[ch] >> 30) & 1u, bin = ea[ch] >> 31;
             const int xv = (int)lane_get((uint32_t)xid, (int)(eq[ch] >> 16) & 63);
             if (s < P.S && (ain || bin)) {
-                const int f = (int)(ea[ch] >> (2 * SH));
+                const int f = (int)((ea[ch] >> (2 * SH)) & ((1u << (30 - 2 * SH)) - 1u));
                 const int nf = m.sc.fnew[f];
                 if (nf >= 0) {
                     const int pos = m.sc.fbase[f] + (int)(eq[ch] & 0xffffu);
