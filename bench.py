@@ -127,7 +127,8 @@ def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_ra
     out = {"workload": f"first {count} requests of a Poisson stream at {rate:g} req/s: 1e3..2e6 "
                        "generators (log-uniform) in the unit cube, 50% uniform / 50% 8-cluster "
                        "sigma 0.05, 30% Lloyd jobs of 10-50 iterations; grid per job by "
-                       f"vd_grid_for_points at its default target (0.35 per bin); HBM budget {budget_gb:g} GiB, 64 slots; "
+                       "vd_grid_for_points at its default target (0.35 per bin); "
+                       f"HBM budget {budget_gb:g} GiB, 64 slots; "
                        "jobs of up to 65,536 points in the small-job lane, larger ones stepped on a worker",
            "lloyd_jobs": int(sum(q.iterations > 1 for q in reqs[:count])),
            "note": "the offered load is far above one GPU's capacity at 2000 req/s, so its latency "

@@ -175,7 +175,8 @@ void grid_for(const double lo[3], const double hi[3], size_t n, double per_bin, 
     double e[3], vol = 1.0;
     for (int a = 0; a < 3; ++a) { e[a] = hi[a] - lo[a]; vol *= e[a]; }
     const double h = std::cbrt(vol * per_bin / (double)std::max<size_t>(n, 1));
-    for (int a = 0; a < 3; ++a) out[a] = std::min<int64_t>(16384, std::max<int64_t>(1, (int64_t)std::llround(e[a] / h)));
+    for (int a = 0; a < 3; ++a)
+        out[a] = std::min<int64_t>(16384, std::max<int64_t>(1, (int64_t)std::llround(e[a] / h)));
 }
 
 // Grid for uneven input. The density seen by the average generator is the mean over generators of
@@ -921,7 +922,8 @@ int engine_calculate(Engine &E) {
                 VD_HIP(hipStreamWaitEvent(E.stream2, E.ev2, 0));
                 for (int tier = 0; tier < NT; ++tier) {
                     if (!nlist[tier]) continue;
-                    if (nsh[tier]) VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier, E.stream2));
+                    if (nsh[tier])
+                        VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier, E.stream2));
                     if (nlist[tier] > nsh[tier])
                         VD_TRY(run_cells(E, tier, list[tier] + nsh[tier], nlist[tier] - nsh[tier], nullptr, false,
                                          1 + NT + tier));
