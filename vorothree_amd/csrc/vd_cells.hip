@@ -1087,18 +1087,18 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
     for (int wi = 0; wi < nwalls; ++wi) {
         const WallDev &W = D.walls[wi];
         for (uint32_t wp = 0;; ++wp) {
-            double pp[3], pn[3];
+            WallPlane q{};
             bool has;
             if (W.kind == kWallTube) {
                 const double *tp = A.tube_planes + 7 * ((size_t)i * A.ntube + tk);
                 has = wp == 0 && tp[6] != 0.0;
-                for (int k = 0; k < 3; ++k) { pp[k] = tp[k]; pn[k] = tp[3 + k]; }
+                q = WallPlane{tp[0], tp[1], tp[2], tp[3], tp[4], tp[5]};
             } else {
-                has = wall_plane(W, wp, gx, gy, gz, pp, pn);
+                has = wall_plane(W, wp, gx, gy, gz, q);
             }
             if (!has) break;
             double wr2 = 0.0;
-            const int rc = clip_any<C>(m, P, r, Plane{pp[0], pp[1], pp[2], pn[0], pn[1], pn[2]}, W.id, false,
+            const int rc = clip_any<C>(m, P, r, Plane{q.px, q.py, q.pz, q.nx, q.ny, q.nz}, W.id, false,
                                        gx, gy, gz, wr2);
             if (rc < 0) {
                 status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
@@ -1299,26 +1299,26 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
                     const bool tube = W.kind == kWallTube;
                     const bool use = W.kind != kWallPlane && W.kind != kWallPolyhedron && ((mi >> w) & 1u) &&
                                      ((mj >> w) & 1u);
-                    double pp[3], pn[3];
+                    WallPlane q{};
                     bool has = false;
                     if (use) {
                         if (tube) {
                             const double *tp = A.tube_planes + 7 * (gj * A.ntube + tk);
                             has = tp[6] != 0.0;
-                            for (int k = 0; k < 3; ++k) { pp[k] = tp[k]; pn[k] = tp[3 + k]; }
+                            q = WallPlane{tp[0], tp[1], tp[2], tp[3], tp[4], tp[5]};
                         } else {
-                            has = wall_plane(W, 0u, nx, ny, nz, pp, pn);
+                            has = wall_plane(W, 0u, nx, ny, nz, q);
                         }
                     }
                     if (tube) ++tk;
                     if (!has) continue;
                     double dot = 0.0;
-                    dot += (gx - pp[0]) * pn[0];
-                    dot += (gy - pp[1]) * pn[1];
-                    dot += (gz - pp[2]) * pn[2];
+                    dot += (gx - q.px) * q.nx;
+                    dot += (gy - q.py) * q.ny;
+                    dot += (gz - q.pz) * q.nz;
                     if (!(dot <= 1e-9)) continue;
                     double wr2 = 0.0;
-                    const int rc = clip_any<C>(m, P, r, Plane{pp[0], pp[1], pp[2], pn[0], pn[1], pn[2]}, W.id,
+                    const int rc = clip_any<C>(m, P, r, Plane{q.px, q.py, q.pz, q.nx, q.ny, q.nz}, W.id,
                                                false, gx, gy, gz, wr2);
                     if (rc < 0) {
                         status = (rc == kFace255) ? kCellFace255 : kCellCapacity;

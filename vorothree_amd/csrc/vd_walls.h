@@ -109,99 +109,106 @@ __device__ inline bool wall_contains(const WallDev &w, double x, double y, doubl
     }
 }
 
+// A cut plane: a point on it and the normal pointing out of the valid region. Named fields rather
+// than arrays keep it in registers where it is filled branch by branch.
+struct WallPlane { double px, py, pz, nx, ny, nz; };
+
 // Plane j of WallGeometry::cut for generator g (point p, normal n pointing out of the valid
 // region); false when the wall emits no plane j. Every kind but the polyhedron emits at most one
 // plane; the tube's plane is built by the caller from its closest point (tube_plane).
 // wall_3d.rs:38-42, 80-100, 145-170, 216-258, 311-363, 703-707.
-__device__ inline bool wall_plane(const WallDev &w, uint32_t j, double gx, double gy, double gz, double p[3],
-                                  double n[3]) {
+__device__ inline bool wall_plane(const WallDev &w, uint32_t j, double gx, double gy, double gz,
+                                  WallPlane &out) {
+    double qpx, qpy, qpz, qnx, qny, qnz;   // set case by case, written to out once
     if (w.kind == kWallPolyhedron) {
         if (j >= w.npts) return false;
         const double *q = w.pts + 6 * (size_t)j;
-        p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
-        n[0] = q[3]; n[1] = q[4]; n[2] = q[5];
-        return true;
-    }
-    if (j > 0) return false;
-    const double dx = gx - w.a[0], dy = gy - w.a[1], dz = gz - w.a[2];
-    switch (w.kind) {
-    case kWallPlane:
-        p[0] = w.a[0]; p[1] = w.a[1]; p[2] = w.a[2];
-        n[0] = -w.b[0]; n[1] = -w.b[1]; n[2] = -w.b[2];
-        return true;
-    case kWallSphere: {
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-        if (dist == 0.0) return false;
-        const double sc = w.r / dist;
-        p[0] = w.a[0] + dx * sc; p[1] = w.a[1] + dy * sc; p[2] = w.a[2] + dz * sc;
-        n[0] = dx / dist; n[1] = dy / dist; n[2] = dz / dist;
-        return true;
-    }
-    case kWallCylinder: {
-        const double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
-        const double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
-        const double dist = sqrt(px * px + py * py + pz * pz);
-        if (dist == 0.0) return false;
-        const double sc = w.r / dist;
-        p[0] = w.a[0] + dot * w.b[0] + px * sc;
-        p[1] = w.a[1] + dot * w.b[1] + py * sc;
-        p[2] = w.a[2] + dot * w.b[2] + pz * sc;
-        n[0] = px / dist; n[1] = py / dist; n[2] = pz / dist;
-        return true;
-    }
-    case kWallCone: {
-        const double h = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
-        const double px = dx - h * w.b[0], py = dy - h * w.b[1], pz = dz - h * w.b[2];
-        const double r = sqrt(px * px + py * py + pz * pz);
-        if (r == 0.0) return false;
-        const double rx = px / r, ry = py / r, rz = pz / r;
-        const double dist = r * w.cs - h * w.sn;
-        const double p2r = r - dist * w.cs;
-        const double p2h = h + dist * w.sn;
-        if (p2h < 0.0) {
-            const double dt = sqrt(dx * dx + dy * dy + dz * dz);
-            if (dt == 0.0) return false;
-            p[0] = w.a[0]; p[1] = w.a[1]; p[2] = w.a[2];
-            n[0] = dx / dt; n[1] = dy / dt; n[2] = dz / dt;
-            return true;
+        qpx = q[0]; qpy = q[1]; qpz = q[2];
+        qnx = q[3]; qny = q[4]; qnz = q[5];
+    } else {
+        if (j > 0) return false;
+        const double dx = gx - w.a[0], dy = gy - w.a[1], dz = gz - w.a[2];
+        switch (w.kind) {
+        case kWallPlane:
+            qpx = w.a[0]; qpy = w.a[1]; qpz = w.a[2];
+            qnx = -w.b[0]; qny = -w.b[1]; qnz = -w.b[2];
+            break;
+        case kWallSphere: {
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist == 0.0) return false;
+            const double sc = w.r / dist;
+            qpx = w.a[0] + dx * sc; qpy = w.a[1] + dy * sc; qpz = w.a[2] + dz * sc;
+            qnx = dx / dist; qny = dy / dist; qnz = dz / dist;
+            break;
         }
-        p[0] = w.a[0] + p2h * w.b[0] + p2r * rx;
-        p[1] = w.a[1] + p2h * w.b[1] + p2r * ry;
-        p[2] = w.a[2] + p2h * w.b[2] + p2r * rz;
-        n[0] = w.cs * rx - w.sn * w.b[0];
-        n[1] = w.cs * ry - w.sn * w.b[1];
-        n[2] = w.cs * rz - w.sn * w.b[2];
-        return true;
-    }
-    case kWallTorus: {
-        const double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
-        const double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
-        const double dist_perp = sqrt(px * px + py * py + pz * pz);
-        double ux, uy, uz;
-        if (dist_perp < 1e-9) {
-            // on the axis: any direction perpendicular to it
-            double tx = 1.0, ty = 0.0;
-            const double tz = 0.0;
-            if (fabs(w.b[0]) > 0.9) { tx = 0.0; ty = 1.0; }
-            const double td = tx * w.b[0] + ty * w.b[1] + tz * w.b[2];
-            const double ax = tx - td * w.b[0], ay = ty - td * w.b[1], az = tz - td * w.b[2];
-            const double len = sqrt(ax * ax + ay * ay + az * az);
-            if (len == 0.0) return false;
-            ux = ax / len; uy = ay / len; uz = az / len;
-        } else {
-            ux = px / dist_perp; uy = py / dist_perp; uz = pz / dist_perp;
+        case kWallCylinder: {
+            const double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+            const double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
+            const double dist = sqrt(px * px + py * py + pz * pz);
+            if (dist == 0.0) return false;
+            const double sc = w.r / dist;
+            qpx = w.a[0] + dot * w.b[0] + px * sc;
+            qpy = w.a[1] + dot * w.b[1] + py * sc;
+            qpz = w.a[2] + dot * w.b[2] + pz * sc;
+            qnx = px / dist; qny = py / dist; qnz = pz / dist;
+            break;
         }
-        const double cx = w.a[0] + ux * w.r, cy = w.a[1] + uy * w.r, cz = w.a[2] + uz * w.r;
-        const double vx = gx - cx, vy = gy - cy, vz = gz - cz;
-        const double dc = sqrt(vx * vx + vy * vy + vz * vz);
-        if (dc == 0.0) return false;
-        n[0] = vx / dc; n[1] = vy / dc; n[2] = vz / dc;
-        p[0] = cx + n[0] * w.r2; p[1] = cy + n[1] * w.r2; p[2] = cz + n[2] * w.r2;
-        return true;
+        case kWallCone: {
+            const double h = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+            const double px = dx - h * w.b[0], py = dy - h * w.b[1], pz = dz - h * w.b[2];
+            const double r = sqrt(px * px + py * py + pz * pz);
+            if (r == 0.0) return false;
+            const double rx = px / r, ry = py / r, rz = pz / r;
+            const double dist = r * w.cs - h * w.sn;
+            const double p2r = r - dist * w.cs;
+            const double p2h = h + dist * w.sn;
+            if (p2h < 0.0) {
+                const double dt = sqrt(dx * dx + dy * dy + dz * dz);
+                if (dt == 0.0) return false;
+                qpx = w.a[0]; qpy = w.a[1]; qpz = w.a[2];
+                qnx = dx / dt; qny = dy / dt; qnz = dz / dt;
+                break;
+            }
+            qpx = w.a[0] + p2h * w.b[0] + p2r * rx;
+            qpy = w.a[1] + p2h * w.b[1] + p2r * ry;
+            qpz = w.a[2] + p2h * w.b[2] + p2r * rz;
+            qnx = w.cs * rx - w.sn * w.b[0];
+            qny = w.cs * ry - w.sn * w.b[1];
+            qnz = w.cs * rz - w.sn * w.b[2];
+            break;
+        }
+        case kWallTorus: {
+            const double dot = dx * w.b[0] + dy * w.b[1] + dz * w.b[2];
+            const double px = dx - dot * w.b[0], py = dy - dot * w.b[1], pz = dz - dot * w.b[2];
+            const double dist_perp = sqrt(px * px + py * py + pz * pz);
+            double ux, uy, uz;
+            if (dist_perp < 1e-9) {
+                // on the axis: any direction perpendicular to it
+                double tx = 1.0, ty = 0.0;
+                const double tz = 0.0;
+                if (fabs(w.b[0]) > 0.9) { tx = 0.0; ty = 1.0; }
+                const double td = tx * w.b[0] + ty * w.b[1] + tz * w.b[2];
+                const double ax = tx - td * w.b[0], ay = ty - td * w.b[1], az = tz - td * w.b[2];
+                const double len = sqrt(ax * ax + ay * ay + az * az);
+                if (len == 0.0) return false;
+                ux = ax / len; uy = ay / len; uz = az / len;
+            } else {
+                ux = px / dist_perp; uy = py / dist_perp; uz = pz / dist_perp;
+            }
+            const double cx = w.a[0] + ux * w.r, cy = w.a[1] + uy * w.r, cz = w.a[2] + uz * w.r;
+            const double vx = gx - cx, vy = gy - cy, vz = gz - cz;
+            const double dc = sqrt(vx * vx + vy * vy + vz * vz);
+            if (dc == 0.0) return false;
+            qnx = vx / dc; qny = vy / dc; qnz = vz / dc;
+            qpx = cx + qnx * w.r2; qpy = cy + qny * w.r2; qpz = cz + qnz * w.r2;
+            break;
+        }
+        default:
+            return false;
+        }
     }
-    default:
-        return false;
-    }
+    out = WallPlane{qpx, qpy, qpz, qnx, qny, qnz};
+    return true;
 }
 
 // The tube's cut plane from the closest point c of generator g (wall_3d.rs:448-456, 807-815,
