@@ -276,7 +276,9 @@ def batched_vs_plain(count: int, steps: int, tune=None) -> dict:
 
     # each tessellation of the workload is computed once, so the plain path launches eagerly; the
     # rerun of the same tessellations every step would let them replay graphs, reported apart
+    progress(f"batched: {count} tessellations in one batch done; plain, one engine each")
     tp, cells_p = plain(False)
+    progress("batched: plain done; plain with graph replay")
     tpg, _ = plain(True)
     dev.close()
     bins = sum(int(np.prod(g)) for g in grids)
@@ -305,6 +307,10 @@ def copy_probe(mib: int) -> None:
     rc = _hip.hip().hipMemcpy(dst.ptr, src.ptr, n * 8, 3)
     _hip.synchronize()
     print(json.dumps({"copy_probe_bytes": n * 8, "rc": rc}), flush=True)
+
+
+def progress(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -367,6 +373,7 @@ def main():
     assert np.array_equal(tg.volumes().view(np.uint64), t.volumes().view(np.uint64))
     tg.close()
 
+    progress(f"plain: {value:.4g} cells/s, graph replay {graph_rate:.4g} cells/s")
     # algorithmic work of one cell launch, counted by the oracle over the same cells
     cpu = None
     flops_per_cell = None
@@ -440,10 +447,15 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if cpu is not None:
+        progress(f"cpu baseline: {cpu['value']:.4g} cells/s on {a.cpu_threads} threads")
     if not a.no_batched:
+        progress("batched: grids of about 2 generators per bin")
         out["batched"] = batched_vs_plain(a.batch_count, a.batch_steps)
+        progress("batched: autotuned grids")
         out["batched"]["autotuned"] = batched_vs_plain(a.batch_count, a.batch_steps, BATCH_TUNE)
     if not a.no_clustered:
+        progress(f"clustered: {a.clustered_n} points")
         out["clustered"] = clustered(a.clustered_n, [0.25, 0.35, 0.7], 3)
     if not a.no_serve:
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]

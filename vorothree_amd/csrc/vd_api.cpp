@@ -17,6 +17,7 @@
 #include <limits>
 #include <atomic>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -372,8 +373,8 @@ struct Engine {
     int device = 0, cus = 1;
     int occ[kTiers] = {1, 1, 1, 1};
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;     // retry launches of cells crossing empty space, beside the rest
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    hipStream_t stream2 = nullptr;     // side stream (shared, see side_stream) for far-cell retries
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int keep_cells = 1;
 
     DevBuf d_desc, d_offs, d_raw, d_raw_tess, d_flags, d_pos, d_scan, d_gat;
@@ -413,9 +414,6 @@ int engine_init(Engine &E) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&E.ev0);
     if (e == hipSuccess) e = hipEventCreate(&E.ev1);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&E.stream2, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&E.ev2, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&E.ev3, hipEventDisableTiming);
     for (int k = 0; k < kTiers && e == hipSuccess; ++k) e = vd::cells_occupancy(k, &E.occ[k]);
     if (e == hipSuccess) e = hipHostMalloc((void **)&E.h_counts, 64 * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) return fail(VD_EDEVICE, std::string("device setup: ") + hipGetErrorString(e));
@@ -446,15 +444,38 @@ void engine_free(Engine &E) {
     for (auto &kv : E.tables) release(kv.second.buf);
     E.tables.clear();
     E.shapes.clear();
-    if (E.stream2) (void)hipStreamSynchronize(E.stream2);
     if (E.ev0) (void)hipEventDestroy(E.ev0);
     if (E.ev1) (void)hipEventDestroy(E.ev1);
-    if (E.ev2) (void)hipEventDestroy(E.ev2);
-    if (E.ev3) (void)hipEventDestroy(E.ev3);
     if (E.stream) (void)hipStreamDestroy(E.stream);
-    if (E.stream2) (void)hipStreamDestroy(E.stream2);
-    E.ev0 = E.ev1 = E.ev2 = E.ev3 = nullptr;
+    E.ev0 = E.ev1 = nullptr;
     E.stream = E.stream2 = nullptr;
+}
+
+// Side streams for the retry launches of cells crossing empty space: kSideStreams per device for
+// the whole process, handed to engines in turn at their first such retry and never destroyed. Each
+// engine owning a second stream would double the streams of a caller that holds thousands of
+// engines (one per small tessellation), and every stream holds runtime queues and memory. Engines
+// that share a side stream stay correct: each orders its own launches on it with its own events;
+// only their far-cell retries queue behind each other.
+constexpr int kSideStreams = 4;
+std::mutex g_side_mu;
+std::map<int, std::vector<hipStream_t>> g_side;
+unsigned g_side_next = 0;
+
+int engine_stream2(Engine &E) {
+    if (E.stream2) return VD_OK;
+    std::lock_guard<std::mutex> lock(g_side_mu);
+    std::vector<hipStream_t> &pool = g_side[E.device];
+    if (pool.size() < (size_t)kSideStreams) {
+        hipStream_t st = nullptr;
+        const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail(VD_EDEVICE, std::string("side stream: ") + hipGetErrorString(e));
+        pool.push_back(st);
+        E.stream2 = st;
+    } else {
+        E.stream2 = pool[g_side_next++ % pool.size()];
+    }
+    return VD_OK;
 }
 
 int scan(Engine &E, const uint32_t *in, uint32_t *out, uint32_t n) {
@@ -711,9 +732,10 @@ int enqueue_binning(Engine &E) {
 }
 
 // One cell launch over d_list on stream st (the engine's stream by default), drawing cells from work
-// counter `slot` of d_queue (launches that may run at the same time use different counters).
+// counter `slot` of d_queue (launches that may run at the same time use different counters), with
+// at most max_waves waves (0: the engine's CUs times the tier's occupancy).
 int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms, bool probe = false,
-              int slot = 0, hipStream_t st = nullptr) {
+              int slot = 0, hipStream_t st = nullptr, int max_waves = 0) {
     if (!st) st = E.stream;
     vd::CellArgs a{};
     a.desc = E.d_desc.as<vd::TessDesc>();
@@ -746,6 +768,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.cell_off = E.d_cell_off.as<uint64_t>();
     VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>() + slot, 0u, 1, st));
     int waves = std::max(1, E.cus - E.reserve_cus) * E.occ[tier];
+    if (max_waves > 0 && waves > max_waves) waves = max_waves;
     if ((uint32_t)waves > nlist) waves = (int)nlist;
     // arena chunks per wave: up to kArenaChunk[k] entries, at most an eighth of the capacity over
     // all waves (the unused tails of the last chunks are holes in the arena)
@@ -916,21 +939,38 @@ int engine_calculate(Engine &E) {
                 oshell[0][0] = E.d_out1.as<uint32_t>();
             } else {
                 if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 12, E.stream));
-                // group 1 (cells crossing empty space: few, long walks) runs on the second stream,
-                // so the rest of the round fills the CUs its long-running waves leave idle
-                VD_HIP(hipEventRecord(E.ev2, E.stream));
-                VD_HIP(hipStreamWaitEvent(E.stream2, E.ev2, 0));
+                // group 1 (cells crossing empty space: few, long walks) runs on a side stream, so
+                // the rest of the round fills the CUs its long-running waves leave idle; ev0 and ev1
+                // (timing events of the first round's launch, free in later rounds) order it
+                bool side = false;
+                for (int tier = 0; tier < NT; ++tier) side = side || nsh[tier] > 0;
+                if (side) {
+                    VD_TRY(engine_stream2(E));
+                    VD_HIP(hipEventRecord(E.ev0, E.stream));
+                    VD_HIP(hipStreamWaitEvent(E.stream2, E.ev0, 0));
+                }
+                // with CUs held back for another lane, the two streams' launches split the CUs this
+                // engine may fill: group 1 takes up to half, as many as its cells need
+                int c1 = 0;
+                const int avail = std::max(2, E.cus - E.reserve_cus);
+                if (side && E.reserve_cus > 0)
+                    for (int tier = 0; tier < NT; ++tier)
+                        c1 = std::max(c1, (int)std::min<uint32_t>(avail / 2,
+                                                                  (nsh[tier] + E.occ[tier] - 1) / E.occ[tier]));
                 for (int tier = 0; tier < NT; ++tier) {
                     if (!nlist[tier]) continue;
                     if (nsh[tier])
-                        VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier, E.stream2));
+                        VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier, E.stream2,
+                                         c1 * E.occ[tier]));
                     if (nlist[tier] > nsh[tier])
                         VD_TRY(run_cells(E, tier, list[tier] + nsh[tier], nlist[tier] - nsh[tier], nullptr, false,
-                                         1 + NT + tier));
+                                         1 + NT + tier, nullptr, c1 ? (avail - c1) * E.occ[tier] : 0));
                     if (tier > 0) E.stats[1] += nlist[tier];
                 }
-                VD_HIP(hipEventRecord(E.ev3, E.stream2));
-                VD_HIP(hipStreamWaitEvent(E.stream, E.ev3, 0));
+                if (side) {
+                    VD_HIP(hipEventRecord(E.ev1, E.stream2));
+                    VD_HIP(hipStreamWaitEvent(E.stream, E.ev1, 0));
+                }
                 VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 8 * NT, E.stream));
                 for (int tier = 0; tier < NT; ++tier) {
                     for (int g = 0; g < 2; ++g) {
