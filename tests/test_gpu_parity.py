@@ -278,6 +278,58 @@ def test_voids_extend_once_and_probe_on_every_later_calculate():
         assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), i
 
 
+def test_many_handles_share_side_streams():
+    # 48 handles alive at once, most with cells next to voids whose reruns go to a side stream;
+    # the handles share the library's few side streams per device, also when two threads
+    # calculate at the same time; every handle keeps the oracle's bits
+    import threading
+    n = 30_000
+    seeds = (57, 58, 59)
+    lo, hi = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    grid = W.grid_for(n, lo, hi, 0.35)
+    pts, ref = {}, {}
+    for s in seeds:
+        pts[s] = W.clustered_points(W.PCG32(s), n, 8, 0.05, 1.0)
+        o = OracleTess(lo, hi, *grid)
+        o.set_generators(pts[s])
+        o.calculate(threads=8)
+        ref[s] = o.reduced()[0]
+    b = BoundingBox(lo, hi)
+    hs = []
+    for k in range(48):
+        t = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=False)
+        t.set_generators(pts[seeds[k % 3]])
+        hs.append(t)
+    try:
+        extended = 0
+        for t in hs:
+            t.calculate()
+            extended += int(t.stats()[2] >= 1)
+        assert extended >= 24, f"only {extended} of 48 handles extended their shell prefix"
+        for k, t in enumerate(hs):
+            assert np.array_equal(bits(t.volumes()), bits(ref[seeds[k % 3]])), k
+        errs = []
+
+        def run(part):
+            try:
+                for t in part:
+                    t.calculate()
+            except Exception as e:      # reported by the main thread
+                errs.append(e)
+
+        th = [threading.Thread(target=run, args=(hs[j::2],)) for j in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        assert not errs, errs
+        for k, t in enumerate(hs):
+            assert np.array_equal(bits(t.volumes()), bits(ref[seeds[k % 3]])), k
+    finally:
+        for t in hs:
+            t.close()
+
+
 @pytest.mark.parametrize("lo,hi,grid,k", [
     ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (16, 16, 16), 7),
     ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (20, 20, 20), 64),       # the whole table

@@ -124,23 +124,29 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x) {
 // Maximum with the reference's update rule `if d2 > m { m = d2 }` (NaN never wins).
 __device__ inline double gt_max(double m, double d2) { return d2 > m ? d2 : m; }
 
-template <int CTRL, int ROW>
-__device__ inline double dpp0_d(double x) {
-    const long long b = __double_as_longlong(x);
-    const uint32_t lo = dpp0<CTRL, ROW>((uint32_t)(b & 0xffffffffll));
-    const uint32_t hi = dpp0<CTRL, ROW>((uint32_t)((unsigned long long)b >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+__device__ inline uint32_t umax32(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+// Maximum over the 64 lanes (row shifts, then row broadcasts 15 and 31; 0 fills lanes without a
+// DPP source); each step is one v_max_u32 with a DPP operand.
+__device__ inline uint32_t wave_max_u32(uint32_t x) {
+    x = umax32(x, dpp0<0x111, 0xf>(x));
+    x = umax32(x, dpp0<0x112, 0xf>(x));
+    x = umax32(x, dpp0<0x114, 0xf>(x));
+    x = umax32(x, dpp0<0x118, 0xf>(x));
+    x = umax32(x, dpp0<0x142, 0xa>(x));
+    x = umax32(x, dpp0<0x143, 0xc>(x));
+    return rlu(x, 63);
 }
 
-// Maximum over the wave of non-negative values (0.0 fills lanes without a DPP source).
+// Maximum over the wave of values built by gt_max from +0.0: never NaN and never -0.0, so their
+// IEEE order is the unsigned order of their bit patterns. The high words are reduced first, then
+// the low words of the lanes that hold the largest high word; the result has the bits of the
+// largest value.
 __device__ inline double wave_max(double x) {
-    x = gt_max(x, dpp0_d<0x111, 0xf>(x));
-    x = gt_max(x, dpp0_d<0x112, 0xf>(x));
-    x = gt_max(x, dpp0_d<0x114, 0xf>(x));
-    x = gt_max(x, dpp0_d<0x118, 0xf>(x));
-    x = gt_max(x, dpp0_d<0x142, 0xa>(x));
-    x = gt_max(x, dpp0_d<0x143, 0xc>(x));
-    return rld(x, 63);
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t hi = wave_max_u32((uint32_t)(b >> 32));
+    const uint32_t lo = wave_max_u32((uint32_t)(b >> 32) == hi ? (uint32_t)b : 0u);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
@@ -601,7 +607,10 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     const int c = P.cur, o = c;
     if (NV > 1 && P.F > 64) return kGeneric;
 
-    // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362)
+    // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362). Lanes past
+    //    the polytope hold d = 0, which is neither beyond kTol nor below -kTol. Ballots are of
+    //    single compares, combined with scalar operations (a ballot of a combined predicate costs
+    //    two more vector instructions).
     double d[NV];
     bool va[NV];
     VMask<NV> gtm, ltm, inm;
@@ -610,13 +619,13 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     for (int k = 0; k < NV; ++k) {
         va[k] = 64 * k + l < P.V;
         d[k] = va[k] ? (r.x[k] - pl.px) * pl.nx + (r.y[k] - pl.py) * pl.ny + (r.z[k] - pl.pz) * pl.nz : 0.0;
-        gtm.w[k] = ballot(va[k] && d[k] > kTol);
+        gtm.w[k] = ballot(d[k] > kTol);
         anyg |= gtm.w[k] != 0;
     }
     if (!anyg) return kNoChange;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        ltm.w[k] = ballot(va[k] && d[k] < -kTol);
+        ltm.w[k] = ballot(d[k] < -kTol);
         anyl |= ltm.w[k] != 0;
     }
     if (!anyl) {
@@ -625,7 +634,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         return kModified;
     }
 #pragma unroll
-    for (int k = 0; k < NV; ++k) inm.w[k] = ballot(va[k] && d[k] <= kTol);
+    for (int k = 0; k < NV; ++k) inm.w[k] = ballot(d[k] <= kTol) & ballot(va[k]);
     const int nk = inm.count();
     double md2 = 0.0;
 #pragma unroll
@@ -640,6 +649,10 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     // 2. slots: output entry counts (prefix in slot order) and plane crossings (:378-452).
     //    Per chunk and lane two registers: ea = a | b << SH | face << 2 SH | a kept << 30 | b kept << 31 and
     //    eq = output prefix | crossing index << 16.
+    //    Slot, key and step-6 reads below index whole 64-lane chunks of arrays of SM (or KM = 64)
+    //    entries without a guard, then select: a guarded read is a branch whose predicate the
+    //    compiler then carries through a vector register.
+    static_assert(64 * NCH <= C::SM && C::KM >= 64, "chunked reads stay inside the slot arrays");
     const int nch = (P.S + 63) >> 6;
     uint32_t ea[NCH];
     uint32_t eq[NCH];
@@ -651,18 +664,19 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         if (ch < nch) {
             const int s = 64 * ch + l;
             const bool sa = s < P.S;
-            const uint32_t e = sa ? m.sedge[c][s] : 0u;
+            const uint32_t e0 = m.sedge[c][s], f0 = m.sface[c][s];
+            const uint32_t e = sa ? e0 : 0u;
             const uint32_t a = e & 0xffffu, b = e >> 16;
             const bool ain = inm.test(a), bin = inm.test(b);
-            ea[ch] = a | (b << SH) | ((sa ? (uint32_t)m.sface[c][s] : 0u) << (2 * SH)) | ((uint32_t)ain << 30) |
-                     ((uint32_t)bin << 31);
-            const uint64_t b0 = ballot(sa && ain);
-            const uint64_t b1 = ballot(sa && !ain && bin);
+            ea[ch] = a | (b << SH) | ((sa ? f0 : 0u) << (2 * SH)) | ((uint32_t)ain << 30) | ((uint32_t)bin << 31);
+            const uint64_t sam = ballot(sa), ainm = ballot(ain), binm = ballot(bin);
+            const uint64_t b0 = sam & ainm;
+            const uint64_t b1 = sam & ~ainm & binm;
             const uint32_t pp = (uint32_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
             ptot += popc(b0) + 2 * popc(b1);
             if (sa) m.sc.P[s] = (uint16_t)pp;
             const bool cr = sa && (ain != bin);
-            const uint64_t cm = ballot(cr);
+            const uint64_t cm = sam & (ainm ^ binm);
             const int ci = K + below_lane(cm);
             eq[ch] = pp | ((uint32_t)(ci & 0xffff) << 16);
             if (cr && ci < 64) {
@@ -682,13 +696,13 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     const int pfs = fa ? (int)m.sc.P[fs] : 0;
     const int tot = fa ? (int)m.sc.P[fe] - pfs : 0;
     const bool sv = fa && tot >= 3;
-    const uint64_t svm = ballot(sv);
+    const uint64_t svm = ballot(tot >= 3);   // tot = 0 past the faces
     int Fn = popc(svm);
     const uint32_t tv = sv ? (uint32_t)tot : 0u;
     const uint32_t inc = wave_incl_scan(tv);
     const int off = (int)(inc - tv);
     int Spre = (int)rlu(inc, 63);
-    if (ballot(sv && tot > 255)) return kFace255;
+    if (ballot(tot > 255)) return kFace255;
     if (Fn + 1 > C::FM || Spre > C::SM) return kOverflow;
     if (fa) {
         const int nf = below_lane(svm);
@@ -704,19 +718,26 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     if (l == 0) m.fstart[o][Fn] = (uint16_t)Spre;
 
     // 4. crossings: the first occurrence of an edge key in slot order creates the vertex
-    //    (:391-417); new vertices are numbered in that order after the kept ones
+    //    (:391-417); new vertices are numbered in that order after the kept ones. Each pass takes
+    //    the lowest crossing not numbered yet, which is the first occurrence of its key, and gives
+    //    every crossing with that key the next new index: one pass per new vertex. Keys are below
+    //    0xffff0000 (vertex indices are 16-bit and a < b), so lanes without a crossing never match.
     const bool ka = l < K;
-    const uint32_t key = ka ? m.sc.ckey[l] : 0xffffffffu;
-    int first = l;
-    for (int cj = 0; cj < K; ++cj) {
-        const uint32_t kj = rlu(key, cj);
-        if (cj < first && kj == key) first = cj;
+    const uint32_t key0 = m.sc.ckey[l];
+    const uint32_t key = ka ? key0 : 0xffffffffu;
+    uint64_t rest = ballot(ka), fm = 0;
+    int nint = 0, xid = 0;
+    while (rest && nint < 64) {     // every pass clears lane f: at most K <= 64 passes
+        const int f = ffs64(rest);
+        const bool mine = key == rlu(key, f);
+        const uint64_t same = ballot(mine);   // holds lane f itself
+        rest &= ~same;
+        fm |= 1ull << f;
+        if (mine) xid = nk + nint;
+        ++nint;
     }
-    const bool isf = ka && first == l;
-    const uint64_t fm = ballot(isf);
-    const int nint = popc(fm);
-    const int myx = nk + below_lane(fm);
-    const int xid = (int)lane_get((uint32_t)myx, first);
+    const bool isf = (fm >> l) & 1ull;
+    const int myx = xid;
     if (nk + nint > C::VM) return kOverflow;
     double x = 0.0, y = 0.0, z = 0.0;
     if (isf) {
@@ -782,14 +803,16 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     // 6. successor of every new slot; lid segments (v, u) for intersection pairs (:459-465)
     int M = 0;
     for (int base = 0; base < Spre; base += 64) {
-        const int p = base + l;
+        const int p = base + l;            // < SM: Spre <= SM, a multiple of 64
         const bool pa = p < Spre;
-        const uint32_t ua = pa ? m.sc.fidxn[p] : 0u;
-        const int np = pa ? m.sc.nextp[p] : 0;
-        const uint32_t vb = pa ? m.sc.fidxn[np] : 0u;
+        const uint32_t u0 = m.sc.fidxn[p], n0 = m.sc.nextp[p];
+        const uint32_t ua = pa ? u0 : 0u;
+        const int np = pa ? (int)n0 : 0;   // stale past Spre: read at 0 instead
+        const uint32_t v0 = m.sc.fidxn[np];
+        const uint32_t vb = pa ? v0 : 0u;
         if (pa) m.sedge[o][p] = ua | (vb << 16);
         const bool sg = pa && (int)ua >= nk && (int)vb >= nk;
-        const uint64_t gm = ballot(sg);
+        const uint64_t gm = ballot((int)ua >= nk) & ballot((int)vb >= nk);   // ua = vb = 0 < nk past Spre
         if (sg && M + below_lane(gm) < C::KM) m.sc.seg[M + below_lane(gm)] = vb | (ua << 16);
         M += popc(gm);
     }
@@ -1201,7 +1224,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
             // next candidate of the window that passes the distance test (tessellation.rs:458)
             const double lim = 4.0 * r2;
             const uint64_t above = last < 0 ? ~0ull : (last >= 63 ? 0ull : ~((2ull << last) - 1ull));
-            const uint64_t pm = wvm & wcm & above & ballot(wj != li && !(wdsq > lim));
+            const uint64_t pm = wvm & wcm & above & ballot(wj != li) & ballot(!(wdsq > lim));
             if (pm == 0) {
                 // window exhausted: find where the walk continues
                 win = false;
@@ -1272,10 +1295,10 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
             if (todo) wcm |= pretest<C>(m, P, (todo >> l) & 1ull, wd0, wd1, wd2, gx, gy, gz, P.nk, P.V);
         }
         const double lim = 4.0 * r2;
-        kbc = ffs64(ballot((l > kc) && (bend || bd2 > lim)));
+        kbc = ffs64(ballot(l > kc) & (ballot(bend) | ballot(bd2 > lim)));
         const int wk = (int)(wko & 0xffu);
         const double wbd = lane_get_d(bdist, wk);   // bin distance of the candidate's entry
-        wvm &= ballot(l <= cl || wk == kc || (wk < kbc && wbd <= lim));
+        wvm &= ballot(l <= cl) | ballot(wk == kc) | (ballot(wk < kbc) & ballot(wbd <= lim));
     }
     wave_sync();
 
