@@ -828,19 +828,22 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         }
         wave_sync();
         const uint32_t mv = (l < nint) ? m.sc.lmap[l] : 0u;
+        // the walk runs over lane indices (vertex - nk): lane j holds the lane of new vertex
+        // nk + j's successor, or 0xffff where the map has none
+        const uint32_t nxl = mv ? (mv & 0xffffu) - (uint32_t)nk : 0xffffu;
         const uint32_t s0 = m.sc.seg[0];
-        const int start = (int)(s0 & 0xffffu);
-        int cv = (int)(s0 >> 16);
+        const int start = (int)(s0 & 0xffffu) - nk;
+        int cv = (int)(s0 >> 16) - nk;
         uint32_t lidv = (uint32_t)start;
         int len = 1;
         while (cv != start && len <= M) {
             if (len >= 64) return kGeneric;
             if (l == len) lidv = (uint32_t)cv;
             ++len;
-            const uint32_t e = rlu(mv, cv - nk);
-            cv = e ? (int)(e & 0xffffu) : 0xffff;
+            cv = (int)rlu(nxl, cv);
             if (cv == 0xffff) break;
         }
+        lidv += (uint32_t)nk;
         if (len >= 3) {
             if (Spre + len > C::SM || Fn + 1 > C::FM) return kOverflow;
             const uint32_t nxt = lane_get(lidv, (l + 1 >= len) ? 0 : l + 1);
