@@ -639,7 +639,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     double md2 = 0.0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        if (va[k]) m.sc.dist[64 * k + l] = d[k];
+        m.sc.dist[64 * k + l] = d[k];   // 0 past the polytope, never read there
         if (withgen && ((inm.w[k] >> l) & 1ull)) {
             const double dx = r.x[k] - gx, dy = r.y[k] - gy, dz = r.z[k] - gz;
             md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
@@ -674,7 +674,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             const uint64_t b1 = sam & ~ainm & binm;
             const uint32_t pp = (uint32_t)(ptot + below_lane(b0) + 2 * below_lane(b1));
             ptot += popc(b0) + 2 * popc(b1);
-            if (sa) m.sc.P[s] = (uint16_t)pp;
+            m.sc.P[s] = (uint16_t)pp;   // past P.S never read; P[P.S] is written below
             const bool cr = sa && (ain != bin);
             const uint64_t cm = sam & (ainm ^ binm);
             const int ci = K + below_lane(cm);
@@ -810,7 +810,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         const int np = pa ? (int)n0 : 0;   // stale past Spre: read at 0 instead
         const uint32_t v0 = m.sc.fidxn[np];
         const uint32_t vb = pa ? v0 : 0u;
-        if (pa) m.sedge[o][p] = ua | (vb << 16);
+        m.sedge[o][p] = ua | (vb << 16);   // past Spre: the lid's slots, or unused
         const bool sg = pa && (int)ua >= nk && (int)vb >= nk;
         const uint64_t gm = ballot((int)ua >= nk) & ballot((int)vb >= nk);   // ua = vb = 0 < nk past Spre
         if (sg && M + below_lane(gm) < C::KM) m.sc.seg[M + below_lane(gm)] = vb | (ua << 16);
@@ -820,7 +820,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
 
     // 7. lid polygon (:471-495); lane j holds the map entry of new vertex nk + j
     if (M > 0) {
-        if (l < nint) m.sc.lmap[l] = 0u;
+        m.sc.lmap[l] = 0u;   // KM >= 64 entries
         wave_sync();
         for (int t = l; t < M; t += 64) {
             const uint32_t sg = m.sc.seg[t];
