@@ -8,7 +8,9 @@
  *     the message of the calling thread's last failure;
  *   - output buffers are caller-owned host memory; the library owns all device memory;
  *   - one handle is not safe for concurrent use (the reference takes &mut self); separate handles
- *     may be used from separate threads and run on their own HIP streams;
+ *     may be used from separate threads and run on their own HIP streams (reruns of cells that
+ *     reach across empty space also use one of a few side streams per device that the library
+ *     shares between handles);
  *   - where the reference panics (wall id > WALL_ID_MAX) the call returns VD_EINVAL; where it
  *     returns None (get_cell out of range) the call returns VD_ERANGE.
  */
