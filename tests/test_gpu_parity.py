@@ -136,6 +136,26 @@ def test_degenerate_inputs():
     assert g0.count_cells() == 0 and g0.get_cell(0) is None
 
 
+@pytest.mark.parametrize("jitter", [0.0, 1e-10, 1e-7])
+@pytest.mark.parametrize("wall", [False, True])
+def test_lattice_bit_exact(jitter, wall):
+    # a 10^3 lattice, exact or jittered around the clip tolerance: candidate planes through existing
+    # vertices (distances at or near +-1e-9), four-fold vertices and crossings that share edges
+    n = 10
+    ax = (np.arange(n) + 0.5) / n
+    pts = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), axis=-1).reshape(-1, 3)
+    if jitter:
+        pts = pts + W.PCG32(11).uniform(len(pts), (-jitter,) * 3, (jitter,) * 3)
+    walls = [("sphere", -1000, (0.5, 0.5, 0.5), 0.45)] if wall else []
+    for grid in ((n, n, n), (7, 7, 7)):
+        g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), grid, pts, walls)
+        vol = check_all_cells(g, o)
+        if not wall:
+            # vertices within the clip tolerance (1e-9) of a plane are kept on both sides, so a
+            # jittered lattice overlaps by a tolerance-sized volume, as the oracle does
+            assert abs(vol.sum() - 1.0) < (1e-12 if jitter == 0.0 else 1e-6)
+
+
 def test_uniform_sphere_50k_bit_exact():
     w = W.config2(n=50_000, grid=37)
     g, o = build_pair(w.lo, w.hi, w.grid, w.points, w.walls)
