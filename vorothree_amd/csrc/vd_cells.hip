@@ -69,7 +69,6 @@ __device__ inline void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "w
 #else
 __device__ inline void wave_sync() { __syncthreads(); }
 #endif
-__device__ inline uint64_t lt_mask() { return (1ull << lane_id()) - 1ull; }
 // Set bits of m below this lane (v_mbcnt_lo / v_mbcnt_hi): the prefix count of a ballot.
 __device__ inline int below_lane(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
