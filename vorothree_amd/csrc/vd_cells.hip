@@ -612,11 +612,12 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     //    two more vector instructions).
     double d[NV];
     bool va[NV];
-    VMask<NV> gtm, ltm, inm;
+    VMask<NV> gtm, ltm, inm, vam;
     bool anyg = false, anyl = false;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         va[k] = 64 * k + l < P.V;
+        vam.w[k] = ballot(va[k]);
         d[k] = va[k] ? (r.x[k] - pl.px) * pl.nx + (r.y[k] - pl.py) * pl.ny + (r.z[k] - pl.pz) * pl.nz : 0.0;
         gtm.w[k] = ballot(d[k] > kTol);
         anyg |= gtm.w[k] != 0;
@@ -633,7 +634,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         return kModified;
     }
 #pragma unroll
-    for (int k = 0; k < NV; ++k) inm.w[k] = ballot(d[k] <= kTol) & ballot(va[k]);
+    for (int k = 0; k < NV; ++k) inm.w[k] = ballot(d[k] <= kTol) & vam.w[k];
     const int nk = inm.count();
     double md2 = 0.0;
 #pragma unroll
@@ -811,7 +812,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         const uint32_t vb = pa ? v0 : 0u;
         m.sedge[o][p] = ua | (vb << 16);   // past Spre: the lid's slots, or unused
         const bool sg = pa && (int)ua >= nk && (int)vb >= nk;
-        const uint64_t gm = ballot((int)ua >= nk) & ballot((int)vb >= nk);   // ua = vb = 0 < nk past Spre
+        const uint64_t gm = ballot(sg);
         if (sg && M + below_lane(gm) < C::KM) m.sc.seg[M + below_lane(gm)] = vb | (ua << 16);
         M += popc(gm);
     }
