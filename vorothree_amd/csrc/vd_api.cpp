@@ -373,7 +373,7 @@ struct Engine {
     int device = 0, cus = 1;
     int occ[kTiers] = {1, 1, 1, 1};
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;     // side stream (shared, see side_stream) for far-cell retries
+    hipStream_t stream2 = nullptr;     // side stream for far-cell retries (shared: engine_stream2)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int keep_cells = 1;
 

@@ -652,7 +652,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     //    Slot, key and step-6 reads below index whole 64-lane chunks of arrays of SM (or KM = 64)
     //    entries without a guard, then select: a guarded read is a branch whose predicate the
     //    compiler then carries through a vector register.
-    static_assert(64 * NCH <= C::SM && C::KM >= 64, "chunked reads stay inside the slot arrays");
+    static_assert(64 * NCH <= C::SM && C::SM % 64 == 0 && C::KM >= 64, "chunked reads stay inside the slot arrays");
     const int nch = (P.S + 63) >> 6;
     uint32_t ea[NCH];
     uint32_t eq[NCH];
