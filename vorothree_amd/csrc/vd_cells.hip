@@ -1056,8 +1056,8 @@ __device__ inline int probe_inert(const ShellCode *shell, uint32_t nshell, const
     return first;
 }
 
-// One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports. The
-// walls and the neighbour candidates feed a single clip site (one inlined copy of the clip).
+// One cell: compute_cell (tessellation.rs:414-475) then the reductions and the exports. The wall
+// planes and the neighbour candidates each have a clip site (two inlined copies of the clip).
 // PROBE: skip inert runs of shell batches in bulk (probe_inert). The generic tiers always do; the
 // fast tiers only in the instantiation that reruns cells whose walk ran past the shell prefix,
 // because the probe's registers slow the common walk down.
@@ -1268,7 +1268,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
         }
 
         double nr2 = 0.0;
-        const int rc = clip_any<C>(m, P, r, pl, nid, withgen, gx, gy, gz, nr2);   // the clip site
+        const int rc = clip_any<C>(m, P, r, pl, nid, withgen, gx, gy, gz, nr2);   // tessellation.rs:460
         if (rc < 0) {
             status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
             break;
