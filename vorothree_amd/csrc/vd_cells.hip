@@ -1268,7 +1268,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChun
         }
 
         double nr2 = 0.0;
-        const int rc = clip_any<C>(m, P, r, pl, nid, withgen, gx, gy, gz, nr2);   // tessellation.rs:460
+        const int rc = clip_any<C>(m, P, r, pl, nid, withgen, gx, gy, gz, nr2);   // tessellation.rs:462-463
         if (rc < 0) {
             status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
             break;
