@@ -389,6 +389,9 @@ struct Engine {
     int seal = 0;                      // cell launches run the seal mode
     bool trunc0 = false;               // the first launch sees only the first prefix of far geometries
     int reserve_cus = 0;               // CUs the persistent cell launches leave to other engines
+    // far-cell reruns on a side stream beside the rest of their round (off in the server's lanes:
+    // with two lanes sharing the GPU the extra overlap cost more than it gained, measured)
+    bool side_streams = true;
     uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
     uint64_t cap_cells = 0;            // cells the arena capacities were sized for
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
@@ -732,10 +735,9 @@ int enqueue_binning(Engine &E) {
 }
 
 // One cell launch over d_list on stream st (the engine's stream by default), drawing cells from work
-// counter `slot` of d_queue (launches that may run at the same time use different counters), with
-// at most max_waves waves (0: the engine's CUs times the tier's occupancy).
+// counter `slot` of d_queue (launches that may run at the same time use different counters).
 int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms, bool probe = false,
-              int slot = 0, hipStream_t st = nullptr, int max_waves = 0) {
+              int slot = 0, hipStream_t st = nullptr) {
     if (!st) st = E.stream;
     vd::CellArgs a{};
     a.desc = E.d_desc.as<vd::TessDesc>();
@@ -768,7 +770,6 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.cell_off = E.d_cell_off.as<uint64_t>();
     VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>() + slot, 0u, 1, st));
     int waves = std::max(1, E.cus - E.reserve_cus) * E.occ[tier];
-    if (max_waves > 0 && waves > max_waves) waves = max_waves;
     if ((uint32_t)waves > nlist) waves = (int)nlist;
     // arena chunks per wave: up to kArenaChunk[k] entries, at most an eighth of the capacity over
     // all waves (the unused tails of the last chunks are holes in the arena)
@@ -943,28 +944,21 @@ int engine_calculate(Engine &E) {
                 // the rest of the round fills the CUs its long-running waves leave idle; ev0 and ev1
                 // (timing events of the first round's launch, free in later rounds) order it
                 bool side = false;
-                for (int tier = 0; tier < NT; ++tier) side = side || nsh[tier] > 0;
+                if (E.side_streams)
+                    for (int tier = 0; tier < NT; ++tier) side = side || nsh[tier] > 0;
                 if (side) {
                     VD_TRY(engine_stream2(E));
                     VD_HIP(hipEventRecord(E.ev0, E.stream));
                     VD_HIP(hipStreamWaitEvent(E.stream2, E.ev0, 0));
                 }
-                // with CUs held back for another lane, the two streams' launches split the CUs this
-                // engine may fill: group 1 takes up to half, as many as its cells need
-                int c1 = 0;
-                const int avail = std::max(2, E.cus - E.reserve_cus);
-                if (side && E.reserve_cus > 0)
-                    for (int tier = 0; tier < NT; ++tier)
-                        c1 = std::max(c1, (int)std::min<uint32_t>(avail / 2,
-                                                                  (nsh[tier] + E.occ[tier] - 1) / E.occ[tier]));
                 for (int tier = 0; tier < NT; ++tier) {
                     if (!nlist[tier]) continue;
                     if (nsh[tier])
-                        VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier, E.stream2,
-                                         c1 * E.occ[tier]));
+                        VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier,
+                                         side ? E.stream2 : nullptr));
                     if (nlist[tier] > nsh[tier])
                         VD_TRY(run_cells(E, tier, list[tier] + nsh[tier], nlist[tier] - nsh[tier], nullptr, false,
-                                         1 + NT + tier, nullptr, c1 ? (avail - c1) * E.occ[tier] : 0));
+                                         1 + NT + tier));
                     if (tier > 0) E.stats[1] += nlist[tier];
                 }
                 if (side) {
@@ -1880,6 +1874,7 @@ int vd_server_create(size_t hbm_budget, int max_slots, vd_server **out) {
     int rc = VD_OK;
     for (Lane &L : S->lanes) {
         L.E.keep_cells = 0;
+        L.E.side_streams = false;
         if (!rc) rc = engine_init(L.E);
     }
     if (rc) {
