@@ -168,6 +168,10 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
 #define VD_OPT_ARENA 3      /* initial full-cell arena per cell, in vertices (default 40; 3/5 as many
                                faces and 3 times as many face indices): a launch that fills it
                                doubles it and runs again, with identical results */
+#define VD_OPT_FAR_LIST 4   /* cells that reach across empty space and would walk about this many
+                               shell entries or more (default 262144; 0: never) rerun over a list
+                               of only the entries whose bins hold generators, in shell order:
+                               the same candidates in the same order, so identical results */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
 
 /* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cell runs

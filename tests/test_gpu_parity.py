@@ -298,6 +298,37 @@ def test_voids_extend_once_and_probe_on_every_later_calculate():
         assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), i
 
 
+@pytest.mark.parametrize("n,keep", [(30_000, False), (12_000, True)])
+def test_far_cell_lists_bit_exact(n, keep):
+    # cells next to the voids rerun over per-cell lists of the shell entries whose bins hold
+    # generators: forced for every such cell (1), the default, and never (0) all give the oracle's
+    # bits, full cells included
+    pts = W.clustered_points(W.PCG32(71), n, 8, 0.05, 1.0)
+    grid = W.grid_for(n, (0, 0, 0), (1, 1, 1), 0.35)
+    o = OracleTess((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), *grid)
+    o.set_generators(pts)
+    o.calculate(threads=8)
+    vol, cen, nf, _ = o.reduced()
+    for min_entries in (1, None, 0):
+        g, _ = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), grid, pts, keep_cells=keep)
+        if min_entries is not None:
+            g.set_far_list(min_entries)
+        g.calculate()
+        assert g.stats()[2] >= 1, "no cell reached across empty space"
+        assert np.array_equal(bits(g.volumes()), bits(vol)), min_entries
+        assert np.array_equal(bits(g.centroids()), bits(cen)), min_entries
+        assert np.array_equal(g.face_counts(), nf), min_entries
+        off, ids = g.neighbors()
+        for i in range(0, n, 53):
+            assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), (min_entries, i)
+        if keep:
+            for i in range(0, n, 97):
+                assert_same_cell(g.get_cell(i), o.get_cell(i), i)
+        g.calculate()                       # the extended prefix from the first launch
+        assert np.array_equal(bits(g.volumes()), bits(vol)), min_entries
+        g.close()
+
+
 def test_many_handles_share_side_streams():
     # 48 handles alive at once, most with cells next to voids whose reruns go to a side stream;
     # the handles share the library's few side streams per device, also when two threads

@@ -27,6 +27,13 @@ namespace {
 
 constexpr int kTiers = 4;   // polytope capacity tiers of the cell kernel (vd_cells.hip)
 constexpr uint64_t kArenaChunk[3] = {512, 256, 1536};   // vertices, faces, face indices per wave chunk
+// Far cells whose rerun walks about this many shell entries or more walk a per-cell list of the
+// entries that hold generators (k_far_lists); at most kFarListCells cells per launch and
+// kFarListEntries list entries in all (the cells with the most entries to walk first), beyond
+// which cells walk the shell prefix as before.
+constexpr double kFarListMinDefault = 262144.0;
+constexpr uint32_t kFarListCells = 256;
+constexpr uint64_t kFarListEntries = uint64_t(1) << 27;
 
 thread_local std::string g_err;
 
@@ -392,6 +399,7 @@ struct Engine {
     // far-cell reruns on a side stream beside the rest of their round (off in the server's lanes:
     // with two lanes sharing the GPU the extra overlap cost more than it gained, measured)
     bool side_streams = true;
+    double far_list_min = kFarListMinDefault;   // walk length for a per-cell list (0: never)
     uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
     uint64_t cap_cells = 0;            // cells the arena capacities were sized for
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
@@ -735,9 +743,12 @@ int enqueue_binning(Engine &E) {
 }
 
 // One cell launch over d_list on stream st (the engine's stream by default), drawing cells from work
-// counter `slot` of d_queue (launches that may run at the same time use different counters).
+// counter `slot` of d_queue (launches that may run at the same time use different counters), with
+// optional per-cell shell lists (far_lists) for the entries of d_list.
 int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float *ms, bool probe = false,
-              int slot = 0, hipStream_t st = nullptr) {
+              int slot = 0, hipStream_t st = nullptr, const vd::ShellCode *lshell = nullptr,
+              const uint64_t *lshell_off = nullptr, const uint32_t *lshell_n = nullptr,
+              const double *lshell_next = nullptr) {
     if (!st) st = E.stream;
     vd::CellArgs a{};
     a.desc = E.d_desc.as<vd::TessDesc>();
@@ -757,6 +768,10 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.cells = d_list;
     a.ncells = nlist;
     a.queue = E.d_queue.as<uint32_t>() + slot;
+    a.lshell = lshell;
+    a.lshell_off = lshell_off;
+    a.lshell_n = lshell_n;
+    a.lshell_next = lshell_next;
     a.vol = E.d_vol.as<double>();
     a.cen = E.d_cen.as<double>();
     a.nface = E.d_nface.as<uint32_t>();
@@ -800,6 +815,110 @@ int enqueue_round0(Engine &E, uint32_t n, float *ms) {
                               E.d_out1.as<uint32_t>(), E.d_counts4.as<uint32_t>(), E.stream));
     VD_HIP(hipMemcpyAsync(E.h_counts, E.d_counts4.p, 4 * kTiers * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           E.stream));
+    return VD_OK;
+}
+
+// Per-cell shell lists (k_far_lists) for the far cells of d_list[0, n) whose reruns walk about
+// E.far_list_min shell entries or more, each over the prefix entries its rerun can visit (key <=
+// 4 r^2 of its stop, with a margin for rounding). Built on E.stream and finished on return. use:
+// whether any cell got one; lsh holds the codes, and per entry of d_list lof the offset, lnn the
+// length (kNoList: none) and lnx the key after the last entry of its list. The caller releases
+// the four buffers after the launch.
+int far_lists(Engine &E, const uint32_t *d_list, uint32_t n, DevBuf &lsh, DevBuf &lof, DevBuf &lnn, DevBuf &lnx,
+              bool &use) {
+    use = false;
+    if (E.far_list_min <= 0.0 || n == 0) return VD_OK;
+    DevBuf flags, lim, sel, slim, cpd, counts, offs, gidx, at;
+    auto done = [&]() {
+        DevBuf *tmp[] = {&flags, &lim, &sel, &slim, &cpd, &counts, &offs, &gidx, &at};
+        for (DevBuf *b : tmp) release(*b);
+    };
+    VD_TRY(ensure(flags, (size_t)n * 4));
+    VD_TRY(ensure(lim, (size_t)n * 4));
+    VD_TRY(ensure(lnx, (size_t)n * 8));
+    const vd::TessDesc *desc = E.d_desc.as<vd::TessDesc>();
+    const uint32_t *gt = E.d_gen_tess.as<uint32_t>();
+    VD_HIP(vd::launch_far_pick(d_list, n, E.d_status.as<int32_t>(), E.d_vol.as<double>(), gt, desc,
+                               E.far_list_min, flags.as<uint32_t>(), lim.as<uint32_t>(), lnx.as<double>(),
+                               E.stream));
+    std::vector<uint32_t> hf(n), hl(n), hlim(n);
+    VD_HIP(hipMemcpyAsync(hf.data(), flags.p, (size_t)n * 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipMemcpyAsync(hlim.data(), lim.p, (size_t)n * 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipMemcpyAsync(hl.data(), d_list, (size_t)n * 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    // selected cells and their chunks: cp[s] = first block of cell s
+    std::vector<uint32_t> sq, hsel, hslim, cp(1, 0u);
+    uint64_t blocks = 0;
+    // the cells with the most entries to walk first, at most kFarListCells of them
+    std::vector<uint32_t> cand;
+    for (uint32_t q = 0; q < n; ++q)
+        if (hf[q] && hlim[q]) cand.push_back(q);
+    std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return hlim[a] > hlim[b]; });
+    if (cand.size() > kFarListCells) cand.resize(kFarListCells);
+    uint64_t entries = 0;   // bounds the kept entries, whose offsets are 32-bit
+    for (uint32_t q : cand) {
+        const uint32_t lq = hlim[q];
+        const uint64_t c = (lq + vd::kListChunk - 1) / vd::kListChunk;
+        if (blocks + c >= (uint64_t(1) << 31) || entries + lq > 0xffffffffull) break;
+        entries += lq;
+        sq.push_back(q);
+        hsel.push_back(hl[q]);
+        hslim.push_back(lq);
+        blocks += c;
+        cp.push_back((uint32_t)blocks);
+    }
+    if (sq.empty()) { done(); return VD_OK; }
+    const uint32_t nsel = (uint32_t)sq.size(), nb = (uint32_t)blocks;
+    VD_TRY(ensure(sel, (size_t)nsel * 4));
+    VD_TRY(ensure(slim, (size_t)nsel * 4));
+    VD_TRY(ensure(cpd, (size_t)(nsel + 1) * 4));
+    VD_TRY(ensure(counts, (size_t)nb * 4));
+    VD_TRY(ensure(offs, (size_t)nb * 4));
+    VD_HIP(hipMemcpyAsync(sel.p, hsel.data(), (size_t)nsel * 4, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(hipMemcpyAsync(slim.p, hslim.data(), (size_t)nsel * 4, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(hipMemcpyAsync(cpd.p, cp.data(), (size_t)(nsel + 1) * 4, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(vd::launch_far_lists(sel.as<uint32_t>(), slim.as<uint32_t>(), cpd.as<uint32_t>(), nsel, nb, desc, gt,
+                                E.d_gen.as<double>(), E.d_bin_start.as<uint32_t>(), counts.as<uint32_t>(),
+                                nullptr, nullptr, E.stream));
+    VD_TRY(scan(E, counts.as<uint32_t>(), offs.as<uint32_t>(), nb));
+    // first list entry of each selected cell (the scanned count of its first block) and the total
+    std::vector<uint64_t> hidx(nsel);
+    for (uint32_t k = 0; k < nsel; ++k) hidx[k] = cp[k];
+    VD_TRY(ensure(gidx, (size_t)nsel * 8));
+    VD_TRY(ensure(at, (size_t)nsel * 4));
+    VD_HIP(hipMemcpyAsync(gidx.p, hidx.data(), (size_t)nsel * 8, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(vd::launch_gather_u32(offs.as<uint32_t>(), gidx.as<uint64_t>(), nsel, at.as<uint32_t>(), E.stream));
+    std::vector<uint32_t> start(nsel);
+    uint32_t tail[2] = {0, 0};
+    VD_HIP(hipMemcpyAsync(start.data(), at.p, (size_t)nsel * 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipMemcpyAsync(&tail[0], offs.as<uint32_t>() + nb - 1, 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipMemcpyAsync(&tail[1], counts.as<uint32_t>() + nb - 1, 4, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    const uint64_t all = (uint64_t)tail[0] + tail[1];
+    // the cells (in the order above) whose lists fit in kFarListEntries: a prefix of the blocks
+    uint32_t keep = nsel;
+    while (keep > 0 && (keep < nsel ? (uint64_t)start[keep] : all) > kFarListEntries) --keep;
+    if (keep == 0) { done(); return VD_OK; }
+    const uint64_t total = keep < nsel ? (uint64_t)start[keep] : all;
+    // one spare entry: a walk's bulk skip reads entry 0 of its list even when the list is empty
+    VD_TRY(ensure(lsh, (size_t)(total + 1) * sizeof(vd::ShellCode)));
+    VD_HIP(vd::launch_far_lists(sel.as<uint32_t>(), slim.as<uint32_t>(), cpd.as<uint32_t>(), keep, cp[keep], desc,
+                                gt, E.d_gen.as<double>(), E.d_bin_start.as<uint32_t>(), nullptr,
+                                offs.as<uint32_t>(), lsh.as<vd::ShellCode>(), E.stream));
+    std::vector<uint64_t> hoff(n, 0);
+    std::vector<uint32_t> hn(n, vd::kNoList);
+    for (uint32_t k = 0; k < keep; ++k) {
+        const uint64_t end = k + 1 < keep ? start[k + 1] : total;
+        hoff[sq[k]] = start[k];
+        hn[sq[k]] = (uint32_t)(end - start[k]);
+    }
+    VD_TRY(ensure(lof, (size_t)n * 8));
+    VD_TRY(ensure(lnn, (size_t)n * 4));
+    VD_HIP(hipMemcpyAsync(lof.p, hoff.data(), (size_t)n * 8, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(hipMemcpyAsync(lnn.p, hn.data(), (size_t)n * 4, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    done();
+    use = true;
     return VD_OK;
 }
 
@@ -928,6 +1047,8 @@ int engine_calculate(Engine &E) {
             // 2 arena, 3 error}], and the lists of capacity and shell failures per group and tier
             uint32_t counts[8 * NT] = {};
             DevBuf out[4 * NT];
+            DevBuf flist[4 * NT];   // per-cell shell lists of the far cells of each tier (far_lists)
+            bool lon[NT] = {};
             const uint32_t *ocap[2][NT] = {}, *oshell[2][NT] = {};
             auto cnt = [&](int g, int tier, int what) { return counts[4 * (NT * g + tier) + what]; };
             if (round == 0 && attempt == 0) {
@@ -943,6 +1064,11 @@ int engine_calculate(Engine &E) {
                 // group 1 (cells crossing empty space: few, long walks) runs on a side stream, so
                 // the rest of the round fills the CUs its long-running waves leave idle; ev0 and ev1
                 // (timing events of the first round's launch, free in later rounds) order it
+                // lists first: the side stream's launches are ordered after everything before ev0
+                for (int tier = 0; tier < NT; ++tier)
+                    if (nsh[tier])
+                        VD_TRY(far_lists(E, list[tier], nsh[tier], flist[4 * tier], flist[4 * tier + 1],
+                                         flist[4 * tier + 2], flist[4 * tier + 3], lon[tier]));
                 bool side = false;
                 if (E.side_streams)
                     for (int tier = 0; tier < NT; ++tier) side = side || nsh[tier] > 0;
@@ -955,7 +1081,11 @@ int engine_calculate(Engine &E) {
                     if (!nlist[tier]) continue;
                     if (nsh[tier])
                         VD_TRY(run_cells(E, tier, list[tier], nsh[tier], nullptr, true, 1 + tier,
-                                         side ? E.stream2 : nullptr));
+                                         side ? E.stream2 : nullptr,
+                                         lon[tier] ? flist[4 * tier].as<vd::ShellCode>() : nullptr,
+                                         lon[tier] ? flist[4 * tier + 1].as<uint64_t>() : nullptr,
+                                         lon[tier] ? flist[4 * tier + 2].as<uint32_t>() : nullptr,
+                                         lon[tier] ? flist[4 * tier + 3].as<double>() : nullptr));
                     if (nlist[tier] > nsh[tier])
                         VD_TRY(run_cells(E, tier, list[tier] + nsh[tier], nlist[tier] - nsh[tier], nullptr, false,
                                          1 + NT + tier));
@@ -1079,6 +1209,7 @@ int engine_calculate(Engine &E) {
                 nsh[tier] = nnsh[tier];
             }
             for (DevBuf &b : out) release(b);
+            for (DevBuf &b : flist) release(b);
         }
         for (DevBuf &b : L) release(b);
         if (!arena_full) break;
@@ -1648,6 +1779,11 @@ int vd_tess_set_option(vd_tess *t, int key, int64_t value) {
         if (value < 1) return fail(VD_EINVAL, "at least one arena vertex per cell");
         t->E.arena_hint = (uint64_t)value;
         t->E.cap_cells = 0;   // resize at the next calculate
+        return VD_OK;
+    }
+    if (key == VD_OPT_FAR_LIST) {
+        if (value < 0) return fail(VD_EINVAL, "walk length must not be negative");
+        t->E.far_list_min = (double)value;
         return VD_OK;
     }
     return fail(VD_EINVAL, "unknown option");

@@ -1062,16 +1062,19 @@ __device__ inline int probe_inert(const ShellCode *shell, uint32_t nshell, const
 // fast tiers only in the instantiation that reruns cells whose walk ran past the shell prefix,
 // because the probe's registers slow the common walk down.
 template <class C, bool SEAL, bool PROBE>
-__device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, ArenaChunks &ac) {
+__device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t q, ArenaChunks &ac) {
     const int l = lane_id_here();
     const TessDesc &D = A.desc[A.gen_tess[i]];   // this cell's tessellation (wave-uniform)
     const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
     const uint32_t li = (uint32_t)(i - D.gen_off);   // index within its tessellation
     const GridGeom &G = D.grid;
-    const ShellCode *shell = D.shell;
+    // a per-cell list (k_far_lists) replaces the member's prefix for cells that reach across
+    // empty space: the same candidates in the same order
+    const bool own = A.lshell && A.lshell_n[q] != kNoList;
+    const ShellCode *shell = own ? A.lshell + A.lshell_off[q] : D.shell;
     const ShellPack pk = D.pack;
-    const uint32_t nshell = D.nshell;
-    const double shell_next = D.shell_next_d2;
+    const uint32_t nshell = own ? A.lshell_n[q] : D.nshell;
+    const double shell_next = own ? A.lshell_next[q] : D.shell_next_d2;
     const uint32_t *bin_start = A.bin_start + D.bin_off;
     const int nwalls = D.nwalls;
     Poly P;
@@ -1430,7 +1433,7 @@ void k_cells(CellArgs A) {
         if (lane_id() == 0) q = atomicAdd(A.queue, 1u);
         q = first_lane_u32(q);
         if (q >= A.ncells) break;
-        process_cell<C, SEAL, PROBE>(m, A, A.cells[q], ac);
+        process_cell<C, SEAL, PROBE>(m, A, A.cells[q], q, ac);
     }
     // entries the wave's cells occupy (the chunks' unused tails excluded)
     if (lane_id() == 0)
@@ -1449,7 +1452,127 @@ using Tier1 = Cap<128, 384, 64, 64, true, 111, 4>;
 using Tier2 = Cap<256, 768, 128, 384, false, 255, 1>;
 using Tier3 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
 
+// Work-list entries whose cell stopped at the end of its shell prefix and whose walk covers about
+// min_entries bins or more: 4/3 pi (2r)^3 over the bin volume (a size estimate only; the lists
+// change no result). Their limit: the first prefix entry with key > L = 4 r^2 (1 + kListMargin)
+// (keys do not decrease), and the key their list ends at. A walk whose limit reaches L at the end
+// of its list reports it ran past it, as at the end of a prefix, and reruns on a longer list.
+__global__ void k_far_pick(const uint32_t *list, uint32_t n, const int32_t *status, const double *vol,
+                           const uint32_t *gen_tess, const TessDesc *desc, double min_entries, uint32_t *flags,
+                           uint32_t *limit, double *next) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t i = list[q];
+    uint32_t f = 0u, lim = 0u;
+    double nk = 0.0;
+    if (status[i] == kCellShell) {
+        const TessDesc &D = desc[gen_tess[i]];
+        const GridGeom &G = D.grid;
+        const double r2 = vol[i];
+        f = 33.510321638291124 * r2 * sqrt(r2) >= min_entries * G.cs[0] * G.cs[1] * G.cs[2];
+        if (f) {
+            const double l4 = 4.0 * r2 * (1.0 + kListMargin);
+            uint32_t lo = 0u, hi = D.nshell;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2u;
+                int dx, dy, dz;
+                shell_unpack(D.shell[mid], D.pack, dx, dy, dz);
+                if (shell_key(dx, dy, dz, G.cs) > l4) hi = mid;
+                else lo = mid + 1u;
+            }
+            lim = lo;
+            nk = lo < D.nshell ? l4 : D.shell_next_d2;
+        }
+    }
+    flags[q] = f;
+    limit[q] = lim;
+    next[q] = nk;
+}
+
+// Per-cell shell lists: block b takes kListChunk entries of one selected cell, 4 consecutive
+// entries per thread, and keeps those below the cell's limit whose bin lies in the grid and holds
+// a generator. An empty or out-of-grid entry gives the walk no candidate and keys do not decrease
+// along the prefix, so a walk over the kept entries visits the same candidates in the same order
+// and ends where the walk over the prefix ends: at the first kept entry past its limit, or at the
+// end of the list, whose next key (L for a list cut below the prefix's end, else the prefix's next
+// key) decides as at the end of a prefix whether the list was long enough.
+__global__ __launch_bounds__(256) void k_far_lists(const uint32_t *cells, const uint32_t *limit,
+                                                   const uint32_t *cp, uint32_t nsel, const TessDesc *desc,
+                                                   const uint32_t *gen_tess, const double *gen,
+                                                   const uint32_t *bin_start, uint32_t *counts,
+                                                   const uint32_t *offs, ShellCode *out) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0u, hi = nsel;   // the selected cell s with cp[s] <= b < cp[s + 1]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (cp[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t s = lo, chunk = b - cp[s], end = limit[s];
+    const int t = (int)threadIdx.x, w = t >> 6, l = t & 63;
+    const uint32_t i = cells[s];
+    const TessDesc &D = desc[gen_tess[i]];
+    const GridGeom &G = D.grid;
+    const double gx = gen[3 * (size_t)i], gy = gen[3 * (size_t)i + 1], gz = gen[3 * (size_t)i + 2];
+    const int ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
+    const int iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
+    const int iz = (int)axis_bin(gz, G.lo[2], G.s[2], G.lim[2]);
+    const uint32_t *bs = bin_start + D.bin_off;
+    ShellCode code[4];
+    uint32_t keep = 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t e = chunk * kListChunk + 4u * (uint32_t)t + (uint32_t)j;
+        code[j] = 0u;
+        if (e < end) {
+            code[j] = D.shell[e];
+            int dx, dy, dz;
+            shell_unpack(code[j], D.pack, dx, dy, dz);
+            const int bx = ix + dx, by = iy + dy, bz = iz + dz;
+            if (bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2]) {
+                const uint32_t bb = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] +
+                                    (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
+                if (bs[bb + 1] > bs[bb]) keep |= 1u << j;
+            }
+        }
+    }
+    const uint32_t cnt = (uint32_t)__popc(keep);
+    const uint32_t inc = wave_incl_scan(cnt);
+    if (l == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = inc - cnt;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    if (!out) {
+        if (t == 0) counts[b] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        return;
+    }
+    uint64_t o = (uint64_t)offs[b] + before;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if ((keep >> j) & 1u) out[o++] = code[j];
+}
+
 }  // namespace
+
+hipError_t launch_far_pick(const uint32_t *list, uint32_t n, const int32_t *status, const double *vol,
+                           const uint32_t *gen_tess, const TessDesc *desc, double min_entries,
+                           uint32_t *flags, uint32_t *limit, double *next, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_far_pick<<<(n + 255) / 256, 256, 0, st>>>(list, n, status, vol, gen_tess, desc, min_entries, flags, limit,
+                                                next);
+    return hipGetLastError();
+}
+
+hipError_t launch_far_lists(const uint32_t *cells, const uint32_t *limit, const uint32_t *cp, uint32_t nsel,
+                            uint32_t blocks, const TessDesc *desc, const uint32_t *gen_tess, const double *gen,
+                            const uint32_t *bin_start, uint32_t *counts, const uint32_t *offs, ShellCode *out,
+                            hipStream_t st) {
+    if (nsel == 0 || blocks == 0) return hipSuccess;
+    static_assert(kListChunk == 4 * 256, "one block covers a chunk, 4 entries per thread");
+    k_far_lists<<<blocks, 256, 0, st>>>(cells, limit, cp, nsel, desc, gen_tess, gen, bin_start, counts, offs, out);
+    return hipGetLastError();
+}
 
 int tier_count() { return 4; }
 

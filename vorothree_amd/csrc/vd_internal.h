@@ -98,6 +98,12 @@ struct CellArgs {
     const uint32_t *cells;
     uint32_t ncells;
     uint32_t *queue;              // atomic work counter (zeroed before launch)
+    // per-cell shell lists (optional, nullptr: none): work-list entry q walks the lshell_n[q]
+    // codes from lshell[lshell_off[q]] instead of its member's prefix, unless lshell_n[q] is kNoList
+    const ShellCode *lshell;
+    const uint64_t *lshell_off;
+    const uint32_t *lshell_n;
+    const double *lshell_next;    // key after each list's last entry (as TessDesc::shell_next_d2)
 
     // reduced outputs, indexed by global generator index
     double *vol;
@@ -133,6 +139,27 @@ enum : int32_t {
     kCellArena = 3,        // output arena full: grow and retry
     kCellFace255 = -1,     // a face reached 256 vertices (the reference's u8 count would wrap)
 };
+
+// ---- per-cell shell lists for cells that reach across empty space (vd_cells.hip) -------------
+// launch_far_pick: flags[q] = 1 for work-list entries whose cell stopped at the end of its shell
+// prefix (status kCellShell, its r^2 in vol) and whose walk covers about min_entries bins or more;
+// limit[q] = the number of prefix entries with key <= L = 4 r^2 (1 + kListMargin), the ones its
+// rerun can visit (its radius is r at the stop and does not grow beyond rounding), and next[q] the
+// key its list ends at: L when that cuts the prefix, else the prefix's own next key.
+// launch_far_lists: block b takes chunk b - cp[s] of kListChunk entries of selected cell s
+// (cp[s] <= b < cp[s + 1]; entries below limit[s]) and counts (out == nullptr) or writes, from
+// offs[b] (the exclusive scan of the counts), the codes of the entries whose bin lies in the grid
+// and holds a generator, in prefix order.
+constexpr uint32_t kNoList = 0xffffffffu;
+constexpr uint32_t kListChunk = 1024;
+constexpr double kListMargin = 1e-9;
+hipError_t launch_far_pick(const uint32_t *list, uint32_t n, const int32_t *status, const double *vol,
+                           const uint32_t *gen_tess, const TessDesc *desc, double min_entries,
+                           uint32_t *flags, uint32_t *limit, double *next, hipStream_t st);
+hipError_t launch_far_lists(const uint32_t *cells, const uint32_t *limit, const uint32_t *cp, uint32_t nsel,
+                            uint32_t blocks, const TessDesc *desc, const uint32_t *gen_tess, const double *gen,
+                            const uint32_t *bin_start, uint32_t *counts, const uint32_t *offs, ShellCode *out,
+                            hipStream_t st);
 
 // ---- generator pipeline (vd_grid.hip) ------------------------------------------------------
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);

@@ -505,6 +505,11 @@ class Tessellation:
         """Initial full-cell arena per cell (VD_OPT_ARENA); a calculate that fills it grows it."""
         check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_ARENA, int(vertices_per_cell)))
 
+    def set_far_list(self, min_entries: int) -> None:
+        """Walk length (shell entries) from which cells reaching across empty space rerun over a
+        list of the occupied entries only (VD_OPT_FAR_LIST; 0: never). Results are identical."""
+        check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_FAR_LIST, int(min_entries)))
+
     # -- queries -------------------------------------------------------------------------------
     def count_generators(self) -> int:
         return int(lib().vd_tess_count_generators(self._h))
