@@ -1069,8 +1069,9 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
     const uint32_t li = (uint32_t)(i - D.gen_off);   // index within its tessellation
     const GridGeom &G = D.grid;
     // a per-cell list (k_far_lists) replaces the member's prefix for cells that reach across
-    // empty space: the same candidates in the same order
-    const bool own = A.lshell && A.lshell_n[q] != kNoList;
+    // empty space: the same candidates in the same order. Only reruns of such cells have lists,
+    // and they run in the skip-ahead instantiation.
+    const bool own = PROBE && A.lshell && A.lshell_n[q] != kNoList;
     const ShellCode *shell = own ? A.lshell + A.lshell_off[q] : D.shell;
     const ShellPack pk = D.pack;
     const uint32_t nshell = own ? A.lshell_n[q] : D.nshell;
