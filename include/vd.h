@@ -235,6 +235,7 @@ int vd_batch_get_cell(const vd_batch *b, uint32_t t, size_t i, uint64_t sizes[3]
 #define VD_BOPT_KEEP_CELLS 1 /* as VD_OPT_KEEP_CELLS; default 0 for batches */
 #define VD_BOPT_PER_BIN 2    /* generators-per-bin target of vd_batch_add's grid choice, x1000 (default 350) */
 #define VD_BOPT_GRAPHS 3     /* as VD_OPT_GRAPHS (default 1) */
+#define VD_BOPT_FAR_LIST 4   /* as VD_OPT_FAR_LIST (default 262144) */
 int vd_batch_set_option(vd_batch *b, int key, int64_t value);
 int vd_batch_get_stats(const vd_batch *b, double out[10]);
 /* Times vd_batch_calculate for each candidate generators-per-bin target (grids re-derived with

@@ -149,3 +149,35 @@ def test_sparse_member_extends_only_its_prefix():
         off, ids = b.neighbors(t)
         for i in range(0, len(vol), 11):
             assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), (t, i)
+
+
+@pytest.mark.parametrize("far_list", [1, 0])
+def test_far_lists_in_batch_members(far_list):
+    # members with cells next to empty space, behind a uniform member (their bins, generators and
+    # outputs start at non-zero offsets): their far cells rerun over per-cell lists (forced) or
+    # over their prefixes (off), and every member equals the oracle bit for bit
+    uni = W.Workload("uniform", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (14, 14, 14),
+                     W.PCG32(81).uniform(2500, (0, 0, 0), (1, 1, 1)))
+    voids = [W.Workload(f"void-{k}", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (30, 30, 30),
+                        W.clustered_points(W.PCG32(82 + k), 9000, 8, 0.04, 1.0)) for k in range(2)]
+    sparse = W.Workload("sparse", (0.0, 0.0, 0.0), (2.0, 1.0, 1.5), (40, 20, 30),
+                        W.PCG32(85).uniform(60, (0, 0, 0), (2.0, 1.0, 1.5)))
+    ws = [uni, voids[0], sparse, voids[1]]
+    b = Batch(keep_cells=True, far_list=far_list)
+    for w in ws:
+        b.add(BoundingBox(w.lo, w.hi), w.grid, walls_of(w), w.points)
+    b.calculate()
+    assert b.stats()[2] > 0, "no cell reached across empty space"
+    for t, w in enumerate(ws):
+        o = oracle_of(w, w.grid)
+        o.calculate(threads=8)
+        vol, cen, nf, _ = o.reduced()
+        assert np.array_equal(bits(b.volumes(t)), bits(vol)), t
+        assert np.array_equal(bits(b.centroids(t)), bits(cen)), t
+        assert np.array_equal(b.face_counts(t), nf), t
+        off, ids = b.neighbors(t)
+        for i in range(0, len(vol), 13):
+            assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(i).face_neighbors), (t, i)
+        for i in range(0, len(vol), 211):
+            assert_same_cell(b.get_cell(t, i), o.get_cell(i), (t, i))
+    b.close()

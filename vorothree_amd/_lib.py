@@ -24,6 +24,7 @@ VD_OPT_FAR_LIST = 4
 VD_BOPT_KEEP_CELLS = 1
 VD_BOPT_PER_BIN = 2
 VD_BOPT_GRAPHS = 3
+VD_BOPT_FAR_LIST = 4
 VD_SOPT_PER_BIN = 1
 VD_SOPT_SMALL_JOB = 2
 

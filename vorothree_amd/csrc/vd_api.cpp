@@ -1967,6 +1967,11 @@ int vd_batch_set_option(vd_batch *b, int key, int64_t value) {
         b->E.graphs = value ? 1 : 0;
         return VD_OK;
     }
+    if (key == VD_BOPT_FAR_LIST) {
+        if (value < 0) return fail(VD_EINVAL, "walk length must not be negative");
+        b->E.far_list_min = (double)value;
+        return VD_OK;
+    }
     return fail(VD_EINVAL, "unknown option");
 }
 

@@ -630,7 +630,8 @@ class Batch:
     ``calculate()`` runs every member; results are read per member.
     """
 
-    def __init__(self, keep_cells: bool = False, per_bin: float = 0.35, graphs: bool = True):
+    def __init__(self, keep_cells: bool = False, per_bin: float = 0.35, graphs: bool = True,
+                 far_list: int | None = None):
         h = ctypes.c_void_p()
         check(lib().vd_batch_create(ctypes.byref(h)))
         self._h = h
@@ -639,6 +640,8 @@ class Batch:
         if not graphs:
             check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_GRAPHS, 0))
         check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_PER_BIN, int(round(per_bin * 1000))))
+        if far_list is not None:   # VD_BOPT_FAR_LIST: as Tessellation.set_far_list
+            check(lib().vd_batch_set_option(self._h, _lib.VD_BOPT_FAR_LIST, int(far_list)))
 
     def close(self):
         h = getattr(self, "_h", None)
