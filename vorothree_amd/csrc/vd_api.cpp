@@ -396,8 +396,9 @@ struct Engine {
     int seal = 0;                      // cell launches run the seal mode
     bool trunc0 = false;               // the first launch sees only the first prefix of far geometries
     int reserve_cus = 0;               // CUs the persistent cell launches leave to other engines
-    // far-cell reruns on a side stream beside the rest of their round (off in the server's lanes:
-    // with two lanes sharing the GPU the extra overlap cost more than it gained, measured)
+    // far-cell reruns on a side stream beside the rest of their round, and over per-cell lists
+    // (both off in the server's lanes: with two lanes sharing the GPU each cost more than it gained
+    // on the served mix, measured)
     bool side_streams = true;
     double far_list_min = kFarListMinDefault;   // walk length for a per-cell list (0: never)
     uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
@@ -2016,6 +2017,7 @@ int vd_server_create(size_t hbm_budget, int max_slots, vd_server **out) {
     for (Lane &L : S->lanes) {
         L.E.keep_cells = 0;
         L.E.side_streams = false;
+        L.E.far_list_min = 0.0;   // measured: the served mix ran faster without the lists
         if (!rc) rc = engine_init(L.E);
     }
     if (rc) {
