@@ -1109,8 +1109,6 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
     int noc = 0;            // consecutive candidate clips without a change
     uint64_t wcm = ~0ull;   // window lanes whose candidate may cut (pre-test), all without it
 
-    bool walking = false;
-
     // walls (tessellation.rs:429-436): every plane each wall's cut emits, in order; the cell is
     // done once a wall has left it empty
     uint32_t tk = 0;        // tube walls seen so far (their planes come from a pre-pass)
@@ -1139,23 +1137,23 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
         if (status != kCellOk || P.V == 0) break;
     }
 
-    for (;;) {
+    // the neighbour walk, for a cell the walls left non-empty (tessellation.rs:438-472)
+    const bool walk = P.V != 0 && status == kCellOk;
+    if (walk) {
+        r2 = poly_max_radius_sq<C>(m, P, gx, gy, gz);   // tessellation.rs:438
+        ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
+        iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
+        iz = (int)axis_bin(gz, G.lo[2], G.s[2], G.lim[2]);
+        rx = (gx - G.lo[0]) * G.s[0] - (double)ix;
+        ry = (gy - G.lo[1]) * G.s[1] - (double)iy;
+        rz = (gz - G.lo[2]) * G.s[2] - (double)iz;
+    }
+    while (walk) {
         Plane pl;
         int32_t nid = 0;
         bool withgen = false;
         int cl = -1;
         {
-            if (!walking) {
-                if (P.V == 0 || status != kCellOk) break;
-                r2 = poly_max_radius_sq<C>(m, P, gx, gy, gz);   // tessellation.rs:438
-                ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
-                iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
-                iz = (int)axis_bin(gz, G.lo[2], G.s[2], G.lim[2]);
-                rx = (gx - G.lo[0]) * G.s[0] - (double)ix;
-                ry = (gy - G.lo[1]) * G.s[1] - (double)iy;
-                rz = (gz - G.lo[2]) * G.s[2] - (double)iz;
-                walking = true;
-            }
             if (!win) {
                 if (load) {
                     if ((!C::FAST || PROBE) && inert >= 2) {
