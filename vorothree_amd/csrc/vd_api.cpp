@@ -75,6 +75,14 @@ void release(DevBuf &b) {
     b.bytes = 0;
 }
 
+// Releases n buffers when the scope ends, on every return path (release is idempotent, so an
+// explicit release before the end is fine).
+struct ScopedBufs {
+    DevBuf *b;
+    size_t n;
+    ~ScopedBufs() { for (size_t k = 0; k < n; ++k) release(b[k]); }
+};
+
 int ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return VD_OK;
@@ -829,11 +837,10 @@ int far_lists(Engine &E, const uint32_t *d_list, uint32_t n, DevBuf &lsh, DevBuf
               bool &use) {
     use = false;
     if (E.far_list_min <= 0.0 || n == 0) return VD_OK;
-    DevBuf flags, lim, sel, slim, cpd, counts, offs, gidx, at;
-    auto done = [&]() {
-        DevBuf *tmp[] = {&flags, &lim, &sel, &slim, &cpd, &counts, &offs, &gidx, &at};
-        for (DevBuf *b : tmp) release(*b);
-    };
+    DevBuf tmp[9];
+    const ScopedBufs keep_tmp{tmp, 9};   // released on every return (after the last sync)
+    DevBuf &flags = tmp[0], &lim = tmp[1], &sel = tmp[2], &slim = tmp[3], &cpd = tmp[4], &counts = tmp[5],
+           &offs = tmp[6], &gidx = tmp[7], &at = tmp[8];
     VD_TRY(ensure(flags, (size_t)n * 4));
     VD_TRY(ensure(lim, (size_t)n * 4));
     VD_TRY(ensure(lnx, (size_t)n * 8));
@@ -868,7 +875,7 @@ int far_lists(Engine &E, const uint32_t *d_list, uint32_t n, DevBuf &lsh, DevBuf
         blocks += c;
         cp.push_back((uint32_t)blocks);
     }
-    if (sq.empty()) { done(); return VD_OK; }
+    if (sq.empty()) return VD_OK;
     const uint32_t nsel = (uint32_t)sq.size(), nb = (uint32_t)blocks;
     VD_TRY(ensure(sel, (size_t)nsel * 4));
     VD_TRY(ensure(slim, (size_t)nsel * 4));
@@ -899,7 +906,7 @@ int far_lists(Engine &E, const uint32_t *d_list, uint32_t n, DevBuf &lsh, DevBuf
     // the cells (in the order above) whose lists fit in kFarListEntries: a prefix of the blocks
     uint32_t keep = nsel;
     while (keep > 0 && (keep < nsel ? (uint64_t)start[keep] : all) > kFarListEntries) --keep;
-    if (keep == 0) { done(); return VD_OK; }
+    if (keep == 0) return VD_OK;
     const uint64_t total = keep < nsel ? (uint64_t)start[keep] : all;
     // one spare entry: a walk's bulk skip reads entry 0 of its list even when the list is empty
     VD_TRY(ensure(lsh, (size_t)(total + 1) * sizeof(vd::ShellCode)));
@@ -918,7 +925,6 @@ int far_lists(Engine &E, const uint32_t *d_list, uint32_t n, DevBuf &lsh, DevBuf
     VD_HIP(hipMemcpyAsync(lof.p, hoff.data(), (size_t)n * 8, hipMemcpyHostToDevice, E.stream));
     VD_HIP(hipMemcpyAsync(lnn.p, hn.data(), (size_t)n * 4, hipMemcpyHostToDevice, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
-    done();
     use = true;
     return VD_OK;
 }
@@ -1038,6 +1044,7 @@ int engine_calculate(Engine &E) {
         // those cells first (group 1, nsh of them), then the others (group 0).
         constexpr int NT = kTiers;
         DevBuf L[NT];
+        const ScopedBufs keep_l{L, NT};
         const uint32_t *list[NT] = {E.d_items.as<uint32_t>()};
         uint32_t nlist[NT] = {(uint32_t)n};
         uint32_t nsh[NT] = {};
@@ -1049,6 +1056,7 @@ int engine_calculate(Engine &E) {
             uint32_t counts[8 * NT] = {};
             DevBuf out[4 * NT];
             DevBuf flist[4 * NT];   // per-cell shell lists of the far cells of each tier (far_lists)
+            const ScopedBufs keep_out{out, 4 * NT}, keep_flist{flist, 4 * NT};
             bool lon[NT] = {};
             const uint32_t *ocap[2][NT] = {}, *oshell[2][NT] = {};
             auto cnt = [&](int g, int tier, int what) { return counts[4 * (NT * g + tier) + what]; };
@@ -1178,6 +1186,7 @@ int engine_calculate(Engine &E) {
             // k-1; group 0 = the capacity failures of group 0 of tier k-1
             auto far_tier = [](int tier) { return tier < kFarTier ? kFarTier : tier; };
             DevBuf next[NT];
+            const ScopedBufs keep_next{next, NT};
             uint32_t nnext[NT] = {}, nnsh[NT] = {};
             for (int tier = 0; tier < NT; ++tier) {
                 uint32_t parts[2 * NT + 2] = {};
