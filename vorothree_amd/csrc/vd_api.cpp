@@ -1627,6 +1627,7 @@ static int tess_add_wall(vd_tess *t, const vd_wall &w) {
     const uint64_t n = E.ngen;
     if (n == 0) return upload_desc(E);
     DevBuf keep;
+    const ScopedBufs keep_g{&keep, 1};
     VD_TRY(ensure(keep, n * 24));
     VD_HIP(hipMemcpyAsync(keep.p, E.d_gen.p, n * 24, hipMemcpyDeviceToDevice, E.stream));
     int rc = ensure(E.d_raw, (n + 1) * 24);
