@@ -172,6 +172,9 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
                                shell entries or more (default 262144; 0: never) rerun over a list
                                of only the entries whose bins hold generators, in shell order:
                                the same candidates in the same order, so identical results */
+#define VD_OPT_ATTEMPTS 5   /* launches, each with twice the arena of the one before, after which
+                               calculate() gives up with VD_ECAPACITY and keeps no results
+                               (default 8, at most 62) */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
 
 /* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cell runs

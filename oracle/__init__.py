@@ -47,6 +47,9 @@ def _load():
         "or_tess_get_bins": (None, [_vp, _vp, _vp, _vp]),
         "or_tess_bin_index": (ctypes.c_size_t, [_vp, _dp]),
         "or_tess_shell_order": (ctypes.c_size_t, [_vp, _vp, _vp, ctypes.c_size_t]),
+        "or_tess_shell_prefix": (ctypes.c_size_t, [_vp, ctypes.c_double, _vp, _vp, ctypes.c_size_t, _vp]),
+        "or_tess_shell_mode": (ctypes.c_size_t, [_vp, ctypes.c_int]),
+        "or_tess_shell_len": (ctypes.c_size_t, [_vp]),
         "or_tess_calculate": (None, [_vp, _vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]),
         "or_tess_count_cells": (ctypes.c_size_t, [_vp]),
         "or_tess_relax": (None, [_vp]),
@@ -57,6 +60,7 @@ def _load():
         "or_cell_sizes": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp]),
         "or_cell_get": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]),
         "or_tess_reduced": (None, [_vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp]),
+        "or_tess_neighbors": (None, [_vp, _vp, ctypes.c_size_t, _vp, _vp, ctypes.c_size_t]),
         "or_cell_new": (_vp, [_dp, _dp]),
         "or_cell_free": (None, [_vp]),
         "or_cell_clip": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int32, _vp, _dp]),
@@ -205,6 +209,24 @@ class OracleTess:
         self._lib.or_tess_shell_order(self._h, _ptr(dxyz), _ptr(d2), m)
         return dxyz, d2
 
+    def shell_prefix(self, thr):
+        """The stable shell order restricted to keys <= thr, built the way the walks extend it:
+        (dxyz, keys, key of the first entry left out)."""
+        nxt = np.zeros(1, dtype=np.float64)
+        total = int(self._lib.or_tess_shell_prefix(self._h, float(thr), None, None, 0, _ptr(nxt)))
+        dxyz = np.empty((total, 3), dtype=np.int32)
+        d2 = np.empty(total, dtype=np.float64)
+        self._lib.or_tess_shell_prefix(self._h, float(thr), _ptr(dxyz), _ptr(d2), total, _ptr(nxt))
+        return dxyz, d2, float(nxt[0])
+
+    def shell_mode(self, full: bool) -> int:
+        """Walks start from the whole table (True) or from the lazily grown prefix (False)."""
+        return int(self._lib.or_tess_shell_mode(self._h, int(bool(full))))
+
+    def shell_len(self) -> int:
+        """Length of the prefix the walks use now."""
+        return int(self._lib.or_tess_shell_len(self._h))
+
     def calculate(self, subset=None, threads=1, brute=False):
         if subset is None:
             self._lib.or_tess_calculate(self._h, None, 0, int(threads), int(bool(brute)))
@@ -248,6 +270,20 @@ class OracleTess:
         work = np.empty((m, 9), dtype=np.uint64)
         self._lib.or_tess_reduced(self._h, ip, m, _ptr(vol), _ptr(cen), _ptr(nf), _ptr(work))
         return vol, cen, nf, work
+
+
+    def neighbors(self, idx=None):
+        """Neighbour ids of the listed cells (or all) in face order, as CSR (offsets, ids)."""
+        if idx is None:
+            m, ip = self.count_cells(), None
+        else:
+            idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+            m, ip = idx.size, _ptr(idx)
+        off = np.empty(m + 1, dtype=np.uint64)
+        self._lib.or_tess_neighbors(self._h, ip, m, _ptr(off), None, 0)
+        ids = np.empty(int(off[-1]), dtype=np.int32)
+        self._lib.or_tess_neighbors(self._h, ip, m, _ptr(off), _ptr(ids), ids.size)
+        return off, ids
 
 
 class OracleStandaloneCell:

@@ -12,6 +12,9 @@
 // One documented deviation from the reference: the bin shell order is sorted STABLY by its
 // min-distance key (ties keep the z-outer / y / x-inner generation order). The reference uses
 // sort_unstable_by (src/algorithm/algo_3d_grid.rs:76), whose tie order is that of Rust's ipnsort.
+// The order is built as a prefix that grows as the walks need it (the whole table of a 474^3 grid
+// would hold 8.6e8 entries); walks over it visit exactly the entries a walk over the whole table
+// visits.
 //
 // Pinned by: the Voro++ fixtures (vertex counts and positions, face counts), closed forms (box,
 // lattices, plane-clipped boxes), a brute-force all-pairs clipping on small inputs, topology
@@ -25,6 +28,8 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
+#include <mutex>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -477,40 +482,95 @@ void wall_cut(const Wall &w, const double g[3], Emit emit, Work *wk) {
 // ---------------------------------------------------------------------------------------------
 struct ShellEntry { int32_t dx, dy, dz; double d2; };
 
+// A prefix of the stable shell order: every entry whose key is <= thr, in order, and the key of
+// the first entry after them (+inf when the prefix is the whole table). The whole table has
+// (2n+1)^3 entries (8.6e8 at 474^3), so only the prefix the walks reach is built: a walk that
+// reaches the end of the prefix while the next key is still within its limit asks for a longer
+// one. Keys do not decrease along the stable order, so every prefix is the start of the table
+// and a longer one starts with the same entries. A built prefix is immutable; calculate()'s
+// threads share it and swap in the longer one under the grid's lock.
+struct ShellPrefix {
+    std::vector<ShellEntry> e;
+    double thr = 0.0;
+    double next = std::numeric_limits<double>::infinity();
+};
+
 struct Grid {
     int64_t n[3];
-    double s[3], lim[3], lo[3];
+    double s[3], lim[3], lo[3], cs[3];
     std::vector<uint32_t> bin_start;  // CSR of the bin lists (each in ascending generator index)
     std::vector<uint32_t> bin_items;
     std::vector<uint32_t> gen_bin;
-    std::vector<ShellEntry> order;
+    mutable std::shared_ptr<const ShellPrefix> pre;   // guarded by mu
+    std::unique_ptr<std::mutex> mu{new std::mutex};
 };
 
-void grid_new(Grid &G, int64_t nx, int64_t ny, int64_t nz, const double lo[3], const double hi[3]) {
-    G.n[0] = nx; G.n[1] = ny; G.n[2] = nz;
-    double cs[3];
-    for (int k = 0; k < 3; ++k) {
-        G.s[k] = (double)G.n[k] / (hi[k] - lo[k]);   // :48-50
-        cs[k] = 1.0 / G.s[k];                        // :52-54
-        G.lim[k] = (double)G.n[k] - 1e-5;            // :85-87
-        G.lo[k] = lo[k];
-    }
-    auto axis = [](int64_t d, double c) -> double {  // :61-66
+// Key of shell offset (dx, dy, dz), algo_3d_grid.rs:61-66,72: the squared distance between the
+// nearest faces of two bins that far apart (0 along an axis with d = 0).
+inline double shell_key(const double cs[3], int64_t x, int64_t y, int64_t z) {
+    auto axis = [](int64_t d, double c) -> double {
         if (d > 0) return (double)(d - 1) * c;
         if (d < 0) return (double)(-d - 1) * c;
         return 0.0;
     };
-    G.order.clear();
-    G.order.reserve((size_t)(2 * nx + 1) * (2 * ny + 1) * (2 * nz + 1));
-    for (int64_t z = -nz; z <= nz; ++z)        // :68-75, z outer, x inner
-        for (int64_t y = -ny; y <= ny; ++y)
-            for (int64_t x = -nx; x <= nx; ++x) {
-                double mx = axis(x, cs[0]), my = axis(y, cs[1]), mz = axis(z, cs[2]);
-                G.order.push_back({(int32_t)x, (int32_t)y, (int32_t)z, mx * mx + my * my + mz * mz});
+    const double mx = axis(x, cs[0]), my = axis(y, cs[1]), mz = axis(z, cs[2]);
+    return mx * mx + my * my + mz * mz;
+}
+
+// The stable shell order restricted to keys <= thr (thr = +inf: the whole table), :61-76 with a
+// stable sort. Offsets with |d_a| > m_a, m_a = floor(sqrt(thr) / cs_a) + 3, have an axis term of
+// at least ((floor + 3) cs_a)^2 > thr, so the box |d_a| <= m_a holds every kept entry; it also
+// holds the axis entry d_a = m_a, whose key is no larger than any key outside the box, so the
+// smallest key > thr inside the box is the next key of the whole table. The box is enumerated
+// z-outer, y, x-inner like the table, so the stable sort keeps the table's tie order.
+std::shared_ptr<const ShellPrefix> shell_build(const Grid &G, double thr) {
+    int64_t m[3];
+    for (int k = 0; k < 3; ++k) {
+        const double q = std::sqrt(thr) / G.cs[k];
+        m[k] = !(q < (double)G.n[k]) ? G.n[k] : std::min<int64_t>(G.n[k], (int64_t)q + 3);
+    }
+    auto P = std::make_shared<ShellPrefix>();
+    P->thr = thr;
+    P->e.reserve((size_t)(2 * m[0] + 1) * (2 * m[1] + 1) * (2 * m[2] + 1));
+    for (int64_t z = -m[2]; z <= m[2]; ++z)
+        for (int64_t y = -m[1]; y <= m[1]; ++y)
+            for (int64_t x = -m[0]; x <= m[0]; ++x) {
+                const double d2 = shell_key(G.cs, x, y, z);
+                if (d2 <= thr) P->e.push_back({(int32_t)x, (int32_t)y, (int32_t)z, d2});
+                else if (d2 < P->next) P->next = d2;
             }
     // :76 sorts unstably; this restatement sorts stably (ties keep generation order).
-    std::stable_sort(G.order.begin(), G.order.end(),
+    std::stable_sort(P->e.begin(), P->e.end(),
                      [](const ShellEntry &a, const ShellEntry &b) { return a.d2 < b.d2; });
+    return P;
+}
+
+std::shared_ptr<const ShellPrefix> grid_prefix(const Grid &G) {
+    std::lock_guard<std::mutex> lk(*G.mu);
+    return G.pre;
+}
+
+// A prefix holding every entry with key <= need (at least four times the old threshold, so a
+// cell's walk extends it a few times at most).
+std::shared_ptr<const ShellPrefix> grid_extend(const Grid &G, double need) {
+    std::lock_guard<std::mutex> lk(*G.mu);
+    if (G.pre->thr >= need || G.pre->next == std::numeric_limits<double>::infinity()) return G.pre;
+    G.pre = shell_build(G, std::max(need, 4.0 * G.pre->thr));
+    return G.pre;
+}
+
+void grid_new(Grid &G, int64_t nx, int64_t ny, int64_t nz, const double lo[3], const double hi[3]) {
+    G.n[0] = nx; G.n[1] = ny; G.n[2] = nz;
+    double csmax = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        G.s[k] = (double)G.n[k] / (hi[k] - lo[k]);   // :48-50
+        G.cs[k] = 1.0 / G.s[k];                      // :52-54
+        G.lim[k] = (double)G.n[k] - 1e-5;            // :85-87
+        G.lo[k] = lo[k];
+        csmax = std::max(csmax, G.cs[k]);
+    }
+    // the first prefix: keys up to (4 * the largest bin edge)^2, extended as walks need
+    G.pre = shell_build(G, 16.0 * csmax * csmax);
 }
 
 inline size_t axis_index(double x, double lo, double s, double lim) {  // :102-104
@@ -560,7 +620,18 @@ void grid_visit(const Grid &G, const std::vector<double> &gen, size_t index, con
         if (d < 0) return ((double)(-(d + 1)) + rel) * c;
         return 0.0;
     };
-    for (const ShellEntry &e : G.order) {
+    std::shared_ptr<const ShellPrefix> pre = grid_prefix(G);
+    for (size_t k = 0;;) {
+        if (k == pre->e.size()) {
+            // end of the prefix: the table's next entry ends the walk, or a longer prefix is needed
+            if (pre->next > 4.0 * r2) {
+                if (w && pre->next != std::numeric_limits<double>::infinity()) w->entries++;
+                break;
+            }
+            pre = grid_extend(G, 4.0 * r2);
+            continue;
+        }
+        const ShellEntry &e = pre->e[k++];
         if (w) w->entries++;
         if (e.d2 > 4.0 * r2) break;  // :170
         const int64_t bx = ix + e.dx, by = iy + e.dy, bz = iz + e.dz;
@@ -789,17 +860,44 @@ void or_tess_get_bins(void *t, uint32_t *bin_start, uint32_t *bin_items, uint32_
 
 size_t or_tess_bin_index(void *t, const double q[3]) { return grid_bin_index(((Tess *)t)->grid, q); }
 
+// Stable shell order restricted to keys <= thr (thr = +inf: the whole table) as (dx, dy, dz,
+// min_d2) -> caller buffers of size cap; *next receives the key of the first entry left out.
+// Returns the prefix length.
+size_t or_tess_shell_prefix(void *t, double thr, int32_t *dxyz, double *d2, size_t cap, double *next) {
+    Tess &T = *(Tess *)t;
+    auto P = shell_build(T.grid, thr);
+    size_t m = std::min(cap, P->e.size());
+    for (size_t k = 0; k < m; ++k) {
+        dxyz[3 * k] = P->e[k].dx;
+        dxyz[3 * k + 1] = P->e[k].dy;
+        dxyz[3 * k + 2] = P->e[k].dz;
+        d2[k] = P->e[k].d2;
+    }
+    if (next) *next = P->next;
+    return P->e.size();
+}
+
 // Full stable shell order (dx, dy, dz, min_d2) -> caller buffers of size cap.
 size_t or_tess_shell_order(void *t, int32_t *dxyz, double *d2, size_t cap) {
+    return or_tess_shell_prefix(t, std::numeric_limits<double>::infinity(), dxyz, d2, cap, nullptr);
+}
+
+// Walks start from the whole table (full = 1) instead of the lazily extended prefix (full = 0),
+// the construction the tests compare the lazy one against. Returns the length of the prefix the
+// walks now start from.
+size_t or_tess_shell_mode(void *t, int full) {
     Tess &T = *(Tess *)t;
-    size_t m = std::min(cap, T.grid.order.size());
-    for (size_t k = 0; k < m; ++k) {
-        dxyz[3 * k] = T.grid.order[k].dx;
-        dxyz[3 * k + 1] = T.grid.order[k].dy;
-        dxyz[3 * k + 2] = T.grid.order[k].dz;
-        d2[k] = T.grid.order[k].d2;
-    }
-    return T.grid.order.size();
+    std::lock_guard<std::mutex> lk(*T.grid.mu);
+    double csmax = std::max(T.grid.cs[0], std::max(T.grid.cs[1], T.grid.cs[2]));
+    T.grid.pre = shell_build(T.grid, full ? std::numeric_limits<double>::infinity() : 16.0 * csmax * csmax);
+    return T.grid.pre->e.size();
+}
+
+// Length of the prefix the walks use now (it grows as walks need).
+size_t or_tess_shell_len(void *t) {
+    Tess &T = *(Tess *)t;
+    std::lock_guard<std::mutex> lk(*T.grid.mu);
+    return T.grid.pre->e.size();
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -970,6 +1068,22 @@ int or_cell_get(void *t, size_t i, double *v, uint32_t *fc, uint32_t *fi, int32_
     if (fi) for (size_t k = 0; k < c.fidx.size(); ++k) fi[k] = c.fidx[k];
     if (fn) std::memcpy(fn, c.fnb.data(), c.fnb.size() * 4);
     return 0;
+}
+
+// Neighbour ids of the listed cells (or all when idx == NULL) in face order, as CSR: off has
+// m + 1 entries; ids (room for cap) may be NULL to query the offsets only.
+void or_tess_neighbors(void *t, const int64_t *idx, size_t m, uint64_t *off, int32_t *ids, size_t cap) {
+    Tess &T = *(Tess *)t;
+    const size_t total = idx ? m : T.cells.size();
+    uint64_t o = 0;
+    for (size_t k = 0; k < total; ++k) {
+        const Poly &c = T.cells[idx ? (size_t)idx[k] : k];
+        off[k] = o;
+        if (ids)
+            for (size_t f = 0; f < c.fnb.size() && o + f < cap; ++f) ids[o + f] = c.fnb[f];
+        o += c.fnb.size();
+    }
+    off[total] = o;
 }
 
 // Per-cell reductions for the listed cells (or all when idx == NULL): volume, centroid, face

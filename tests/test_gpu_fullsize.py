@@ -1,0 +1,89 @@
+"""GPU parity at the full sizes of the 1M-uniform-sphere and 4M-clustered workloads, and at 300k
+points of the clustered distribution.
+
+Every cell's reductions (volume, centroid, face count) and neighbour ids (face order) are compared
+bit for bit with the oracle run over all cells; full cells (vertices, face arrays) are compared on
+a 20,000-cell sample. The clustered workload is 80% of the points from 64 Gaussian clusters
+(sigma 0.02) and 20% uniform, seed 3, on the grid the density rule picks for it
+(vd_grid_for_points at 0.35 generators per bin: 192^3 at 300k, 474^3 at 4M). The oracle grows
+its shell order lazily, as the library does, so grids of this size fit in memory.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import assert_same_cell, bits
+from oracle import OracleTess
+from vorothree_amd import (Algorithm3DGrid, BoundingBox, SphereGeometry, Tessellation, Wall,
+                           grid_for_points)
+from vorothree_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def run_both(w, grid, walls=()):
+    b = BoundingBox(w.lo, w.hi)
+    g = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=True)
+    o = OracleTess(w.lo, w.hi, *grid)
+    for wl in walls:
+        g.add_wall(Wall(wl[1], SphereGeometry(wl[2], wl[3])))
+        o.add_wall_sphere(wl[1], wl[2], wl[3])
+    g.set_generators(w.points)
+    o.set_generators(w.points)
+    g.calculate()
+    o.calculate(threads=THREADS)
+    return g, o
+
+
+def compare_all(g, o, sample=None):
+    """Bit-exact reductions and neighbour ids of every cell; full cells of a sample (or all)."""
+    n = o.count_cells()
+    assert g.count_cells() == n
+    assert np.array_equal(bits(g.generators()), bits(o.generators()))
+    vol, cen, nf, _ = o.reduced()
+    gv = g.volumes()
+    assert np.array_equal(bits(gv), bits(vol)), f"{np.count_nonzero(bits(gv) != bits(vol))} volumes differ"
+    assert np.array_equal(bits(g.centroids()), bits(cen))
+    assert np.array_equal(g.face_counts(), nf)
+    goff, gids = g.neighbors()
+    ooff, oids = o.neighbors()
+    assert np.array_equal(goff.astype(np.uint64), ooff) and np.array_equal(gids, oids)
+    idx = range(n) if sample is None else np.sort(np.random.default_rng(7).choice(n, sample, replace=False))
+    for i in idx:
+        assert_same_cell(g.get_cell(int(i)), o.get_cell(int(i)), int(i))
+    return gv, vol
+
+
+def test_full_config2_all_cells():
+    # the timed path of the headline workload: full cells kept, 524,199 cells
+    w = W.config2()
+    g, o = run_both(w, w.grid, w.walls)
+    gv, vol = compare_all(g, o, sample=20_000)
+    ball = 4.0 / 3.0 * np.pi * 0.125
+    assert gv.sum() == vol.sum()
+    assert gv.sum() >= ball and (gv.sum() - ball) / ball < 0.01
+
+
+def test_config3_distribution_300k_bit_exact():
+    w = W.config3(n=300_000)
+    grid = grid_for_points(BoundingBox(w.lo, w.hi), w.points, 0.35)
+    assert tuple(grid) == (192, 192, 192)
+    g, o = run_both(w, grid)
+    gv, vol = compare_all(g, o)
+    assert gv.sum() == vol.sum()
+    assert abs(gv.sum() - 1.0) < 1e-8
+
+
+def test_full_config3_all_cells():
+    w = W.config3()
+    grid = grid_for_points(BoundingBox(w.lo, w.hi), w.points, 0.35)
+    assert tuple(grid) == (474, 474, 474)
+    g, o = run_both(w, grid)
+    gv, vol = compare_all(g, o, sample=20_000)
+    # the fan triangulation is origin-relative (cell_3d_faces.rs:106-143): cluster cells of
+    # 1e-9 volume carry 1e-16 absolute rounding each, so the sum is 1 only to about 1e-7
+    assert gv.sum() == vol.sum()
+    assert abs(gv.sum() - 1.0) < 1e-7

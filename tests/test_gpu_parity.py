@@ -174,28 +174,6 @@ def test_clustered_60k_bit_exact():
     assert st[0] == 60_000
 
 
-def test_full_config2_sample_and_sums():
-    w = W.config2()
-    g, o = build_pair(w.lo, w.hi, w.grid, w.points, w.walls, keep_cells=False)
-    g.calculate()
-    n = g.count_cells()
-    assert n == o.count_generators()
-    rng = np.random.default_rng(7)
-    sample = np.sort(rng.choice(n, size=20_000, replace=False))
-    o.calculate(subset=sample, threads=16)
-    vol, cen, nf, _ = o.reduced(sample)
-    gv, gc, gf = g.volumes(), g.centroids(), g.face_counts()
-    assert np.array_equal(bits(gv[sample]), bits(vol))
-    assert np.array_equal(bits(gc[sample]), bits(cen))
-    assert np.array_equal(gf[sample], nf)
-    off, ids = g.neighbors()
-    for k, i in enumerate(sample[:2000]):
-        assert np.array_equal(ids[off[i]:off[i + 1]], o.get_cell(int(i)).face_neighbors)
-    ball = 4.0 / 3.0 * np.pi * 0.125
-    total = gv.sum()
-    assert total >= ball and (total - ball) / ball < 0.01
-
-
 def test_relax_matches_oracle():
     w = W.config1()
     g, o = build_pair(w.lo, w.hi, w.grid, w.points)
@@ -262,6 +240,26 @@ def test_arena_growth_keeps_cells():
     off_a, ids_a = small.neighbors()
     off_b, ids_b = ref.neighbors()
     assert np.array_equal(off_a, off_b) and np.array_equal(ids_a, ids_b)
+
+
+def test_arena_gives_up_with_an_error_and_no_results():
+    # one launch with an arena of one vertex per cell cannot hold the cells: calculate() reports
+    # VD_ECAPACITY and leaves no cells to read (no half-written arena offsets); with the default
+    # attempts the same handle then computes every cell as the oracle does
+    from vorothree_amd._lib import VD_ECAPACITY, VD_ESTATE, VdError
+    w = W.config2(n=20_000, grid=27)
+    g, o = build_pair(w.lo, w.hi, w.grid, w.points, w.walls)
+    g.set_arena(1)
+    g.set_attempts(1)
+    with pytest.raises(VdError) as ei:
+        g.calculate()
+    assert ei.value.code == VD_ECAPACITY
+    assert g.count_cells() == 0 and g.get_cell(0) is None
+    with pytest.raises(VdError) as es:
+        g.volumes()
+    assert es.value.code == VD_ESTATE
+    g.set_attempts(8)
+    check_all_cells(g, o)
 
 
 def test_voids_extend_once_and_probe_on_every_later_calculate():

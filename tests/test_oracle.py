@@ -174,6 +174,54 @@ def test_stable_shell_order_properties():
     assert np.all(d2[:27] == 0.0)
 
 
+@pytest.mark.parametrize("lo,hi,grid", [((0, 0, 0), (1, 2, 3), (3, 4, 5)),
+                                        ((0, 0, 0), (1, 1, 1), (7, 7, 7)),
+                                        ((-1, 0, 2), (2, 0.5, 3), (12, 5, 9))])
+def test_lazy_shell_prefix_is_start_of_full_table(lo, hi, grid):
+    """The prefix the walks grow (every key <= thr, and the first key left out) is, entry for
+    entry, the start of the whole stable table, for thresholds at, between and beyond its keys."""
+    t = OracleTess(lo, hi, *grid)
+    fd, fk = t.shell_order()
+    keys = np.unique(fk)
+    thrs = list(keys[::max(1, len(keys) // 40)]) + list((keys[:-1] + keys[1:])[::7] / 2.0)
+    thrs += [0.0, keys[-1], keys[-1] * 2.0, float("inf")]
+    for thr in thrs:
+        d, k, nxt = t.shell_prefix(thr)
+        m = int(np.count_nonzero(fk <= thr))
+        assert len(k) == m, thr
+        assert np.array_equal(d, fd[:m]) and np.array_equal(k.view(np.uint64), fk[:m].view(np.uint64))
+        assert nxt == (fk[m] if m < len(fk) else float("inf")), thr
+
+
+@pytest.mark.parametrize("case", ["sparse", "clusters"])
+def test_lazy_shell_walk_equals_full_table_walk(case):
+    """Cells computed over the lazily grown prefix are bit-identical to cells computed over the
+    whole table, and examine the same number of shell entries."""
+    g = W.PCG32(77)
+    if case == "sparse":
+        pts, grid = g.uniform(60, (0, 0, 0), (1, 1, 1)), (20, 20, 20)
+    else:
+        pts, grid = W.clustered_points(g, 3000, 4, 0.03, 1.0), (18, 18, 18)
+    cells = []
+    for full in (True, False):
+        t = OracleTess((0, 0, 0), (1, 1, 1), *grid)
+        first = t.shell_mode(full)
+        t.set_generators(pts)
+        t.calculate(threads=4)
+        if not full:
+            assert t.shell_len() > first, "the walks never grew the prefix"
+        vol, cen, nf, work = t.reduced()
+        cells.append((vol, cen, nf, work[:, 5], [t.get_cell(i) for i in range(t.count_cells())]))
+    (va, ca, na, ea, xa), (vb, cb, nb, eb, xb) = cells
+    assert np.array_equal(va.view(np.uint64), vb.view(np.uint64))
+    assert np.array_equal(ca.view(np.uint64), cb.view(np.uint64))
+    assert np.array_equal(na, nb) and np.array_equal(ea, eb)
+    for p, q in zip(xa, xb):
+        assert np.array_equal(p.vertices.view(np.uint64), q.vertices.view(np.uint64))
+        assert np.array_equal(p.face_indices, q.face_indices)
+        assert np.array_equal(p.face_neighbors, q.face_neighbors)
+
+
 # --- brute force and invariants --------------------------------------------------------------
 
 @pytest.mark.parametrize("seed,n,walls", [(31, 120, ()), (32, 150, ("sphere",)), (33, 100, ("plane",))])

@@ -21,6 +21,7 @@ VD_OPT_KEEP_CELLS = 1
 VD_OPT_GRAPHS = 2
 VD_OPT_ARENA = 3
 VD_OPT_FAR_LIST = 4
+VD_OPT_ATTEMPTS = 5
 VD_BOPT_KEEP_CELLS = 1
 VD_BOPT_PER_BIN = 2
 VD_BOPT_GRAPHS = 3

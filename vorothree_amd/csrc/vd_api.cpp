@@ -410,6 +410,7 @@ struct Engine {
     bool side_streams = true;
     double far_list_min = kFarListMinDefault;   // walk length for a per-cell list (0: never)
     uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
+    int max_attempts = 8;              // launches with doubled arenas before giving up (VD_OPT_ATTEMPTS)
     uint64_t cap_cells = 0;            // cells the arena capacities were sized for
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
     uint32_t ntube = 0;                // tube-wall slots per generator
@@ -442,6 +443,9 @@ int engine_init(Engine &E) {
 }
 
 void engine_free(Engine &E) {
+    // the side stream may hold this engine's far-cell launches when a calculate failed before
+    // its stream waited for them
+    if (E.stream2) (void)hipStreamSynchronize(E.stream2);
     if (E.stream) (void)hipStreamSynchronize(E.stream);
     DevBuf *bufs[] = {&E.d_desc, &E.d_offs, &E.d_raw, &E.d_raw_tess, &E.d_flags, &E.d_pos, &E.d_scan,
                       &E.d_gat, &E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start,
@@ -1026,7 +1030,10 @@ int engine_calculate(Engine &E) {
         E.trunc0 = far;
         VD_TRY(upload_desc(E));
     }
-    for (int attempt = 0; attempt < 8; ++attempt) {
+    // every cell computed within the attempts and rounds; otherwise calculate() fails and
+    // leaves no results (cells could be left unfinished, with arena offsets never written)
+    bool done = false;
+    for (int attempt = 0; attempt < E.max_attempts && !done; ++attempt) {
         if (E.keep_cells) {
             VD_TRY(ensure(E.d_vtx, E.cap[0] * 24));
             VD_TRY(ensure(E.d_fcnt, E.cap[1]));
@@ -1222,9 +1229,16 @@ int engine_calculate(Engine &E) {
             for (DevBuf &b : flist) release(b);
         }
         for (DevBuf &b : L) release(b);
-        if (!arena_full) break;
-        for (int k = 0; k < 3; ++k) E.cap[k] *= 2;
+        if (arena_full) {
+            for (int k = 0; k < 3; ++k) E.cap[k] *= 2;
+            continue;
+        }
+        if (pending()) return fail(VD_ECAPACITY, "cells still need reruns after 64 rounds");
+        done = true;
     }
+    if (!done)
+        return fail(VD_ECAPACITY, "the cell arenas filled up on every attempt (" + std::to_string(E.max_attempts) +
+                                      " launches, each with twice the arena of the one before)");
     uint64_t top[6] = {0, 0, 0, 0, 0, 0};
     VD_HIP(hipMemcpyAsync(top, E.d_top.p, sizeof(top), hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
@@ -1620,27 +1634,33 @@ static int tess_add_wall(vd_tess *t, const vd_wall &w) {
     Engine &E = t->E;
     Member &m = E.mem[0];
     if ((int)m.walls.size() >= vd::kMaxWalls) return fail(VD_EINVAL, "too many walls (max 8)");
+    // prune_outside_generators (tessellation.rs:175-194): the current generators are filtered
+    // again by every wall. They become the raw input first (filtering them by the walls they
+    // already passed keeps them all, and re-binning an unchanged set gives the same bins), so a
+    // failure below leaves the handle as it was, without the new wall.
+    const uint64_t n = E.ngen;
+    if (n) {
+        DevBuf keep;
+        const ScopedBufs keep_g{&keep, 1};
+        VD_TRY(ensure(keep, n * 24));
+        VD_HIP(hipMemcpyAsync(keep.p, E.d_gen.p, n * 24, hipMemcpyDeviceToDevice, E.stream));
+        VD_TRY(ensure(E.d_raw, (n + 1) * 24));
+        VD_HIP(hipMemcpyAsync(E.d_raw.p, keep.p, n * 24, hipMemcpyDeviceToDevice, E.stream));
+        VD_HIP(hipStreamSynchronize(E.stream));
+        m.nraw = n;
+        engine_layout(E);
+    }
     m.walls.push_back(d);
     m.wdata.push_back(std::move(data));
-    // prune_outside_generators (tessellation.rs:175-194): the current generators are filtered
-    // again by every wall (re-binning an unchanged set gives the same bins)
-    const uint64_t n = E.ngen;
-    if (n == 0) return upload_desc(E);
-    DevBuf keep;
-    const ScopedBufs keep_g{&keep, 1};
-    VD_TRY(ensure(keep, n * 24));
-    VD_HIP(hipMemcpyAsync(keep.p, E.d_gen.p, n * 24, hipMemcpyDeviceToDevice, E.stream));
-    int rc = ensure(E.d_raw, (n + 1) * 24);
-    if (!rc) {
-        hipError_t e = hipMemcpyAsync(E.d_raw.p, keep.p, n * 24, hipMemcpyDeviceToDevice, E.stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(E.stream);
-        rc = e == hipSuccess ? VD_OK : fail(VD_EDEVICE, hipGetErrorString(e));
+    const int rc = n ? engine_prepare(E) : upload_desc(E);
+    if (rc) {
+        const std::string msg = g_err;
+        m.walls.pop_back();
+        m.wdata.pop_back();
+        (void)(n ? engine_prepare(E) : upload_desc(E));
+        return fail(rc, msg);
     }
-    release(keep);
-    if (rc) return rc;
-    m.nraw = n;
-    engine_layout(E);
-    return engine_prepare(E);
+    return VD_OK;
 }
 
 int vd_tess_add_wall_plane(vd_tess *t, int32_t id, const double point[3], const double normal[3]) {
@@ -1795,6 +1815,11 @@ int vd_tess_set_option(vd_tess *t, int key, int64_t value) {
     if (key == VD_OPT_FAR_LIST) {
         if (value < 0) return fail(VD_EINVAL, "walk length must not be negative");
         t->E.far_list_min = (double)value;
+        return VD_OK;
+    }
+    if (key == VD_OPT_ATTEMPTS) {
+        if (value < 1 || value > 62) return fail(VD_EINVAL, "attempts must be in [1, 62]");
+        t->E.max_attempts = (int)value;
         return VD_OK;
     }
     return fail(VD_EINVAL, "unknown option");

@@ -510,6 +510,11 @@ class Tessellation:
         list of the occupied entries only (VD_OPT_FAR_LIST; 0: never). Results are identical."""
         check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_FAR_LIST, int(min_entries)))
 
+    def set_attempts(self, attempts: int) -> None:
+        """Launches, each with twice the arena of the one before, after which calculate() gives up
+        with VD_ECAPACITY and keeps no results (VD_OPT_ATTEMPTS, default 8)."""
+        check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_ATTEMPTS, int(attempts)))
+
     # -- queries -------------------------------------------------------------------------------
     def count_generators(self) -> int:
         return int(lib().vd_tess_count_generators(self._h))
