@@ -303,6 +303,28 @@ int vd_server_stats(const vd_server *s, double out[4]);
                                the plain target instead of three times it: relaxation fills them. */
 int vd_server_set_option(vd_server *s, int key, int64_t value);
 
+/* ---- Without device allocation (no counterpart in the reference) --------------------------
+ * For callers that cannot allocate device memory: code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc. One tessellation, as vd_tess_create + vd_tess_add_wall (walls in
+ * order) + vd_tess_set_generators_device + vd_tess_calculate compute it, with the reduced outputs:
+ * every intermediate buffer is carved from a static device array of VD_STATIC_SCRATCH_BYTES that
+ * each GPU the library runs on holds until the process ends. No hipMalloc, hipFree, pinned memory,
+ * event, stream or graph is created. d_xyz (3n doubles), d_vol (n), d_cen (3n) and d_nface (n) are
+ * device memory of the caller; *ncells receives the cells kept by the walls, whose results fill
+ * the first *ncells entries. stream: a hipStream_t of the caller (NULL: the null stream); the call
+ * returns after its work on it has finished.
+ * Limits: at most VD_STATIC_MAX_POINTS points and VD_STATIC_MAX_BINS bins (VD_EINVAL beyond);
+ * inputs that need more scratch than the array holds during the computation (a shell prefix grown
+ * for a very sparse grid, many reruns) return VD_ENOMEM. The array's contents are left from the
+ * previous call and cleared where they must be; calls that overlap on one GPU (from several
+ * threads, or on several streams) share the array and race: serialise them. */
+#define VD_STATIC_SCRATCH_BYTES (96u << 20)
+#define VD_STATIC_MAX_POINTS 262144u
+#define VD_STATIC_MAX_BINS 2097152u
+int vd_calculate_static(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny, uint32_t nz,
+                        const vd_wall *walls, int nwalls, const double *d_xyz, size_t n, double *d_vol,
+                        double *d_cen, uint32_t *d_nface, size_t *ncells, void *stream);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char *vd_last_error(void);
 

@@ -107,3 +107,17 @@ def test_grid_for_points_rule():
     box = BoundingBox((0.0, 0.0, 0.0), (2.0, 1.0, 0.5))
     g_a = grid_for_points(box, W.PCG32(93).uniform(100_000, (0, 0, 0), (2.0, 1.0, 0.5)), 0.35)
     assert g_a[0] > g_a[1] > g_a[2]                                    # proportional to the edges
+
+
+def test_static_entry_rejects_sizes_beyond_its_maximum():
+    # vd_calculate_static checks its limits before it touches a device
+    from vorothree_amd import calculate_static
+    from vorothree_amd._lib import VD_EINVAL, VD_STATIC_MAX_BINS, VD_STATIC_MAX_POINTS, VdError
+    b = BoundingBox((0, 0, 0), (1, 1, 1))
+    with pytest.raises(VdError) as e1:
+        calculate_static(b, (10, 10, 10), [], 0x1000, VD_STATIC_MAX_POINTS + 1, 0x1000, 0x1000, 0x1000)
+    assert e1.value.code == VD_EINVAL and "at most" in str(e1.value)
+    k = int(round(VD_STATIC_MAX_BINS ** (1 / 3))) + 1
+    with pytest.raises(VdError) as e2:
+        calculate_static(b, (k, k, k), [], 0x1000, 100, 0x1000, 0x1000, 0x1000)
+    assert e2.value.code == VD_EINVAL and "bins" in str(e2.value)

@@ -22,6 +22,7 @@ from .tessellation import (  # noqa: F401
     TrefoilKnotGeometry,
     Wall,
     box_side,
+    calculate_static,
     grid_for,
     grid_for_points,
     shell_prefix,

@@ -56,6 +56,18 @@ class DeviceArray:
         if self.nbytes and hip().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes, 1) != 0:
             raise RuntimeError("hipMemcpy H2D failed")
 
+    @classmethod
+    def empty(cls, nbytes: int) -> "DeviceArray":
+        """Uninitialised device memory of nbytes bytes."""
+        return cls(np.empty((max(int(nbytes), 8) + 7) // 8, dtype=np.float64))
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        """The first count elements of dtype, copied to the host."""
+        out = np.empty(int(count), dtype=dtype)
+        if out.nbytes and hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy D2H failed")
+        return out
+
     def close(self):
         if self.ptr:
             hip().hipFree(self.ptr)

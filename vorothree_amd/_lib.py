@@ -22,6 +22,9 @@ VD_OPT_GRAPHS = 2
 VD_OPT_ARENA = 3
 VD_OPT_FAR_LIST = 4
 VD_OPT_ATTEMPTS = 5
+VD_STATIC_SCRATCH_BYTES = 96 << 20
+VD_STATIC_MAX_POINTS = 262144
+VD_STATIC_MAX_BINS = 2097152
 VD_BOPT_KEEP_CELLS = 1
 VD_BOPT_PER_BIN = 2
 VD_BOPT_GRAPHS = 3
@@ -92,6 +95,8 @@ SIGNATURES = {
     "vd_batch_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_batch_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "vd_grid_for": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_double, _vp]),
+    "vd_calculate_static": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _vp, ctypes.c_int, _vp, _sz, _vp, _vp,
+                                           _vp, ctypes.POINTER(_sz), _vp]),
     "vd_grid_for_points": (ctypes.c_int, [_vp, _vp, _vp, _sz, ctypes.c_double, _vp]),
     "vd_server_create": (ctypes.c_int, [_sz, ctypes.c_int, ctypes.POINTER(_vp)]),
     "vd_server_destroy": (None, [_vp]),

@@ -59,6 +59,7 @@ int fail(int code, const std::string &msg) {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool carved = false;   // carved from a static arena: never passed to hipFree
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
@@ -66,13 +67,29 @@ struct DevBuf {
 // for the whole device, work that other threads have in flight included.
 thread_local std::vector<DevBuf> *g_defer = nullptr;
 
+// While g_static is set, ensure() carves buffers from that device range (a bump allocator) instead
+// of calling hipMalloc, and release() returns only the most recently carved buffer to it.
+struct StaticArena {
+    char *base;
+    size_t cap, top;
+};
+thread_local StaticArena *g_static = nullptr;
+
 void release(DevBuf &b) {
     if (b.p) {
-        if (g_defer) g_defer->push_back(b);
-        else (void)hipFree(b.p);
+        if (b.carved) {
+            char *end = static_cast<char *>(b.p) + b.bytes;
+            if (g_static && end == g_static->base + g_static->top)
+                g_static->top = (size_t)(static_cast<char *>(b.p) - g_static->base);
+        } else if (g_defer) {
+            g_defer->push_back(b);
+        } else {
+            (void)hipFree(b.p);
+        }
     }
     b.p = nullptr;
     b.bytes = 0;
+    b.carved = false;
 }
 
 // Releases n buffers when the scope ends, on every return path (release is idempotent, so an
@@ -87,6 +104,17 @@ int ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return VD_OK;
     release(b);
+    if (g_static) {
+        const size_t at = (g_static->top + 255) & ~(size_t)255;
+        if (at > g_static->cap || bytes > g_static->cap - at)
+            return fail(VD_ENOMEM, "static scratch exhausted: " + std::to_string(bytes) + " more bytes needed at " +
+                                       std::to_string(at) + " of " + std::to_string(g_static->cap));
+        b.p = g_static->base + at;
+        b.bytes = bytes;
+        b.carved = true;
+        g_static->top = at + bytes;
+        return VD_OK;
+    }
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -409,6 +437,11 @@ struct Engine {
     // on the served mix, measured)
     bool side_streams = true;
     double far_list_min = kFarListMinDefault;   // walk length for a per-cell list (0: never)
+    bool keep_nb = true;               // keep every cell's neighbour ids (reduced mode)
+    // vd_calculate_static: buffers carved from the static scratch, the caller's stream (never
+    // destroyed), no events, no pinned memory, no graphs, no side streams
+    bool static_mode = false;
+    uint32_t h_local[64] = {};         // the outcome counts' host copy in static mode
     uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
     int max_attempts = 8;              // launches with doubled arenas before giving up (VD_OPT_ATTEMPTS)
     uint64_t cap_cells = 0;            // cells the arena capacities were sized for
@@ -463,14 +496,14 @@ void engine_free(Engine &E) {
     for (DevBuf *b : post) release(*b);
     if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
     E.gexec = nullptr;
-    if (E.h_counts) (void)hipHostFree(E.h_counts);
+    if (E.h_counts && !E.static_mode) (void)hipHostFree(E.h_counts);
     E.h_counts = nullptr;
     for (auto &kv : E.tables) release(kv.second.buf);
     E.tables.clear();
     E.shapes.clear();
     if (E.ev0) (void)hipEventDestroy(E.ev0);
     if (E.ev1) (void)hipEventDestroy(E.ev1);
-    if (E.stream) (void)hipStreamDestroy(E.stream);
+    if (E.stream && !E.static_mode) (void)hipStreamDestroy(E.stream);
     E.ev0 = E.ev1 = nullptr;
     E.stream = E.stream2 = nullptr;
 }
@@ -792,7 +825,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.status = E.d_status.as<int32_t>();
     a.vtx = E.keep_cells ? E.d_vtx.as<double>() : nullptr;
     a.fcnt = E.d_fcnt.as<uint8_t>();
-    a.fnb = E.d_fnb.as<int32_t>();
+    a.fnb = (E.keep_cells || E.keep_nb) ? E.d_fnb.as<int32_t>() : nullptr;
     a.fidx = E.d_fidx.as<uint16_t>();
     a.arena_top = E.d_top.as<uint64_t>();
     a.cell_off = E.d_cell_off.as<uint64_t>();
@@ -989,7 +1022,7 @@ int run_round0(Engine &E, uint32_t n, float *ms) {
         VD_HIP(hipEventElapsedTime(ms, E.ev0, E.ev1));
         E.graph_launches += 1;
     } else {
-        VD_TRY(enqueue_round0(E, n, ms));
+        VD_TRY(enqueue_round0(E, n, E.static_mode ? nullptr : ms));
         VD_HIP(hipStreamSynchronize(E.stream));
     }
     E.binned = true;
@@ -1008,7 +1041,7 @@ int engine_calculate(Engine &E) {
     VD_TRY(ensure(E.d_nface, n * 4));
     VD_TRY(ensure(E.d_nvert, n * 4));
     VD_TRY(ensure(E.d_status, n * 4));
-    VD_TRY(ensure(E.d_cell_off, n * 24));
+    if (E.keep_cells || E.keep_nb) VD_TRY(ensure(E.d_cell_off, n * 24));
     VD_TRY(ensure(E.d_top, 64));
     VD_TRY(ensure(E.d_queue, 64));
     VD_TRY(ensure(E.d_counts4, 8 * kTiers * sizeof(uint32_t)));
@@ -1039,7 +1072,7 @@ int engine_calculate(Engine &E) {
             VD_TRY(ensure(E.d_fcnt, E.cap[1]));
             VD_TRY(ensure(E.d_fidx, E.cap[2] * 2));
         }
-        VD_TRY(ensure(E.d_fnb, E.cap[1] * 4));
+        if (E.keep_cells || E.keep_nb) VD_TRY(ensure(E.d_fnb, E.cap[1] * 4));
         VD_TRY(ensure(E.d_out0, n * 4));
         VD_TRY(ensure(E.d_out1, n * 4));
 
@@ -1484,6 +1517,43 @@ double now_s() {
 
 }  // namespace
 
+// ------------------------------------------------------------------------------------------
+// vd_calculate_static: one tessellation computed in a file-scope device array instead of
+// hipMalloc'd buffers. The array's name is a host-side stand-in; its device address on each GPU is
+// taken once with hipGetSymbolAddress (the device must be current) and used wherever host code
+// hands the memory to the device.
+// ------------------------------------------------------------------------------------------
+static_assert(VD_STATIC_SCRATCH_BYTES % 256 == 0, "whole 256-byte blocks");
+static __device__ __align__(256) unsigned char scratch[VD_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_ptr;   // device -> address of that device's copy of `scratch`
+
+int scratch_address(int device, char **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it == g_scratch_ptr.end()) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(VD_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(device, static_cast<char *>(ptr)).first;
+    }
+    *out = it->second;
+    return VD_OK;
+}
+
+// Sets g_static for the scope of one vd_calculate_static call.
+struct StaticScope {
+    StaticArena arena;
+    explicit StaticScope(char *base) : arena{base, VD_STATIC_SCRATCH_BYTES, 0} { g_static = &arena; }
+    ~StaticScope() { g_static = nullptr; }
+};
+
+}  // namespace
+
 // One lane of the server: an engine (with its own stream) and the jobs it runs. The lane of large
 // jobs runs its steps on a worker thread, so that small jobs, stepped by the calling thread on the
 // other lane, do not wait for them; its cell launches leave kReservedCUs to the small lane. While
@@ -1590,6 +1660,72 @@ int vd_grid_for_points(const double bmin[3], const double bmax[3], const double 
     int64_t g[3];
     density_grid(bmin, bmax, xyz, n, per_bin, false, g);
     for (int a = 0; a < 3; ++a) out[a] = (uint32_t)g[a];
+    return VD_OK;
+}
+
+int vd_calculate_static(const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny, uint32_t nz,
+                        const vd_wall *walls, int nwalls, const double *d_xyz, size_t n, double *d_vol,
+                        double *d_cen, uint32_t *d_nface, size_t *ncells, void *stream) {
+    // every check that needs no device comes first
+    VD_TRY(check_grid(bmin, bmax, nx, ny, nz));
+    if (!ncells || (n && (!d_xyz || !d_vol || !d_cen || !d_nface)) || nwalls < 0 || (nwalls && !walls))
+        return fail(VD_EINVAL, "NULL argument");
+    if (n > VD_STATIC_MAX_POINTS)
+        return fail(VD_EINVAL, "vd_calculate_static takes at most " + std::to_string(VD_STATIC_MAX_POINTS) +
+                                   " points, got " + std::to_string(n));
+    if ((uint64_t)nx * ny * nz > VD_STATIC_MAX_BINS)
+        return fail(VD_EINVAL, "vd_calculate_static takes at most " + std::to_string(VD_STATIC_MAX_BINS) +
+                                   " bins, got " + std::to_string((uint64_t)nx * ny * nz));
+    if (nwalls > vd::kMaxWalls) return fail(VD_EINVAL, "too many walls (max 8)");
+    Member m;
+    for (int a = 0; a < 3; ++a) { m.lo[a] = bmin[a]; m.hi[a] = bmax[a]; }
+    m.n[0] = nx; m.n[1] = ny; m.n[2] = nz;
+    m.grid = make_grid(m.lo, m.hi, m.n);
+    for (int k = 0; k < nwalls; ++k) {
+        vd::WallDev d;
+        std::vector<double> data;
+        VD_TRY(make_wall(walls[k], d, data));
+        m.walls.push_back(d);
+        m.wdata.push_back(std::move(data));
+    }
+    *ncells = 0;
+    Engine E;
+    E.static_mode = true;
+    E.stream = static_cast<hipStream_t>(stream);
+    E.h_counts = E.h_local;
+    E.graphs = 0;
+    E.side_streams = false;
+    E.far_list_min = 0.0;
+    E.keep_cells = 0;
+    E.keep_nb = false;
+    VD_HIP(hipGetDevice(&E.device));
+    VD_HIP(hipDeviceGetAttribute(&E.cus, hipDeviceAttributeMultiprocessorCount, E.device));
+    for (int k = 0; k < kTiers; ++k) {
+        VD_HIP(vd::cells_occupancy(k, &E.occ[k]));
+        E.occ[k] = std::max(1, E.occ[k]);
+    }
+    char *base = nullptr;
+    VD_TRY(scratch_address(E.device, &base));
+    const StaticScope scope(base);
+    struct Free {   // the engine's carved buffers go back before the scope ends
+        Engine &E;
+        ~Free() { engine_free(E); }
+    } free_engine{E};
+    E.mem.push_back(m);
+    VD_TRY(ensure(E.d_raw, (n + 1) * 24));
+    if (n) VD_HIP(hipMemcpyAsync(E.d_raw.p, d_xyz, n * 24, hipMemcpyDeviceToDevice, E.stream));
+    E.mem[0].nraw = n;
+    engine_layout(E);
+    VD_TRY(engine_prepare(E));
+    VD_TRY(engine_calculate(E));
+    const uint64_t kept = E.ngen;
+    if (kept) {
+        VD_HIP(hipMemcpyAsync(d_vol, E.d_vol.p, kept * 8, hipMemcpyDeviceToDevice, E.stream));
+        VD_HIP(hipMemcpyAsync(d_cen, E.d_cen.p, kept * 24, hipMemcpyDeviceToDevice, E.stream));
+        VD_HIP(hipMemcpyAsync(d_nface, E.d_nface.p, kept * 4, hipMemcpyDeviceToDevice, E.stream));
+    }
+    VD_HIP(hipStreamSynchronize(E.stream));
+    *ncells = kept;
     return VD_OK;
 }
 

@@ -1409,7 +1409,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                 A.cell_off[3 * (size_t)i + 2] = oi;
             }
         }
-    } else {
+    } else if (A.fnb) {
         // reduced mode keeps the neighbour ids (face order) only
         const uint64_t of = arena_take(A, ac, 1, (uint64_t)P.F);
         if (of + P.F > A.arena_cap[1]) {

@@ -112,7 +112,8 @@ struct CellArgs {
     uint32_t *nvert;              // vertex count (0 = empty cell)
     int32_t *status;              // 0 ok, >0 needs a retry, <0 error
 
-    // full cell export (optional: vtx == nullptr keeps only the neighbour ids in fnb)
+    // full cell export (optional: vtx == nullptr keeps only the neighbour ids in fnb, and with
+    // fnb == nullptr too only the reductions are written)
     double *vtx;                  // arena of vertices, 3 doubles each
     uint8_t *fcnt;                // arena of face vertex counts
     int32_t *fnb;                 // arena of face neighbour ids (same indexing as fcnt)

@@ -627,6 +627,24 @@ def _vd_wall(wall: Wall):
     return w
 
 
+def calculate_static(bounds: BoundingBox, grid, walls, d_points: int, n: int, d_vol: int, d_cen: int,
+                     d_nface: int, stream: int = 0) -> int:
+    """vd_calculate_static: one tessellation (walls in order, then the n points at device address
+    d_points) computed without device allocation, in the library's static scratch array; volumes,
+    centroids and face counts go to the caller's device buffers d_vol, d_cen and d_nface. Returns
+    the number of cells kept by the walls."""
+    ws = (_lib.VdWall * max(1, len(walls)))()
+    for k, w in enumerate(walls):
+        ws[k] = _vd_wall(w)
+    kept = ctypes.c_size_t(0)
+    nx, ny, nz = (int(g) for g in grid)
+    check(lib().vd_calculate_static(_d3(bounds.min), _d3(bounds.max), nx, ny, nz, ws, len(walls),
+                                    ctypes.c_void_p(d_points), int(n), ctypes.c_void_p(d_vol),
+                                    ctypes.c_void_p(d_cen), ctypes.c_void_p(d_nface), ctypes.byref(kept),
+                                    ctypes.c_void_p(stream) if stream else None))
+    return int(kept.value)
+
+
 class Batch:
     """Many independent tessellations computed in one persistent launch (vd_batch_* in vd.h).
 
