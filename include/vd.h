@@ -90,6 +90,30 @@ int vd_tess_add_wall_sphere(vd_tess *t, int32_t id, const double center[3], doub
 /* add_wall for any wall kind of vd_wall (the same pruning as the two calls above). */
 int vd_tess_add_wall(vd_tess *t, const vd_wall *wall);
 
+/* A wall whose geometry stays with the caller: Wall::new(id, Box<dyn WallGeometry>) for any
+ * geometry (src/wall/mod.rs:24-31, trait :53-71), evaluated on the host through these callbacks.
+ *   contains(ctx, p): WallGeometry::contains (nonzero: p is in the valid region);
+ *   cut(ctx, g, planes, cap): WallGeometry::cut for generator g: writes up to cap planes, 6
+ *     doubles each (a point on the plane, the normal pointing out of the valid region), in the
+ *     order the geometry emits them, and returns how many it emits; a count above cap makes the
+ *     library call again with room for all (at most VD_WALL_MAX_POINTS per generator);
+ *   planar: WallGeometry::is_planar (calculate_sealed and prune_boundaries treat non-planar walls
+ *     as curved).
+ * The callbacks run on the calling thread during vd_tess_add_wall_planes_per_generator,
+ * vd_tess_set_generators(_device) and vd_tess_relax: contains for every input point, cut for every
+ * kept generator. ctx must stay valid while the handle holds the wall. */
+typedef struct vd_wall_host {
+    void *ctx;
+    int (*contains)(void *ctx, const double p[3]);
+    int (*cut)(void *ctx, const double g[3], double *planes, int cap);
+    int planar;
+} vd_wall_host;
+
+/* add_wall(Wall::new(id, Box::new(geometry))) for a geometry of the caller's: the prune of
+ *   src/tessellation.rs:175-200 with wall->contains, and at every calculate() the planes of
+ *   wall->cut for each generator clip its cell in wall order (tessellation.rs:429-436). */
+int vd_tess_add_wall_planes_per_generator(vd_tess *t, int32_t id, const vd_wall_host *wall);
+
 /* clear_walls(), src/tessellation.rs:203-205 (generators are kept as they are). */
 int vd_tess_clear_walls(vd_tess *t);
 

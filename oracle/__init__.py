@@ -20,6 +20,8 @@ _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
+_HOST_CONTAINS = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp)
+_HOST_CUT = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp, ctypes.c_int)
 
 
 def build() -> None:
@@ -41,6 +43,7 @@ def _load():
         "or_tess_add_wall_sphere": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_double]),
         "or_tess_add_wall": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int, _dp, _dp, ctypes.c_double,
                                             ctypes.c_double, _dp, ctypes.c_uint32, ctypes.c_int]),
+        "or_tess_add_wall_host": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _HOST_CONTAINS, _HOST_CUT, ctypes.c_int]),
         "or_tess_set_generators": (None, [_vp, _dp, ctypes.c_size_t]),
         "or_tess_count_generators": (ctypes.c_size_t, [_vp]),
         "or_tess_get_generators": (None, [_vp, _dp]),
@@ -153,6 +156,27 @@ class OracleTess:
                                         p.ctypes.data_as(_dp), npts, 1 if closed else 0)
         if rc != 0:
             raise ValueError("Wall ID must be <= -1000" if rc == -1 else "unknown wall kind")
+
+    def add_wall_host(self, wid, geometry):
+        """Wall::new(id, Box<dyn WallGeometry>) for a Python geometry with contains(p) and cut(g)
+        (a list of (point, normal)); is_planar() optional. Called back from the oracle's threads."""
+        def contains(_ctx, p):
+            return 1 if geometry.contains((p[0], p[1], p[2])) else 0
+
+        def cut(_ctx, g, out, cap):
+            planes = list(geometry.cut((g[0], g[1], g[2])))
+            for k, (pt, nm) in enumerate(planes[:cap]):
+                for c in range(3):
+                    out[6 * k + c] = float(pt[c])
+                    out[6 * k + 3 + c] = float(nm[c])
+            return len(planes)
+
+        cbs = (_HOST_CONTAINS(contains), _HOST_CUT(cut))
+        self.__dict__.setdefault("_host", []).append(cbs)
+        planar = getattr(geometry, "is_planar", None)
+        if self._lib.or_tess_add_wall_host(self._h, int(wid), None, cbs[0], cbs[1],
+                                           1 if (planar and planar()) else 0) != 0:
+            raise ValueError("Wall ID must be <= -1000")
 
     def calculate_sealed(self, threads=1):
         """Tessellation::calculate_sealed (tessellation.rs:234-318)."""

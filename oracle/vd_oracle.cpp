@@ -297,15 +297,25 @@ uint64_t poly_triangles(const Poly &c) {
 // ---------------------------------------------------------------------------------------------
 // Walls. Restates src/wall/mod.rs:15-49 and src/wall/wall_3d.rs:8-101 (Plane and Sphere only).
 // ---------------------------------------------------------------------------------------------
+// A wall whose geometry lives with the caller (kind 7): WallGeometry::contains and ::cut
+// (src/wall/mod.rs:53-71) called back through these pointers; planar is is_planar().
+using HostContains = int (*)(void *ctx, const double p[3]);
+using HostCut = int (*)(void *ctx, const double g[3], double *planes, int cap);
+
 struct Wall {
     int32_t id;
-    int kind;       // 0 plane, 1 sphere, 2 cylinder, 3 cone, 4 torus, 5 convex polyhedron, 6 tube
+    int kind;       // 0 plane, 1 sphere, 2 cylinder, 3 cone, 4 torus, 5 convex polyhedron, 6 tube,
+                    // 7 host callbacks
     double a[3];    // plane point / sphere, cylinder, torus centre / cone tip
     double b[3];    // plane unit normal (pointing into the valid region) / unit axis
     double r;       // sphere, cylinder, tube radius / torus major radius / cone half-angle
     double r2;      // torus minor radius
     std::vector<double> pts;   // polyhedron planes (point, outward normal) / tube samples (xyz)
     bool closed;               // tube: the last sample joins the first
+    void *hctx = nullptr;      // host callbacks (kind 7)
+    HostContains hcontains = nullptr;
+    HostCut hcut = nullptr;
+    bool planar = false;
 };
 
 // get_closest_point of the sampled-curve walls: TrefoilKnotGeometry (wall_3d.rs:408-438, always
@@ -341,6 +351,7 @@ void tube_closest(const Wall &w, const double q[3], double c[3]) {
 
 // WallGeometry::contains, src/wall/wall_3d.rs.
 bool wall_contains(const Wall &w, const double q[3]) {
+    if (w.kind == 7) return w.hcontains(w.hctx, q) != 0;   // wall/mod.rs:34-36
     double dx = q[0] - w.a[0], dy = q[1] - w.a[1], dz = q[2] - w.a[2];
     switch (w.kind) {
     case 0: return (dx * w.b[0] + dy * w.b[1] + dz * w.b[2]) >= 0.0;     // :31-36
@@ -383,6 +394,16 @@ bool wall_contains(const Wall &w, const double q[3]) {
 // WallGeometry::cut: every plane (point, outward normal) the wall emits for generator g, in order.
 template <class Emit>
 void wall_cut(const Wall &w, const double g[3], Emit emit, Work *wk) {
+    if (w.kind == 7) {   // wall/mod.rs:38-40: every plane the caller's cut emits, in order
+        std::vector<double> buf(6 * 16);
+        int m = w.hcut(w.hctx, g, buf.data(), 16);
+        if (m > 16) {
+            buf.resize(6 * (size_t)m);
+            m = w.hcut(w.hctx, g, buf.data(), m);
+        }
+        for (int k = 0; k < m; ++k) emit(&buf[6 * (size_t)k], &buf[6 * (size_t)k + 3]);
+        return;
+    }
     double dx = g[0] - w.a[0], dy = g[1] - w.a[1], dz = g[2] - w.a[2];
     switch (w.kind) {
     case 0: {  // :38-42
@@ -840,6 +861,16 @@ int or_tess_add_wall(void *t, int32_t id, int kind, const double a[3], const dou
     return 0;
 }
 
+// Wall::new(id, Box<dyn WallGeometry>) for a geometry of the caller's, evaluated through callbacks.
+int or_tess_add_wall_host(void *t, int32_t id, void *ctx, HostContains contains, HostCut cut, int planar) {
+    if (id > -1000) return -1;
+    Wall w{};
+    w.id = id; w.kind = 7;
+    w.hctx = ctx; w.hcontains = contains; w.hcut = cut; w.planar = planar != 0;
+    tess_add_wall(*(Tess *)t, w);
+    return 0;
+}
+
 void or_tess_set_generators(void *t, const double *xyz, size_t n) {
     tess_set_generators(*(Tess *)t, xyz, n);
 }
@@ -904,7 +935,9 @@ size_t or_tess_shell_len(void *t) {
 // Post-passes. Restate src/tessellation.rs:234-385 and Cell3DFaces::shared_vertices
 // (src/cell/cell_3d_faces.rs:536-568).
 // ---------------------------------------------------------------------------------------------
-bool wall_planar(const Wall &w) { return w.kind == 0 || w.kind == 5; }   // wall_3d.rs:44, 709
+bool wall_planar(const Wall &w) {   // wall_3d.rs:44, 709; mod.rs:70 for the caller's geometry
+    return w.kind == 0 || w.kind == 5 || (w.kind == 7 && w.planar);
+}
 
 // Neighbour generators per cell in face order (deduplicated for prune_boundaries) and the ids of
 // the curved walls it touches (tessellation.rs:244-264, 330-350).

@@ -394,3 +394,29 @@ def test_device_shell_prefix_equals_host(lo, hi, grid, k):
     assert np.array_equal(d_host, d_dev)
     assert np.array_equal(bits(k_host), bits(k_dev))
     assert bits(np.array([n_host])) == bits(np.array([n_dev]))
+
+
+@pytest.mark.parametrize("edge", [4.64e-4, 7.94e-4, 1e-3, 1e-4])
+def test_centroid_guard_matches_oracle(edge):
+    # |sum det| = 6V against 1e-9 (cell_3d_faces.rs:195-197): boxes on both sides of the guard
+    lo = (0.25, 0.5, 0.75)
+    hi = tuple(x + edge for x in lo)
+    pts = np.array([[lo[0] + 0.3 * edge, lo[1] + 0.6 * edge, lo[2] + 0.5 * edge]])
+    g, o = build_pair(lo, hi, (1, 1, 1), pts)
+    check_all_cells(g, o)
+    zero = 6.0 * edge ** 3 < 1e-9
+    assert np.array_equal(g.centroids()[0], [0.0, 0.0, 0.0]) == zero
+
+
+def test_empty_cell_and_relax_match_oracle():
+    # a generator outside the box (no bounding-box check, tessellation.rs:43-67) ends empty; relax
+    # keeps its position (tessellation.rs:486-490)
+    pts = np.array([[5.0, 0.5, 0.5], [0.5, 0.5, 0.5], [0.2, 0.3, 0.4], [0.8, 0.7, 0.1]])
+    g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (2, 2, 2), pts)
+    check_all_cells(g, o)
+    assert g.get_cell(0).is_empty() and g.volumes()[0] == 0.0
+    g.relax()
+    o.relax()
+    assert np.array_equal(bits(g.generators()), bits(o.generators()))
+    assert np.array_equal(g.generators().reshape(-1, 3)[0], pts[0])
+    check_all_cells(g, o)

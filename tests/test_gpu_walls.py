@@ -126,3 +126,72 @@ def test_tube_wall_under_graph_replay():
         assert np.array_equal(bits(g.volumes()), bits(e.volumes()))
         assert np.array_equal(bits(g.centroids()), bits(e.centroids()))
     assert g.stats()[8] >= 1
+
+
+# --- geometries of the caller's, through vd_tess_add_wall_planes_per_generator ------------------
+
+def _compare(g, o):
+    n = o.count_cells()
+    assert g.count_cells() == n
+    assert np.array_equal(bits(g.generators()), bits(o.generators()))
+    for i in range(n):
+        assert_same_cell(g.get_cell(i), o.get_cell(i), i)
+    vol, cen, nf, _ = o.reduced()
+    assert np.array_equal(bits(g.volumes()), bits(vol))
+    assert np.array_equal(bits(g.centroids()), bits(cen))
+
+
+def test_host_sphere_equals_builtin_and_oracle():
+    from host_geometries import PySphere
+    pts = W.PCG32(1301).uniform(6000, (0, 0, 0), (1, 1, 1))
+    host = Tessellation(UNIT, Algorithm3DGrid(16, 16, 16, UNIT))
+    builtin = Tessellation(UNIT, Algorithm3DGrid(16, 16, 16, UNIT))
+    o = OracleTess((0, 0, 0), (1, 1, 1), 16, 16, 16)
+    host.add_wall(Wall(-1000, PySphere((0.5, 0.5, 0.5), 0.45)))
+    builtin.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.45)))
+    o.add_wall_host(-1000, PySphere((0.5, 0.5, 0.5), 0.45))
+    for t in (host, builtin, o):
+        t.set_generators(pts)
+    for step in range(2):
+        host.calculate_sealed()
+        builtin.calculate_sealed()
+        o.calculate_sealed(threads=4)
+        _compare(host, o)
+        _compare(builtin, o)
+        assert np.array_equal(host.seal_log(), o.seal_log()) and len(o.seal_log()) > 0
+        assert np.array_equal(builtin.seal_log(), o.seal_log())
+        host.relax()
+        builtin.relax()
+        o.relax()
+
+
+@pytest.mark.parametrize("walls_first", [True, False])
+def test_host_wall_with_varying_plane_counts_bit_exact(walls_first):
+    # 1 to 8 planes per generator, beside a built-in curved wall; sealing clips by every plane of
+    # the neighbour's cut
+    from host_geometries import PyNearFacets
+    pts = W.PCG32(1302 + int(walls_first)).uniform(8000, (0, 0, 0), (1, 1, 1))
+    g = Tessellation(UNIT, Algorithm3DGrid(18, 18, 18, UNIT))
+    o = OracleTess((0, 0, 0), (1, 1, 1), 18, 18, 18)
+
+    def walls():
+        g.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.49)))
+        o.add_wall_sphere(-1000, (0.5, 0.5, 0.5), 0.49)
+        g.add_wall(Wall(-1001, PyNearFacets((0.5, 0.5, 0.5), 0.6)))
+        o.add_wall_host(-1001, PyNearFacets((0.5, 0.5, 0.5), 0.6))
+
+    if walls_first:
+        walls()
+    g.set_generators(pts)
+    o.set_generators(pts)
+    if not walls_first:
+        walls()
+    g.calculate_sealed()
+    o.calculate_sealed(threads=4)
+    _compare(g, o)
+    assert np.array_equal(g.seal_log(), o.seal_log())
+    g.relax()
+    o.relax()
+    g.calculate()
+    o.calculate(threads=4)
+    _compare(g, o)

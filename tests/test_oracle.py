@@ -299,3 +299,43 @@ def test_relax_keeps_count_and_moves_to_centroids():
     t.relax()
     assert t.count_generators() == 1000
     assert np.array_equal(t.generators().reshape(-1, 3), cen)
+
+
+@pytest.mark.parametrize("edge,zero", [(4.64e-4, True), (7.94e-4, False), (1e-3, False), (1e-4, True)])
+def test_centroid_guard_is_on_six_volumes(edge, zero):
+    """cell_3d_faces.rs:195-197 returns [0, 0, 0] when |sum det| = 6V < 1e-9: a box of edge 4.64e-4
+    (6V = 6.0e-10) gives the origin, one of edge 7.94e-4 (V = 5.0e-10, 6V = 3.0e-9) its centre. A
+    guard on V instead of 6V would zero the 7.94e-4 box too."""
+    lo = (0.25, 0.5, 0.75)
+    hi = tuple(x + edge for x in lo)
+    t = OracleTess(lo, hi, 1, 1, 1)
+    t.set_generators([[lo[0] + 0.3 * edge, lo[1] + 0.6 * edge, lo[2] + 0.5 * edge]])
+    t.calculate()
+    vol, cen, _, _ = t.reduced()
+    assert vol[0] == pytest.approx(edge ** 3, rel=1e-6)
+    if zero:
+        assert np.array_equal(cen[0], [0.0, 0.0, 0.0])
+    else:
+        # the fan is origin-relative: tv = 6V carries cancellation error of about 1e-16 / 6V
+        # relative, so the centre comes out to about 1e-8 of the box's distance from the origin
+        assert np.allclose(cen[0], [x + 0.5 * edge for x in lo], rtol=0, atol=1e-3 * edge)
+
+
+def test_generator_outside_the_box_has_an_empty_cell_and_relax_keeps_it():
+    """set_generators has no bounding-box check (tessellation.rs:43-67): a generator at x = 5 in the
+    unit box is clipped away entirely by its neighbours' bisectors (cell_3d_faces.rs:320-326), so
+    its cell is empty (volume 0, centroid [0, 0, 0]) and relax keeps its position
+    (tessellation.rs:486-490) while the others move to their centroids."""
+    pts = np.array([[5.0, 0.5, 0.5], [0.5, 0.5, 0.5], [0.2, 0.3, 0.4], [0.8, 0.7, 0.1]])
+    t = OracleTess((0, 0, 0), (1, 1, 1), 2, 2, 2)
+    t.set_generators(pts)
+    t.calculate()
+    c0 = t.get_cell(0)
+    assert c0.vertices.size == 0 and c0.face_counts.size == 0
+    vol, cen, _, _ = t.reduced()
+    assert vol[0] == 0.0 and np.array_equal(cen[0], [0.0, 0.0, 0.0])
+    assert abs(vol[1:].sum() - 1.0) < 1e-12
+    t.relax()
+    g = t.generators().reshape(-1, 3)
+    assert np.array_equal(g[0], pts[0])
+    assert np.array_equal(g[1:], cen[1:])

@@ -196,3 +196,55 @@ def test_prune_vertices_lie_on_both_faces():
         dj = np.sum((v - gen[j]) ** 2, axis=1)
         assert np.all(np.abs(di - dj) < 1e-9)
         assert np.all(np.linalg.norm(v - 0.5, axis=1) >= R - 1e-9)
+
+
+# --- geometries of the caller's (WallGeometry implemented outside the crate) -------------------
+
+def _cells(t):
+    vol, cen, nf, _ = t.reduced()
+    return vol, cen, nf, [t.get_cell(i) for i in range(t.count_cells())]
+
+
+def _same(a, b):
+    va, ca, na, xa = a
+    vb, cb, nb, xb = b
+    assert np.array_equal(va.view(np.uint64), vb.view(np.uint64))
+    assert np.array_equal(ca.view(np.uint64), cb.view(np.uint64))
+    assert np.array_equal(na, nb)
+    for p, q in zip(xa, xb):
+        assert np.array_equal(p.vertices.view(np.uint64), q.vertices.view(np.uint64))
+        assert np.array_equal(p.face_indices, q.face_indices)
+        assert np.array_equal(p.face_neighbors, q.face_neighbors)
+
+
+def test_host_sphere_equals_builtin_sphere():
+    """The callback path (contains on every point, cut per generator, curved for sealing) gives the
+    built-in SphereGeometry's cells and seal log bit for bit."""
+    from host_geometries import PySphere
+    pts = W.PCG32(1201).uniform(4000, (0, 0, 0), (1, 1, 1))
+    a = OracleTess((0, 0, 0), (1, 1, 1), 14, 14, 14)
+    b = OracleTess((0, 0, 0), (1, 1, 1), 14, 14, 14)
+    a.add_wall_sphere(-1000, (0.5, 0.5, 0.5), 0.45)
+    b.add_wall_host(-1000, PySphere((0.5, 0.5, 0.5), 0.45))
+    for t in (a, b):
+        t.set_generators(pts)
+        t.calculate_sealed(threads=4)
+    assert a.count_generators() == b.count_generators() > 0
+    _same(_cells(a), _cells(b))
+    assert np.array_equal(a.seal_log(), b.seal_log()) and len(a.seal_log()) > 0
+
+
+def test_host_octahedron_equals_builtin_polyhedron_and_fills_it():
+    from host_geometries import PyOctahedron, octahedron_planes
+    pts = W.PCG32(1202).uniform(3000, (0, 0, 0), (1, 1, 1))
+    planes = octahedron_planes((0.5, 0.5, 0.5), 0.45)
+    a = OracleTess((0, 0, 0), (1, 1, 1), 12, 12, 12)
+    b = OracleTess((0, 0, 0), (1, 1, 1), 12, 12, 12)
+    a.add_wall(-1000, 5, pts=[list(q) + list(n) for q, n in planes])
+    b.add_wall_host(-1000, PyOctahedron((0.5, 0.5, 0.5), 0.45))
+    for t in (a, b):
+        t.set_generators(pts)
+        t.calculate(threads=4)
+    _same(_cells(a), _cells(b))
+    vol, _, _, _ = b.reduced()
+    assert abs(vol.sum() - 4.0 / 3.0 * 0.45 ** 3) < 1e-12

@@ -40,6 +40,17 @@ class VdWall(ctypes.Structure):
                 ("pts", ctypes.c_void_p), ("npts", ctypes.c_uint32), ("closed", ctypes.c_int32)]
 
 
+# struct vd_wall_host of include/vd.h: a wall whose geometry is evaluated by the caller
+HOST_CONTAINS = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
+HOST_CUT = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double), ctypes.c_int)
+
+
+class VdWallHost(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("contains", HOST_CONTAINS), ("cut", HOST_CUT),
+                ("planar", ctypes.c_int)]
+
+
 VD_WALL_PLANE, VD_WALL_SPHERE, VD_WALL_CYLINDER, VD_WALL_CONE, VD_WALL_TORUS = 0, 1, 2, 3, 4
 VD_WALL_POLYHEDRON, VD_WALL_TUBE = 5, 6
 
@@ -95,6 +106,7 @@ SIGNATURES = {
     "vd_batch_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_batch_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "vd_grid_for": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_double, _vp]),
+    "vd_tess_add_wall_planes_per_generator": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(VdWallHost)]),
     "vd_calculate_static": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _vp, ctypes.c_int, _vp, _sz, _vp, _vp,
                                            _vp, ctypes.POINTER(_sz), _vp]),
     "vd_grid_for_points": (ctypes.c_int, [_vp, _vp, _vp, _sz, ctypes.c_double, _vp]),

@@ -7,6 +7,7 @@
 // one persistent launch.
 #include "vd.h"
 #include "vd_internal.h"
+#include "vd_walls.h"
 
 #include <algorithm>
 #include <chrono>
@@ -339,6 +340,7 @@ struct Member {
     vd::GridGeom grid{};
     std::vector<vd::WallDev> walls;
     std::vector<std::vector<double>> wdata;   // per wall: polyhedron planes or tube samples
+    std::vector<vd_wall_host> host;           // the caller's walls (kind 7), by their npts index
     int shape = -1;
     uint64_t raw_off = 0, nraw = 0, gen_off = 0, ngen = 0, bin_off = 0, nbins = 0;
 };
@@ -447,6 +449,12 @@ struct Engine {
     uint64_t cap_cells = 0;            // cells the arena capacities were sized for
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
     uint32_t ntube = 0;                // tube-wall slots per generator
+    // the caller's walls (vd_tess_add_wall_planes_per_generator): their contains verdicts per raw
+    // point and cut planes per generator, evaluated on the host at every set_generators
+    uint32_t nhost = 0;                // such walls per generator (slots of d_host_off)
+    DevBuf d_hflags, d_host_off, d_host_planes;
+    std::vector<uint32_t> host_max;    // per such wall: most planes it emits for one generator
+    uint32_t seal_slots = vd::kMaxWalls;   // seal-log entries per neighbour
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
     uint64_t nraw = 0, ngen = 0, nbins = 0;
     bool prepared = false, calculated = false;
@@ -491,6 +499,9 @@ void engine_free(Engine &E) {
     release(E.d_out1);
     release(E.d_wdata);
     release(E.d_tube);
+    release(E.d_hflags);
+    release(E.d_host_off);
+    release(E.d_host_planes);
     DevBuf *post[] = {&E.d_smask, &E.d_scnt, &E.d_soff, &E.d_snb, &E.d_slog, &E.d_slogcnt, &E.d_pcnt, &E.d_poff,
                       &E.d_pi, &E.d_pp};
     for (DevBuf *b : post) release(*b);
@@ -699,6 +710,47 @@ void engine_layout(Engine &E) {
     E.nbins = b;
 }
 
+// The cut planes of the caller's walls for every generator (WallGeometry::cut, src/wall/mod.rs:38-40),
+// evaluated on the host: generator g's planes of its member's k-th such wall are
+// d_host_planes[6 q], d_host_off[g * nhost + k] <= q < d_host_off[g * nhost + k + 1].
+int host_planes(Engine &E) {
+    const uint64_t ng = E.ngen, nh = E.nhost;
+    std::vector<double> gen(3 * ng);
+    if (ng) VD_HIP(hipMemcpyAsync(gen.data(), E.d_gen.p, ng * 24, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    std::vector<uint64_t> off(ng * nh + 1, 0);
+    std::vector<double> planes;
+    std::vector<double> buf(6 * 16);
+    E.host_max.assign(nh, 0u);
+    for (const Member &m : E.mem) {
+        for (uint64_t g = m.gen_off; g < m.gen_off + m.ngen; ++g) {
+            for (uint64_t k = 0; k < nh; ++k) {
+                if (k < m.host.size()) {
+                    const vd_wall_host &cb = m.host[k];
+                    int c = cb.cut(cb.ctx, &gen[3 * g], buf.data(), (int)(buf.size() / 6));
+                    if (c > (int)(buf.size() / 6)) {
+                        if (c > VD_WALL_MAX_POINTS) return fail(VD_EINVAL, "a wall's cut emitted too many planes");
+                        buf.resize(6 * (size_t)c);
+                        c = cb.cut(cb.ctx, &gen[3 * g], buf.data(), c);
+                    }
+                    if (c < 0 || c > (int)(buf.size() / 6)) return fail(VD_EINVAL, "a wall's cut returned a bad count");
+                    planes.insert(planes.end(), buf.begin(), buf.begin() + 6 * (size_t)c);
+                    E.host_max[k] = std::max<uint32_t>(E.host_max[k], (uint32_t)c);
+                }
+                off[g * nh + k + 1] = planes.size() / 6;
+            }
+        }
+    }
+    VD_TRY(ensure(E.d_host_off, off.size() * 8));
+    VD_TRY(ensure(E.d_host_planes, std::max<size_t>(planes.size(), 6) * 8));
+    VD_HIP(hipMemcpyAsync(E.d_host_off.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, E.stream));
+    if (!planes.empty())
+        VD_HIP(hipMemcpyAsync(E.d_host_planes.p, planes.data(), planes.size() * 8, hipMemcpyHostToDevice,
+                              E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    return VD_OK;
+}
+
 // set_generators for every member from E.d_raw (tessellation.rs:43-67): wall filter in input order,
 // compaction, then the stable counting-sort binning (algo_3d_grid.rs:111-130).
 int engine_prepare(Engine &E) {
@@ -717,6 +769,28 @@ int engine_prepare(Engine &E) {
     VD_HIP(vd::launch_member_of(E.d_offs.as<uint64_t>(), (uint32_t)M, E.d_raw_tess.as<uint32_t>(), n, E.stream));
     bool walls = false;
     for (const Member &m : E.mem) walls |= !m.walls.empty();
+    // the caller's walls: contains on the host for every raw point, in wall order
+    E.nhost = 0;
+    for (const Member &m : E.mem) E.nhost = std::max<uint32_t>(E.nhost, (uint32_t)m.host.size());
+    const uint32_t *hflags = nullptr;
+    if (E.nhost && n) {
+        std::vector<double> raw(3 * n);
+        VD_HIP(hipMemcpyAsync(raw.data(), E.d_raw.p, n * 24, hipMemcpyDeviceToHost, E.stream));
+        VD_HIP(hipStreamSynchronize(E.stream));
+        std::vector<uint32_t> hf(n + 1, 1u);
+        for (const Member &m : E.mem) {
+            if (m.host.empty()) continue;
+            for (uint64_t g = m.raw_off; g < m.raw_off + m.nraw; ++g)
+                for (const vd::WallDev &w : m.walls)
+                    if (w.kind == 7) {
+                        const vd_wall_host &cb = m.host[w.npts];
+                        if (!cb.contains(cb.ctx, &raw[3 * g])) { hf[g] = 0u; break; }
+                    }
+        }
+        VD_TRY(ensure(E.d_hflags, (n + 1) * 4));
+        VD_HIP(hipMemcpyAsync(E.d_hflags.p, hf.data(), (n + 1) * 4, hipMemcpyHostToDevice, E.stream));
+        hflags = E.d_hflags.as<uint32_t>();
+    }
     // descriptors with the raw layout (the wall filter reads the walls)
     for (Member &m : E.mem) { m.gen_off = m.raw_off; m.ngen = m.nraw; }
     VD_TRY(upload_desc(E));
@@ -733,7 +807,7 @@ int engine_prepare(Engine &E) {
         VD_TRY(ensure(E.d_pos, (n + 1) * 4));
         VD_HIP(vd::launch_fill_u32(E.d_flags.as<uint32_t>(), 0u, n + 1, E.stream));
         VD_HIP(vd::launch_wall_flags(E.d_raw.as<double>(), n, E.d_desc.as<vd::TessDesc>(),
-                                     E.d_raw_tess.as<uint32_t>(), E.d_flags.as<uint32_t>(), E.stream));
+                                     E.d_raw_tess.as<uint32_t>(), hflags, E.d_flags.as<uint32_t>(), E.stream));
         VD_TRY(scan(E, E.d_flags.as<uint32_t>(), E.d_pos.as<uint32_t>(), (uint32_t)(n + 1)));
         VD_HIP(vd::launch_compact(E.d_raw.as<double>(), E.d_raw_tess.as<uint32_t>(), E.d_flags.as<uint32_t>(),
                                   E.d_pos.as<uint32_t>(), n, E.d_gen.as<double>(), E.d_gen_tess.as<uint32_t>(),
@@ -752,6 +826,8 @@ int engine_prepare(Engine &E) {
         E.ngen = kept[M];
         VD_TRY(upload_desc(E));
     }
+    // the caller's walls: cut planes of every kept generator, as CSR over (generator, wall)
+    if (E.nhost) VD_TRY(host_planes(E));
     // buffers of the binning (it runs at the start of calculate, see enqueue_binning)
     const uint64_t nb = E.nbins, ng = E.ngen;
     VD_TRY(ensure(E.d_counts, (nb + 1) * 4));
@@ -804,6 +880,10 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.bin_start = E.d_bin_start.as<uint32_t>();
     a.tube_planes = E.d_tube.as<double>();
     a.ntube = E.ntube;
+    a.host_off = E.nhost ? E.d_host_off.as<uint64_t>() : nullptr;
+    a.host_planes = E.nhost ? E.d_host_planes.as<double>() : nullptr;
+    a.nhost = E.nhost;
+    a.seal_slots = E.seal_slots;
     if (E.seal) {
         a.seal_mask = E.d_smask.as<uint32_t>();
         a.seal_off = E.d_soff.as<uint32_t>();
@@ -971,12 +1051,13 @@ int far_lists(Engine &E, const uint32_t *d_list, uint32_t n, DevBuf &lsh, DevBuf
 std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
                                E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.reserve_cus,
-                               (uint64_t)E.mem.size(), E.ntube, (uint64_t)E.seal};
+                               (uint64_t)E.mem.size(), E.ntube, (uint64_t)E.seal, E.nhost, E.seal_slots};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
                             &E.d_top, &E.d_queue, &E.d_counts4, &E.d_out0, &E.d_out1, &E.d_tube,
-                            &E.d_wdata, &E.d_smask, &E.d_soff, &E.d_snb, &E.d_slog, &E.d_slogcnt};
+                            &E.d_wdata, &E.d_smask, &E.d_soff, &E.d_snb, &E.d_slog, &E.d_slogcnt,
+                            &E.d_host_off, &E.d_host_planes};
     for (const DevBuf *b : bufs) k.push_back(reinterpret_cast<uint64_t>(b->p));
     k.push_back(reinterpret_cast<uint64_t>(E.h_counts));
     const size_t words = E.desc.size() * sizeof(vd::TessDesc) / 8;
@@ -1303,7 +1384,14 @@ int engine_cell_topology(Engine &E, bool lists) {
     VD_HIP(hipMemcpyAsync(&total, E.d_soff.as<uint32_t>() + n, 4, hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
     VD_TRY(ensure(E.d_snb, ((size_t)total + 1) * 4));
-    VD_TRY(ensure(E.d_slog, ((size_t)total + 1) * 3 * vd::kMaxWalls * 4));
+    // log room per neighbour: one entry per curved-wall plane the neighbour's generator emits
+    uint32_t slots = 0;
+    for (const Member &m : E.mem)
+        for (const vd::WallDev &w : m.walls)
+            if (!vd::wall_planar(w))
+                slots += w.kind == 7 ? (w.npts < E.host_max.size() ? E.host_max[w.npts] : 1u) : 1u;
+    E.seal_slots = std::max<uint32_t>((uint32_t)vd::kMaxWalls, slots);
+    VD_TRY(ensure(E.d_slog, ((size_t)total + 1) * 3 * E.seal_slots * 4));
     VD_TRY(ensure(E.d_slogcnt, (n + 1) * 4));
     VD_HIP(vd::launch_cell_neighbors(E.d_desc.as<vd::TessDesc>(), E.d_gen_tess.as<uint32_t>(), n,
                                      E.d_nface.as<uint32_t>(), E.d_cell_off.as<uint64_t>(), E.d_fnb.as<int32_t>(),
@@ -1329,13 +1417,13 @@ int engine_calculate_sealed(Engine &E, std::vector<int32_t> &log) {
     VD_HIP(hipMemcpyAsync(cnt.data(), E.d_slogcnt.p, n * 4, hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipMemcpyAsync(off.data(), E.d_soff.p, (n + 1) * 4, hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
-    std::vector<int32_t> all((size_t)off[n] * 3 * vd::kMaxWalls);
+    std::vector<int32_t> all((size_t)off[n] * 3 * E.seal_slots);
     if (!all.empty())
         VD_HIP(hipMemcpyAsync(all.data(), E.d_slog.p, all.size() * 4, hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
     for (uint64_t i = 0; i < n; ++i)
         for (uint32_t k = 0; k < cnt[i]; ++k)
-            for (int c = 0; c < 3; ++c) log.push_back(all[(size_t)off[i] * 3 * vd::kMaxWalls + 3 * k + c]);
+            for (int c = 0; c < 3; ++c) log.push_back(all[(size_t)off[i] * 3 * E.seal_slots + 3 * k + c]);
     return VD_OK;
 }
 
@@ -1763,10 +1851,9 @@ void vd_tess_destroy(vd_tess *t) {
     delete t;
 }
 
-static int tess_add_wall(vd_tess *t, const vd_wall &w) {
-    vd::WallDev d;
-    std::vector<double> data;
-    VD_TRY(make_wall(w, d, data));
+// add_wall (tessellation.rs:175-200) of a wall in device form; host: the caller's callbacks for
+// a wall of kind 7 (nullptr otherwise).
+static int tess_add_wall_dev(vd_tess *t, vd::WallDev d, std::vector<double> data, const vd_wall_host *host) {
     Engine &E = t->E;
     Member &m = E.mem[0];
     if ((int)m.walls.size() >= vd::kMaxWalls) return fail(VD_EINVAL, "too many walls (max 8)");
@@ -1786,6 +1873,10 @@ static int tess_add_wall(vd_tess *t, const vd_wall &w) {
         m.nraw = n;
         engine_layout(E);
     }
+    if (host) {
+        d.npts = (uint32_t)m.host.size();
+        m.host.push_back(*host);
+    }
     m.walls.push_back(d);
     m.wdata.push_back(std::move(data));
     const int rc = n ? engine_prepare(E) : upload_desc(E);
@@ -1793,10 +1884,28 @@ static int tess_add_wall(vd_tess *t, const vd_wall &w) {
         const std::string msg = g_err;
         m.walls.pop_back();
         m.wdata.pop_back();
+        if (host) m.host.pop_back();
         (void)(n ? engine_prepare(E) : upload_desc(E));
         return fail(rc, msg);
     }
     return VD_OK;
+}
+
+static int tess_add_wall(vd_tess *t, const vd_wall &w) {
+    vd::WallDev d;
+    std::vector<double> data;
+    VD_TRY(make_wall(w, d, data));
+    return tess_add_wall_dev(t, d, std::move(data), nullptr);
+}
+
+int vd_tess_add_wall_planes_per_generator(vd_tess *t, int32_t id, const vd_wall_host *wall) {
+    if (!t || !wall || !wall->contains || !wall->cut) return fail(VD_EINVAL, "NULL argument");
+    if (id > VD_WALL_ID_MAX) return fail(VD_EINVAL, "Wall ID must be <= -1000");   // wall/mod.rs:25-27
+    vd::WallDev d{};
+    d.id = id;
+    d.kind = vd::kWallHost;
+    d.closed = wall->planar ? 1 : 0;
+    return tess_add_wall_dev(t, d, {}, wall);
 }
 
 int vd_tess_add_wall_plane(vd_tess *t, int32_t id, const double point[3], const double normal[3]) {
@@ -1827,6 +1936,7 @@ int vd_tess_clear_walls(vd_tess *t) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
     t->E.mem[0].walls.clear();
     t->E.mem[0].wdata.clear();
+    t->E.mem[0].host.clear();
     t->E.calculated = false;
     return upload_desc(t->E);
 }

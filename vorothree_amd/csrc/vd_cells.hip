@@ -1121,6 +1121,15 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                 const double *tp = A.tube_planes + 7 * ((size_t)i * A.ntube + tk);
                 has = wp == 0 && tp[6] != 0.0;
                 q = WallPlane{tp[0], tp[1], tp[2], tp[3], tp[4], tp[5]};
+            } else if (W.kind == kWallHost) {
+                // the caller's cut planes for this generator, evaluated on the host
+                const uint64_t *ho = A.host_off + (size_t)i * A.nhost + W.npts;
+                const uint64_t pq = ho[0] + wp;
+                has = pq < ho[1];
+                if (has) {
+                    const double *hp = A.host_planes + 6 * pq;
+                    q = WallPlane{hp[0], hp[1], hp[2], hp[3], hp[4], hp[5]};
+                }
             } else {
                 has = wall_plane(W, wp, gx, gy, gz, q);
             }
@@ -1315,7 +1324,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
         uint32_t nlog = 0;
         if (status == kCellOk && mi) {
             const uint32_t q0 = A.seal_off[i], q1 = A.seal_off[i + 1];
-            int32_t *lg = A.seal_log + 3 * (size_t)kMaxWalls * q0;
+            int32_t *lg = A.seal_log + 3 * (size_t)A.seal_slots * q0;
             for (uint32_t q = q0; q < q1 && status == kCellOk; ++q) {
                 const uint32_t j = A.seal_nb[q];
                 const size_t gj = D.gen_off + j;
@@ -1325,21 +1334,28 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                 for (int w = 0; w < nwalls; ++w) {
                     const WallDev &W = D.walls[w];
                     const bool tube = W.kind == kWallTube;
-                    const bool use = W.kind != kWallPlane && W.kind != kWallPolyhedron && ((mi >> w) & 1u) &&
-                                     ((mj >> w) & 1u);
+                    const bool use = !wall_planar(W) && ((mi >> w) & 1u) && ((mj >> w) & 1u);
+                    // every plane the wall's cut emits for the neighbour's generator, in order
+                    // (one for the built-in curved walls)
+                    for (uint32_t wp = 0; use; ++wp) {
                     WallPlane q{};
                     bool has = false;
-                    if (use) {
-                        if (tube) {
-                            const double *tp = A.tube_planes + 7 * (gj * A.ntube + tk);
-                            has = tp[6] != 0.0;
-                            q = WallPlane{tp[0], tp[1], tp[2], tp[3], tp[4], tp[5]};
-                        } else {
-                            has = wall_plane(W, 0u, nx, ny, nz, q);
+                    if (tube) {
+                        const double *tp = A.tube_planes + 7 * (gj * A.ntube + tk);
+                        has = wp == 0 && tp[6] != 0.0;
+                        q = WallPlane{tp[0], tp[1], tp[2], tp[3], tp[4], tp[5]};
+                    } else if (W.kind == kWallHost) {
+                        const uint64_t *ho = A.host_off + gj * A.nhost + W.npts;
+                        const uint64_t pq = ho[0] + wp;
+                        has = pq < ho[1];
+                        if (has) {
+                            const double *hp = A.host_planes + 6 * pq;
+                            q = WallPlane{hp[0], hp[1], hp[2], hp[3], hp[4], hp[5]};
                         }
+                    } else {
+                        has = wall_plane(W, wp, nx, ny, nz, q);
                     }
-                    if (tube) ++tk;
-                    if (!has) continue;
+                    if (!has) break;
                     double dot = 0.0;
                     dot += (gx - q.px) * q.nx;
                     dot += (gy - q.py) * q.ny;
@@ -1360,6 +1376,9 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                         }
                         ++nlog;
                     }
+                    }
+                    if (tube) ++tk;
+                    if (status != kCellOk) break;
                 }
             }
         }

@@ -32,13 +32,16 @@ __global__ void k_member_of(const uint64_t *__restrict__ offs, uint32_t nmem, ui
     tess[g] = lo;
 }
 
+// Keeps point g iff every wall contains it (tessellation.rs:43-67); hflags (optional) holds the
+// verdict of the caller's walls, evaluated on the host.
 __global__ void k_wall_flags(const double *__restrict__ xyz, uint64_t n, const TessDesc *__restrict__ desc,
-                             const uint32_t *__restrict__ tess, uint32_t *__restrict__ flags) {
+                             const uint32_t *__restrict__ tess, const uint32_t *__restrict__ hflags,
+                             uint32_t *__restrict__ flags) {
     const uint64_t g = gid();
     if (g >= n) return;
     const TessDesc &D = desc[tess[g]];
     const double x = xyz[3 * g], y = xyz[3 * g + 1], z = xyz[3 * g + 2];
-    uint32_t in = 1;
+    uint32_t in = hflags ? hflags[g] : 1u;
     for (int k = 0; k < D.nwalls; ++k)
         if (!wall_contains(D.walls[k], x, y, z)) { in = 0; break; }
     flags[g] = in;
@@ -70,10 +73,6 @@ __global__ void k_tube_planes(const double *__restrict__ gen, uint64_t n, const 
 
 // ---- post-passes: calculate_sealed / prune_boundaries (tessellation.rs:234-385) -------------
 
-__device__ inline bool wall_planar(int32_t kind) {   // WallGeometry::is_planar (wall_3d.rs:44, 709)
-    return kind == kWallPlane || kind == kWallPolyhedron;
-}
-
 // The walls whose id is in a cell's curved-wall set: for every face whose neighbour id is not a
 // generator of the tessellation, the first wall with that id decides (walls.iter().find); if it is
 // curved, the id joins the set (tessellation.rs:248-257), and every wall with that id gets its bit.
@@ -86,7 +85,7 @@ __device__ inline uint32_t curved_wall_mask(const TessDesc &D, const int32_t *fl
         int w0 = -1;
         for (int w = 0; w < D.nwalls; ++w)
             if (D.walls[w].id == fn) { w0 = w; break; }
-        if (w0 < 0 || wall_planar(D.walls[w0].kind)) continue;
+        if (w0 < 0 || wall_planar(D.walls[w0])) continue;
         for (int w = 0; w < D.nwalls; ++w)
             if (D.walls[w].id == fn) m |= 1u << w;
     }
@@ -153,7 +152,7 @@ __global__ void k_prune(const TessDesc *__restrict__ desc, const uint32_t *__res
             const uint32_t mj = mask[D.gen_off + (uint32_t)j];
             for (int w = 0; w < D.nwalls; ++w) {
                 const WallDev &W = D.walls[w];
-                if (wall_planar(W.kind) || !((mi >> w) & 1u) || !((mj >> w) & 1u)) continue;
+                if (wall_planar(W) || !((mi >> w) & 1u) || !((mj >> w) & 1u)) continue;
                 // shared_vertices(W.id, j)
                 uint32_t c = 0;
                 uint32_t s = 0;
@@ -414,9 +413,9 @@ hipError_t launch_member_of(const uint64_t *offs, uint32_t nmem, uint32_t *tess,
 }
 
 hipError_t launch_wall_flags(const double *xyz, uint64_t n, const TessDesc *desc, const uint32_t *tess,
-                             uint32_t *flags, hipStream_t st) {
+                             const uint32_t *hflags, uint32_t *flags, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    k_wall_flags<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, desc, tess, flags);
+    k_wall_flags<<<blocks_for(n), kBlock, 0, st>>>(xyz, n, desc, tess, hflags, flags);
     return hipGetLastError();
 }
 

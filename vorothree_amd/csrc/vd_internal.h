@@ -42,7 +42,10 @@ __host__ __device__ inline double shell_key(int dx, int dy, int dz, const double
 }
 
 // A wall as the kernels see it (src/wall/wall_3d.rs). kind: 0 plane, 1 sphere, 2 cylinder, 3 cone,
-// 4 torus, 5 convex polyhedron, 6 tube around a sampled curve (the vd_wall kinds of vd.h).
+// 4 torus, 5 convex polyhedron, 6 tube around a sampled curve (the vd_wall kinds of vd.h), 7 a
+// geometry of the caller's (vd_tess_add_wall_planes_per_generator): its contains verdicts and cut
+// planes are evaluated on the host per generator; npts is its index among the member's such walls
+// and closed its is_planar flag.
 struct WallDev {
     int32_t id;
     int32_t kind;
@@ -93,6 +96,11 @@ struct CellArgs {
     const uint32_t *bin_start;    // concatenated bins + 1 (positions in spos)
     const double *tube_planes;    // per generator and tube wall: plane point, normal, emitted flag
     uint32_t ntube;               // tube-wall slots per generator (0: no member has one)
+    // planes of the caller's walls (kind 7): generator g's planes of its k-th such wall are
+    // host_planes[6 q] for host_off[g * nhost + k] <= q < host_off[g * nhost + k + 1]
+    const uint64_t *host_off;
+    const double *host_planes;
+    uint32_t nhost;
 
     // work list: generator indices (global) to compute
     const uint32_t *cells;
@@ -127,8 +135,9 @@ struct CellArgs {
     const uint32_t *seal_mask;    // per cell: walls (bit = wall index) of its curved-wall set
     const uint32_t *seal_off;     // per cell: its neighbour list in seal_nb (n + 1 offsets)
     const uint32_t *seal_nb;      // member-local neighbour indices in face order
-    int32_t *seal_log;            // (cell, neighbour, wall id) per modifying clip, kMaxWalls
-                                  // entries per neighbour from 3 * kMaxWalls * seal_off[i]
+    int32_t *seal_log;            // (cell, neighbour, wall id) per modifying clip, seal_slots
+                                  // entries per neighbour from 3 * seal_slots * seal_off[i]
+    uint32_t seal_slots;
     uint32_t *seal_logcnt;        // per cell: log entries written
 };
 
@@ -167,7 +176,7 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_member_of(const uint64_t *offs, uint32_t nmem, uint32_t *tess, uint64_t n,
                             hipStream_t st);   // tess[g] = member t with offs[t] <= g < offs[t+1]
 hipError_t launch_wall_flags(const double *xyz, uint64_t n, const TessDesc *desc,
-                             const uint32_t *tess, uint32_t *flags, hipStream_t st);
+                             const uint32_t *tess, const uint32_t *hflags, uint32_t *flags, hipStream_t st);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *tmp,
                            size_t tmp_words, hipStream_t st);
 size_t scan_tmp_words(uint32_t n);

@@ -17,7 +17,14 @@ enum : int32_t {
     kWallTorus = 4,
     kWallPolyhedron = 5,
     kWallTube = 6,
+    kWallHost = 7,   // the caller's geometry: contains and cut evaluated on the host
 };
+
+// WallGeometry::is_planar (wall_3d.rs:44, 709; mod.rs:66-70 for the caller's geometry, whose flag
+// is in `closed`).
+__host__ __device__ inline bool wall_planar(const WallDev &w) {
+    return w.kind == kWallPlane || w.kind == kWallPolyhedron || (w.kind == kWallHost && w.closed);
+}
 
 // Segments of a tube's sample polyline: n in a closed loop, n - 1 open (wall_3d.rs:412, 769, 915).
 __device__ inline uint32_t tube_segments(const WallDev &w) {
@@ -120,6 +127,7 @@ struct WallPlane { double px, py, pz, nx, ny, nz; };
 __device__ inline bool wall_plane(const WallDev &w, uint32_t j, double gx, double gy, double gz,
                                   WallPlane &out) {
     double qpx, qpy, qpz, qnx, qny, qnz;   // set case by case, written to out once
+    if (w.kind == kWallHost) return false;   // its planes come from the host, per generator
     if (w.kind == kWallPolyhedron) {
         if (j >= w.npts) return false;
         const double *q = w.pts + 6 * (size_t)j;

@@ -303,12 +303,57 @@ _GEOMETRIES = (PlaneGeometry, SphereGeometry, CylinderGeometry, ConeGeometry, To
                ConvexPolyhedronGeometry, _Tube)
 
 
+def _is_host_geometry(g) -> bool:
+    """A geometry of the caller's (the WallGeometry trait, src/wall/mod.rs:53-71): ``contains(p)``
+    and ``cut(g)`` returning the (point, normal) planes for generator g, normals pointing out of
+    the valid region; ``is_planar()`` is optional (False by default, as in the trait)."""
+    return callable(getattr(g, "contains", None)) and callable(getattr(g, "cut", None))
+
+
+class HostWallCallbacks:
+    """ctypes callbacks (struct vd_wall_host) around a geometry of the caller's. An exception raised
+    by the geometry is kept and re-raised by ``check_error`` after the library call returns."""
+
+    def __init__(self, geometry):
+        self.geometry = geometry
+        self.error = None
+
+        def contains(_ctx, p):
+            try:
+                return 1 if self.geometry.contains((p[0], p[1], p[2])) else 0
+            except Exception as e:   # noqa: BLE001 - re-raised by check_error
+                self.error = self.error or e
+                return 0
+
+        def cut(_ctx, g, out, cap):
+            try:
+                planes = list(self.geometry.cut((g[0], g[1], g[2])))
+            except Exception as e:   # noqa: BLE001 - re-raised by check_error
+                self.error = self.error or e
+                return 0
+            for k, (pt, nm) in enumerate(planes[:cap]):
+                for c in range(3):
+                    out[6 * k + c] = float(pt[c])
+                    out[6 * k + 3 + c] = float(nm[c])
+            return len(planes)
+
+        self._contains = _lib.HOST_CONTAINS(contains)
+        self._cut = _lib.HOST_CUT(cut)
+        planar = getattr(geometry, "is_planar", None)
+        self.struct = _lib.VdWallHost(None, self._contains, self._cut, 1 if (planar and planar()) else 0)
+
+    def check_error(self):
+        if self.error is not None:
+            e, self.error = self.error, None
+            raise e
+
+
 class Wall:
     def __init__(self, id: int, geometry):  # noqa: A002
         if id > WALL_ID_MAX:
             raise ValueError(f"Wall ID must be <= {WALL_ID_MAX}")
-        if not isinstance(geometry, _GEOMETRIES):
-            raise TypeError("unsupported wall geometry")
+        if not isinstance(geometry, _GEOMETRIES) and not _is_host_geometry(geometry):
+            raise TypeError("a wall geometry is a built-in one or has contains(p) and cut(g)")
         self._id = int(id)
         self.geometry = geometry
 
@@ -416,6 +461,7 @@ class Tessellation:
         self.bounds = bounds
         self.algorithm = algorithm
         self._walls = []
+        self._host = []   # callbacks of the caller's wall geometries, alive while the handle holds them
         h = ctypes.c_void_p()
         check(lib().vd_tess_create(_d3(bounds.min), _d3(bounds.max), algorithm.grid_res_x,
                                    algorithm.grid_res_y, algorithm.grid_res_z, ctypes.byref(h)))
@@ -441,15 +487,26 @@ class Tessellation:
     def set_generators(self, generators) -> None:
         a = np.ascontiguousarray(np.asarray(generators, dtype=np.float64).reshape(-1))
         n = a.size // 3
-        check(lib().vd_tess_set_generators(self._h, _ptr(a), n))
+        rc = lib().vd_tess_set_generators(self._h, _ptr(a), n)
+        self._host_errors()
+        check(rc)
 
     def set_generators_device(self, ptr: int, n: int) -> None:
         """Points already in device memory (e.g. a torch CUDA tensor's data_ptr())."""
-        check(lib().vd_tess_set_generators_device(self._h, ctypes.c_void_p(int(ptr)), int(n)))
+        rc = lib().vd_tess_set_generators_device(self._h, ctypes.c_void_p(int(ptr)), int(n))
+        self._host_errors()
+        check(rc)
 
     def add_wall(self, wall: Wall) -> None:
         g = wall.geometry
-        if isinstance(g, PlaneGeometry):
+        if not isinstance(g, _GEOMETRIES):
+            # any other geometry: contains and cut evaluated here, on the host, through callbacks
+            cb = HostWallCallbacks(g)
+            rc = lib().vd_tess_add_wall_planes_per_generator(self._h, wall.id(), ctypes.byref(cb.struct))
+            cb.check_error()
+            check(rc)
+            self._host.append(cb)
+        elif isinstance(g, PlaneGeometry):
             check(lib().vd_tess_add_wall_plane(self._h, wall.id(), _d3(g.point), _d3(g.normal_raw)))
         elif isinstance(g, SphereGeometry):
             check(lib().vd_tess_add_wall_sphere(self._h, wall.id(), _d3(g.center), g.radius))
@@ -461,12 +518,19 @@ class Tessellation:
     def clear_walls(self) -> None:
         check(lib().vd_tess_clear_walls(self._h))
         self._walls = []
+        self._host = []
+
+    def _host_errors(self):
+        for cb in self._host:
+            cb.check_error()
 
     def calculate(self) -> None:
         check(lib().vd_tess_calculate(self._h))
 
     def relax(self) -> None:
-        check(lib().vd_tess_relax(self._h))
+        rc = lib().vd_tess_relax(self._h)
+        self._host_errors()
+        check(rc)
 
     def calculate_sealed(self) -> None:
         """calculate(), then the curved-wall sealing pass (vd_tess_calculate_sealed)."""
