@@ -29,7 +29,9 @@ extern "C" {
 #define VD_ERANGE (-2)    /* index out of range (reference: None) */
 #define VD_ENOMEM (-3)    /* device or host allocation failed */
 #define VD_EDEVICE (-4)   /* HIP runtime error */
-#define VD_ECAPACITY (-5) /* a cell exceeded every polytope capacity tier */
+#define VD_ECAPACITY (-5) /* a cell exceeded every polytope capacity tier (32,767 vertices, 65,535
+                             face slots), has a face of 256 or more vertices (the reference's u8
+                             face count would wrap), or reruns did not finish */
 #define VD_ESTATE (-6)    /* call out of order (e.g. results before calculate) */
 
 #define VD_WALL_ID_MAX (-1000) /* src/wall/mod.rs:8 */

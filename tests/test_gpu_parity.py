@@ -194,16 +194,20 @@ def fibonacci_sphere(n, r, c):
                      c[2] + r * np.cos(phi)], axis=1)
 
 
-@pytest.mark.parametrize("nshell", [70, 400])
+@pytest.mark.parametrize("nshell", [70, 400, 1500])
 def test_large_cells_move_to_larger_tiers(nshell):
     # a generator surrounded by nshell points on a sphere: its cell has about nshell faces
-    # (70 -> past the 64-vertex fast tier; 400 -> past the 256-vertex tier)
+    # (70 -> past the 64-vertex fast tier; 400 -> past the 256-vertex tier; 1500 -> about 3,000
+    # vertices, past the 1,024-vertex LDS tier into the global-memory tier)
     bg = W.PCG32(51).uniform(300, (0, 0, 0), (1, 1, 1))
     pts = np.concatenate([[[0.5, 0.5, 0.5]], fibonacci_sphere(nshell, 0.2, (0.5, 0.5, 0.5)), bg])
     g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (5, 5, 5), pts)
     check_all_cells(g, o)
     assert g.stats()[1] > 0, "no cell reached a larger tier"
-    assert g.face_counts()[0] >= nshell // 2
+    # the background points inside the shell cut some of its faces away
+    assert g.face_counts()[0] >= (nshell // 2 if nshell < 1000 else 600)
+    if nshell >= 1500:
+        assert o.reduced([0])[3][0, 6] > 1024, "the cell never outgrew the LDS tiers"
 
 
 def test_sparse_grid_extends_shell_prefix():
@@ -420,3 +424,20 @@ def test_empty_cell_and_relax_match_oracle():
     assert np.array_equal(bits(g.generators()), bits(o.generators()))
     assert np.array_equal(g.generators().reshape(-1, 3)[0], pts[0])
     check_all_cells(g, o)
+
+
+def test_face_of_256_vertices_is_an_error():
+    # 300 generators on a circle around one in the xy plane: the centre cell's box faces at z = 0
+    # and z = 1 get 300 vertices. The reference stores face counts as u8 (cell_3d_faces.rs:455,
+    # 491), which wraps at 256 and corrupts the cell; the library reports VD_ECAPACITY instead.
+    from vorothree_amd._lib import VD_ECAPACITY, VdError
+    a = 2.0 * np.pi * np.arange(300) / 300
+    ring = np.stack([0.5 + 0.2 * np.cos(a), 0.5 + 0.2 * np.sin(a), np.full(300, 0.5)], axis=1)
+    pts = np.concatenate([[[0.5, 0.5, 0.5]], ring])
+    b = BoundingBox((0, 0, 0), (1, 1, 1))
+    t = Tessellation(b, Algorithm3DGrid(4, 4, 4, b))
+    t.set_generators(pts)
+    with pytest.raises(VdError) as e:
+        t.calculate()
+    assert e.value.code == VD_ECAPACITY and "256" in str(e.value)
+    assert t.count_cells() == 0

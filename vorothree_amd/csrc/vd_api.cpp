@@ -26,7 +26,9 @@
 
 namespace {
 
-constexpr int kTiers = 4;   // polytope capacity tiers of the cell kernel (vd_cells.hip)
+constexpr int kTiers = 5;   // polytope capacity tiers of the cell kernel (vd_cells.hip)
+constexpr int kGlobalTier = 4;      // the tier whose polytopes live in global memory
+constexpr int kGlobalSlots = 32;    // its waves (one polytope slot each)
 constexpr uint64_t kArenaChunk[3] = {512, 256, 1536};   // vertices, faces, face indices per wave chunk
 // Far cells whose rerun walks about this many shell entries or more walk a per-cell list of the
 // entries that hold generators (k_far_lists); at most kFarListCells cells per launch and
@@ -416,7 +418,7 @@ struct Engine {
     std::map<GeomKey, ShellTable> tables;   // device shell prefixes by grid geometry
     uint64_t epoch = 0;
     int device = 0, cus = 1;
-    int occ[kTiers] = {1, 1, 1, 1};
+    int occ[kTiers] = {1, 1, 1, 1, 1};
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;     // side stream for far-cell retries (shared: engine_stream2)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -453,6 +455,7 @@ struct Engine {
     // point and cut planes per generator, evaluated on the host at every set_generators
     uint32_t nhost = 0;                // such walls per generator (slots of d_host_off)
     DevBuf d_hflags, d_host_off, d_host_planes;
+    DevBuf d_gmem;                     // polytope slots of the global-memory tier
     std::vector<uint32_t> host_max;    // per such wall: most planes it emits for one generator
     uint32_t seal_slots = vd::kMaxWalls;   // seal-log entries per neighbour
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
@@ -500,6 +503,7 @@ void engine_free(Engine &E) {
     release(E.d_wdata);
     release(E.d_tube);
     release(E.d_hflags);
+    release(E.d_gmem);
     release(E.d_host_off);
     release(E.d_host_planes);
     DevBuf *post[] = {&E.d_smask, &E.d_scnt, &E.d_soff, &E.d_snb, &E.d_slog, &E.d_slogcnt, &E.d_pcnt, &E.d_poff,
@@ -911,6 +915,11 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.cell_off = E.d_cell_off.as<uint64_t>();
     VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>() + slot, 0u, 1, st));
     int waves = std::max(1, E.cus - E.reserve_cus) * E.occ[tier];
+    if (tier == kGlobalTier) {
+        waves = std::min(waves, kGlobalSlots);
+        VD_TRY(ensure(E.d_gmem, (size_t)kGlobalSlots * vd::tier_global_bytes()));
+        a.gmem = E.d_gmem.p;
+    }
     if ((uint32_t)waves > nlist) waves = (int)nlist;
     // arena chunks per wave: up to kArenaChunk[k] entries, at most an eighth of the capacity over
     // all waves (the unused tails of the last chunks are holes in the arena)

@@ -69,6 +69,15 @@ __device__ inline void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "w
 #else
 __device__ inline void wave_sync() { __syncthreads(); }
 #endif
+// The same ordering point for a tier whose polytope lives in global memory: a workgroup-scope
+// fence, which waits for the wave's outstanding vector memory accesses, so a value one lane
+// stores is seen by a later load of another lane.
+template <class C>
+__device__ inline void poly_sync() {
+    if constexpr (C::GLOBAL) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    else wave_sync();
+}
+
 // Set bits of m below this lane (v_mbcnt_lo / v_mbcnt_hi): the prefix count of a ballot.
 __device__ inline int below_lane(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -156,7 +165,7 @@ __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
     return (uint32_t)v;
 }
 
-template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int TM_, int WAVES_>
+template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int TM_, int WAVES_, bool GLOBAL_ = false>
 struct Cap {
     static constexpr int VM = VM_;      // vertices
     static constexpr int SM = SM_;      // face slots (flattened face indices)
@@ -165,6 +174,7 @@ struct Cap {
     static constexpr bool FAST = FAST_; // register fast path (VM <= 64) instead of the generic one
     static constexpr int TM = TM_;      // triangles staged in LDS for the volume/centroid folds
     static constexpr int WAVES = WAVES_;  // occupancy target (waves per SIMD) for the compiler
+    static constexpr bool GLOBAL = GLOBAL_;  // the polytope lives in global memory, not LDS
     static constexpr int NV = FAST_ ? (VM_ + 63) / 64 : 1;   // register vertices per lane
     // polytope buffers: the fast path rewrites the polytope in place, the generic one ping-pongs
     static constexpr int NB = FAST_ ? 1 : 2;
@@ -336,7 +346,7 @@ __device__ void poly_box(Mem<C> &m, Poly &P, const double lo[3], const double hi
     if (l < 7) m.fstart[0][l] = (uint16_t)(4 * l);
     if (l < 6) m.fnb[0][l] = -(int32_t)((0x214365u >> (4 * l)) & 0xf);  // -5 -6 -3 -4 -1 -2
     P.cur = 0; P.V = 8; P.F = 6; P.S = 24; P.nk = 0;
-    wave_sync();
+    poly_sync<C>();
 }
 
 // Half-space clip keeping (x - p).n <= 1e-9 (cell_3d_faces.rs:288-503). Returns kNoChange,
@@ -382,7 +392,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         r2 = 0.0;
         return kModified;
     }
-    wave_sync();
+    poly_sync<C>();
 
     // 2. per slot: output entry count and plane crossings (:378-452)
     int K = 0, ptot = 0;
@@ -410,7 +420,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     }
     if (l == 0) m.sc.P[P.S] = (uint16_t)ptot;
     if (K > C::KM || ptot > 0xffff) return kOverflow;
-    wave_sync();
+    poly_sync<C>();
 
     // 3. per face: surviving faces (>= 3 entries, :454-456) and their new offsets
     int Fn = 0, Spre = 0;
@@ -479,12 +489,12 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         }
     }
     if (nk + nint > C::VM) return kOverflow;
-    wave_sync();
+    poly_sync<C>();
     for (int ci = l; ci < K; ci += 64) {
         const uint16_t v = m.sc.cxid[ci];
         if (v & 0x8000) m.sc.cxid[ci] = m.sc.cxid[v & 0x7fff];
     }
-    wave_sync();
+    poly_sync<C>();
 
     // 5. emit the surviving faces' entries (:386-451, :466)
     int Kb = 0;
@@ -517,7 +527,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         }
     }
     if (l == 0) m.fstart[o][Fn] = (uint16_t)Spre;
-    wave_sync();
+    poly_sync<C>();
 
     // 6. successor of every new slot; lid segments (v, u) for intersection pairs (:459-465)
     int M = 0;
@@ -538,12 +548,12 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
     // 7. lid polygon (:471-495): map[first] = second (later segments overwrite), walk from seg[0]
     if (M > 0) {
         for (int t = l; t < nint; t += 64) m.sc.lmap[t] = 0u;
-        wave_sync();
+        poly_sync<C>();
         for (int t = l; t < M; t += 64) {
             const uint32_t sg = m.sc.seg[t];
             atomicMax(&m.sc.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
         }
-        wave_sync();
+        poly_sync<C>();
         const uint32_t s0 = m.sc.seg[0];
         const int start = (int)(s0 & 0xffffu);
         int cv = (int)(s0 >> 16);
@@ -559,7 +569,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
         }
         if (len >= 3) {
             if (Spre + len > C::SM || len > 255) return len > 255 ? kFace255 : kOverflow;
-            wave_sync();
+            poly_sync<C>();
             for (int t = l; t < len; t += 64) {
                 const uint32_t u = m.sc.lid[t], w = m.sc.lid[(t + 1) % len];
                 m.sedge[o][Spre + t] = u | (w << 16);
@@ -574,7 +584,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
             Spre += len;
         }
     }
-    wave_sync();
+    poly_sync<C>();
     P.cur = o;
     P.V = nk + nint;
     P.F = Fn;
@@ -688,7 +698,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     }
     if (K > 64) return kGeneric;
     if (l == 0) m.sc.P[P.S] = (uint16_t)ptot;
-    wave_sync();
+    poly_sync<C>();
 
     // 3. faces: survivors (>= 3 entries, :454-456) and their new offsets
     const bool fa = l < P.F;
@@ -757,7 +767,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
         }
     }
-    wave_sync();
+    poly_sync<C>();
     // the old vertices are read: store the kept ones at [0, nk) and the new ones after them
     int kbase = 0;
 #pragma unroll
@@ -769,7 +779,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         kbase += popc(inm.w[k]);
     }
     if (isf) { m.vx[o][myx] = x; m.vy[o][myx] = y; m.vz[o][myx] = z; }
-    wave_sync();
+    poly_sync<C>();
 
     // 5. emit the entries of every surviving face (:386-451), with each entry's successor slot
 #pragma unroll
@@ -798,7 +808,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             }
         }
     }
-    wave_sync();
+    poly_sync<C>();
 
     // 6. successor of every new slot; lid segments (v, u) for intersection pairs (:459-465)
     int M = 0;
@@ -821,12 +831,12 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     // 7. lid polygon (:471-495); lane j holds the map entry of new vertex nk + j
     if (M > 0) {
         m.sc.lmap[l] = 0u;   // KM >= 64 entries
-        wave_sync();
+        poly_sync<C>();
         for (int t = l; t < M; t += 64) {
             const uint32_t sg = m.sc.seg[t];
             atomicMax(&m.sc.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
         }
-        wave_sync();
+        poly_sync<C>();
         const uint32_t mv = (l < nint) ? m.sc.lmap[l] : 0u;
         // the walk runs over lane indices (vertex - nk): lane j holds the lane of new vertex
         // nk + j's successor, or 0xffff where the map has none
@@ -860,7 +870,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             Spre += len;
         }
     }
-    wave_sync();
+    poly_sync<C>();
     P.cur = o;
     P.V = nk + nint;
     P.F = Fn;
@@ -949,12 +959,12 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
         }
     }
     if (staged) {
-        wave_sync();
+        poly_sync<C>();
         double acc = 0.0;
         if (l < 4)
             for (int t = 0; t < T; ++t) acc += m.tri[l][t];
         tv = rld(acc, 0); sx = rld(acc, 1); sy = rld(acc, 2); sz = rld(acc, 3);
-        wave_sync();
+        poly_sync<C>();
     }
     vol = fabs(tv / 6.0);
     if (fabs(tv) < 1e-9) {
@@ -1216,7 +1226,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                         m.winoff[pos + t] = (uint16_t)(skip + t);
                     }
                 }
-                wave_sync();
+                poly_sync<C>();
                 const bool wa = l < nwin;
                 const uint32_t wq = wa ? m.winq[l] : 0u;
                 wko = wa ? ((uint32_t)m.winent[l] | ((uint32_t)m.winoff[l] << 8)) : 0u;
@@ -1233,7 +1243,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                           ? pretest<C>(m, P, wa && wj != li, wd0, wd1, wd2, gx, gy, gz, 0, P.V) : ~0ull;
                 last = -1;
                 win = true;
-                wave_sync();
+                poly_sync<C>();
             }
             // next candidate of the window that passes the distance test (tessellation.rs:458)
             const double lim = 4.0 * r2;
@@ -1314,7 +1324,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
         const double wbd = lane_get_d(bdist, wk);   // bin distance of the candidate's entry
         wvm &= ballot(l <= cl) | ballot(wk == kc) | (ballot(wk < kbc) & ballot(wbd <= lim));
     }
-    wave_sync();
+    poly_sync<C>();
 
     if constexpr (SEAL) {
         // calculate_sealed (tessellation.rs:266-315): for every neighbour of the previous pass, in
@@ -1391,7 +1401,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
     if (lo == 0) A.status[i] = status;
     if (status != kCellOk) {
         if (status == kCellShell && lo == 0) A.vol[i] = r2;
-        wave_sync();
+        poly_sync<C>();
         return;
     }
     double vol, cen[3];
@@ -1438,13 +1448,11 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
             if (lo == 0) A.cell_off[3 * (size_t)i + 1] = of;
         }
     }
-    wave_sync();
+    poly_sync<C>();
 }
 
 template <class C, bool SEAL, bool PROBE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C::WAVES, 8)))
-void k_cells(CellArgs A) {
-    __shared__ Mem<C> m;
+__device__ inline void cells_loop(Mem<C> &m, const CellArgs &A) {
     ArenaChunks ac;
     for (;;) {
         uint32_t q = 0;
@@ -1459,6 +1467,21 @@ void k_cells(CellArgs A) {
             if (ac.used[k]) atomicAdd((unsigned long long *)&A.arena_top[3 + k], (unsigned long long)ac.used[k]);
 }
 
+template <class C, bool SEAL, bool PROBE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C::WAVES, 8)))
+void k_cells(CellArgs A) {
+    __shared__ Mem<C> m;
+    cells_loop<C, SEAL, PROBE>(m, A);
+}
+
+// The global-memory tier: wave b builds its cells in slot b of A.gmem (the launch has at most as
+// many waves as slots).
+template <class C, bool SEAL>
+__global__ __launch_bounds__(64) void k_cells_global(CellArgs A) {
+    Mem<C> &m = reinterpret_cast<Mem<C> *>(A.gmem)[blockIdx.x];
+    cells_loop<C, SEAL, true>(m, A);
+}
+
 // Tier 0: register fast path (one vertex per lane), small LDS (many waves per CU). Tier 1: the
 // same path with two vertices per lane (128 vertices). Tiers 2-3: generic path for cells that
 // outgrow the tier before, with 256 and 1024 vertices (the last one wave per CU).
@@ -1469,6 +1492,11 @@ using Tier0 = Cap<64, 192, 48, 64, true, 107, VD_TIER0_WAVES>;
 using Tier1 = Cap<128, 384, 64, 64, true, 111, 4>;
 using Tier2 = Cap<256, 768, 128, 384, false, 255, 1>;
 using Tier3 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
+// Tier 4: the generic path with its polytope in global memory, for cells beyond LDS. Face-slot
+// offsets are 16-bit in every tier and a crossing's vertex id shares 16 bits with a flag in
+// clip_generic, so it holds up to 32,767 vertices and faces, 65,535 face slots and 32,767
+// crossings per clip (the reference's own face indices are u16, cell_3d_faces.rs:29).
+using TierG = Cap<32767, 65535, 32767, 32767, false, 511, 1, true>;
 
 // Work-list entries whose cell stopped at the end of its shell prefix and whose walk covers about
 // min_entries bins or more: 4/3 pi (2r)^3 over the bin volume (a size estimate only; the lists
@@ -1592,27 +1620,31 @@ hipError_t launch_far_lists(const uint32_t *cells, const uint32_t *limit, const 
     return hipGetLastError();
 }
 
-int tier_count() { return 4; }
+int tier_count() { return 5; }
 
 int tier_max_vertices(int tier) {
-    return tier == 0 ? Tier0::VM : tier == 1 ? Tier1::VM : tier == 2 ? Tier2::VM : Tier3::VM;
+    return tier == 0 ? Tier0::VM : tier == 1 ? Tier1::VM : tier == 2 ? Tier2::VM : tier == 3 ? Tier3::VM : TierG::VM;
 }
 
 size_t tier_lds_bytes(int tier) {
     return tier == 0 ? sizeof(Mem<Tier0>) : tier == 1 ? sizeof(Mem<Tier1>)
-         : tier == 2 ? sizeof(Mem<Tier2>) : sizeof(Mem<Tier3>);
+         : tier == 2 ? sizeof(Mem<Tier2>) : tier == 3 ? sizeof(Mem<Tier3>) : 0;
 }
+
+size_t tier_global_bytes() { return sizeof(Mem<TierG>); }
 
 template <bool SEAL, bool PROBE>
 void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st) {
     if (tier == 0) k_cells<Tier0, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 1) k_cells<Tier1, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 2) k_cells<Tier2, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
-    else k_cells<Tier3, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
+    else if (tier == 3) k_cells<Tier3, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
+    else k_cells_global<TierG, SEAL><<<grid_waves, 64, 0, st>>>(a);
 }
 
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal, bool probe) {
     if (a.ncells == 0) return hipSuccess;
+    if (tier == 4 && !a.gmem) return hipErrorInvalidValue;
     if (seal) {
         if (probe) launch_tier<true, true>(tier, a, grid_waves, st);
         else launch_tier<true, false>(tier, a, grid_waves, st);
@@ -1630,7 +1662,9 @@ hipError_t cells_occupancy(int tier, int *blocks_per_cu) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1, false, false>, 64, 0);
     if (tier == 2)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier2, false, true>, 64, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier3, false, true>, 64, 0);
+    if (tier == 3)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier3, false, true>, 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells_global<TierG, false>, 64, 0);
 }
 
 }  // namespace vd

@@ -130,6 +130,7 @@ struct CellArgs {
     uint64_t arena_cap[3];
     uint32_t arena_chunk[3];      // entries a wave reserves at a time (cells are carved from them)
     uint64_t *cell_off;           // 3 per cell: vertex, face, index offsets
+    void *gmem;                   // the global-memory tier's polytopes: one slot per wave
 
     // seal mode (calculate_sealed, tessellation.rs:234-318): cell topology of the previous pass
     const uint32_t *seal_mask;    // per cell: walls (bit = wall index) of its curved-wall set
@@ -233,12 +234,14 @@ hipError_t shell_finish(const ShellPack &pack, const double cs[3], uint32_t w, u
 // ---- cell kernel (vd_cells.hip) ------------------------------------------------------------
 // tier: 0 = register fast path with a small LDS polytope (many waves per CU); 1 = the fast path
 // with two vertices per lane (128 vertices); 2, 3 = generic path with LDS polytopes of 256 and 1024
-// vertices (the last one wave per CU)
+// vertices (the last one wave per CU); 4 = the generic path with the polytope in global memory
+// (a.gmem: grid_waves slots of tier_global_bytes() each), up to 65,535 vertices
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal = false,
                         bool probe = false);
 hipError_t cells_occupancy(int tier, int *blocks_per_cu);
 int tier_count();
 int tier_max_vertices(int tier);
 size_t tier_lds_bytes(int tier);
+size_t tier_global_bytes();
 
 }  // namespace vd
