@@ -317,6 +317,14 @@ int vd_server_result(const vd_server *s, uint64_t job, double *volumes, double *
                      double *generators, size_t n);
 /* Forgets a finished or refused job. */
 int vd_server_release(vd_server *s, uint64_t job);
+/* Device memory of the server, counted at every allocation and free it makes (lanes, jobs, spare
+ * buffers): [0] bytes held now, [1] the most it has held at once, [2] its fixed reserve (the
+ * lanes' global-memory tier slots), [3] the budget. Admission keeps [1] within [3]: each job is
+ * charged an upper bound of what it can hold while it runs (its points, its share of a lane's
+ * buffers, its grid's bins and its shell table at the largest with the table's construction
+ * scratch), a lane that kept buffers from earlier steps counts with what it holds, and a budget
+ * at or below the reserve is refused by vd_server_create. */
+int vd_server_memory(const vd_server *s, uint64_t out[4]);
 /* [0] steps, [1] cells computed, [2] budget bytes in use, [3] budget. */
 int vd_server_stats(const vd_server *s, double out[4]);
 #define VD_SOPT_SMALL_JOB 2 /* jobs with at most this many points (default 65536; 0: none) run in a

@@ -97,9 +97,20 @@ def test_small_job_runs_beside_a_large_one():
     assert np.array_equal(bits(gen.reshape(-1)), bits(o.generators()))
 
 
+def job_charge(grid, pts, iterations=2):
+    """What the server charges a job (the budget bytes it holds while the job runs alone)."""
+    s = Server(hbm_budget=8 << 30)
+    j = s.submit(UNIT, grid, [], pts, iterations=iterations)
+    s.step()
+    charge = s.pending()["bytes"]
+    run_until_done(s, [j])
+    return charge, s.memory()["reserve"]
+
+
 def test_budget_admission_and_refusal():
     small = [job_points(120 + k, 4000) for k in range(3)]
-    s = Server(hbm_budget=1_500_000)      # one 4000-point job (about 1.2 MB) at a time
+    charge, reserve = job_charge((12, 12, 12), small[0])
+    s = Server(hbm_budget=reserve + charge + charge // 2)   # one such job at a time
     jobs = [s.submit(UNIT, (12, 12, 12), [], p, iterations=2) for p in small]
     s.step()
     p = s.pending()
@@ -108,6 +119,7 @@ def test_budget_admission_and_refusal():
     for j in jobs:
         assert s.info(j)["state"] == Server.FINISHED
     assert s.pending()["bytes"] == 0
+    assert s.memory()["peak"] <= s.memory()["budget"]
     with pytest.raises(_lib.VdError) as e:
         s.submit(UNIT, None, [], job_points(130, 200_000))
     assert e.value.code == _lib.VD_ENOMEM
@@ -116,3 +128,47 @@ def test_budget_admission_and_refusal():
     run_until_done(s, [j])
     for j0 in jobs:
         s.release(j0)
+    # a budget that cannot hold the server's fixed reserve is refused
+    with pytest.raises(_lib.VdError) as e:
+        Server(hbm_budget=reserve)
+    assert e.value.code == _lib.VD_EINVAL
+
+
+def test_peak_memory_within_budget_on_clustered_lloyd_job():
+    # 300k points in 8 clusters (sigma 0.05), 6 Lloyd rounds on the density-rule grid: the first
+    # calculate's cells next to the voids grow the shell prefix. The budget holds the reserve and
+    # the job's charge with 10 % to spare; the device's free memory, sampled by a second thread
+    # during the steps, never drops by more than the budget, and the server's own count agrees.
+    import threading
+    from vorothree_amd import _hip
+    pts = job_points(140, 300_000, clustered=True)
+    charge, reserve = job_charge(None, pts, iterations=6)
+    budget = reserve + charge + charge // 10
+    s = Server(hbm_budget=budget)
+    # the same kind of job, smaller, first: module loads, the runtime's scratch for the tiers the
+    # clusters reach, and its pools are then in place before the measured run
+    warm = s.submit(UNIT, None, [], job_points(141, 20_000, clustered=True), iterations=2)
+    run_until_done(s, [warm])
+    s.release(warm)
+    _hip.synchronize()
+    free0 = _hip.mem_free()
+    held0 = s.memory()["held"]
+    low = [free0]
+    stop = threading.Event()
+
+    def sample():
+        while not stop.is_set():
+            low[0] = min(low[0], _hip.mem_free())
+
+    th = threading.Thread(target=sample)
+    th.start()
+    try:
+        j = s.submit(UNIT, None, [], pts, iterations=6)
+        run_until_done(s, [j], max_steps=200)
+    finally:
+        stop.set()
+        th.join()
+    m = s.memory()
+    assert s.info(j)["state"] == Server.FINISHED and s.info(j)["runs"] == 6
+    assert m["peak"] <= budget
+    assert held0 + (free0 - low[0]) <= budget

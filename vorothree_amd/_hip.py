@@ -26,6 +26,8 @@ def hip():
         h.hipDeviceSynchronize.restype = ctypes.c_int
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
         h.hipGetDeviceCount.restype = ctypes.c_int
+        h.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+        h.hipMemGetInfo.restype = ctypes.c_int
         _hip = h
     return _hip
 
@@ -35,6 +37,14 @@ def device_count() -> int:
     if hip().hipGetDeviceCount(ctypes.byref(n)) != 0:
         return 0
     return n.value
+
+
+def mem_free() -> int:
+    """Free device memory of the current device in bytes (hipMemGetInfo)."""
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    if hip().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return int(free.value)
 
 
 def synchronize() -> None:

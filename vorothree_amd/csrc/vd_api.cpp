@@ -59,10 +59,24 @@ int fail(int code, const std::string &msg) {
         if (rc_ != VD_OK) return rc_; \
     } while (0)
 
+// Device bytes held by one owner (a server: its lanes' engines, its jobs' buffers), and their
+// high-water mark. Every hipMalloc'd DevBuf remembers the account that was current when it was
+// allocated and gives its bytes back when it is freed, on whatever thread that happens.
+struct AllocAccount {
+    std::atomic<int64_t> bytes{0}, peak{0};
+    void add(int64_t d) {
+        const int64_t v = bytes.fetch_add(d) + d;
+        int64_t p = peak.load();
+        while (v > p && !peak.compare_exchange_weak(p, v)) {}
+    }
+};
+thread_local AllocAccount *g_acct = nullptr;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     bool carved = false;   // carved from a static arena: never passed to hipFree
+    AllocAccount *acct = nullptr;
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
@@ -85,14 +99,16 @@ void release(DevBuf &b) {
             if (g_static && end == g_static->base + g_static->top)
                 g_static->top = (size_t)(static_cast<char *>(b.p) - g_static->base);
         } else if (g_defer) {
-            g_defer->push_back(b);
+            g_defer->push_back(b);   // still held: its account is charged until it is freed
         } else {
             (void)hipFree(b.p);
+            if (b.acct) b.acct->add(-(int64_t)b.bytes);
         }
     }
     b.p = nullptr;
     b.bytes = 0;
     b.carved = false;
+    b.acct = nullptr;
 }
 
 // Releases n buffers when the scope ends, on every return path (release is idempotent, so an
@@ -124,6 +140,8 @@ int ensure(DevBuf &b, size_t bytes) {
         return fail(VD_ENOMEM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
     }
     b.bytes = bytes;
+    b.acct = g_acct;
+    if (g_acct) g_acct->add((int64_t)bytes);
     return VD_OK;
 }
 
@@ -442,6 +460,8 @@ struct Engine {
     bool side_streams = true;
     double far_list_min = kFarListMinDefault;   // walk length for a per-cell list (0: never)
     bool keep_nb = true;               // keep every cell's neighbour ids (reduced mode)
+    size_t shell_cache_bytes = kShellCacheBytes;   // shell tables of other geometries kept up to this
+    int global_slots = kGlobalSlots;   // polytope slots (waves) of the global-memory tier
     // vd_calculate_static: buffers carved from the static scratch, the caller's stream (never
     // destroyed), no events, no pinned memory, no graphs, no side streams
     bool static_mode = false;
@@ -486,11 +506,9 @@ int engine_init(Engine &E) {
     return VD_OK;
 }
 
-void engine_free(Engine &E) {
-    // the side stream may hold this engine's far-cell launches when a calculate failed before
-    // its stream waited for them
-    if (E.stream2) (void)hipStreamSynchronize(E.stream2);
-    if (E.stream) (void)hipStreamSynchronize(E.stream);
+// Releases every device buffer of the engine (its shell tables and graph too); the engine stays
+// usable: the next set_generators or calculate allocates again. The caller has synchronised.
+void engine_release_buffers(Engine &E) {
     DevBuf *bufs[] = {&E.d_desc, &E.d_offs, &E.d_raw, &E.d_raw_tess, &E.d_flags, &E.d_pos, &E.d_scan,
                       &E.d_gat, &E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start,
                       &E.d_fill, &E.d_items, &E.d_spos, &E.d_shell_tmp, &E.d_mflags, &E.d_mrmax, &E.d_vol,
@@ -511,11 +529,23 @@ void engine_free(Engine &E) {
     for (DevBuf *b : post) release(*b);
     if (E.gexec) (void)hipGraphExecDestroy(E.gexec);
     E.gexec = nullptr;
-    if (E.h_counts && !E.static_mode) (void)hipHostFree(E.h_counts);
-    E.h_counts = nullptr;
+    E.last_key.clear();
+    E.gkey.clear();
     for (auto &kv : E.tables) release(kv.second.buf);
     E.tables.clear();
     E.shapes.clear();
+    E.cap_cells = 0;
+    E.prepared = E.calculated = E.binned = false;
+}
+
+void engine_free(Engine &E) {
+    // the side stream may hold this engine's far-cell launches when a calculate failed before
+    // its stream waited for them
+    if (E.stream2) (void)hipStreamSynchronize(E.stream2);
+    if (E.stream) (void)hipStreamSynchronize(E.stream);
+    engine_release_buffers(E);
+    if (E.h_counts && !E.static_mode) (void)hipHostFree(E.h_counts);
+    E.h_counts = nullptr;
     if (E.ev0) (void)hipEventDestroy(E.ev0);
     if (E.ev1) (void)hipEventDestroy(E.ev1);
     if (E.stream && !E.static_mode) (void)hipStreamDestroy(E.stream);
@@ -631,7 +661,7 @@ int engine_shells(Engine &E) {
     // least recently used tables of other geometries go first when the cache grows too large
     size_t bytes = 0;
     for (auto &kv : E.tables) bytes += kv.second.buf.bytes;
-    while (bytes > kShellCacheBytes) {
+    while (bytes > E.shell_cache_bytes) {
         auto victim = E.tables.end();
         for (auto it = E.tables.begin(); it != E.tables.end(); ++it)
             if (it->second.epoch != E.epoch && (victim == E.tables.end() || it->second.epoch < victim->second.epoch))
@@ -916,8 +946,8 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     VD_HIP(vd::launch_fill_u32(E.d_queue.as<uint32_t>() + slot, 0u, 1, st));
     int waves = std::max(1, E.cus - E.reserve_cus) * E.occ[tier];
     if (tier == kGlobalTier) {
-        waves = std::min(waves, kGlobalSlots);
-        VD_TRY(ensure(E.d_gmem, (size_t)kGlobalSlots * vd::tier_global_bytes()));
+        waves = std::min(waves, E.global_slots);
+        VD_TRY(ensure(E.d_gmem, (size_t)E.global_slots * vd::tier_global_bytes()));
         a.gmem = E.d_gmem.p;
     }
     if ((uint32_t)waves > nlist) waves = (int)nlist;
@@ -1604,9 +1634,30 @@ struct Job {
     bool uploaded = false;         // pts holds the input (the lane uploads `host` at its next step)
 };
 
-// Device bytes one generator occupies across the engine's buffers (input, generators, bins,
-// bin-ordered copy, outputs, neighbour arena) plus the job's own copy, with headroom.
-constexpr size_t kBytesPerPoint = 24 + 24 + 24 + 4 + 4 + 4 + 4 + 32 + 8 + 24 + 4 + 4 + 4 + 24 + 24 * 4 + 16;
+// Device bytes a job can hold while it runs, an upper bound: its point buffer; its share of a lane
+// engine's buffers in one step (the input and compacted generators with their member, flag and
+// position words, bin and item ids, the bin-ordered copy, the reductions and status, the retry
+// lists of a round, the relaxed copy, tube-wall planes); its grid's bins and scan scratch; its
+// walls' data and descriptor; and the shell-prefix table of its grid at its largest (every offset
+// of the grid's enumeration box) with the scratch that table's construction needs. Server lanes
+// keep no neighbour ids or full cells (no cell arenas) and no per-cell far lists; their
+// global-memory tier slots are the server's fixed reserve.
+size_t job_bytes(const Member &m, uint64_t n) {
+    uint64_t ntube = 0;
+    for (const vd::WallDev &w : m.walls) ntube += w.kind == VD_WALL_TUBE;
+    const uint64_t per_point = 24 + 24 + 4 + 4 + 4 + 24 + 4 + 4 + 4 + 32 + 8 + 24 + 4 + 4 + 4 + 8 + 16 + 24 + 56 * ntube;
+    const uint64_t bins = (uint64_t)m.n[0] * m.n[1] * m.n[2];
+    size_t b = (n + 1) * per_point + (bins + 1) * 12 + vd::scan_tmp_words((uint32_t)std::max(bins, n) + 1) * 4;
+    for (const std::vector<double> &d : m.wdata) b += d.size() * 8;
+    b += 4096;
+    const int64_t nmax = std::max(m.n[0], std::max(m.n[1], m.n[2]));
+    const int64_t kfull = (int64_t)std::ceil(std::sqrt(3.0) * (double)nmax) + 3;
+    uint64_t box = 1;
+    for (int a = 0; a < 3; ++a) box *= (uint64_t)(2 * std::min(kfull, m.n[a]) + 1);
+    box = std::min<uint64_t>(box, 0x7fffffffull);
+    b += box * sizeof(vd::ShellCode) + 32 + vd::shell_scratch_bytes((uint32_t)box);
+    return b;
+}
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -1658,6 +1709,7 @@ struct StaticScope {
 // the outcome (job states, budget) when it collects the step.
 struct Lane {
     Engine E;
+    AllocAccount *acct = nullptr;    // the server's: every device buffer of the lane is charged to it
     std::vector<Job *> active;       // jobs of the running or next step
     std::vector<uint64_t> cells;     // per active job: cells of the step's calculate
     std::thread worker;
@@ -1675,9 +1727,20 @@ constexpr uint64_t kSmallJobDefault = 65536;   // jobs with at most this many po
 constexpr int kReservedCUs = VD_RESERVED_CUS;
 constexpr size_t kSpareBuffers = 64;
 
+// Makes an account current on this thread for a scope.
+struct AcctScope {
+    AllocAccount *prev;
+    explicit AcctScope(AllocAccount *a) : prev(g_acct) { g_acct = a; }
+    ~AcctScope() { g_acct = prev; }
+};
+
+constexpr int kServerGlobalSlots = 2;   // global-memory tier slots of each server lane
+
 struct vd_server {
     Lane lanes[2];                 // 0: large jobs (worker thread), 1: small jobs (calling thread)
-    size_t budget = 0, used = 0;
+    AllocAccount acct;             // every device byte the server holds (lanes, jobs, spare buffers)
+    size_t reserve = 0;            // fixed part of the budget: the lanes' global-memory tier slots
+    size_t budget = 0, used = 0;   // used: the admitted jobs' job_bytes
     int max_slots = 64;
     uint64_t next_id = 1;
     std::map<uint64_t, Job *> jobs;
@@ -2305,11 +2368,18 @@ int vd_server_create(size_t hbm_budget, int max_slots, vd_server **out) {
     S->max_slots = max_slots;
     int rc = VD_OK;
     for (Lane &L : S->lanes) {
+        L.acct = &S->acct;
         L.E.keep_cells = 0;
+        L.E.keep_nb = false;      // results are volumes, centroids and generators
         L.E.side_streams = false;
         L.E.far_list_min = 0.0;   // measured: the served mix ran faster without the lists
+        L.E.shell_cache_bytes = 0;   // tables of geometries no job of the step uses are freed
+        L.E.global_slots = kServerGlobalSlots;
         if (!rc) rc = engine_init(L.E);
     }
+    S->reserve = 2 * (size_t)kServerGlobalSlots * vd::tier_global_bytes();
+    if (!rc && hbm_budget <= S->reserve)
+        rc = fail(VD_EINVAL, "HBM budget below the server's fixed reserve of " + std::to_string(S->reserve) + " bytes");
     if (rc) {
         for (Lane &L : S->lanes) engine_free(L.E);
         delete S;
@@ -2328,6 +2398,15 @@ void vd_server_destroy(vd_server *S) {
     for (DevBuf &b : S->deferred) release(b);
     for (Lane &L : S->lanes) engine_free(L.E);
     delete S;
+}
+
+int vd_server_memory(const vd_server *S, uint64_t out[4]) {
+    if (!S || !out) return fail(VD_EINVAL, "NULL argument");
+    out[0] = (uint64_t)std::max<int64_t>(0, S->acct.bytes.load());
+    out[1] = (uint64_t)std::max<int64_t>(0, S->acct.peak.load());
+    out[2] = S->reserve;
+    out[3] = S->budget;
+    return VD_OK;
 }
 
 int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], uint32_t nx, uint32_t ny,
@@ -2359,10 +2438,10 @@ int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], u
     J->host.assign(xyz, xyz + 3 * n);
     J->npts = n;
     J->lane = n <= S->small_max ? 1 : 0;
-    J->bytes = n * kBytesPerPoint + (uint64_t)J->m.n[0] * J->m.n[1] * J->m.n[2] * 12 + 4096;
+    J->bytes = job_bytes(J->m, n);
     J->t_submit = now_s();
     J->id = S->next_id++;
-    if (J->bytes > S->budget) {
+    if (J->bytes + S->reserve > S->budget) {
         J->state = 3;   // can never be admitted: refuse it now instead of blocking the queue
         S->jobs[J->id] = J;
         *job = J->id;
@@ -2379,6 +2458,7 @@ int vd_server_submit(vd_server *S, const double bmin[3], const double bmax[3], u
 // the others' generators relaxed in place (tessellation.rs:482-495; their next input never leaves
 // the device). Touches only the lane's engine and jobs, so it can run on a worker thread.
 int lane_run(Lane &L) {
+    const AcctScope scope(L.acct);
     Engine &E = L.E;
     for (Job *J : L.active)
         if (!J->uploaded && J->npts)
@@ -2469,6 +2549,7 @@ void lane_collect(vd_server *S, Lane &L, uint32_t *finished) {
 // A device buffer of at least `bytes` for a job's points: the smallest spare one that fits, else a
 // new allocation.
 int take_buffer(vd_server *S, size_t bytes, DevBuf &out) {
+    const AcctScope scope(&S->acct);
     size_t best = S->spare.size();
     for (size_t k = 0; k < S->spare.size(); ++k)
         if (S->spare[k].bytes >= bytes && (best == S->spare.size() || S->spare[k].bytes < S->spare[best].bytes))
@@ -2500,8 +2581,24 @@ int collect_large(vd_server *S, uint32_t *finished) {
 int vd_server_step(vd_server *S, uint32_t *finished) {
     if (!S) return fail(VD_EINVAL, "NULL handle");
     if (finished) *finished = 0;
+    // admission: the admitted jobs' bounds plus the reserve, or what the server holds now if that
+    // is more (buffers an idle lane kept from earlier steps, spare point buffers), must leave room
+    // for the next job's bound. When they do not, an idle lane's buffers and the spare buffers are
+    // freed first (only while no large step runs: hipFree waits for the whole device).
+    auto committed = [S]() {
+        return std::max<int64_t>((int64_t)(S->used + S->reserve), S->acct.bytes.load());
+    };
+    if (!S->pending.empty() && committed() + (int64_t)S->pending.front()->bytes > (int64_t)S->budget &&
+        !S->lanes[0].running.load()) {
+        for (Lane &L : S->lanes)
+            if (L.active.empty() && (&L != &S->lanes[0] || S->admitted0.empty())) engine_release_buffers(L.E);
+        for (DevBuf &b : S->spare) release(b);
+        S->spare.clear();
+        for (DevBuf &b : S->deferred) release(b);
+        S->deferred.clear();
+    }
     while (!S->pending.empty() && (int)S->nslots < S->max_slots &&
-           S->used + S->pending.front()->bytes <= S->budget) {
+           committed() + (int64_t)S->pending.front()->bytes <= (int64_t)S->budget) {
         Job *J = S->pending.front();
         S->pending.pop_front();
         VD_TRY(take_buffer(S, (J->npts + 1) * 24, J->pts));

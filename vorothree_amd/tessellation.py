@@ -904,3 +904,10 @@ class Server:
         out = np.zeros(4, dtype=np.float64)
         check(lib().vd_server_stats(self._h, _ptr(out)))
         return {"steps": int(out[0]), "cells": int(out[1]), "bytes": int(out[2]), "budget": int(out[3])}
+
+    def memory(self):
+        """Device memory the server holds now, the most it held at once, its fixed reserve and its
+        budget, in bytes (vd_server_memory)."""
+        out = np.zeros(4, dtype=np.uint64)
+        check(lib().vd_server_memory(self._h, _ptr(out)))
+        return {"held": int(out[0]), "peak": int(out[1]), "reserve": int(out[2]), "budget": int(out[3])}
