@@ -94,18 +94,51 @@ def serve_run(reqs, rate: float, budget_gb: float) -> dict:
     wall = time.monotonic() - t0
     lat = []
     cells = 0
+    grids = []
     for r, j in zip(reqs, ids):
         inf = s.info(j)
         lat.append(inf["finish"] - (t0 + r.arrival))
         cells += inf["total_cells"]
+        grids.append(s.grid(j))
     lat = np.asarray(lat)
     st = s.stats()
+    mem = s.memory()
     s.close()
     print(f"serve {rate:g} req/s: done in {wall:.2f} s", file=sys.stderr, flush=True)
     return {"rate_req_s": rate, "requests": len(reqs), "cells": int(cells), "wall_s": wall,
             "cells_per_s": cells / wall, "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "latency_max_s": float(lat.max()),
-            "server_steps": st["steps"]}
+            "server_steps": st["steps"], "device_bytes_peak": mem["peak"], "budget_bytes": mem["budget"],
+            "_grids": grids}
+
+
+def serve_plain(reqs, grids) -> dict:
+    """The same requests without the server: one Tessellation per job, in arrival order, each run
+    eagerly to completion (set_generators from the host, calculate, and for a Lloyd job relax +
+    calculate for its remaining rounds), on the grid the server used for it."""
+    from vorothree_amd import Algorithm3DGrid, BoundingBox, Tessellation, _hip
+
+    unit = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    _hip.synchronize()
+    t0 = time.perf_counter()
+    cells = 0
+    last = t0
+    for k, (r, grid) in enumerate(zip(reqs, grids)):
+        t = Tessellation(unit, Algorithm3DGrid(*grid, unit), keep_cells=False)
+        t.set_generators(r.points)
+        t.calculate()
+        cells += t.count_cells()
+        for _ in range(r.iterations - 1):
+            t.relax()
+            t.calculate()
+            cells += t.count_cells()
+        t.close()
+        if time.perf_counter() - last > 10.0:
+            last = time.perf_counter()
+            print(f"serve plain: {k + 1} of {len(reqs)} jobs", file=sys.stderr, flush=True)
+    _hip.synchronize()
+    dt = time.perf_counter() - t0
+    return {"cells": int(cells), "wall_s": dt, "cells_per_s": cells / dt}
 
 
 def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_rates) -> dict:
@@ -120,10 +153,14 @@ def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_ra
     reqs = W.config5(count=max(count, sweep_count), rate=rate)
     print(f"serve: {len(reqs)} requests generated", file=sys.stderr, flush=True)
     main_run = serve_run(reqs[:count], rate, budget_gb)
+    progress("serve: the same requests through the plain path, one tessellation per job")
+    plain = serve_plain(reqs[:count], main_run.pop("_grids"))
     sweep = []
     for r in sweep_rates:
         scaled = [dataclasses.replace(q, arrival=q.arrival * rate / r) for q in reqs[:sweep_count]]
-        sweep.append(serve_run(scaled, r, budget_gb))
+        run = serve_run(scaled, r, budget_gb)
+        run.pop("_grids")
+        sweep.append(run)
     out = {"workload": f"first {count} requests of a Poisson stream at {rate:g} req/s: 1e3..2e6 "
                        "generators (log-uniform) in the unit cube, 50% uniform / 50% 8-cluster "
                        "sigma 0.05, 30% Lloyd jobs of 10-50 iterations; grid per job by "
@@ -134,6 +171,8 @@ def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_ra
            "note": "the offered load is far above one GPU's capacity at 2000 req/s, so its latency "
                    "is mostly queueing; the sweep lowers the rate on the first requests"}
     out.update(main_run)
+    out["plain"] = plain
+    out["speedup_served_vs_plain"] = main_run["cells_per_s"] / plain["cells_per_s"]
     out["sweep"] = sweep
     return out
 
@@ -309,6 +348,20 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes": n * 8, "rc": rc}), flush=True)
 
 
+def host_cpu() -> str:
+    """CPU model of the host (first 'model name' of /proc/cpuinfo) and its logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
+
+
 def progress(msg: str) -> None:
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
@@ -381,7 +434,9 @@ def main():
         from oracle import OracleTess
         o = OracleTess(w.lo, w.hi, *w.grid)
         o.add_wall_sphere(-1000, (0.5, 0.5, 0.5), 0.5)
-        o.set_generators(w.points)
+        c0 = time.perf_counter()
+        o.set_generators(w.points)            # wall filter + binning (single thread, as the reference)
+        bdt = time.perf_counter() - c0
         c0 = time.perf_counter()
         o.calculate(threads=a.cpu_threads)
         cdt = time.perf_counter() - c0
@@ -398,7 +453,10 @@ def main():
                          f"tessellation, one calculate() on {a.cpu_threads} threads "
                          f"(std::thread, dynamic chunks of 64 cells); host nproc={os.cpu_count()}",
                "single_thread": {"value": len(sub) / sdt, "unit": "cells/s", "cores": 1,
-                                 "sample": f"every {max(1, ncpu // 50_000)}th cell ({len(sub)} cells)"}}
+                                 "sample": f"every {max(1, ncpu // 50_000)}th cell ({len(sub)} cells)"},
+               "host_cpu": host_cpu(), "set_generators_ms": bdt * 1e3, "calculate_ms": cdt * 1e3,
+               "flops_per_cell": None}
+        cpu["flops_per_cell"] = flops_per_cell
         del o
 
     # algorithmic HBM bytes of the launch: per cell the generator, its bin item and bin offset read

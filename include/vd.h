@@ -311,6 +311,8 @@ int vd_server_pending(const vd_server *s, uint64_t counts[3]);
 /* state: 0 queued, 1 running, 2 finished, 3 refused; info: cells of the last calculate,
  * calculate() runs done, cells over all runs; times: submit, admit, finish (seconds, steady clock). */
 int vd_server_job_info(const vd_server *s, uint64_t job, int32_t *state, uint64_t info[3], double times[3]);
+/* The grid a job runs on (its own, or the one vd_grid_for_points picked for it at submission). */
+int vd_server_job_grid(const vd_server *s, uint64_t job, uint32_t out[3]);
 /* Results of a finished job's last calculate(): volumes, centroids (3 per cell) and the generators
  * it ran on (3 per cell); NULL pointers are skipped. */
 int vd_server_result(const vd_server *s, uint64_t job, double *volumes, double *centroids,

@@ -120,6 +120,7 @@ SIGNATURES = {
     "vd_server_release": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "vd_server_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_server_memory": (ctypes.c_int, [_vp, _vp]),
+    "vd_server_job_grid": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp]),
     "vd_server_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
     "vd_last_error": (ctypes.c_char_p, []),
 }

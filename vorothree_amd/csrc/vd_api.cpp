@@ -2400,6 +2400,14 @@ void vd_server_destroy(vd_server *S) {
     delete S;
 }
 
+int vd_server_job_grid(const vd_server *S, uint64_t job, uint32_t out[3]) {
+    if (!S || !out) return fail(VD_EINVAL, "NULL argument");
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return fail(VD_ERANGE, "unknown job");
+    for (int a = 0; a < 3; ++a) out[a] = (uint32_t)it->second->m.n[a];
+    return VD_OK;
+}
+
 int vd_server_memory(const vd_server *S, uint64_t out[4]) {
     if (!S || !out) return fail(VD_EINVAL, "NULL argument");
     out[0] = (uint64_t)std::max<int64_t>(0, S->acct.bytes.load());

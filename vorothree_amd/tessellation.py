@@ -889,6 +889,12 @@ class Server:
                 "total_cells": int(info[2]), "submit": float(times[0]), "admit": float(times[1]),
                 "finish": float(times[2])}
 
+    def grid(self, job: int):
+        """The grid the job runs on (vd_server_job_grid)."""
+        out = np.zeros(3, dtype=np.uint32)
+        check(lib().vd_server_job_grid(self._h, int(job), _ptr(out)))
+        return tuple(int(x) for x in out)
+
     def result(self, job: int):
         n = self.info(job)["cells"]
         vol = np.empty(n, dtype=np.float64)
