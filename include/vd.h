@@ -201,9 +201,21 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
 #define VD_OPT_ATTEMPTS 5   /* launches, each with twice the arena of the one before, after which
                                calculate() gives up with VD_ECAPACITY and keeps no results
                                (default 8, at most 62) */
+/* Launch shape of the cell kernels; every choice gives identical results. */
+#define VD_OPT_T0_WAVES 6   /* resident waves per SIMD of the small-LDS tier's kernel: 5, 6
+                               (default) or 7, each a separate compiled instantiation */
+#define VD_OPT_QUEUE_GRAIN 7 /* cells a wave takes from the work queue per atomic, 1 (default) to 64 */
+#define VD_OPT_FIRST_TIER 8 /* tier of the first launch over every cell: 0 (default, 64 vertices)
+                               or 1 (128 vertices); cells that outgrow it move up as usual */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
+/* Times calculate() under the launch shapes (coordinate descent over VD_OPT_T0_WAVES {5, 6, 7},
+ * VD_OPT_QUEUE_GRAIN {1, 2, 4, 8}, VD_OPT_FIRST_TIER {0, 1}, from the current shape), keeps the
+ * fastest and writes it to picks[3] (waves, grain, first tier). Generators must be set; about ten
+ * calculates run, and the results of the last one are kept (identical under every shape). No
+ * counterpart in the reference. */
+int vd_tess_autotune_launch(vd_tess *t, int picks[3]);
 
-/* Diagnostics of the last calculate(): [0] cells computed in the small-LDS tier, [1] cell runs
+/* Diagnostics of the last calculate(): [0] cells of the first launch (its tier: VD_OPT_FIRST_TIER), [1] cell runs
  * in the larger tiers, [2] shell-prefix extensions, [3] entries of the shell prefixes,
  * [4] time in microseconds of the first cell launch (of the whole replayed first round when a
  * graph ran), [5] total calculate time (us), [6] vertices and [7] face-index entries written to
@@ -265,11 +277,16 @@ int vd_batch_get_cell(const vd_batch *b, uint32_t t, size_t i, uint64_t sizes[3]
 #define VD_BOPT_PER_BIN 2    /* generators-per-bin target of vd_batch_add's grid choice, x1000 (default 350) */
 #define VD_BOPT_GRAPHS 3     /* as VD_OPT_GRAPHS (default 1) */
 #define VD_BOPT_FAR_LIST 4   /* as VD_OPT_FAR_LIST (default 262144) */
+#define VD_BOPT_T0_WAVES 5   /* as VD_OPT_T0_WAVES */
+#define VD_BOPT_QUEUE_GRAIN 6 /* as VD_OPT_QUEUE_GRAIN */
+#define VD_BOPT_FIRST_TIER 7 /* as VD_OPT_FIRST_TIER */
 int vd_batch_set_option(vd_batch *b, int key, int64_t value);
 int vd_batch_get_stats(const vd_batch *b, double out[10]);
 /* Times vd_batch_calculate for each candidate generators-per-bin target (grids re-derived with
  * vd_grid_for for the members added with a zero grid), keeps the fastest, and returns it. */
 int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *best);
+/* vd_tess_autotune_launch for a batch (its grids as they are: run vd_batch_autotune first). */
+int vd_batch_autotune_launch(vd_batch *b, int picks[3]);
 
 /* The grid rule of the autotuner: bins per axis proportional to the box edges with about
  * per_bin generators per bin, at least 1 per axis. */

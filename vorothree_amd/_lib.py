@@ -22,6 +22,9 @@ VD_OPT_GRAPHS = 2
 VD_OPT_ARENA = 3
 VD_OPT_FAR_LIST = 4
 VD_OPT_ATTEMPTS = 5
+VD_OPT_T0_WAVES = 6
+VD_OPT_QUEUE_GRAIN = 7
+VD_OPT_FIRST_TIER = 8
 VD_STATIC_SCRATCH_BYTES = 96 << 20
 VD_STATIC_MAX_POINTS = 262144
 VD_STATIC_MAX_BINS = 2097152
@@ -29,6 +32,9 @@ VD_BOPT_KEEP_CELLS = 1
 VD_BOPT_PER_BIN = 2
 VD_BOPT_GRAPHS = 3
 VD_BOPT_FAR_LIST = 4
+VD_BOPT_T0_WAVES = 5
+VD_BOPT_QUEUE_GRAIN = 6
+VD_BOPT_FIRST_TIER = 7
 VD_SOPT_PER_BIN = 1
 VD_SOPT_SMALL_JOB = 2
 
@@ -84,6 +90,7 @@ SIGNATURES = {
     "vd_tess_get_neighbors": (ctypes.c_int, [_vp, _vp, _sz, _vp, _sz]),
     "vd_tess_get_cell": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "vd_tess_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
+    "vd_tess_autotune_launch": (ctypes.c_int, [_vp, _vp]),
     "vd_tess_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_shell_prefix": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _sz, _vp, _vp]),
     "vd_shell_prefix_device": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _sz, _vp, _vp]),
@@ -105,6 +112,7 @@ SIGNATURES = {
     "vd_batch_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
     "vd_batch_get_stats": (ctypes.c_int, [_vp, _vp]),
     "vd_batch_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    "vd_batch_autotune_launch": (ctypes.c_int, [_vp, _vp]),
     "vd_grid_for": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_double, _vp]),
     "vd_tess_add_wall_planes_per_generator": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(VdWallHost)]),
     "vd_calculate_static": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, _vp, ctypes.c_int, _vp, _sz, _vp, _vp,

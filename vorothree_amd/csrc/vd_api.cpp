@@ -467,6 +467,12 @@ struct Engine {
     bool static_mode = false;
     uint32_t h_local[64] = {};         // the outcome counts' host copy in static mode
     uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
+    // launch shape of the cell kernels (VD_OPT_T0_WAVES, VD_OPT_QUEUE_GRAIN, VD_OPT_FIRST_TIER):
+    // the tier-0 instantiation by resident waves per SIMD, list entries per queue atomic, and the
+    // tier the first round runs every cell in. Any choice gives the same results.
+    int t0_waves = vd::kTier0WavesDefault;
+    uint32_t qgrain = 1;
+    int first_tier = 0;
     int max_attempts = 8;              // launches with doubled arenas before giving up (VD_OPT_ATTEMPTS)
     uint64_t cap_cells = 0;            // cells the arena capacities were sized for
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
@@ -499,7 +505,7 @@ int engine_init(Engine &E) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&E.ev0);
     if (e == hipSuccess) e = hipEventCreate(&E.ev1);
-    for (int k = 0; k < kTiers && e == hipSuccess; ++k) e = vd::cells_occupancy(k, &E.occ[k]);
+    for (int k = 0; k < kTiers && e == hipSuccess; ++k) e = vd::cells_occupancy(k, &E.occ[k], E.t0_waves);
     if (e == hipSuccess) e = hipHostMalloc((void **)&E.h_counts, 64 * sizeof(uint32_t), hipHostMallocDefault);
     if (e != hipSuccess) return fail(VD_EDEVICE, std::string("device setup: ") + hipGetErrorString(e));
     for (int k = 0; k < kTiers; ++k) E.occ[k] = std::max(1, E.occ[k]);
@@ -928,6 +934,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     a.cells = d_list;
     a.ncells = nlist;
     a.queue = E.d_queue.as<uint32_t>() + slot;
+    a.qgrain = E.qgrain;
     a.lshell = lshell;
     a.lshell_off = lshell_off;
     a.lshell_n = lshell_n;
@@ -959,7 +966,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
         a.arena_chunk[k] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kArenaChunk[k], share));
     }
     if (ms) VD_HIP(hipEventRecord(E.ev0, st));
-    VD_HIP(vd::launch_cells(tier, a, waves, st, E.seal != 0, probe));
+    VD_HIP(vd::launch_cells(tier, a, waves, st, E.seal != 0, probe, E.t0_waves));
     if (ms) {
         VD_HIP(hipEventRecord(E.ev1, st));
         VD_HIP(hipEventSynchronize(E.ev1));
@@ -968,16 +975,17 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
     return VD_OK;
 }
 
-// Launches of the first round of a calculate: the binning (if the generators changed), the tier-0
-// cell launch over every cell, the collect of its retry lists and the readback of the outcome
-// counts into pinned memory. Launches and copies only, so the sequence can be captured.
+// Launches of the first round of a calculate: the binning (if the generators changed), the cell
+// launch over every cell in the first tier (tier 0 unless VD_OPT_FIRST_TIER says otherwise), the
+// collect of its retry lists and the readback of the outcome counts into pinned memory. Launches
+// and copies only, so the sequence can be captured.
 int enqueue_round0(Engine &E, uint32_t n, float *ms) {
     if (!E.binned) VD_TRY(enqueue_binning(E));
     VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 12, E.stream));
-    VD_TRY(run_cells(E, 0, E.d_items.as<uint32_t>(), n, ms));
+    VD_TRY(run_cells(E, E.first_tier, E.d_items.as<uint32_t>(), n, ms));
     VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 4 * kTiers, E.stream));
     VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), E.d_items.as<uint32_t>(), n, E.d_out0.as<uint32_t>(),
-                              E.d_out1.as<uint32_t>(), E.d_counts4.as<uint32_t>(), E.stream));
+                              E.d_out1.as<uint32_t>(), E.d_counts4.as<uint32_t>() + 4 * E.first_tier, E.stream));
     VD_HIP(hipMemcpyAsync(E.h_counts, E.d_counts4.p, 4 * kTiers * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           E.stream));
     return VD_OK;
@@ -1090,7 +1098,8 @@ int far_lists(Engine &E, const uint32_t *d_list, uint32_t n, DevBuf &lsh, DevBuf
 std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
                                E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.reserve_cus,
-                               (uint64_t)E.mem.size(), E.ntube, (uint64_t)E.seal, E.nhost, E.seal_slots};
+                               (uint64_t)E.mem.size(), E.ntube, (uint64_t)E.seal, E.nhost, E.seal_slots,
+                               (uint64_t)E.t0_waves, E.qgrain, (uint64_t)E.first_tier};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
@@ -1196,7 +1205,7 @@ int engine_calculate(Engine &E) {
         VD_TRY(ensure(E.d_out0, n * 4));
         VD_TRY(ensure(E.d_out1, n * 4));
 
-        // Tier 0 (register fast path, small LDS) over every cell in bin order. A cell that
+        // Tier 0 (register fast path, small LDS; or the first tier set) over every cell in bin order. A cell that
         // outgrows its tier moves to the next one; a cell whose walk runs past the shell prefix
         // reruns in its tier after the prefixes are extended. Reruns start from scratch and give
         // the same result. Cells that once ran past their prefix reach across empty space: they
@@ -1226,8 +1235,8 @@ int engine_calculate(Engine &E) {
                 E.stats[4] = 1000.0 * ms;
                 E.stats[0] = (double)n;
                 std::memcpy(counts, E.h_counts, 4 * NT * sizeof(uint32_t));
-                ocap[0][0] = E.d_out0.as<uint32_t>();
-                oshell[0][0] = E.d_out1.as<uint32_t>();
+                ocap[0][E.first_tier] = E.d_out0.as<uint32_t>();
+                oshell[0][E.first_tier] = E.d_out1.as<uint32_t>();
             } else {
                 if (round == 0) VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 12, E.stream));
                 // group 1 (cells crossing empty space: few, long walks) runs on a side stream, so
@@ -1755,6 +1764,62 @@ struct vd_server {
     uint64_t steps = 0, cells = 0;
 };
 
+// Launch shape of the cell kernels (VD_OPT_T0_WAVES, VD_OPT_QUEUE_GRAIN, VD_OPT_FIRST_TIER).
+enum LaunchKnob : int { kKnobT0Waves = 0, kKnobGrain = 1, kKnobFirstTier = 2 };
+
+int engine_set_launch(Engine &E, int knob, int64_t value) {
+    if (knob == kKnobT0Waves) {
+        if (value < 5 || value > 7 || !vd::tier0_waves_valid((int)value))
+            return fail(VD_EINVAL, "tier-0 waves per SIMD must be 5, 6 or 7");
+        int occ = 1;
+        VD_HIP(vd::cells_occupancy(0, &occ, (int)value));
+        E.t0_waves = (int)value;
+        E.occ[0] = std::max(1, occ);
+        return VD_OK;
+    }
+    if (knob == kKnobGrain) {
+        if (value < 1 || value > 64) return fail(VD_EINVAL, "queue grain must be in [1, 64]");
+        E.qgrain = (uint32_t)value;
+        return VD_OK;
+    }
+    if (value < 0 || value > 1) return fail(VD_EINVAL, "first tier must be 0 or 1");
+    E.first_tier = (int)value;
+    return VD_OK;
+}
+
+int64_t engine_launch_value(const Engine &E, int knob) {
+    return knob == kKnobT0Waves ? E.t0_waves : knob == kKnobGrain ? (int64_t)E.qgrain : E.first_tier;
+}
+
+// Coordinate descent over the launch shape: from the current one, each knob in turn takes the
+// candidate under which run() (one calculate, returning its time in microseconds) was fastest,
+// the other knobs held. Every shape gives the same results, so whichever calculate ran last left
+// valid ones. picks: tier-0 waves per SIMD, queue grain, first tier.
+template <class Run>
+int engine_tune_launch(Engine &E, Run run, int picks[3]) {
+    static const std::vector<int64_t> cand[3] = {{5, 6, 7}, {1, 2, 4, 8}, {0, 1}};
+    double cur = 0.0;
+    VD_TRY(run(&cur));   // warm: shell tables, buffers
+    VD_TRY(run(&cur));
+    for (int k = 0; k < 3; ++k) {
+        const int64_t keep = engine_launch_value(E, k);
+        int64_t bv = keep;
+        double bt = cur;
+        for (int64_t v : cand[k]) {
+            if (v == keep) continue;
+            VD_TRY(engine_set_launch(E, k, v));
+            double us = 0.0;
+            VD_TRY(run(&us));
+            if (us < bt) { bt = us; bv = v; }
+        }
+        VD_TRY(engine_set_launch(E, k, bv));
+        cur = bt;
+    }
+    if (picks)
+        for (int k = 0; k < 3; ++k) picks[k] = (int)engine_launch_value(E, k);
+    return VD_OK;
+}
+
 extern "C" {
 
 const char *vd_last_error(void) { return g_err.c_str(); }
@@ -1861,7 +1926,7 @@ int vd_calculate_static(const double bmin[3], const double bmax[3], uint32_t nx,
     VD_HIP(hipGetDevice(&E.device));
     VD_HIP(hipDeviceGetAttribute(&E.cus, hipDeviceAttributeMultiprocessorCount, E.device));
     for (int k = 0; k < kTiers; ++k) {
-        VD_HIP(vd::cells_occupancy(k, &E.occ[k]));
+        VD_HIP(vd::cells_occupancy(k, &E.occ[k], E.t0_waves));
         E.occ[k] = std::max(1, E.occ[k]);
     }
     char *base = nullptr;
@@ -2043,6 +2108,16 @@ int vd_tess_calculate(vd_tess *t) {
     return engine_calculate(t->E);
 }
 
+int vd_tess_autotune_launch(vd_tess *t, int picks[3]) {
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    if (!t->E.prepared) return fail(VD_ESTATE, "generators are not set");
+    return engine_tune_launch(t->E, [&](double *us) {
+        VD_TRY(vd_tess_calculate(t));
+        *us = t->E.stats[5];
+        return VD_OK;
+    }, picks);
+}
+
 int vd_tess_calculate_sealed(vd_tess *t) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
     t->prune_log.clear();
@@ -2140,6 +2215,9 @@ int vd_tess_set_option(vd_tess *t, int key, int64_t value) {
         t->E.max_attempts = (int)value;
         return VD_OK;
     }
+    if (key == VD_OPT_T0_WAVES) return engine_set_launch(t->E, kKnobT0Waves, value);
+    if (key == VD_OPT_QUEUE_GRAIN) return engine_set_launch(t->E, kKnobGrain, value);
+    if (key == VD_OPT_FIRST_TIER) return engine_set_launch(t->E, kKnobFirstTier, value);
     return fail(VD_EINVAL, "unknown option");
 }
 
@@ -2326,6 +2404,9 @@ int vd_batch_set_option(vd_batch *b, int key, int64_t value) {
         b->E.far_list_min = (double)value;
         return VD_OK;
     }
+    if (key == VD_BOPT_T0_WAVES) return engine_set_launch(b->E, kKnobT0Waves, value);
+    if (key == VD_BOPT_QUEUE_GRAIN) return engine_set_launch(b->E, kKnobGrain, value);
+    if (key == VD_BOPT_FIRST_TIER) return engine_set_launch(b->E, kKnobFirstTier, value);
     return fail(VD_EINVAL, "unknown option");
 }
 
@@ -2358,6 +2439,16 @@ int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *bes
     E.prepared = E.calculated = false;
     if (best) *best = best_r;
     return VD_OK;
+}
+
+int vd_batch_autotune_launch(vd_batch *b, int picks[3]) {
+    if (!b) return fail(VD_EINVAL, "NULL handle");
+    if (b->E.mem.empty()) return fail(VD_ESTATE, "the batch has no members");
+    return engine_tune_launch(b->E, [&](double *us) {
+        VD_TRY(vd_batch_calculate(b));
+        *us = b->E.stats[5];
+        return VD_OK;
+    }, picks);
 }
 
 // ---- vd_server -----------------------------------------------------------------------------

@@ -165,14 +165,13 @@ __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
     return (uint32_t)v;
 }
 
-template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int TM_, int WAVES_, bool GLOBAL_ = false>
+template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int WAVES_, bool GLOBAL_ = false>
 struct Cap {
     static constexpr int VM = VM_;      // vertices
     static constexpr int SM = SM_;      // face slots (flattened face indices)
     static constexpr int FM = FM_;      // faces
     static constexpr int KM = KM_;      // plane-crossing slots per clip
     static constexpr bool FAST = FAST_; // register fast path (VM <= 64) instead of the generic one
-    static constexpr int TM = TM_;      // triangles staged in LDS for the volume/centroid folds
     static constexpr int WAVES = WAVES_;  // occupancy target (waves per SIMD) for the compiler
     static constexpr bool GLOBAL = GLOBAL_;  // the polytope lives in global memory, not LDS
     static constexpr int NV = FAST_ ? (VM_ + 63) / 64 : 1;   // register vertices per lane
@@ -228,20 +227,27 @@ struct ClipScratch<C, false> {
     uint16_t lid[C::SM];
 };
 
+// One gather's window staging: written, then read back into registers before any clip.
+struct WinStage {
+    uint32_t q[64];     // bin slot of window lane w's candidate
+    uint16_t ent[64];   // batch lane of its shell entry
+    uint16_t off[64];   // its offset within the entry's bin
+};
+
 template <class C>
 struct Mem {
     double vx[C::NB][C::VM], vy[C::NB][C::VM], vz[C::NB][C::VM];
     uint32_t sedge[C::NB][C::SM];
     int32_t fnb[C::NB][C::FM];
-    uint32_t winq[64];
     uint16_t sface[C::NB][C::SM];
     uint16_t fstart[C::NB][C::FM + 1];
-    uint16_t winent[64];
-    uint16_t winoff[64];
-    // Scratch of one clip; at the end of a cell the same bytes stage the fold terms.
+    // Bytes of three phases that never overlap: one clip's scratch, one window gather's staging,
+    // and at the end of a cell the fold terms of one 64-slot chunk. Keeping them in one place is
+    // what holds tier 0's LDS per wave under 5 KiB (more resident waves per SIMD).
     union {
         ClipScratch<C> sc;
-        double tri[4][C::TM + 1];   // +1 keeps the four rows in different LDS banks
+        WinStage win;
+        double tri[4][65];   // +1 keeps the four rows in different LDS banks
     };
 };
 
@@ -904,24 +910,20 @@ __device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double
 }
 
 // Volume and centroid (cell_3d_faces.rs:106-205). Every face has >= 3 entries, so slot p of face
-// f at position 1 <= i <= count-2 is triangle p - 2f - 1 of the fan order. Triangle terms are
-// computed lane-parallel and staged in LDS; lanes 0-3 then fold det, det*x, det*y, det*z left in
-// that order. More than TM triangles fall back to a fold by lane reads.
+// f at position 1 <= i <= count-2 is triangle p - 2f - 1 of the fan order, and the triangles of one
+// 64-slot chunk are consecutive. Per chunk, triangle terms are computed lane-parallel and staged in
+// LDS; lanes 0-3 then fold det, det*x, det*y, det*z left in triangle order, chunk after chunk.
 template <class C>
 __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]) {
     const int l = lane_id_here();
     const int c = P.cur;
-    const int T = P.S - 2 * P.F;
-    const bool staged = T <= C::TM;
-    double tv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    double acc = 0.0;
     for (int base = 0; base < P.S; base += 64) {
         const int p = base + l;
-        const bool act = p < P.S;
         double det = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
         bool tri = false;
-        int f = 0;
-        if (act) {
-            f = m.sface[c][p];
+        if (p < P.S) {
+            const int f = m.sface[c][p];
             const int fs = m.fstart[c][f], fe = m.fstart[c][f + 1];
             const int cnt = fe - fs, i = p - fs;
             tri = cnt >= 3 && i >= 1 && i <= cnt - 2;
@@ -941,31 +943,19 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
                 wz = det * (az + bz + qz);
             }
         }
-        if (staged) {
-            if (tri) {
-                const int t = p - 2 * f - 1;
-                m.tri[0][t] = det; m.tri[1][t] = wx; m.tri[2][t] = wy; m.tri[3][t] = wz;
-            }
-        } else {
-            uint64_t tm = ballot(tri);
-            while (tm) {
-                const int t = ffs64(tm);
-                tm &= tm - 1;
-                tv += rld(det, t);
-                sx += rld(wx, t);
-                sy += rld(wy, t);
-                sz += rld(wz, t);
-            }
+        const uint64_t tm = ballot(tri);
+        if (tri) {
+            const int j = below_lane(tm);   // the chunk's triangles in order
+            m.tri[0][j] = det; m.tri[1][j] = wx; m.tri[2][j] = wy; m.tri[3][j] = wz;
         }
-    }
-    if (staged) {
         poly_sync<C>();
-        double acc = 0.0;
-        if (l < 4)
-            for (int t = 0; t < T; ++t) acc += m.tri[l][t];
-        tv = rld(acc, 0); sx = rld(acc, 1); sy = rld(acc, 2); sz = rld(acc, 3);
+        if (l < 4) {
+            const int nt = popc(tm);
+            for (int j = 0; j < nt; ++j) acc += m.tri[l][j];
+        }
         poly_sync<C>();
     }
+    const double tv = rld(acc, 0), sx = rld(acc, 1), sy = rld(acc, 2), sz = rld(acc, 3);
     vol = fabs(tv / 6.0);
     if (fabs(tv) < 1e-9) {
         cen[0] = cen[1] = cen[2] = 0.0;
@@ -1221,15 +1211,15 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                 const int nwin = wtot < 64u ? (int)wtot : 64;
                 if (take) {
                     for (uint32_t t = 0; t < cnt && pos + t < 64u; ++t) {
-                        m.winq[pos + t] = bs + skip + t;
-                        m.winent[pos + t] = (uint16_t)l;
-                        m.winoff[pos + t] = (uint16_t)(skip + t);
+                        m.win.q[pos + t] = bs + skip + t;
+                        m.win.ent[pos + t] = (uint16_t)l;
+                        m.win.off[pos + t] = (uint16_t)(skip + t);
                     }
                 }
                 poly_sync<C>();
                 const bool wa = l < nwin;
-                const uint32_t wq = wa ? m.winq[l] : 0u;
-                wko = wa ? ((uint32_t)m.winent[l] | ((uint32_t)m.winoff[l] << 8)) : 0u;
+                const uint32_t wq = wa ? m.win.q[l] : 0u;
+                wko = wa ? ((uint32_t)m.win.ent[l] | ((uint32_t)m.win.off[l] << 8)) : 0u;
                 wj = 0xffffffffu;
                 wd0 = wd1 = wd2 = wdsq = 0.0;
                 if (wa) {
@@ -1455,11 +1445,13 @@ template <class C, bool SEAL, bool PROBE>
 __device__ inline void cells_loop(Mem<C> &m, const CellArgs &A) {
     ArenaChunks ac;
     for (;;) {
+        // A.qgrain consecutive list entries per queue atomic
         uint32_t q = 0;
-        if (lane_id() == 0) q = atomicAdd(A.queue, 1u);
+        if (lane_id() == 0) q = atomicAdd(A.queue, A.qgrain);
         q = first_lane_u32(q);
         if (q >= A.ncells) break;
-        process_cell<C, SEAL, PROBE>(m, A, A.cells[q], q, ac);
+        const uint32_t qe = A.ncells - q < A.qgrain ? A.ncells : q + A.qgrain;
+        for (; q < qe; ++q) process_cell<C, SEAL, PROBE>(m, A, A.cells[q], q, ac);
     }
     // entries the wave's cells occupy (the chunks' unused tails excluded)
     if (lane_id() == 0)
@@ -1482,21 +1474,21 @@ __global__ __launch_bounds__(64) void k_cells_global(CellArgs A) {
     cells_loop<C, SEAL, true>(m, A);
 }
 
-// Tier 0: register fast path (one vertex per lane), small LDS (many waves per CU). Tier 1: the
-// same path with two vertices per lane (128 vertices). Tiers 2-3: generic path for cells that
-// outgrow the tier before, with 256 and 1024 vertices (the last one wave per CU).
-#ifndef VD_TIER0_WAVES
-#define VD_TIER0_WAVES 5
-#endif
-using Tier0 = Cap<64, 192, 48, 64, true, 107, VD_TIER0_WAVES>;
-using Tier1 = Cap<128, 384, 64, 64, true, 111, 4>;
-using Tier2 = Cap<256, 768, 128, 384, false, 255, 1>;
-using Tier3 = Cap<1024, 2560, 320, 1280, false, 511, 1>;
+// Tier 0: register fast path (one vertex per lane), small LDS (many waves per CU), compiled for
+// 5, 6 and 7 resident waves per SIMD (the register budget each allows; the launch picks one).
+// Tier 1: the same path with two vertices per lane (128 vertices). Tiers 2-3: generic path for
+// cells that outgrow the tier before, with 256 and 1024 vertices (the last one wave per CU).
+template <int W>
+using Tier0W = Cap<64, 192, 48, 64, true, W>;
+using Tier0 = Tier0W<kTier0WavesDefault>;
+using Tier1 = Cap<128, 384, 64, 64, true, 4>;
+using Tier2 = Cap<256, 768, 128, 384, false, 1>;
+using Tier3 = Cap<1024, 2560, 320, 1280, false, 1>;
 // Tier 4: the generic path with its polytope in global memory, for cells beyond LDS. Face-slot
 // offsets are 16-bit in every tier and a crossing's vertex id shares 16 bits with a flag in
 // clip_generic, so it holds up to 32,767 vertices and faces, 65,535 face slots and 32,767
 // crossings per clip (the reference's own face indices are u16, cell_3d_faces.rs:29).
-using TierG = Cap<32767, 65535, 32767, 32767, false, 511, 1, true>;
+using TierG = Cap<32767, 65535, 32767, 32767, false, 1, true>;
 
 // Work-list entries whose cell stopped at the end of its shell prefix and whose walk covers about
 // min_entries bins or more: 4/3 pi (2r)^3 over the bin volume (a size estimate only; the lists
@@ -1634,30 +1626,44 @@ size_t tier_lds_bytes(int tier) {
 size_t tier_global_bytes() { return sizeof(Mem<TierG>); }
 
 template <bool SEAL, bool PROBE>
-void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st) {
-    if (tier == 0) k_cells<Tier0, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st, int t0_waves) {
+    if (tier == 0) {
+        // the skip-ahead reruns never start in tier 0 (they rerun from tier 1): one variant
+        if (PROBE || t0_waves == kTier0WavesDefault) k_cells<Tier0, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+        else if (t0_waves == 5) k_cells<Tier0W<5>, SEAL, false><<<grid_waves, 64, 0, st>>>(a);
+        else k_cells<Tier0W<7>, SEAL, false><<<grid_waves, 64, 0, st>>>(a);
+    }
     else if (tier == 1) k_cells<Tier1, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 2) k_cells<Tier2, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 3) k_cells<Tier3, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
     else k_cells_global<TierG, SEAL><<<grid_waves, 64, 0, st>>>(a);
 }
 
-hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal, bool probe) {
+bool tier0_waves_valid(int w) { return w == 5 || w == 6 || w == 7; }
+
+hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal, bool probe,
+                        int t0_waves) {
     if (a.ncells == 0) return hipSuccess;
     if (tier == 4 && !a.gmem) return hipErrorInvalidValue;
+    if (a.qgrain == 0 || !tier0_waves_valid(t0_waves)) return hipErrorInvalidValue;
     if (seal) {
-        if (probe) launch_tier<true, true>(tier, a, grid_waves, st);
-        else launch_tier<true, false>(tier, a, grid_waves, st);
+        if (probe) launch_tier<true, true>(tier, a, grid_waves, st, t0_waves);
+        else launch_tier<true, false>(tier, a, grid_waves, st, t0_waves);
     } else {
-        if (probe) launch_tier<false, true>(tier, a, grid_waves, st);
-        else launch_tier<false, false>(tier, a, grid_waves, st);
+        if (probe) launch_tier<false, true>(tier, a, grid_waves, st, t0_waves);
+        else launch_tier<false, false>(tier, a, grid_waves, st, t0_waves);
     }
     return hipGetLastError();
 }
 
-hipError_t cells_occupancy(int tier, int *blocks_per_cu) {
-    if (tier == 0)
+hipError_t cells_occupancy(int tier, int *blocks_per_cu, int t0_waves) {
+    if (tier == 0) {
+        if (t0_waves == 5)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<5>, false, false>, 64, 0);
+        if (t0_waves == 7)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<7>, false, false>, 64, 0);
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0, false, false>, 64, 0);
+    }
     if (tier == 1)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1, false, false>, 64, 0);
     if (tier == 2)

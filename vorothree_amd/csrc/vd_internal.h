@@ -106,6 +106,7 @@ struct CellArgs {
     const uint32_t *cells;
     uint32_t ncells;
     uint32_t *queue;              // atomic work counter (zeroed before launch)
+    uint32_t qgrain;              // list entries a wave takes per queue atomic (>= 1)
     // per-cell shell lists (optional, nullptr: none): work-list entry q walks the lshell_n[q]
     // codes from lshell[lshell_off[q]] instead of its member's prefix, unless lshell_n[q] is kNoList
     const ShellCode *lshell;
@@ -236,9 +237,12 @@ hipError_t shell_finish(const ShellPack &pack, const double cs[3], uint32_t w, u
 // with two vertices per lane (128 vertices); 2, 3 = generic path with LDS polytopes of 256 and 1024
 // vertices (the last one wave per CU); 4 = the generic path with the polytope in global memory
 // (a.gmem: grid_waves slots of tier_global_bytes() each), up to 65,535 vertices
+// t0_waves: the tier-0 instantiation, by resident waves per SIMD (5, 6 or 7; same results)
+constexpr int kTier0WavesDefault = 6;
+bool tier0_waves_valid(int w);
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal = false,
-                        bool probe = false);
-hipError_t cells_occupancy(int tier, int *blocks_per_cu);
+                        bool probe = false, int t0_waves = kTier0WavesDefault);
+hipError_t cells_occupancy(int tier, int *blocks_per_cu, int t0_waves = kTier0WavesDefault);
 int tier_count();
 int tier_max_vertices(int tier);
 size_t tier_lds_bytes(int tier);

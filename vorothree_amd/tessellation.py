@@ -579,6 +579,22 @@ class Tessellation:
         with VD_ECAPACITY and keeps no results (VD_OPT_ATTEMPTS, default 8)."""
         check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_ATTEMPTS, int(attempts)))
 
+    def set_launch(self, t0_waves: int | None = None, queue_grain: int | None = None,
+                   first_tier: int | None = None) -> None:
+        """Launch shape of the cell kernels (VD_OPT_T0_WAVES 5/6/7, VD_OPT_QUEUE_GRAIN 1-64,
+        VD_OPT_FIRST_TIER 0/1). Results are identical under every shape."""
+        for key, v in ((_lib.VD_OPT_T0_WAVES, t0_waves), (_lib.VD_OPT_QUEUE_GRAIN, queue_grain),
+                       (_lib.VD_OPT_FIRST_TIER, first_tier)):
+            if v is not None:
+                check(lib().vd_tess_set_option(self._h, key, int(v)))
+
+    def autotune_launch(self) -> dict:
+        """Times calculate() over the launch shapes and keeps the fastest (vd_tess_autotune_launch);
+        returns the picks. Generators must be set."""
+        picks = (ctypes.c_int * 3)()
+        check(lib().vd_tess_autotune_launch(self._h, picks))
+        return {"t0_waves": picks[0], "queue_grain": picks[1], "first_tier": picks[2]}
+
     # -- queries -------------------------------------------------------------------------------
     def count_generators(self) -> int:
         return int(lib().vd_tess_count_generators(self._h))
@@ -767,6 +783,20 @@ class Batch:
         best = ctypes.c_double(0.0)
         check(lib().vd_batch_autotune(self._h, _ptr(c), c.size, ctypes.byref(best)))
         return best.value
+
+    def set_launch(self, t0_waves: int | None = None, queue_grain: int | None = None,
+                   first_tier: int | None = None) -> None:
+        """As Tessellation.set_launch (VD_BOPT_T0_WAVES, VD_BOPT_QUEUE_GRAIN, VD_BOPT_FIRST_TIER)."""
+        for key, v in ((_lib.VD_BOPT_T0_WAVES, t0_waves), (_lib.VD_BOPT_QUEUE_GRAIN, queue_grain),
+                       (_lib.VD_BOPT_FIRST_TIER, first_tier)):
+            if v is not None:
+                check(lib().vd_batch_set_option(self._h, key, int(v)))
+
+    def autotune_launch(self) -> dict:
+        """vd_batch_autotune_launch: the launch shape for the batch's current grids."""
+        picks = (ctypes.c_int * 3)()
+        check(lib().vd_batch_autotune_launch(self._h, picks))
+        return {"t0_waves": picks[0], "queue_grain": picks[1], "first_tier": picks[2]}
 
     def __len__(self):
         return int(lib().vd_batch_count(self._h))
