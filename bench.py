@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--serve-sweep-requests", type=int, default=30)
     p.add_argument("--serve-sweep-rates", type=str, default="5,50,500")
     p.add_argument("--no-clustered", action="store_true", help="skip the 4M clustered workload")
+    p.add_argument("--no-launch-tune", action="store_true",
+                   help="keep the default launch shape instead of vd_*_autotune_launch's pick")
     p.add_argument("--clustered-n", type=int, default=4_000_000)
     return p.parse_args()
 
@@ -187,11 +189,12 @@ def seen_per_bin(points, grid) -> float:
     return float((c.astype(np.float64) ** 2).sum() / len(points))
 
 
-def clustered(n: int, targets, steps: int) -> dict:
+def clustered(n: int, targets, steps: int, launch_tune: bool = True) -> dict:
     """Uneven density: n generators, 80% from 64 Gaussian clusters (sigma 0.02) + 20% uniform, in
     the unit cube, no walls. The grid is tuned over generators-per-bin targets of the density rule
-    (vd_grid_for_points); the best is timed over ``steps`` calculate() calls. Each grid also
-    reports the generators per bin seen by the average generator, sum(c^2) / n over its bins."""
+    (vd_grid_for_points), each with its own launch shape (vd_tess_autotune_launch); the best pair
+    is timed over ``steps`` calculate() calls. Each grid also reports the generators per bin seen
+    by the average generator, sum(c^2) / n over its bins."""
     from vorothree_amd import Algorithm3DGrid, BoundingBox, Tessellation, grid_for_points, _hip
     from vorothree_amd import workloads as W
 
@@ -206,17 +209,20 @@ def clustered(n: int, targets, steps: int) -> dict:
         t = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=False)
         t.set_generators_device(dpts.ptr.value, n)
         t.calculate()                              # builds the shell prefix
+        launch = t.autotune_launch() if launch_tune else None
         _hip.synchronize()
         c0 = time.perf_counter()
         t.calculate()
         dt = time.perf_counter() - c0
         tried.append({"per_bin_target": tgt, "grid": list(grid), "seen_per_bin": seen_per_bin(w.points, grid),
-                      "calc_ms": dt * 1e3})
+                      "launch": launch, "calc_ms": dt * 1e3})
         if best is None or dt < best[0]:
-            best = (dt, tgt, grid)
+            best = (dt, tgt, grid, launch)
         t.close()
-    _, tgt, grid = best
+    _, tgt, grid, launch = best
     t = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=False)
+    if launch:
+        t.set_launch(**launch)
     t.set_generators_device(dpts.ptr.value, n)
     t.calculate()
     _hip.synchronize()
@@ -230,7 +236,7 @@ def clustered(n: int, targets, steps: int) -> dict:
     cells = t.count_cells()
     out = {"workload": f"unit cube, {n} f64 generators: 80% from 64 Gaussian clusters (sigma 0.02, "
                        "re-drawn inside the box) + 20% uniform, seed 3, no walls",
-           "grid": list(grid), "per_bin_target": tgt, "tuning": tried, "cells": cells,
+           "grid": list(grid), "per_bin_target": tgt, "tuning": tried, "launch": launch, "cells": cells,
            "ms_per_step": dt * 1e3, "cells_per_s": cells / dt,
            "path": "set_generators(device) + calculate(), reduced outputs",
            "cells_first_tier": int(st[0]), "cell_runs_larger_tiers": int(st[1]),
@@ -242,7 +248,7 @@ def clustered(n: int, targets, steps: int) -> dict:
 BATCH_TUNE = (0.25, 0.35, 0.5, 1.0, 2.0)
 
 
-def batched_vs_plain(count: int, steps: int, tune=None) -> dict:
+def batched_vs_plain(count: int, steps: int, tune=None, launch_tune: bool = True) -> dict:
     """Mixed-size workload: ``count`` tessellations with log-uniform generator counts in
     [64, 65536], boxes with edges U[0.5, 2], 30% sphere walls, 20% plane walls. Grids: about 2
     generators per bin (``tune`` None), or the batch autotuner's pick among the generators-per-bin
@@ -250,7 +256,9 @@ def batched_vs_plain(count: int, steps: int, tune=None) -> dict:
     members in one vd_batch_calculate (wall filter, binning and one persistent cell launch for
     all). Plain: one Tessellation per member, each running its own set_generators + calculate
     launch sequence, eagerly; the same with graph replay of repeated sequences is reported beside
-    it. All start from points already in device memory."""
+    it. All start from points already in device memory. The launch shape is the batch's
+    vd_batch_autotune_launch pick (``launch_tune``); the plain tessellations keep the library's
+    default shape (one tuned shape for thousands of small launches would not be theirs)."""
     from vorothree_amd import (Algorithm3DGrid, Batch, BoundingBox, PlaneGeometry, SphereGeometry,
                                Tessellation, Wall, _hip)
     from vorothree_amd import workloads as W
@@ -269,6 +277,7 @@ def batched_vs_plain(count: int, steps: int, tune=None) -> dict:
         b.add(BoundingBox(w.lo, w.hi), None if tune else w.grid, walls_of(w), w.points)
     b.upload()
     best = b.autotune(tune) if tune else None
+    launch = b.autotune_launch() if launch_tune else None
     b.calculate()                       # warmup (also builds the shell prefixes)
     _hip.synchronize()
     t0 = time.perf_counter()
@@ -325,7 +334,7 @@ def batched_vs_plain(count: int, steps: int, tune=None) -> dict:
                  if tune else "grids of about 2 generators per bin")
     return {"workload": f"{count} tessellations, generators log-uniform in [64, 65536], "
                         f"boxes U[0.5,2]^3, 30% sphere / 20% plane walls, {grid_note}",
-            "per_bin": best if tune else 2.0, "generators_per_bin": float(offs[-1]) / bins,
+            "per_bin": best if tune else 2.0, "launch": launch, "generators_per_bin": float(offs[-1]) / bins,
             "generators": int(offs[-1]), "cells": cells, "cells_plain": cells_p,
             "batched_cells_per_s": cells / tb, "batched_ms": tb * 1e3, "batched_first_round_ms": kern_b * 1e3,
             "batched_graph_replays": replays_b,
@@ -387,6 +396,11 @@ def main():
     t.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5)))
     dpts = _hip.DeviceArray(w.points)
     _hip.synchronize()
+    launch = None
+    if not a.no_launch_tune:
+        t.set_generators_device(dpts.ptr.value, a.n)
+        launch = t.autotune_launch()
+        progress(f"plain: launch shape {launch}")
 
     def step():
         t.set_generators_device(dpts.ptr.value, a.n)
@@ -411,6 +425,8 @@ def main():
     # the same steps with the first round replayed from a captured graph (end-to-end rate only)
     tg = Tessellation(b, Algorithm3DGrid(*w.grid, b), keep_cells=not a.reduced, graphs=True)
     tg.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5)))
+    if launch:
+        tg.set_launch(**launch)
     for _ in range(max(2, a.warmup)):
         tg.set_generators_device(dpts.ptr.value, a.n)
         tg.calculate()
@@ -477,7 +493,10 @@ def main():
         achieved = cells * flops_per_cell / kern_s / 1e12
         roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                "kernel": "k_cells<Tier0>", "kernel_ms": kern_s * 1e3,
+                "kernel": ("k_cells<Tier1" if launch and launch["first_tier"] == 1
+                           else f"k_cells<Tier0W<{launch['t0_waves'] if launch else 6}>")
+                          + (", skip-ahead>" if launch and launch["first_probe"] else ">"),
+                "kernel_ms": kern_s * 1e3,
                 "flops_per_cell": flops_per_cell,
                 "hbm": {"achieved": bytes_launch / kern_s / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": bytes_launch / kern_s / 1e9 / HBM_PEAK_GBS,
@@ -499,6 +518,12 @@ def main():
                                "grid 100^3, sphere wall c=(0.5,0.5,0.5) r=0.5",
                    "generators": a.n, "cells": cells, "grid": list(w.grid),
                    "keep_cells": not a.reduced, "path": "set_generators(device) + calculate()",
+                   "launch": launch or {"t0_waves": 6, "queue_grain": 1, "first_tier": 0, "first_probe": 0},
+                   "launch_note": "tier-0 waves per SIMD, cells per queue atomic and first tier: "
+                                  + ("vd_tess_autotune_launch's pick on this workload (coordinate "
+                                     "descent over skip-ahead 0/1, waves 5/6/7, grain 1/2/4/8, first "
+                                     "tier 0/1), before the warmup"
+                                     if launch else "the library defaults"),
                    "cells_retried_large_tier": st[1], "shell_prefix_entries": st[3],
                    "graph_replay_cells_per_s": graph_rate, "graph_replays": graph_replays,
                    "volume_sum": float(vol.sum())},
@@ -509,12 +534,13 @@ def main():
         progress(f"cpu baseline: {cpu['value']:.4g} cells/s on {a.cpu_threads} threads")
     if not a.no_batched:
         progress("batched: grids of about 2 generators per bin")
-        out["batched"] = batched_vs_plain(a.batch_count, a.batch_steps)
+        out["batched"] = batched_vs_plain(a.batch_count, a.batch_steps, None, not a.no_launch_tune)
         progress("batched: autotuned grids")
-        out["batched"]["autotuned"] = batched_vs_plain(a.batch_count, a.batch_steps, BATCH_TUNE)
+        out["batched"]["autotuned"] = batched_vs_plain(a.batch_count, a.batch_steps, BATCH_TUNE,
+                                                       not a.no_launch_tune)
     if not a.no_clustered:
         progress(f"clustered: {a.clustered_n} points")
-        out["clustered"] = clustered(a.clustered_n, [0.25, 0.35, 0.7], 3)
+        out["clustered"] = clustered(a.clustered_n, [0.25, 0.35, 0.7], 3, not a.no_launch_tune)
     if not a.no_serve:
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
         out["serve"] = served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates)

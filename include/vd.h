@@ -207,13 +207,16 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
 #define VD_OPT_QUEUE_GRAIN 7 /* cells a wave takes from the work queue per atomic, 1 (default) to 64 */
 #define VD_OPT_FIRST_TIER 8 /* tier of the first launch over every cell: 0 (default, 64 vertices)
                                or 1 (128 vertices); cells that outgrow it move up as usual */
+#define VD_OPT_FIRST_PROBE 9 /* 1: the first launch uses the instantiation that skips runs of
+                               empty shell batches in bulk (cells in sparse regions walk far);
+                               0 (default): the plain walk, fewer registers */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
-/* Times calculate() under the launch shapes (coordinate descent over VD_OPT_T0_WAVES {5, 6, 7},
- * VD_OPT_QUEUE_GRAIN {1, 2, 4, 8}, VD_OPT_FIRST_TIER {0, 1}, from the current shape), keeps the
- * fastest and writes it to picks[3] (waves, grain, first tier). Generators must be set; about ten
- * calculates run, and the results of the last one are kept (identical under every shape). No
- * counterpart in the reference. */
-int vd_tess_autotune_launch(vd_tess *t, int picks[3]);
+/* Times calculate() under the launch shapes (coordinate descent over VD_OPT_FIRST_PROBE {0, 1},
+ * VD_OPT_T0_WAVES {5, 6, 7}, VD_OPT_QUEUE_GRAIN {1, 2, 4, 8}, VD_OPT_FIRST_TIER {0, 1}, from the
+ * current shape), keeps the fastest and writes it to picks[4] (waves, grain, first tier,
+ * skip-ahead). Generators must be set; about a dozen calculates run, and the results of the last
+ * one are kept (identical under every shape). No counterpart in the reference. */
+int vd_tess_autotune_launch(vd_tess *t, int picks[4]);
 
 /* Diagnostics of the last calculate(): [0] cells of the first launch (its tier: VD_OPT_FIRST_TIER), [1] cell runs
  * in the larger tiers, [2] shell-prefix extensions, [3] entries of the shell prefixes,
@@ -280,13 +283,14 @@ int vd_batch_get_cell(const vd_batch *b, uint32_t t, size_t i, uint64_t sizes[3]
 #define VD_BOPT_T0_WAVES 5   /* as VD_OPT_T0_WAVES */
 #define VD_BOPT_QUEUE_GRAIN 6 /* as VD_OPT_QUEUE_GRAIN */
 #define VD_BOPT_FIRST_TIER 7 /* as VD_OPT_FIRST_TIER */
+#define VD_BOPT_FIRST_PROBE 8 /* as VD_OPT_FIRST_PROBE */
 int vd_batch_set_option(vd_batch *b, int key, int64_t value);
 int vd_batch_get_stats(const vd_batch *b, double out[10]);
 /* Times vd_batch_calculate for each candidate generators-per-bin target (grids re-derived with
  * vd_grid_for for the members added with a zero grid), keeps the fastest, and returns it. */
 int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *best);
 /* vd_tess_autotune_launch for a batch (its grids as they are: run vd_batch_autotune first). */
-int vd_batch_autotune_launch(vd_batch *b, int picks[3]);
+int vd_batch_autotune_launch(vd_batch *b, int picks[4]);
 
 /* The grid rule of the autotuner: bins per axis proportional to the box edges with about
  * per_bin generators per bin, at least 1 per axis. */

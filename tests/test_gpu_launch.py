@@ -30,15 +30,16 @@ def oracle_sphere():
     return o
 
 
-@pytest.mark.parametrize("waves,grain,first", [(5, 1, 0), (6, 1, 0), (7, 1, 0), (6, 4, 0), (6, 8, 0),
-                                               (5, 64, 0), (6, 1, 1), (7, 2, 1)])
-def test_launch_shape_bit_exact(oracle_sphere, waves, grain, first):
-    print(f"launch shape waves={waves} grain={grain} first_tier={first}", flush=True)
+@pytest.mark.parametrize("waves,grain,first,probe", [(5, 1, 0, 0), (6, 1, 0, 0), (7, 1, 0, 0), (6, 4, 0, 0),
+                                                     (6, 8, 0, 0), (5, 64, 0, 0), (6, 1, 1, 0), (7, 2, 1, 0),
+                                                     (5, 1, 0, 1), (6, 2, 0, 1), (7, 1, 0, 1), (6, 1, 1, 1)])
+def test_launch_shape_bit_exact(oracle_sphere, waves, grain, first, probe):
+    print(f"launch shape waves={waves} grain={grain} first_tier={first} probe={probe}", flush=True)
     lo, hi, grid, pts = _sphere_case()
     b = BoundingBox(lo, hi)
     g = Tessellation(b, Algorithm3DGrid(*grid, b))
     g.add_wall(Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5)))
-    g.set_launch(t0_waves=waves, queue_grain=grain, first_tier=first)
+    g.set_launch(t0_waves=waves, queue_grain=grain, first_tier=first, first_probe=probe)
     g.set_generators(pts)
     o = oracle_sphere
     for rep in range(2):   # the second calculate replays the first round from a graph
@@ -62,7 +63,7 @@ def test_launch_options_rejected():
     b = BoundingBox((0.0,) * 3, (1.0,) * 3)
     g = Tessellation(b, Algorithm3DGrid(4, 4, 4, b))
     for kw in ({"t0_waves": 4}, {"t0_waves": 8}, {"queue_grain": 0}, {"queue_grain": 65}, {"first_tier": 2},
-               {"first_tier": -1}):
+               {"first_tier": -1}, {"first_probe": 2}):
         with pytest.raises(VdError):
             g.set_launch(**kw)
     g.close()
@@ -78,7 +79,7 @@ def test_tess_autotune_launch_keeps_results(oracle_sphere):
     print("picks", picks, flush=True)
     assert picks["t0_waves"] in (5, 6, 7)
     assert picks["queue_grain"] in (1, 2, 4, 8)
-    assert picks["first_tier"] in (0, 1)
+    assert picks["first_tier"] in (0, 1) and picks["first_probe"] in (0, 1)
     vol, cen, nf, _ = oracle_sphere.reduced()
     assert np.array_equal(bits(g.volumes()), bits(vol))
     g.calculate()
@@ -100,7 +101,8 @@ def test_batch_autotune_launch_keeps_results():
     ref = [bits(bt.volumes(t)).copy() for t in range(len(members))]
     picks = bt.autotune_launch()
     print("batch picks", picks, flush=True)
-    assert picks["t0_waves"] in (5, 6, 7) and picks["queue_grain"] in (1, 2, 4, 8) and picks["first_tier"] in (0, 1)
+    assert picks["t0_waves"] in (5, 6, 7) and picks["queue_grain"] in (1, 2, 4, 8)
+    assert picks["first_tier"] in (0, 1) and picks["first_probe"] in (0, 1)
     bt.calculate()
     for t in range(len(members)):
         assert np.array_equal(bits(bt.volumes(t)), ref[t]), t

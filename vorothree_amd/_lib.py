@@ -25,6 +25,7 @@ VD_OPT_ATTEMPTS = 5
 VD_OPT_T0_WAVES = 6
 VD_OPT_QUEUE_GRAIN = 7
 VD_OPT_FIRST_TIER = 8
+VD_OPT_FIRST_PROBE = 9
 VD_STATIC_SCRATCH_BYTES = 96 << 20
 VD_STATIC_MAX_POINTS = 262144
 VD_STATIC_MAX_BINS = 2097152
@@ -35,6 +36,7 @@ VD_BOPT_FAR_LIST = 4
 VD_BOPT_T0_WAVES = 5
 VD_BOPT_QUEUE_GRAIN = 6
 VD_BOPT_FIRST_TIER = 7
+VD_BOPT_FIRST_PROBE = 8
 VD_SOPT_PER_BIN = 1
 VD_SOPT_SMALL_JOB = 2
 

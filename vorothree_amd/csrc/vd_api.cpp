@@ -467,12 +467,14 @@ struct Engine {
     bool static_mode = false;
     uint32_t h_local[64] = {};         // the outcome counts' host copy in static mode
     uint64_t arena_hint = 40;          // initial arena vertices per cell (VD_OPT_ARENA)
-    // launch shape of the cell kernels (VD_OPT_T0_WAVES, VD_OPT_QUEUE_GRAIN, VD_OPT_FIRST_TIER):
-    // the tier-0 instantiation by resident waves per SIMD, list entries per queue atomic, and the
-    // tier the first round runs every cell in. Any choice gives the same results.
+    // launch shape of the cell kernels (VD_OPT_T0_WAVES, VD_OPT_QUEUE_GRAIN, VD_OPT_FIRST_TIER,
+    // VD_OPT_FIRST_PROBE): the tier-0 instantiation by resident waves per SIMD, list entries per
+    // queue atomic, the tier the first round runs every cell in, and whether that launch uses the
+    // skip-ahead instantiation. Any choice gives the same results.
     int t0_waves = vd::kTier0WavesDefault;
     uint32_t qgrain = 1;
     int first_tier = 0;
+    int first_probe = 0;               // the first launch skips inert shell batches (VD_OPT_FIRST_PROBE)
     int max_attempts = 8;              // launches with doubled arenas before giving up (VD_OPT_ATTEMPTS)
     uint64_t cap_cells = 0;            // cells the arena capacities were sized for
     DevBuf d_smask, d_scnt, d_soff, d_snb, d_slog, d_slogcnt, d_pcnt, d_poff, d_pi, d_pp;
@@ -982,7 +984,7 @@ int run_cells(Engine &E, int tier, const uint32_t *d_list, uint32_t nlist, float
 int enqueue_round0(Engine &E, uint32_t n, float *ms) {
     if (!E.binned) VD_TRY(enqueue_binning(E));
     VD_HIP(vd::launch_fill_u32(reinterpret_cast<uint32_t *>(E.d_top.p), 0u, 12, E.stream));
-    VD_TRY(run_cells(E, E.first_tier, E.d_items.as<uint32_t>(), n, ms));
+    VD_TRY(run_cells(E, E.first_tier, E.d_items.as<uint32_t>(), n, ms, E.first_probe != 0));
     VD_HIP(vd::launch_fill_u32(E.d_counts4.as<uint32_t>(), 0u, 4 * kTiers, E.stream));
     VD_HIP(vd::launch_collect(E.d_status.as<int32_t>(), E.d_items.as<uint32_t>(), n, E.d_out0.as<uint32_t>(),
                               E.d_out1.as<uint32_t>(), E.d_counts4.as<uint32_t>() + 4 * E.first_tier, E.stream));
@@ -1099,7 +1101,8 @@ std::vector<uint64_t> round0_key(const Engine &E, uint32_t n) {
     std::vector<uint64_t> k = {n, E.nbins, E.nraw, (uint64_t)E.binned, (uint64_t)E.keep_cells, E.cap[0], E.cap[1],
                                E.cap[2], (uint64_t)E.occ[0], (uint64_t)E.cus, (uint64_t)E.reserve_cus,
                                (uint64_t)E.mem.size(), E.ntube, (uint64_t)E.seal, E.nhost, E.seal_slots,
-                               (uint64_t)E.t0_waves, E.qgrain, (uint64_t)E.first_tier};
+                               (uint64_t)E.t0_waves, E.qgrain, (uint64_t)E.first_tier,
+                               (uint64_t)E.first_probe};
     const DevBuf *bufs[] = {&E.d_gen, &E.d_gen_tess, &E.d_gen_bin, &E.d_counts, &E.d_bin_start, &E.d_fill,
                             &E.d_items, &E.d_spos, &E.d_scan, &E.d_desc, &E.d_vol, &E.d_cen, &E.d_nface,
                             &E.d_nvert, &E.d_status, &E.d_cell_off, &E.d_vtx, &E.d_fcnt, &E.d_fnb, &E.d_fidx,
@@ -1764,8 +1767,10 @@ struct vd_server {
     uint64_t steps = 0, cells = 0;
 };
 
-// Launch shape of the cell kernels (VD_OPT_T0_WAVES, VD_OPT_QUEUE_GRAIN, VD_OPT_FIRST_TIER).
-enum LaunchKnob : int { kKnobT0Waves = 0, kKnobGrain = 1, kKnobFirstTier = 2 };
+// Launch shape of the cell kernels (VD_OPT_T0_WAVES, VD_OPT_QUEUE_GRAIN, VD_OPT_FIRST_TIER,
+// VD_OPT_FIRST_PROBE).
+enum LaunchKnob : int { kKnobT0Waves = 0, kKnobGrain = 1, kKnobFirstTier = 2, kKnobFirstProbe = 3 };
+constexpr int kLaunchKnobs = 4;
 
 int engine_set_launch(Engine &E, int knob, int64_t value) {
     if (knob == kKnobT0Waves) {
@@ -1782,26 +1787,34 @@ int engine_set_launch(Engine &E, int knob, int64_t value) {
         E.qgrain = (uint32_t)value;
         return VD_OK;
     }
-    if (value < 0 || value > 1) return fail(VD_EINVAL, "first tier must be 0 or 1");
-    E.first_tier = (int)value;
+    if (knob == kKnobFirstTier) {
+        if (value < 0 || value > 1) return fail(VD_EINVAL, "first tier must be 0 or 1");
+        E.first_tier = (int)value;
+        return VD_OK;
+    }
+    if (value < 0 || value > 1) return fail(VD_EINVAL, "first-launch skip-ahead must be 0 or 1");
+    E.first_probe = (int)value;
     return VD_OK;
 }
 
 int64_t engine_launch_value(const Engine &E, int knob) {
-    return knob == kKnobT0Waves ? E.t0_waves : knob == kKnobGrain ? (int64_t)E.qgrain : E.first_tier;
+    return knob == kKnobT0Waves ? E.t0_waves : knob == kKnobGrain ? (int64_t)E.qgrain
+         : knob == kKnobFirstTier ? E.first_tier : E.first_probe;
 }
 
-// Coordinate descent over the launch shape: from the current one, each knob in turn takes the
-// candidate under which run() (one calculate, returning its time in microseconds) was fastest,
-// the other knobs held. Every shape gives the same results, so whichever calculate ran last left
-// valid ones. picks: tier-0 waves per SIMD, queue grain, first tier.
+// Coordinate descent over the launch shape: from the current one, each knob in turn (skip-ahead,
+// waves, grain, first tier) takes the candidate under which run() (one calculate, returning its
+// time in microseconds) was fastest, the other knobs held. Every shape gives the same results, so
+// whichever calculate ran last left valid ones. picks: tier-0 waves per SIMD, queue grain, first
+// tier, first-launch skip-ahead.
 template <class Run>
-int engine_tune_launch(Engine &E, Run run, int picks[3]) {
-    static const std::vector<int64_t> cand[3] = {{5, 6, 7}, {1, 2, 4, 8}, {0, 1}};
+int engine_tune_launch(Engine &E, Run run, int picks[kLaunchKnobs]) {
+    static const std::vector<int64_t> cand[kLaunchKnobs] = {{5, 6, 7}, {1, 2, 4, 8}, {0, 1}, {0, 1}};
+    static const int order[kLaunchKnobs] = {kKnobFirstProbe, kKnobT0Waves, kKnobGrain, kKnobFirstTier};
     double cur = 0.0;
     VD_TRY(run(&cur));   // warm: shell tables, buffers
     VD_TRY(run(&cur));
-    for (int k = 0; k < 3; ++k) {
+    for (int k : order) {
         const int64_t keep = engine_launch_value(E, k);
         int64_t bv = keep;
         double bt = cur;
@@ -1816,7 +1829,7 @@ int engine_tune_launch(Engine &E, Run run, int picks[3]) {
         cur = bt;
     }
     if (picks)
-        for (int k = 0; k < 3; ++k) picks[k] = (int)engine_launch_value(E, k);
+        for (int k = 0; k < kLaunchKnobs; ++k) picks[k] = (int)engine_launch_value(E, k);
     return VD_OK;
 }
 
@@ -2108,7 +2121,7 @@ int vd_tess_calculate(vd_tess *t) {
     return engine_calculate(t->E);
 }
 
-int vd_tess_autotune_launch(vd_tess *t, int picks[3]) {
+int vd_tess_autotune_launch(vd_tess *t, int picks[4]) {
     if (!t) return fail(VD_EINVAL, "NULL handle");
     if (!t->E.prepared) return fail(VD_ESTATE, "generators are not set");
     return engine_tune_launch(t->E, [&](double *us) {
@@ -2218,6 +2231,7 @@ int vd_tess_set_option(vd_tess *t, int key, int64_t value) {
     if (key == VD_OPT_T0_WAVES) return engine_set_launch(t->E, kKnobT0Waves, value);
     if (key == VD_OPT_QUEUE_GRAIN) return engine_set_launch(t->E, kKnobGrain, value);
     if (key == VD_OPT_FIRST_TIER) return engine_set_launch(t->E, kKnobFirstTier, value);
+    if (key == VD_OPT_FIRST_PROBE) return engine_set_launch(t->E, kKnobFirstProbe, value);
     return fail(VD_EINVAL, "unknown option");
 }
 
@@ -2407,6 +2421,7 @@ int vd_batch_set_option(vd_batch *b, int key, int64_t value) {
     if (key == VD_BOPT_T0_WAVES) return engine_set_launch(b->E, kKnobT0Waves, value);
     if (key == VD_BOPT_QUEUE_GRAIN) return engine_set_launch(b->E, kKnobGrain, value);
     if (key == VD_BOPT_FIRST_TIER) return engine_set_launch(b->E, kKnobFirstTier, value);
+    if (key == VD_BOPT_FIRST_PROBE) return engine_set_launch(b->E, kKnobFirstProbe, value);
     return fail(VD_EINVAL, "unknown option");
 }
 
@@ -2441,7 +2456,7 @@ int vd_batch_autotune(vd_batch *b, const double *per_bin, int ncand, double *bes
     return VD_OK;
 }
 
-int vd_batch_autotune_launch(vd_batch *b, int picks[3]) {
+int vd_batch_autotune_launch(vd_batch *b, int picks[4]) {
     if (!b) return fail(VD_EINVAL, "NULL handle");
     if (b->E.mem.empty()) return fail(VD_ESTATE, "the batch has no members");
     return engine_tune_launch(b->E, [&](double *us) {

@@ -227,7 +227,7 @@ struct ClipScratch<C, false> {
     uint16_t lid[C::SM];
 };
 
-// One gather's window staging: written, then read back into registers before any clip.
+// One gather's window staging: written, then read back into registers.
 struct WinStage {
     uint32_t q[64];     // bin slot of window lane w's candidate
     uint16_t ent[64];   // batch lane of its shell entry
@@ -241,12 +241,14 @@ struct Mem {
     int32_t fnb[C::NB][C::FM];
     uint16_t sface[C::NB][C::SM];
     uint16_t fstart[C::NB][C::FM + 1];
-    // Bytes of three phases that never overlap: one clip's scratch, one window gather's staging,
-    // and at the end of a cell the fold terms of one 64-slot chunk. Keeping them in one place is
-    // what holds tier 0's LDS per wave under 5 KiB (more resident waves per SIMD).
+    // (Sharing the union below too would save 512 bytes, but measured slower on walks through
+    // sparse regions, where gathers dominate.)
+    WinStage win;
+    // Bytes of two phases that never overlap: one clip's scratch and, at the end of a cell, the
+    // fold terms of one 64-slot chunk. This keeps tier 0's LDS per wave at 5.5 KiB (up to 7
+    // resident waves per SIMD).
     union {
         ClipScratch<C> sc;
-        WinStage win;
         double tri[4][65];   // +1 keeps the four rows in different LDS banks
     };
 };
@@ -1628,10 +1630,9 @@ size_t tier_global_bytes() { return sizeof(Mem<TierG>); }
 template <bool SEAL, bool PROBE>
 void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st, int t0_waves) {
     if (tier == 0) {
-        // the skip-ahead reruns never start in tier 0 (they rerun from tier 1): one variant
-        if (PROBE || t0_waves == kTier0WavesDefault) k_cells<Tier0, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
-        else if (t0_waves == 5) k_cells<Tier0W<5>, SEAL, false><<<grid_waves, 64, 0, st>>>(a);
-        else k_cells<Tier0W<7>, SEAL, false><<<grid_waves, 64, 0, st>>>(a);
+        if (t0_waves == 5) k_cells<Tier0W<5>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+        else if (t0_waves == 7) k_cells<Tier0W<7>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+        else k_cells<Tier0, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     }
     else if (tier == 1) k_cells<Tier1, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 2) k_cells<Tier2, SEAL, true><<<grid_waves, 64, 0, st>>>(a);

@@ -580,20 +580,20 @@ class Tessellation:
         check(lib().vd_tess_set_option(self._h, _lib.VD_OPT_ATTEMPTS, int(attempts)))
 
     def set_launch(self, t0_waves: int | None = None, queue_grain: int | None = None,
-                   first_tier: int | None = None) -> None:
+                   first_tier: int | None = None, first_probe: int | None = None) -> None:
         """Launch shape of the cell kernels (VD_OPT_T0_WAVES 5/6/7, VD_OPT_QUEUE_GRAIN 1-64,
-        VD_OPT_FIRST_TIER 0/1). Results are identical under every shape."""
+        VD_OPT_FIRST_TIER 0/1, VD_OPT_FIRST_PROBE 0/1). Results are identical under every shape."""
         for key, v in ((_lib.VD_OPT_T0_WAVES, t0_waves), (_lib.VD_OPT_QUEUE_GRAIN, queue_grain),
-                       (_lib.VD_OPT_FIRST_TIER, first_tier)):
+                       (_lib.VD_OPT_FIRST_TIER, first_tier), (_lib.VD_OPT_FIRST_PROBE, first_probe)):
             if v is not None:
                 check(lib().vd_tess_set_option(self._h, key, int(v)))
 
     def autotune_launch(self) -> dict:
         """Times calculate() over the launch shapes and keeps the fastest (vd_tess_autotune_launch);
         returns the picks. Generators must be set."""
-        picks = (ctypes.c_int * 3)()
+        picks = (ctypes.c_int * 4)()
         check(lib().vd_tess_autotune_launch(self._h, picks))
-        return {"t0_waves": picks[0], "queue_grain": picks[1], "first_tier": picks[2]}
+        return {"t0_waves": picks[0], "queue_grain": picks[1], "first_tier": picks[2], "first_probe": picks[3]}
 
     # -- queries -------------------------------------------------------------------------------
     def count_generators(self) -> int:
@@ -785,18 +785,19 @@ class Batch:
         return best.value
 
     def set_launch(self, t0_waves: int | None = None, queue_grain: int | None = None,
-                   first_tier: int | None = None) -> None:
-        """As Tessellation.set_launch (VD_BOPT_T0_WAVES, VD_BOPT_QUEUE_GRAIN, VD_BOPT_FIRST_TIER)."""
+                   first_tier: int | None = None, first_probe: int | None = None) -> None:
+        """As Tessellation.set_launch (VD_BOPT_T0_WAVES, VD_BOPT_QUEUE_GRAIN, VD_BOPT_FIRST_TIER,
+        VD_BOPT_FIRST_PROBE)."""
         for key, v in ((_lib.VD_BOPT_T0_WAVES, t0_waves), (_lib.VD_BOPT_QUEUE_GRAIN, queue_grain),
-                       (_lib.VD_BOPT_FIRST_TIER, first_tier)):
+                       (_lib.VD_BOPT_FIRST_TIER, first_tier), (_lib.VD_BOPT_FIRST_PROBE, first_probe)):
             if v is not None:
                 check(lib().vd_batch_set_option(self._h, key, int(v)))
 
     def autotune_launch(self) -> dict:
         """vd_batch_autotune_launch: the launch shape for the batch's current grids."""
-        picks = (ctypes.c_int * 3)()
+        picks = (ctypes.c_int * 4)()
         check(lib().vd_batch_autotune_launch(self._h, picks))
-        return {"t0_waves": picks[0], "queue_grain": picks[1], "first_tier": picks[2]}
+        return {"t0_waves": picks[0], "queue_grain": picks[1], "first_tier": picks[2], "first_probe": picks[3]}
 
     def __len__(self):
         return int(lib().vd_batch_count(self._h))
