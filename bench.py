@@ -53,6 +53,9 @@ def parse():
     p.add_argument("--serve-sweep-requests", type=int, default=30)
     p.add_argument("--serve-sweep-rates", type=str, default="5,50,500")
     p.add_argument("--no-clustered", action="store_true", help="skip the 4M clustered workload")
+    p.add_argument("--only", choices=["batched", "clustered", "serve"], default=None,
+                   help="run one secondary workload alone and print its JSON object (for profiling)")
+    p.add_argument("--no-plain", action="store_true", help="--only batched: skip the plain comparison legs")
     p.add_argument("--no-launch-tune", action="store_true",
                    help="keep the default launch shape instead of vd_*_autotune_launch's pick")
     p.add_argument("--clustered-n", type=int, default=4_000_000)
@@ -248,7 +251,7 @@ def clustered(n: int, targets, steps: int, launch_tune: bool = True) -> dict:
 BATCH_TUNE = (0.25, 0.35, 0.5, 1.0, 2.0)
 
 
-def batched_vs_plain(count: int, steps: int, tune=None, launch_tune: bool = True) -> dict:
+def batched_vs_plain(count: int, steps: int, tune=None, launch_tune: bool = True, plain_legs: bool = True) -> dict:
     """Mixed-size workload: ``count`` tessellations with log-uniform generator counts in
     [64, 65536], boxes with edges U[0.5, 2], 30% sphere walls, 20% plane walls. Grids: about 2
     generators per bin (``tune`` None), or the batch autotuner's pick among the generators-per-bin
@@ -324,10 +327,12 @@ def batched_vs_plain(count: int, steps: int, tune=None, launch_tune: bool = True
 
     # each tessellation of the workload is computed once, so the plain path launches eagerly; the
     # rerun of the same tessellations every step would let them replay graphs, reported apart
-    progress(f"batched: {count} tessellations in one batch done; plain, one engine each")
-    tp, cells_p = plain(False)
-    progress("batched: plain done; plain with graph replay")
-    tpg, _ = plain(True)
+    tp = tpg = cells_p = None
+    if plain_legs:
+        progress(f"batched: {count} tessellations in one batch done; plain, one engine each")
+        tp, cells_p = plain(False)
+        progress("batched: plain done; plain with graph replay")
+        tpg, _ = plain(True)
     dev.close()
     bins = sum(int(np.prod(g)) for g in grids)
     grid_note = (f"autotuned grids (vd_batch_autotune over {list(tune)} generators per bin; picked {best:g})"
@@ -338,10 +343,10 @@ def batched_vs_plain(count: int, steps: int, tune=None, launch_tune: bool = True
             "generators": int(offs[-1]), "cells": cells, "cells_plain": cells_p,
             "batched_cells_per_s": cells / tb, "batched_ms": tb * 1e3, "batched_first_round_ms": kern_b * 1e3,
             "batched_graph_replays": replays_b,
-            "plain_cells_per_s": cells_p / tp, "plain_ms": tp * 1e3,
-            "speedup_batched_vs_plain": tp / tb,
-            "plain_graph_replay_cells_per_s": cells_p / tpg,
-            "speedup_batched_vs_plain_graph_replay": tpg / tb, "steps": steps}
+            "plain_cells_per_s": cells_p / tp if tp else None, "plain_ms": tp * 1e3 if tp else None,
+            "speedup_batched_vs_plain": tp / tb if tp else None,
+            "plain_graph_replay_cells_per_s": cells_p / tpg if tpg else None,
+            "speedup_batched_vs_plain_graph_replay": tpg / tb if tpg else None, "steps": steps}
 
 
 def copy_probe(mib: int) -> None:
@@ -384,6 +389,18 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}), flush=True)
         sys.exit(2)
+    if a.only == "batched":
+        print(json.dumps(batched_vs_plain(a.batch_count, a.batch_steps, None, not a.no_launch_tune, not a.no_plain)),
+              flush=True)
+        return
+    if a.only == "clustered":
+        print(json.dumps(clustered(a.clustered_n, [0.25, 0.35, 0.7], 3, not a.no_launch_tune)), flush=True)
+        return
+    if a.only == "serve":
+        rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
+        print(json.dumps(served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates)),
+              flush=True)
+        return
 
     from vorothree_amd import Algorithm3DGrid, BoundingBox, SphereGeometry, Tessellation, Wall
     from vorothree_amd import _hip

@@ -562,7 +562,7 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
             atomicMax(&m.sc.lmap[(int)(sg & 0xffffu) - nk], ((uint32_t)(t + 1) << 16) | (sg >> 16));
         }
         poly_sync<C>();
-        const uint32_t s0 = m.sc.seg[0];
+        const uint32_t s0 = first_lane_u32(m.sc.seg[0]);   // wave-uniform: the walk is a scalar loop
         const int start = (int)(s0 & 0xffffu);
         int cv = (int)(s0 >> 16);
         if (l == 0) m.sc.lid[0] = (uint16_t)start;
@@ -849,7 +849,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         // the walk runs over lane indices (vertex - nk): lane j holds the lane of new vertex
         // nk + j's successor, or 0xffff where the map has none
         const uint32_t nxl = mv ? (mv & 0xffffu) - (uint32_t)nk : 0xffffu;
-        const uint32_t s0 = m.sc.seg[0];
+        const uint32_t s0 = first_lane_u32(m.sc.seg[0]);   // wave-uniform: the walk is a scalar loop
         const int start = (int)(s0 & 0xffffu) - nk;
         int cv = (int)(s0 >> 16) - nk;
         uint32_t lidv = (uint32_t)start;
