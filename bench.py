@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--serve-sweep-requests", type=int, default=30)
     p.add_argument("--serve-sweep-rates", type=str, default="5,50,500")
     p.add_argument("--no-clustered", action="store_true", help="skip the 4M clustered workload")
-    p.add_argument("--only", choices=["batched", "clustered", "serve"], default=None,
+    p.add_argument("--only", choices=["batched", "clustered", "serve", "cpu"], default=None,
                    help="run one secondary workload alone and print its JSON object (for profiling)")
     p.add_argument("--no-plain", action="store_true", help="--only batched: skip the plain comparison legs")
     p.add_argument("--no-launch-tune", action="store_true",
@@ -349,6 +349,46 @@ def batched_vs_plain(count: int, steps: int, tune=None, launch_tune: bool = True
             "speedup_batched_vs_plain_graph_replay": tpg / tb if tpg else None, "steps": steps}
 
 
+def cpu_table(threads: int) -> dict:
+    """The CPU restatement of calculate() (oracle/, C++) on this host for the single-tessellation
+    workloads: 1k in [0,10]^3 (grid 10^3), 1M uniform with the sphere wall (grid 100^3) and 4M
+    clustered (grid 474^3, the density rule at 0.35 generators per bin). set_generators (wall filter
+    and binning) runs on one thread as in the reference; calculate on ``threads`` threads and, for a
+    bounded sample of every k-th cell, on one. Median of 3 calculates (1 for the 4M case)."""
+    from oracle import OracleTess
+    from vorothree_amd import workloads as W
+    rows = []
+    cases = [(W.config1(), 3), (W.config2(), 3), (W.config3(grid=(474, 474, 474)), 1)]
+    for w, reps in cases:
+        o = OracleTess(w.lo, w.hi, *w.grid)
+        for wl in w.walls:
+            o.add_wall_sphere(wl[1], wl[2], wl[3])
+        c0 = time.perf_counter()
+        o.set_generators(w.points)
+        bdt = time.perf_counter() - c0
+        ts = []
+        for _ in range(reps):
+            c0 = time.perf_counter()
+            o.calculate(threads=threads)
+            ts.append(time.perf_counter() - c0)
+        cdt = float(np.median(ts))
+        n = o.count_cells()
+        _, _, _, work = o.reduced()
+        step = max(1, n // 50_000)
+        sub = np.arange(0, n, step, dtype=np.uint32)
+        c0 = time.perf_counter()
+        o.calculate(subset=sub, threads=1)
+        sdt = time.perf_counter() - c0
+        rows.append({"workload": w.name, "grid": list(w.grid), "generators": int(len(w.points)), "cells": int(n),
+                     "threads": threads, "set_generators_ms": bdt * 1e3, "calculate_ms": cdt * 1e3,
+                     "cells_per_s": n / cdt, "single_thread_cells_per_s": len(sub) / sdt,
+                     "single_thread_sample": f"every {step}th cell ({len(sub)} cells)",
+                     "flops_per_cell": float(work[:, 0].mean())})
+        progress(f"cpu: {w.name} {n / cdt:.4g} cells/s on {threads} threads")
+        del o
+    return {"host_cpu": host_cpu(), "kind": "port (oracle/, the C++ restatement of calculate())", "rows": rows}
+
+
 def copy_probe(mib: int) -> None:
     """One hipMemcpy device-to-device of a known size, so FETCH_SIZE / WRITE_SIZE counters read
     under rocprofv3 can be scaled to bytes."""
@@ -389,6 +429,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}), flush=True)
         sys.exit(2)
+    if a.only == "cpu":
+        print(json.dumps(cpu_table(a.cpu_threads)), flush=True)
+        return
     if a.only == "batched":
         print(json.dumps(batched_vs_plain(a.batch_count, a.batch_steps, None, not a.no_launch_tune, not a.no_plain)),
               flush=True)

@@ -654,6 +654,8 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
 #pragma unroll
     for (int k = 0; k < NV; ++k) inm.w[k] = ballot(d[k] <= kTol) & vam.w[k];
     const int nk = inm.count();
+    // one vertex per lane: lane v's new index if it is kept (read by lane shuffle in step 5)
+    const uint32_t kidx = NV == 1 ? (uint32_t)below_lane(inm.w[0]) : 0u;
     double md2 = 0.0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -716,10 +718,15 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     const bool sv = fa && tot >= 3;
     const uint64_t svm = ballot(tot >= 3);   // tot = 0 past the faces
     int Fn = popc(svm);
-    const uint32_t tv = sv ? (uint32_t)tot : 0u;
-    const uint32_t inc = wave_incl_scan(tv);
-    const int off = (int)(inc - tv);
-    int Spre = (int)rlu(inc, 63);
+    // a face's output offset is the slot prefix of its first slot unless a face before it drops
+    // 1 or 2 entries (a face left with fewer than 3): a scan only then
+    int off = pfs, Spre = ptot;
+    if (ballot(tot > 0) & ~svm) {
+        const uint32_t tv = sv ? (uint32_t)tot : 0u;
+        const uint32_t inc = wave_incl_scan(tv);
+        off = (int)(inc - tv);
+        Spre = (int)rlu(inc, 63);
+    }
     if (ballot(tot > 255)) return kFace255;
     if (Fn + 1 > C::FM || Spre > C::SM) return kOverflow;
     if (fa) {
@@ -745,15 +752,16 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     const uint32_t key = ka ? key0 : 0xffffffffu;
     uint64_t rest = ballot(ka), fm = 0;
     int nint = 0, xid = 0;
-    while (rest && nint < 64) {     // every pass clears lane f: at most K <= 64 passes
+    while (rest) {     // every pass clears lane f (its own key matches): at most K <= 64 passes
         const int f = ffs64(rest);
         const bool mine = key == rlu(key, f);
-        const uint64_t same = ballot(mine);   // holds lane f itself
+        const uint64_t same = ballot(mine);
         rest &= ~same;
         fm |= 1ull << f;
-        if (mine) xid = nk + nint;
+        if (mine) xid = nint;
         ++nint;
     }
+    xid += nk;   // read from crossing lanes only
     const bool isf = (fm >> l) & 1ull;
     const int myx = xid;
     if (nk + nint > C::VM) return kOverflow;
@@ -797,13 +805,14 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             const uint32_t b = (ea[ch] >> SH) & VMSK;
             const bool ain = (ea[ch] >> 30) & 1u, bin = ea[ch] >> 31;
             const int xv = (int)lane_get((uint32_t)xid, (int)(eq[ch] >> 16) & 63);
+            const uint32_t kb = NV == 1 ? lane_get(kidx, (int)b) : 0u;   // full wave: b's new index
             if (s < P.S && (ain || bin)) {
                 const int f = (int)((ea[ch] >> (2 * SH)) & ((1u << (30 - 2 * SH)) - 1u));
                 const int nf = m.sc.fnew[f];
                 if (nf >= 0) {
                     const int pos = m.sc.fbase[f] + (int)(eq[ch] & 0xffffu);
                     const int ns = m.sc.fnoff[f], ne = m.sc.nend[f];
-                    const uint16_t rb = (uint16_t)inm.below(b);
+                    const uint16_t rb = (uint16_t)(NV == 1 ? kb : (uint32_t)inm.below(b));
                     m.sc.fidxn[pos] = (ain && bin) ? rb : (uint16_t)xv;
                     m.sface[o][pos] = (uint16_t)nf;
                     m.sc.nextp[pos] = (uint16_t)(pos + 1 == ne ? ns : pos + 1);
