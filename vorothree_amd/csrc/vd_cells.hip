@@ -98,6 +98,42 @@ __device__ inline double rld(double v, int l) {
 __device__ inline uint32_t first_lane_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+// Wave-uniform values moved to scalar registers. Loads through the launch's descriptors land in
+// vector registers (the compiler cannot prove that the persistent loop's stores leave those
+// arrays alone), and the walk would hold them in vector registers across every clip.
+__device__ inline double uni_d(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint64_t lo = first_lane_u32((uint32_t)b), hi = first_lane_u32((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)((hi << 32) | lo));
+}
+template <class T>
+__device__ inline T *uni_ptr(T *p) {
+    const uint64_t b = (uint64_t)p;
+    const uint64_t lo = first_lane_u32((uint32_t)b), hi = first_lane_u32((uint32_t)(b >> 32));
+    return reinterpret_cast<T *>((hi << 32) | lo);
+}
+__device__ inline GridGeom uni_grid(const GridGeom &g) {
+    GridGeom u;
+    for (int a = 0; a < 3; ++a) {
+        u.lo[a] = uni_d(g.lo[a]);
+        u.s[a] = uni_d(g.s[a]);
+        u.cs[a] = uni_d(g.cs[a]);
+        u.lim[a] = uni_d(g.lim[a]);
+        u.n[a] = (int32_t)first_lane_u32((uint32_t)g.n[a]);
+    }
+    return u;
+}
+__device__ inline ShellPack uni_pack(const ShellPack &p) {
+    static_assert(sizeof(ShellPack) == 8, "two words");
+    uint32_t w[2];
+    __builtin_memcpy(w, &p, 8);
+    w[0] = first_lane_u32(w[0]);
+    w[1] = first_lane_u32(w[1]);
+    ShellPack u;
+    __builtin_memcpy(&u, w, 8);
+    return u;
+}
+
 __device__ inline uint64_t bcast_u64(uint64_t v) {  // lane 0's value to every lane
     const uint32_t lo = first_lane_u32((uint32_t)v), hi = first_lane_u32((uint32_t)(v >> 32));
     return ((uint64_t)hi << 32) | lo;
@@ -1076,18 +1112,19 @@ template <class C, bool SEAL, bool PROBE>
 __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t q, ArenaChunks &ac) {
     const int l = lane_id_here();
     const TessDesc &D = A.desc[A.gen_tess[i]];   // this cell's tessellation (wave-uniform)
-    const double gx = A.gen[3 * (size_t)i], gy = A.gen[3 * (size_t)i + 1], gz = A.gen[3 * (size_t)i + 2];
-    const uint32_t li = (uint32_t)(i - D.gen_off);   // index within its tessellation
-    const GridGeom &G = D.grid;
+    const double gx = uni_d(A.gen[3 * (size_t)i]), gy = uni_d(A.gen[3 * (size_t)i + 1]),
+                 gz = uni_d(A.gen[3 * (size_t)i + 2]);
+    const uint32_t li = first_lane_u32((uint32_t)(i - D.gen_off));   // index within its tessellation
+    const GridGeom G = uni_grid(D.grid);
     // a per-cell list (k_far_lists) replaces the member's prefix for cells that reach across
     // empty space: the same candidates in the same order. Only reruns of such cells have lists,
     // and they run in the skip-ahead instantiation.
     const bool own = PROBE && A.lshell && A.lshell_n[q] != kNoList;
-    const ShellCode *shell = own ? A.lshell + A.lshell_off[q] : D.shell;
-    const ShellPack pk = D.pack;
-    const uint32_t nshell = own ? A.lshell_n[q] : D.nshell;
-    const double shell_next = own ? A.lshell_next[q] : D.shell_next_d2;
-    const uint32_t *bin_start = A.bin_start + D.bin_off;
+    const ShellCode *shell = uni_ptr(own ? A.lshell + A.lshell_off[q] : D.shell);
+    const ShellPack pk = uni_pack(D.pack);
+    const uint32_t nshell = first_lane_u32(own ? A.lshell_n[q] : D.nshell);
+    const double shell_next = uni_d(own ? A.lshell_next[q] : D.shell_next_d2);
+    const uint32_t *bin_start = uni_ptr(A.bin_start + D.bin_off);
     const int nwalls = D.nwalls;
     Poly P;
     poly_box<C>(m, P, D.box_lo, D.box_hi);
@@ -1161,12 +1198,12 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
     const bool walk = P.V != 0 && status == kCellOk;
     if (walk) {
         r2 = poly_max_radius_sq<C>(m, P, gx, gy, gz);   // tessellation.rs:438
-        ix = (int)axis_bin(gx, G.lo[0], G.s[0], G.lim[0]);
-        iy = (int)axis_bin(gy, G.lo[1], G.s[1], G.lim[1]);
-        iz = (int)axis_bin(gz, G.lo[2], G.s[2], G.lim[2]);
-        rx = (gx - G.lo[0]) * G.s[0] - (double)ix;
-        ry = (gy - G.lo[1]) * G.s[1] - (double)iy;
-        rz = (gz - G.lo[2]) * G.s[2] - (double)iz;
+        ix = (int)first_lane_u32(axis_bin(gx, G.lo[0], G.s[0], G.lim[0]));
+        iy = (int)first_lane_u32(axis_bin(gy, G.lo[1], G.s[1], G.lim[1]));
+        iz = (int)first_lane_u32(axis_bin(gz, G.lo[2], G.s[2], G.lim[2]));
+        rx = uni_d((gx - G.lo[0]) * G.s[0] - (double)ix);
+        ry = uni_d((gy - G.lo[1]) * G.s[1] - (double)iy);
+        rz = uni_d((gz - G.lo[2]) * G.s[2] - (double)iz);
     }
     while (walk) {
         Plane pl;
