@@ -554,7 +554,7 @@ def main():
         roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                 "kernel": ("k_cells<Tier1" if launch and launch["first_tier"] == 1
-                           else f"k_cells<Tier0W<{launch['t0_waves'] if launch else 6}>")
+                           else f"k_cells<Tier0W<{launch['t0_waves'] if launch else 7}>")
                           + (", skip-ahead>" if launch and launch["first_probe"] else ">"),
                 "kernel_ms": kern_s * 1e3,
                 "flops_per_cell": flops_per_cell,
@@ -578,7 +578,7 @@ def main():
                                "grid 100^3, sphere wall c=(0.5,0.5,0.5) r=0.5",
                    "generators": a.n, "cells": cells, "grid": list(w.grid),
                    "keep_cells": not a.reduced, "path": "set_generators(device) + calculate()",
-                   "launch": launch or {"t0_waves": 6, "queue_grain": 1, "first_tier": 0, "first_probe": 0},
+                   "launch": launch or {"t0_waves": 7, "queue_grain": 1, "first_tier": 0, "first_probe": 0},
                    "launch_note": "tier-0 waves per SIMD, cells per queue atomic and first tier: "
                                   + ("vd_tess_autotune_launch's pick on this workload (coordinate "
                                      "descent over skip-ahead 0/1, waves 5/6/7, grain 1/2/4/8, first "

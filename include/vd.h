@@ -202,8 +202,8 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
                                calculate() gives up with VD_ECAPACITY and keeps no results
                                (default 8, at most 62) */
 /* Launch shape of the cell kernels; every choice gives identical results. */
-#define VD_OPT_T0_WAVES 6   /* resident waves per SIMD of the small-LDS tier's kernel: 5, 6
-                               (default) or 7, each a separate compiled instantiation */
+#define VD_OPT_T0_WAVES 6   /* resident waves per SIMD of the small-LDS tier's kernel: 5, 6 or
+                               7 (default), each a separate compiled instantiation */
 #define VD_OPT_QUEUE_GRAIN 7 /* cells a wave takes from the work queue per atomic, 1 (default) to 64 */
 #define VD_OPT_FIRST_TIER 8 /* tier of the first launch over every cell: 0 (default, 64 vertices)
                                or 1 (128 vertices); cells that outgrow it move up as usual */

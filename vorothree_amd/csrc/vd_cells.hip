@@ -297,6 +297,11 @@ struct Plane { double px, py, pz, nx, ny, nz; };
 // polytope, vertices [0, nk) are bit-exact copies of the kept vertices and [nk, V) are new.
 struct Poly { int cur, V, F, S, nk; };
 
+// The buffer holding the current polytope: always 0 in the single-buffer (fast) tiers, which lets
+// the compiler fold every LDS address of a 64-lane chunk into one lane base and immediate offsets.
+template <class C>
+__device__ inline int cur_buf(const Poly &P) { return C::NB == 1 ? 0 : P.cur; }
+
 // A wave's reserved arena ranges [at, end) for vertices, faces and face indices. Cells are carved
 // from them, so the shared arena counters see one atomic per chunk instead of three per cell
 // (with thousands of waves, same-address atomics per cell cost more than the cell's stores).
@@ -334,9 +339,9 @@ __device__ inline void vreg_load(Mem<C> &m, const Poly &P, VReg<C> &r) {
         for (int k = 0; k < C::NV; ++k) {
             const int v = 64 * k + l;
             if (v < P.V) {
-                r.x[k] = m.vx[P.cur][v];
-                r.y[k] = m.vy[P.cur][v];
-                r.z[k] = m.vz[P.cur][v];
+                r.x[k] = m.vx[0][v];   // single buffer (static_assert in clip_fast)
+                r.y[k] = m.vy[0][v];
+                r.z[k] = m.vz[0][v];
             }
         }
     }
@@ -657,7 +662,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     // kept ones are stored after it). A clip that fails part way leaves a mixed polytope, but every
     // failure moves the cell to the next tier, which starts it again.
     static_assert(C::NB == 1, "the fast path rewrites the polytope in place");
-    const int c = P.cur, o = c;
+    const int c = 0, o = 0;
     if (NV > 1 && P.F > 64) return kGeneric;
 
     // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362). Lanes past
@@ -950,7 +955,8 @@ template <class C>
 __device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double gy, double gz) {
     double md = 0.0;  // cell_3d_faces.rs:271-286
     for (int v = lane_id_here(); v < P.V; v += 64) {
-        const double dx = m.vx[P.cur][v] - gx, dy = m.vy[P.cur][v] - gy, dz = m.vz[P.cur][v] - gz;
+        const int c = cur_buf<C>(P);
+        const double dx = m.vx[c][v] - gx, dy = m.vy[c][v] - gy, dz = m.vz[c][v] - gz;
         md = gt_max(md, dx * dx + dy * dy + dz * dz);
     }
     return wave_max(md);
@@ -963,7 +969,7 @@ __device__ double poly_max_radius_sq(Mem<C> &m, const Poly &P, double gx, double
 template <class C>
 __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]) {
     const int l = lane_id_here();
-    const int c = P.cur;
+    const int c = cur_buf<C>(P);
     double acc = 0.0;
     for (int base = 0; base < P.S; base += 64) {
         const int p = base + l;
@@ -1023,7 +1029,8 @@ __device__ inline uint64_t pretest(Mem<C> &m, const Poly &P, bool act, double d0
     bool cut = false;
     if (act) {
         const double px = gx + d0 * 0.5, py = gy + d1 * 0.5, pz = gz + d2 * 0.5;
-        const double *vx = m.vx[P.cur], *vy = m.vy[P.cur], *vz = m.vz[P.cur];
+        const int c = cur_buf<C>(P);
+        const double *vx = m.vx[c], *vy = m.vy[c], *vz = m.vz[c];
         for (int v = v0; v < v1; ++v) {
             const double d = (vx[v] - px) * d0 + (vy[v] - py) * d1 + (vz[v] - pz) * d2;
             cut |= d > kTol;
@@ -1459,7 +1466,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
         if (ov + P.V > A.arena_cap[0] || of + P.F > A.arena_cap[1] || oi + P.S > A.arena_cap[2]) {
             if (lo == 0) A.status[i] = kCellArena;
         } else {
-            const int c = P.cur;
+            const int c = cur_buf<C>(P);
             for (int v = lo; v < P.V; v += 64) {
                 A.vtx[3 * (ov + v)] = m.vx[c][v];
                 A.vtx[3 * (ov + v) + 1] = m.vy[c][v];
@@ -1482,7 +1489,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
         if (of + P.F > A.arena_cap[1]) {
             if (lo == 0) A.status[i] = kCellArena;
         } else {
-            for (int f = lo; f < P.F; f += 64) A.fnb[of + f] = m.fnb[P.cur][f];
+            for (int f = lo; f < P.F; f += 64) A.fnb[of + f] = m.fnb[cur_buf<C>(P)][f];
             if (lo == 0) A.cell_off[3 * (size_t)i + 1] = of;
         }
     }
@@ -1528,7 +1535,7 @@ __global__ __launch_bounds__(64) void k_cells_global(CellArgs A) {
 // cells that outgrow the tier before, with 256 and 1024 vertices (the last one wave per CU).
 template <int W>
 using Tier0W = Cap<64, 192, 48, 64, true, W>;
-using Tier0 = Tier0W<kTier0WavesDefault>;
+using Tier0 = Tier0W<kTier0WavesDefault>;   // sizes (tier_max_vertices, tier_lds_bytes)
 using Tier1 = Cap<128, 384, 64, 64, true, 4>;
 using Tier2 = Cap<256, 768, 128, 384, false, 1>;
 using Tier3 = Cap<1024, 2560, 320, 1280, false, 1>;
@@ -1677,8 +1684,8 @@ template <bool SEAL, bool PROBE>
 void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st, int t0_waves) {
     if (tier == 0) {
         if (t0_waves == 5) k_cells<Tier0W<5>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
-        else if (t0_waves == 7) k_cells<Tier0W<7>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
-        else k_cells<Tier0, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+        else if (t0_waves == 6) k_cells<Tier0W<6>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+        else k_cells<Tier0W<7>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     }
     else if (tier == 1) k_cells<Tier1, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     else if (tier == 2) k_cells<Tier2, SEAL, true><<<grid_waves, 64, 0, st>>>(a);
@@ -1707,9 +1714,9 @@ hipError_t cells_occupancy(int tier, int *blocks_per_cu, int t0_waves) {
     if (tier == 0) {
         if (t0_waves == 5)
             return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<5>, false, false>, 64, 0);
-        if (t0_waves == 7)
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<7>, false, false>, 64, 0);
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0, false, false>, 64, 0);
+        if (t0_waves == 6)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<6>, false, false>, 64, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<7>, false, false>, 64, 0);
     }
     if (tier == 1)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier1, false, false>, 64, 0);
