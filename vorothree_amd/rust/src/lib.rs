@@ -299,6 +299,21 @@ impl Tessellation<3, Cell3DFaces, Algorithm3DGrid> {
         check(unsafe { sys::vd_tess_relax(self.raw) }, "relax");
     }
 
+    /// A library option (`sys::VD_OPT_*`; no counterpart in the reference). Options change what is
+    /// kept or how the kernels are launched, never the results.
+    pub fn set_option(&mut self, key: c_int, value: i64) {
+        check(unsafe { sys::vd_tess_set_option(self.raw, key, value) }, "set_option");
+    }
+
+    /// Times calculate() under the launch shapes and keeps the fastest (vd_tess_autotune_launch):
+    /// [tier-0 waves per SIMD, cells per queue atomic, first tier, first-launch skip-ahead].
+    /// Generators must be set; the results of the last calculate are kept.
+    pub fn autotune_launch(&mut self) -> [i32; 4] {
+        let mut picks = [0 as c_int; 4];
+        check(unsafe { sys::vd_tess_autotune_launch(self.raw, picks.as_mut_ptr()) }, "autotune_launch");
+        picks
+    }
+
     /// map (src/tessellation.rs:391-412): computes every cell and applies f to each. The cells are
     /// computed by one calculate on the device, whose results the handle keeps.
     pub fn map<F, T>(&self, f: F) -> Vec<T>

@@ -16,6 +16,10 @@ pub const VD_OPT_GRAPHS: c_int = 2;
 pub const VD_OPT_ARENA: c_int = 3;
 pub const VD_OPT_FAR_LIST: c_int = 4;
 pub const VD_OPT_ATTEMPTS: c_int = 5;
+pub const VD_OPT_T0_WAVES: c_int = 6;
+pub const VD_OPT_QUEUE_GRAIN: c_int = 7;
+pub const VD_OPT_FIRST_TIER: c_int = 8;
+pub const VD_OPT_FIRST_PROBE: c_int = 9;
 
 pub const VD_WALL_PLANE: i32 = 0;
 pub const VD_WALL_SPHERE: i32 = 1;
@@ -85,6 +89,7 @@ unsafe extern "C" {
     pub fn vd_tess_get_cell(t: *const vd_tess, i: usize, sizes: *mut u64, vertices: *mut f64, face_counts: *mut u32,
                             face_indices: *mut u32, face_neighbors: *mut i32) -> c_int;
     pub fn vd_tess_set_option(t: *mut vd_tess, key: c_int, value: i64) -> c_int;
+    pub fn vd_tess_autotune_launch(t: *mut vd_tess, picks: *mut c_int) -> c_int;
     pub fn vd_tess_get_stats(t: *const vd_tess, out: *mut f64) -> c_int;
     pub fn vd_last_error() -> *const c_char;
 }
