@@ -7,10 +7,12 @@ calculate() (every cell, full cells kept, as the reference's calculate stores th
 are uploaded once before the timed region.
 
 Prints one JSON line. ``roofline`` is the FP64 roofline of the cell kernel (the path has no dense
-contraction, so it is bound by FP64 vector work and latency, not by HBM or matrix cores): the
-algorithmic flops of the launch, counted per cell by the oracle with the same rules, divided by
-the kernel time measured with HIP events on the library's stream. ``cpu_baseline`` times the CPU
-oracle (a C++ restatement of the reference's calculate) on the same workload on the host cores.
+contraction, so neither HBM nor matrix cores bound it): the algorithmic flops of the launch,
+counted per cell by the oracle with the same rules, divided by the kernel time measured with HIP
+events on the library's stream. Its ``issue`` object gives what does bound the kernel, the issue
+rate of vector instructions against one wave64 instruction per SIMD every 4 cycles, from the
+per-cell instruction counts of the committed profile. ``cpu_baseline`` times the CPU oracle (a C++
+restatement of the reference's calculate) on the same workload on the host cores.
 """
 from __future__ import annotations
 
@@ -543,11 +545,15 @@ def main():
     vol = t.volumes()
     bytes_launch = cells * (24 + 4 + 4 + 8 + 24 + 4 + 4 + 4 + 24) + nv * 24 + ns * 2 + nf * 5
     traffic = None
+    issue_counts = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                          "k_cells_uniform1M_sphere_traffic.json")
     if os.path.exists(tfile) and a.n == 1_000_000 and a.grid == 100 and not a.reduced:
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        if prof.get("valu_insts_per_cell"):
+            issue_counts = (prof["valu_insts_per_cell"], prof["salu_insts_per_cell"])
     roof = None
     if flops_per_cell is not None:
         achieved = cells * flops_per_cell / kern_s / 1e12
@@ -561,6 +567,16 @@ def main():
                 "hbm": {"achieved": bytes_launch / kern_s / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": bytes_launch / kern_s / 1e9 / HBM_PEAK_GBS,
                         "bytes_per_launch": bytes_launch}}
+        if issue_counts:
+            # what binds the kernel: vector-instruction issue. Peak: one wave64 VALU instruction
+            # per SIMD every 4 cycles (1024 SIMDs at 2.4 GHz); the instructions per cell are the
+            # SQ_INSTS_VALU / SQ_INSTS_SALU counts of the committed profile of this kernel
+            valu_peak = 1024 * 2.4e9 / 4 / 1e9
+            v_rate = issue_counts[0] * cells / kern_s / 1e9
+            roof["issue"] = {"achieved": v_rate, "peak": valu_peak, "unit": "G VALU wave-instructions/s",
+                             "frac": v_rate / valu_peak, "valu_per_cell": issue_counts[0],
+                             "salu_per_cell": issue_counts[1],
+                             "salu_frac": issue_counts[1] * cells / kern_s / 1e9 / valu_peak}
     out = {
         "metric": METRIC,
         "value": value,
