@@ -57,7 +57,8 @@ def parse():
     p.add_argument("--no-clustered", action="store_true", help="skip the 4M clustered workload")
     p.add_argument("--only", choices=["batched", "clustered", "serve", "cpu"], default=None,
                    help="run one secondary workload alone and print its JSON object (for profiling)")
-    p.add_argument("--no-plain", action="store_true", help="--only batched: skip the plain comparison legs")
+    p.add_argument("--no-plain", action="store_true",
+                   help="--only batched / --only serve: skip the plain comparison legs")
     p.add_argument("--no-launch-tune", action="store_true",
                    help="keep the default launch shape instead of vd_*_autotune_launch's pick")
     p.add_argument("--clustered-n", type=int, default=4_000_000)
@@ -148,7 +149,7 @@ def serve_plain(reqs, grids) -> dict:
     return {"cells": int(cells), "wall_s": dt, "cells_per_s": cells / dt}
 
 
-def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_rates) -> dict:
+def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_rates, with_plain: bool = True) -> dict:
     """Served workload: the first ``count`` requests of a Poisson stream (rate req/s) of
     tessellations, 1e3..2e6 generators, 50% clustered, 30% Lloyd jobs of 10-50 iterations, jobs
     without a grid (the server picks it with vd_grid_for_points). Then a rate sweep over the first
@@ -160,8 +161,11 @@ def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_ra
     reqs = W.config5(count=max(count, sweep_count), rate=rate)
     print(f"serve: {len(reqs)} requests generated", file=sys.stderr, flush=True)
     main_run = serve_run(reqs[:count], rate, budget_gb)
-    progress("serve: the same requests through the plain path, one tessellation per job")
-    plain = serve_plain(reqs[:count], main_run.pop("_grids"))
+    grids = main_run.pop("_grids")
+    plain = None
+    if with_plain:
+        progress("serve: the same requests through the plain path, one tessellation per job")
+        plain = serve_plain(reqs[:count], grids)
     sweep = []
     for r in sweep_rates:
         scaled = [dataclasses.replace(q, arrival=q.arrival * rate / r) for q in reqs[:sweep_count]]
@@ -178,8 +182,9 @@ def served(count: int, rate: float, budget_gb: float, sweep_count: int, sweep_ra
            "note": "the offered load is far above one GPU's capacity at 2000 req/s, so its latency "
                    "is mostly queueing; the sweep lowers the rate on the first requests"}
     out.update(main_run)
-    out["plain"] = plain
-    out["speedup_served_vs_plain"] = main_run["cells_per_s"] / plain["cells_per_s"]
+    if plain is not None:
+        out["plain"] = plain
+        out["speedup_served_vs_plain"] = main_run["cells_per_s"] / plain["cells_per_s"]
     out["sweep"] = sweep
     return out
 
@@ -443,7 +448,8 @@ def main():
         return
     if a.only == "serve":
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
-        print(json.dumps(served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates)),
+        print(json.dumps(served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates,
+                                not a.no_plain)),
               flush=True)
         return
 

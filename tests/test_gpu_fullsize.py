@@ -87,3 +87,33 @@ def test_full_config3_all_cells():
     # 1e-9 volume carry 1e-16 absolute rounding each, so the sum is 1 only to about 1e-7
     assert gv.sum() == vol.sum()
     assert abs(gv.sum() - 1.0) < 1e-7
+
+
+def test_full_config4_batch_members_bit_exact():
+    # the batched workload at its full size in one batch (4,096 members, 64 to 65,536 generators,
+    # grids of about 2 generators per bin, sphere and plane walls, 24.3M cells): every fifth
+    # member (820 members of every size and wall kind, about 4.7M cells) bit for bit against the
+    # oracle run on that member alone: volumes, centroids, face counts, generators and neighbour
+    # ids (a fifth keeps the oracle's share of the test near half a minute)
+    from test_gpu_batch import oracle_of, walls_of
+    from vorothree_amd import Batch
+
+    ws = W.config4()
+    b = Batch()
+    for w in ws:
+        b.add(BoundingBox(w.lo, w.hi), w.grid, walls_of(w), w.points)
+    b.calculate()
+    assert sum(b.info(t)["cells"] for t in range(len(ws))) == 24_349_075
+    bad = []
+    for t in range(0, len(ws), 5):
+        o = oracle_of(ws[t], ws[t].grid)
+        o.calculate(threads=THREADS)
+        vol, cen, nf, _ = o.reduced()
+        goff, gids = b.neighbors(t)
+        ooff, oids = o.neighbors()
+        if not (np.array_equal(bits(b.volumes(t)), bits(vol)) and np.array_equal(bits(b.centroids(t)), bits(cen))
+                and np.array_equal(b.face_counts(t), nf)
+                and np.array_equal(bits(b.generators(t)), bits(o.generators()))
+                and np.array_equal(goff.astype(np.uint64), ooff) and np.array_equal(gids, oids)):
+            bad.append(t)
+    assert not bad, f"members differing from the oracle: {bad[:20]}"

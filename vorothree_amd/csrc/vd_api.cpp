@@ -1559,14 +1559,18 @@ int get_neighbors(const Engine &E, const Member &m, uint64_t *offsets, size_t n,
     for (size_t i = 0; i < nc; ++i) offsets[i + 1] = offsets[i] + nf[i];
     if (!ids) return VD_OK;
     if (cap < offsets[nc]) return fail(VD_EINVAL, "ids buffer too small");
-    std::vector<uint64_t> off(3 * nc);
-    if (nc) VD_HIP(hipMemcpy(off.data(), E.d_cell_off.as<uint64_t>() + 3 * m.gen_off, nc * 24, hipMemcpyDeviceToHost));
-    uint64_t top = 0;
-    VD_HIP(hipMemcpy(&top, E.d_top.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost));
-    std::vector<int32_t> arena(top);
-    if (top) VD_HIP(hipMemcpy(arena.data(), E.d_fnb.p, top * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < nc; ++i)
-        if (nf[i]) std::memcpy(ids + offsets[i], arena.data() + off[3 * i + 1], nf[i] * 4);
+    const uint64_t total = offsets[nc];
+    if (total == 0) return VD_OK;
+    // the member's ids are packed on the device (a batch member's cells are spread over the
+    // shared face arena), then copied once
+    DevBuf tmp[2];
+    const ScopedBufs tmp_g{tmp, 2};
+    VD_TRY(ensure(tmp[0], (nc + 1) * 8));
+    VD_TRY(ensure(tmp[1], total * 4));
+    VD_HIP(hipMemcpy(tmp[0].p, offsets, (nc + 1) * 8, hipMemcpyHostToDevice));
+    VD_HIP(vd::launch_gather_nb(E.d_fnb.as<int32_t>(), E.d_cell_off.as<uint64_t>() + 3 * m.gen_off,
+                                tmp[0].as<uint64_t>(), nc, tmp[1].as<int32_t>(), nullptr));
+    VD_HIP(hipMemcpy(ids, tmp[1].p, total * 4, hipMemcpyDeviceToHost));
     return VD_OK;
 }
 

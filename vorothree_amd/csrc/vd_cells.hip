@@ -83,6 +83,10 @@ __device__ inline int below_lane(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 __device__ inline uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
+// Lanes [0, n) as a wave-uniform mask (n <= 0: none, n >= 64: all).
+__device__ inline uint64_t lanes_below(int n) {
+    return n >= 64 ? ~0ull : n <= 0 ? 0ull : (1ull << n) - 1ull;
+}
 __device__ inline int popc(uint64_t x) { return __popcll(x); }
 __device__ inline int ffs64(uint64_t x) { return x ? (int)__builtin_ctzll(x) : 64; }
 __device__ inline int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
@@ -666,25 +670,24 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     if (NV > 1 && P.F > 64) return kGeneric;
 
     // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362). Lanes past
-    //    the polytope hold d = 0, which is neither beyond kTol nor below -kTol. Ballots are of
-    //    single compares, combined with scalar operations (a ballot of a combined predicate costs
-    //    two more vector instructions).
+    //    the polytope compute a distance from stale register copies; every ballot is masked with
+    //    the polytope's lanes (a scalar mask), so those lanes take no part. Ballots are of single
+    //    compares, combined with scalar operations (a ballot of a combined predicate costs two more
+    //    vector instructions).
     double d[NV];
-    bool va[NV];
     VMask<NV> gtm, ltm, inm, vam;
     bool anyg = false, anyl = false;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        va[k] = 64 * k + l < P.V;
-        vam.w[k] = ballot(va[k]);
-        d[k] = va[k] ? (r.x[k] - pl.px) * pl.nx + (r.y[k] - pl.py) * pl.ny + (r.z[k] - pl.pz) * pl.nz : 0.0;
-        gtm.w[k] = ballot(d[k] > kTol);
+        vam.w[k] = lanes_below(P.V - 64 * k);
+        d[k] = (r.x[k] - pl.px) * pl.nx + (r.y[k] - pl.py) * pl.ny + (r.z[k] - pl.pz) * pl.nz;
+        gtm.w[k] = ballot(d[k] > kTol) & vam.w[k];
         anyg |= gtm.w[k] != 0;
     }
     if (!anyg) return kNoChange;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        ltm.w[k] = ballot(d[k] < -kTol);
+        ltm.w[k] = ballot(d[k] < -kTol) & vam.w[k];
         anyl |= ltm.w[k] != 0;
     }
     if (!anyl) {
@@ -692,16 +695,21 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
         r2 = 0.0;
         return kModified;
     }
+    // kin: this lane's vertex is kept (the lane predicate, so tests of it need no bit extraction)
+    bool kin[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) inm.w[k] = ballot(d[k] <= kTol) & vam.w[k];
+    for (int k = 0; k < NV; ++k) {
+        kin[k] = d[k] <= kTol && 64 * k + l < P.V;
+        inm.w[k] = ballot(kin[k]);
+    }
     const int nk = inm.count();
     // one vertex per lane: lane v's new index if it is kept (read by lane shuffle in step 5)
     const uint32_t kidx = NV == 1 ? (uint32_t)below_lane(inm.w[0]) : 0u;
     double md2 = 0.0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        m.sc.dist[64 * k + l] = d[k];   // 0 past the polytope, never read there
-        if (withgen && ((inm.w[k] >> l) & 1ull)) {
+        m.sc.dist[64 * k + l] = d[k];   // never read past the polytope
+        if (withgen && kin[k]) {
             const double dx = r.x[k] - gx, dy = r.y[k] - gy, dz = r.z[k] - gz;
             md2 = gt_max(md2, dx * dx + dy * dy + dz * dz);
         }
@@ -729,7 +737,8 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
             const uint32_t e = sa ? e0 : 0u;
             const uint32_t a = e & 0xffffu, b = e >> 16;
             const bool ain = inm.test(a), bin = inm.test(b);
-            ea[ch] = a | (b << SH) | ((sa ? f0 : 0u) << (2 * SH)) | ((uint32_t)ain << 30) | ((uint32_t)bin << 31);
+            // (the face field of a lane past the slots is never read)
+            ea[ch] = a | (b << SH) | (f0 << (2 * SH)) | ((uint32_t)ain << 30) | ((uint32_t)bin << 31);
             const uint64_t sam = ballot(sa), ainm = ballot(ain), binm = ballot(bin);
             const uint64_t b0 = sam & ainm;
             const uint64_t b1 = sam & ~ainm & binm;
@@ -829,7 +838,7 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     int kbase = 0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        if ((inm.w[k] >> l) & 1ull) {
+        if (kin[k]) {
             const int ni = kbase + below_lane(inm.w[k]);
             m.vx[o][ni] = r.x[k]; m.vy[o][ni] = r.y[k]; m.vz[o][ni] = r.z[k];
         }
@@ -1135,7 +1144,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
     const int nwalls = D.nwalls;
     Poly P;
     poly_box<C>(m, P, D.box_lo, D.box_hi);
-    VReg<C> r;
+    VReg<C> r{};   // lanes past the polytope keep defined (stale) values
     vreg_load(m, P, r);
     int status = kCellOk;
     double r2 = 0.0;

@@ -214,6 +214,16 @@ __global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint64_t *_
     if (k < n) out[k] = src[idx[k]];
 }
 
+// The neighbour ids of cells [0, n) of one tessellation packed in cell order: cell k's ids start at
+// cell_off[3k + 1] in the face arena and go to out[offs[k], offs[k + 1]).
+__global__ void k_gather_nb(const int32_t *__restrict__ arena, const uint64_t *__restrict__ cell_off,
+                            const uint64_t *__restrict__ offs, uint64_t n, int32_t *__restrict__ out) {
+    const uint64_t k = gid();
+    if (k >= n) return;
+    const uint64_t src = cell_off[3 * k + 1], o = offs[k], c = offs[k + 1] - o;
+    for (uint64_t j = 0; j < c; ++j) out[o + j] = arena[src + j];
+}
+
 // get_bin_index per axis (algo_3d_grid.rs:102-104): clamp((x - min) * s, 0, n - 1e-5) as usize.
 // f64::clamp passes NaN through and `NaN as usize` is 0.
 __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
@@ -429,6 +439,13 @@ hipError_t launch_compact(const double *xyz, const uint32_t *tess_in, const uint
 hipError_t launch_gather_u32(const uint32_t *src, const uint64_t *idx, uint32_t n, uint32_t *out, hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_gather_u32<<<blocks_for(n), kBlock, 0, st>>>(src, idx, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_nb(const int32_t *arena, const uint64_t *cell_off, const uint64_t *offs, uint64_t n,
+                            int32_t *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_gather_nb<<<blocks_for(n), kBlock, 0, st>>>(arena, cell_off, offs, n, out);
     return hipGetLastError();
 }
 

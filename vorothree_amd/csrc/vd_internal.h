@@ -187,6 +187,8 @@ hipError_t launch_compact(const double *xyz, const uint32_t *tess_in, const uint
                           hipStream_t st);
 hipError_t launch_gather_u32(const uint32_t *src, const uint64_t *idx, uint32_t n, uint32_t *out,
                              hipStream_t st);
+hipError_t launch_gather_nb(const int32_t *arena, const uint64_t *cell_off, const uint64_t *offs, uint64_t n,
+                            int32_t *out, hipStream_t st);
 hipError_t launch_bin_count(const double *xyz, uint64_t n, const TessDesc *desc, const uint32_t *tess,
                             uint32_t *gen_bin, uint32_t *counts, hipStream_t st);
 hipError_t launch_bin_scatter(const uint32_t *gen_bin, uint64_t n, const uint32_t *bin_start,
