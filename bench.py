@@ -62,7 +62,13 @@ def parse():
     p.add_argument("--no-launch-tune", action="store_true",
                    help="keep the default launch shape instead of vd_*_autotune_launch's pick")
     p.add_argument("--clustered-n", type=int, default=4_000_000)
+    p.add_argument("--clustered-targets", type=str, default="0.25,0.35,0.7",
+                   help="generators-per-bin targets of the density rule tried for the clustered workload")
     return p.parse_args()
+
+
+def targets_of(a) -> list:
+    return [float(x) for x in a.clustered_targets.split(",") if x]
 
 
 def serve_run(reqs, rate: float, budget_gb: float) -> dict:
@@ -444,7 +450,7 @@ def main():
               flush=True)
         return
     if a.only == "clustered":
-        print(json.dumps(clustered(a.clustered_n, [0.25, 0.35, 0.7], 3, not a.no_launch_tune)), flush=True)
+        print(json.dumps(clustered(a.clustered_n, targets_of(a), 3, not a.no_launch_tune)), flush=True)
         return
     if a.only == "serve":
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
@@ -622,7 +628,7 @@ def main():
                                                        not a.no_launch_tune)
     if not a.no_clustered:
         progress(f"clustered: {a.clustered_n} points")
-        out["clustered"] = clustered(a.clustered_n, [0.25, 0.35, 0.7], 3, not a.no_launch_tune)
+        out["clustered"] = clustered(a.clustered_n, targets_of(a), 3, not a.no_launch_tune)
     if not a.no_serve:
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
         out["serve"] = served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates)
