@@ -609,7 +609,7 @@ def main():
                    "launch": launch or {"t0_waves": 7, "queue_grain": 1, "first_tier": 0, "first_probe": 0},
                    "launch_note": "tier-0 waves per SIMD, cells per queue atomic and first tier: "
                                   + ("vd_tess_autotune_launch's pick on this workload (coordinate "
-                                     "descent over skip-ahead 0/1, waves 5/6/7, grain 1/2/4/8, first "
+                                     "descent over skip-ahead 0/1, waves 5/6/7/8, grain 1/2/4/8, first "
                                      "tier 0/1), before the warmup"
                                      if launch else "the library defaults"),
                    "cells_retried_large_tier": st[1], "shell_prefix_entries": st[3],

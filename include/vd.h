@@ -202,8 +202,9 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
                                calculate() gives up with VD_ECAPACITY and keeps no results
                                (default 8, at most 62) */
 /* Launch shape of the cell kernels; every choice gives identical results. */
-#define VD_OPT_T0_WAVES 6   /* resident waves per SIMD of the small-LDS tier's kernel: 5, 6 or
-                               7 (default), each a separate compiled instantiation */
+#define VD_OPT_T0_WAVES 6   /* resident waves per SIMD of the small-LDS tier's kernel: 5, 6, 7 or
+                               8 (default; a layout under 5 KiB of LDS per wave), each a separate
+                               compiled instantiation */
 #define VD_OPT_QUEUE_GRAIN 7 /* cells a wave takes from the work queue per atomic, 1 (default) to 64 */
 #define VD_OPT_FIRST_TIER 8 /* tier of the first launch over every cell: 0 (default, 64 vertices)
                                or 1 (128 vertices); cells that outgrow it move up as usual */
@@ -212,7 +213,7 @@ int vd_tess_get_cell(const vd_tess *t, size_t i, uint64_t sizes[3], double *vert
                                0 (default): the plain walk, fewer registers */
 int vd_tess_set_option(vd_tess *t, int key, int64_t value);
 /* Times calculate() under the launch shapes (coordinate descent over VD_OPT_FIRST_PROBE {0, 1},
- * VD_OPT_T0_WAVES {5, 6, 7}, VD_OPT_QUEUE_GRAIN {1, 2, 4, 8}, VD_OPT_FIRST_TIER {0, 1}, from the
+ * VD_OPT_T0_WAVES {5, 6, 7, 8}, VD_OPT_QUEUE_GRAIN {1, 2, 4, 8}, VD_OPT_FIRST_TIER {0, 1}, from the
  * current shape), keeps the fastest and writes it to picks[4] (waves, grain, first tier,
  * skip-ahead). Generators must be set; about a dozen calculates run, and the results of the last
  * one are kept (identical under every shape). No counterpart in the reference. */

@@ -1,4 +1,4 @@
-"""Launch shapes of the cell kernels: every tier-0 instantiation (5, 6, 7 waves per SIMD), queue
+"""Launch shapes of the cell kernels: every tier-0 instantiation (5, 6, 7, 8 waves per SIMD), queue
 grain and first tier gives bit-exact the oracle's cells, and the launch autotuners keep results
 identical while picking a valid shape.
 """
@@ -30,9 +30,10 @@ def oracle_sphere():
     return o
 
 
-@pytest.mark.parametrize("waves,grain,first,probe", [(5, 1, 0, 0), (6, 1, 0, 0), (7, 1, 0, 0), (6, 4, 0, 0),
-                                                     (6, 8, 0, 0), (5, 64, 0, 0), (6, 1, 1, 0), (7, 2, 1, 0),
-                                                     (5, 1, 0, 1), (6, 2, 0, 1), (7, 1, 0, 1), (6, 1, 1, 1)])
+@pytest.mark.parametrize("waves,grain,first,probe", [(5, 1, 0, 0), (6, 1, 0, 0), (7, 1, 0, 0), (8, 1, 0, 0),
+                                                     (6, 4, 0, 0), (6, 8, 0, 0), (5, 64, 0, 0), (8, 4, 0, 0),
+                                                     (6, 1, 1, 0), (7, 2, 1, 0), (5, 1, 0, 1), (6, 2, 0, 1),
+                                                     (7, 1, 0, 1), (8, 1, 0, 1), (6, 1, 1, 1)])
 def test_launch_shape_bit_exact(oracle_sphere, waves, grain, first, probe):
     print(f"launch shape waves={waves} grain={grain} first_tier={first} probe={probe}", flush=True)
     lo, hi, grid, pts = _sphere_case()
@@ -62,7 +63,7 @@ def test_launch_shape_bit_exact(oracle_sphere, waves, grain, first, probe):
 def test_launch_options_rejected():
     b = BoundingBox((0.0,) * 3, (1.0,) * 3)
     g = Tessellation(b, Algorithm3DGrid(4, 4, 4, b))
-    for kw in ({"t0_waves": 4}, {"t0_waves": 8}, {"queue_grain": 0}, {"queue_grain": 65}, {"first_tier": 2},
+    for kw in ({"t0_waves": 4}, {"t0_waves": 9}, {"queue_grain": 0}, {"queue_grain": 65}, {"first_tier": 2},
                {"first_tier": -1}, {"first_probe": 2}):
         with pytest.raises(VdError):
             g.set_launch(**kw)
@@ -77,7 +78,7 @@ def test_tess_autotune_launch_keeps_results(oracle_sphere):
     g.set_generators(pts)
     picks = g.autotune_launch()
     print("picks", picks, flush=True)
-    assert picks["t0_waves"] in (5, 6, 7)
+    assert picks["t0_waves"] in (5, 6, 7, 8)
     assert picks["queue_grain"] in (1, 2, 4, 8)
     assert picks["first_tier"] in (0, 1) and picks["first_probe"] in (0, 1)
     vol, cen, nf, _ = oracle_sphere.reduced()
@@ -101,7 +102,7 @@ def test_batch_autotune_launch_keeps_results():
     ref = [bits(bt.volumes(t)).copy() for t in range(len(members))]
     picks = bt.autotune_launch()
     print("batch picks", picks, flush=True)
-    assert picks["t0_waves"] in (5, 6, 7) and picks["queue_grain"] in (1, 2, 4, 8)
+    assert picks["t0_waves"] in (5, 6, 7, 8) and picks["queue_grain"] in (1, 2, 4, 8)
     assert picks["first_tier"] in (0, 1) and picks["first_probe"] in (0, 1)
     bt.calculate()
     for t in range(len(members)):

@@ -1778,8 +1778,8 @@ constexpr int kLaunchKnobs = 4;
 
 int engine_set_launch(Engine &E, int knob, int64_t value) {
     if (knob == kKnobT0Waves) {
-        if (value < 5 || value > 7 || !vd::tier0_waves_valid((int)value))
-            return fail(VD_EINVAL, "tier-0 waves per SIMD must be 5, 6 or 7");
+        if (value < 5 || value > 8 || !vd::tier0_waves_valid((int)value))
+            return fail(VD_EINVAL, "tier-0 waves per SIMD must be 5, 6, 7 or 8");
         int occ = 1;
         VD_HIP(vd::cells_occupancy(0, &occ, (int)value));
         E.t0_waves = (int)value;
@@ -1813,7 +1813,7 @@ int64_t engine_launch_value(const Engine &E, int knob) {
 // tier, first-launch skip-ahead.
 template <class Run>
 int engine_tune_launch(Engine &E, Run run, int picks[kLaunchKnobs]) {
-    static const std::vector<int64_t> cand[kLaunchKnobs] = {{5, 6, 7}, {1, 2, 4, 8}, {0, 1}, {0, 1}};
+    static const std::vector<int64_t> cand[kLaunchKnobs] = {{5, 6, 7, 8}, {1, 2, 4, 8}, {0, 1}, {0, 1}};
     static const int order[kLaunchKnobs] = {kKnobFirstProbe, kKnobT0Waves, kKnobGrain, kKnobFirstTier};
     double cur = 0.0;
     VD_TRY(run(&cur));   // warm: shell tables, buffers

@@ -205,7 +205,7 @@ __device__ inline uint32_t axis_bin(double x, double lo, double s, double lim) {
     return (uint32_t)v;
 }
 
-template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int WAVES_, bool GLOBAL_ = false>
+template <int VM_, int SM_, int FM_, int KM_, bool FAST_, int WAVES_, bool GLOBAL_ = false, bool COMPACT_ = false>
 struct Cap {
     static constexpr int VM = VM_;      // vertices
     static constexpr int SM = SM_;      // face slots (flattened face indices)
@@ -214,6 +214,10 @@ struct Cap {
     static constexpr bool FAST = FAST_; // register fast path (VM <= 64) instead of the generic one
     static constexpr int WAVES = WAVES_;  // occupancy target (waves per SIMD) for the compiler
     static constexpr bool GLOBAL = GLOBAL_;  // the polytope lives in global memory, not LDS
+    // LDS under 5 KiB: the window staging shares the clip scratch's bytes and the volume fold stages
+    // half a slot chunk at a time (for 8 resident waves per SIMD)
+    static constexpr bool COMPACT = COMPACT_;
+    static constexpr int TRI = COMPACT_ ? 32 : 64;   // fold terms staged at a time
     static constexpr int NV = FAST_ ? (VM_ + 63) / 64 : 1;   // register vertices per lane
     // polytope buffers: the fast path rewrites the polytope in place, the generic one ping-pongs
     static constexpr int NB = FAST_ ? 1 : 2;
@@ -274,23 +278,37 @@ struct WinStage {
     uint16_t off[64];   // its offset within the entry's bin
 };
 
+// Bytes of phases that never overlap: one clip's scratch and, at the end of a cell, the fold terms
+// of one 64-slot chunk (half a chunk in the compact layout), and in the compact layout also one
+// gather's window staging (read back into registers before the next clip). The default layout
+// keeps the window staging apart: sharing it saves 512 bytes but measured slower on walks through
+// sparse regions, where gathers dominate.
+template <class C, bool COMPACT = C::COMPACT>
+struct MemScratch {
+    WinStage win;
+    union {
+        ClipScratch<C> sc;
+        double tri[4][C::TRI + 1];   // +1 keeps the four rows in different LDS banks
+    };
+};
 template <class C>
-struct Mem {
+struct MemScratch<C, true> {
+    union {
+        WinStage win;
+        ClipScratch<C> sc;
+        double tri[4][C::TRI + 1];
+    };
+};
+
+// Tier 0's LDS per wave: 5.5 KiB in the default layout (up to 7 resident waves per SIMD), under
+// 5 KiB in the compact one (8).
+template <class C>
+struct Mem : MemScratch<C> {
     double vx[C::NB][C::VM], vy[C::NB][C::VM], vz[C::NB][C::VM];
     uint32_t sedge[C::NB][C::SM];
     int32_t fnb[C::NB][C::FM];
     uint16_t sface[C::NB][C::SM];
     uint16_t fstart[C::NB][C::FM + 1];
-    // (Sharing the union below too would save 512 bytes, but measured slower on walks through
-    // sparse regions, where gathers dominate.)
-    WinStage win;
-    // Bytes of two phases that never overlap: one clip's scratch and, at the end of a cell, the
-    // fold terms of one 64-slot chunk. This keeps tier 0's LDS per wave at 5.5 KiB (up to 7
-    // resident waves per SIMD).
-    union {
-        ClipScratch<C> sc;
-        double tri[4][65];   // +1 keeps the four rows in different LDS banks
-    };
 };
 
 enum ClipRc : int { kNoChange = 0, kModified = 1, kOverflow = -1, kFace255 = -2, kGeneric = -3 };
@@ -1006,16 +1024,18 @@ __device__ void poly_reduce(Mem<C> &m, const Poly &P, double &vol, double cen[3]
             }
         }
         const uint64_t tm = ballot(tri);
-        if (tri) {
-            const int j = below_lane(tm);   // the chunk's triangles in order
-            m.tri[0][j] = det; m.tri[1][j] = wx; m.tri[2][j] = wy; m.tri[3][j] = wz;
+        const int nt = popc(tm), j = below_lane(tm);   // the chunk's triangles in order
+        for (int j0 = 0; j0 < nt; j0 += C::TRI) {      // staged C::TRI at a time
+            if (tri && j >= j0 && j < j0 + C::TRI) {
+                m.tri[0][j - j0] = det; m.tri[1][j - j0] = wx; m.tri[2][j - j0] = wy; m.tri[3][j - j0] = wz;
+            }
+            poly_sync<C>();
+            if (l < 4) {
+                const int ne = nt - j0 < C::TRI ? nt - j0 : C::TRI;
+                for (int k = 0; k < ne; ++k) acc += m.tri[l][k];
+            }
+            poly_sync<C>();
         }
-        poly_sync<C>();
-        if (l < 4) {
-            const int nt = popc(tm);
-            for (int j = 0; j < nt; ++j) acc += m.tri[l][j];
-        }
-        poly_sync<C>();
     }
     const double tv = rld(acc, 0), sx = rld(acc, 1), sy = rld(acc, 2), sz = rld(acc, 3);
     vol = fabs(tv / 6.0);
@@ -1543,7 +1563,7 @@ __global__ __launch_bounds__(64) void k_cells_global(CellArgs A) {
 // Tier 1: the same path with two vertices per lane (128 vertices). Tiers 2-3: generic path for
 // cells that outgrow the tier before, with 256 and 1024 vertices (the last one wave per CU).
 template <int W>
-using Tier0W = Cap<64, 192, 48, 64, true, W>;
+using Tier0W = Cap<64, 192, 48, 64, true, W, false, (W >= 8)>;
 using Tier0 = Tier0W<kTier0WavesDefault>;   // sizes (tier_max_vertices, tier_lds_bytes)
 using Tier1 = Cap<128, 384, 64, 64, true, 4>;
 using Tier2 = Cap<256, 768, 128, 384, false, 1>;
@@ -1694,6 +1714,7 @@ void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st, in
     if (tier == 0) {
         if (t0_waves == 5) k_cells<Tier0W<5>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
         else if (t0_waves == 6) k_cells<Tier0W<6>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
+        else if (t0_waves == 8) k_cells<Tier0W<8>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
         else k_cells<Tier0W<7>, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
     }
     else if (tier == 1) k_cells<Tier1, SEAL, PROBE><<<grid_waves, 64, 0, st>>>(a);
@@ -1702,7 +1723,7 @@ void launch_tier(int tier, const CellArgs &a, int grid_waves, hipStream_t st, in
     else k_cells_global<TierG, SEAL><<<grid_waves, 64, 0, st>>>(a);
 }
 
-bool tier0_waves_valid(int w) { return w == 5 || w == 6 || w == 7; }
+bool tier0_waves_valid(int w) { return w >= 5 && w <= 8; }
 
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal, bool probe,
                         int t0_waves) {
@@ -1725,6 +1746,8 @@ hipError_t cells_occupancy(int tier, int *blocks_per_cu, int t0_waves) {
             return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<5>, false, false>, 64, 0);
         if (t0_waves == 6)
             return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<6>, false, false>, 64, 0);
+        if (t0_waves == 8)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<8>, false, false>, 64, 0);
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cells<Tier0W<7>, false, false>, 64, 0);
     }
     if (tier == 1)

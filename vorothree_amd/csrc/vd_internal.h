@@ -239,8 +239,8 @@ hipError_t shell_finish(const ShellPack &pack, const double cs[3], uint32_t w, u
 // with two vertices per lane (128 vertices); 2, 3 = generic path with LDS polytopes of 256 and 1024
 // vertices (the last one wave per CU); 4 = the generic path with the polytope in global memory
 // (a.gmem: grid_waves slots of tier_global_bytes() each), up to 65,535 vertices
-// t0_waves: the tier-0 instantiation, by resident waves per SIMD (5, 6 or 7; same results)
-constexpr int kTier0WavesDefault = 7;
+// t0_waves: the tier-0 instantiation, by resident waves per SIMD (5, 6, 7 or 8; same results)
+constexpr int kTier0WavesDefault = 8;
 bool tier0_waves_valid(int w);
 hipError_t launch_cells(int tier, const CellArgs &a, int grid_waves, hipStream_t st, bool seal = false,
                         bool probe = false, int t0_waves = kTier0WavesDefault);

@@ -581,7 +581,7 @@ class Tessellation:
 
     def set_launch(self, t0_waves: int | None = None, queue_grain: int | None = None,
                    first_tier: int | None = None, first_probe: int | None = None) -> None:
-        """Launch shape of the cell kernels (VD_OPT_T0_WAVES 5/6/7, VD_OPT_QUEUE_GRAIN 1-64,
+        """Launch shape of the cell kernels (VD_OPT_T0_WAVES 5/6/7/8, VD_OPT_QUEUE_GRAIN 1-64,
         VD_OPT_FIRST_TIER 0/1, VD_OPT_FIRST_PROBE 0/1). Results are identical under every shape."""
         for key, v in ((_lib.VD_OPT_T0_WAVES, t0_waves), (_lib.VD_OPT_QUEUE_GRAIN, queue_grain),
                        (_lib.VD_OPT_FIRST_TIER, first_tier), (_lib.VD_OPT_FIRST_PROBE, first_probe)):
