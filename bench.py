@@ -572,7 +572,7 @@ def main():
         roof = {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                 "kernel": ("k_cells<Tier1" if launch and launch["first_tier"] == 1
-                           else f"k_cells<Tier0W<{launch['t0_waves'] if launch else 7}>")
+                           else f"k_cells<Tier0W<{launch['t0_waves'] if launch else 8}>")
                           + (", skip-ahead>" if launch and launch["first_probe"] else ">"),
                 "kernel_ms": kern_s * 1e3,
                 "flops_per_cell": flops_per_cell,
