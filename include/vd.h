@@ -126,6 +126,17 @@ int vd_tess_set_generators(vd_tess *t, const double *xyz, size_t n);
 /* The same with the points already in device memory (HBM); d_xyz is not retained. */
 int vd_tess_set_generators_device(vd_tess *t, const double *d_xyz, size_t n);
 
+/* set_generator(index, &generator), src/tessellation.rs:71-92: moves generator `index` to xyz if
+ * every wall contains xyz (otherwise nothing changes and the call returns VD_OK, as the reference
+ * ignores it); the bins follow update_generator (src/algorithm/algo_3d_grid.rs:132-143: swap_remove
+ * from the old bin, push to the end of the new one), which changes the visiting order of later
+ * calculate() calls exactly as in the reference. An index past the generators returns VD_ERANGE
+ * (the reference ignores it). As after set_generators, count_cells() is 0 until the next
+ * calculate(). Each call copies 24 bytes to the device and synchronises; the first one after a
+ * binning reads the generators back once to rebuild the bins on the host (O(n)), and a move
+ * between bins shifts the host bin lists (O(n) memmove); the bins go up once, at calculate(). */
+int vd_tess_set_generator(vd_tess *t, size_t index, const double xyz[3]);
+
 /* calculate(), src/tessellation.rs:213-230: every cell, results kept on the device. */
 int vd_tess_calculate(vd_tess *t);
 

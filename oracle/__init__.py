@@ -45,6 +45,7 @@ def _load():
                                             ctypes.c_double, _dp, ctypes.c_uint32, ctypes.c_int]),
         "or_tess_add_wall_host": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _HOST_CONTAINS, _HOST_CUT, ctypes.c_int]),
         "or_tess_set_generators": (None, [_vp, _dp, ctypes.c_size_t]),
+        "or_tess_set_generator": (ctypes.c_int, [_vp, ctypes.c_size_t, _dp]),
         "or_tess_count_generators": (ctypes.c_size_t, [_vp]),
         "or_tess_get_generators": (None, [_vp, _dp]),
         "or_tess_get_bins": (None, [_vp, _vp, _vp, _vp]),
@@ -204,6 +205,11 @@ class OracleTess:
     def set_generators(self, xyz):
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))
         self._lib.or_tess_set_generators(self._h, a.ctypes.data_as(_dp), a.size // 3)
+
+    def set_generator(self, index: int, q) -> int:
+        """set_generator (tessellation.rs:71-92): 0 moved, 1 index past the generators, 2 a wall
+        does not contain q (the last two change nothing)."""
+        return int(self._lib.or_tess_set_generator(self._h, int(index), _d3(q)))
 
     def count_generators(self) -> int:
         return int(self._lib.or_tess_count_generators(self._h))

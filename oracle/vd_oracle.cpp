@@ -720,6 +720,41 @@ void tess_add_wall(Tess &T, const Wall &w) {  // tessellation.rs:175-200
     }
 }
 
+// set_generator, tessellation.rs:71-92, with update_generator, algo_3d_grid.rs:132-143. Returns 0
+// when the generator moved, 1 for an index past the generators and 2 for a position a wall does
+// not contain (the reference ignores both calls).
+int tess_set_generator(Tess &T, size_t index, const double q[3]) {
+    const size_t offset = index * 3;
+    if (offset + 3 > T.gen.size()) return 1;                  // :73-75
+    for (const Wall &w : T.walls)
+        if (!wall_contains(w, q)) return 2;                    // :77-81
+    Grid &G = T.grid;
+    const uint32_t nb = (uint32_t)grid_bin_index(G, q);       // :133
+    const uint32_t ob = G.gen_bin[index];                      // :134
+    if (nb != ob) {                                            // :136
+        const size_t nbins = G.bin_start.size() - 1;
+        std::vector<std::vector<uint32_t>> bins(nbins);        // grid_bins
+        for (size_t b = 0; b < nbins; ++b)
+            bins[b].assign(G.bin_items.begin() + G.bin_start[b], G.bin_items.begin() + G.bin_start[b + 1]);
+        std::vector<uint32_t> &old = bins[ob];
+        const auto it = std::find(old.begin(), old.end(), (uint32_t)index);   // :137
+        if (it != old.end()) {                                 // :138 swap_remove
+            *it = old.back();
+            old.pop_back();
+        }
+        bins[nb].push_back((uint32_t)index);                   // :140
+        G.gen_bin[index] = nb;                                 // :141
+        uint32_t k = 0;
+        for (size_t b = 0; b < nbins; ++b) {
+            G.bin_start[b] = k;
+            for (uint32_t v : bins[b]) G.bin_items[k++] = v;
+        }
+        G.bin_start[nbins] = k;
+    }
+    std::memcpy(&T.gen[offset], q, 3 * sizeof(double));       // :89-91
+    return 0;
+}
+
 // compute_cell, tessellation.rs:414-475. brute != 0 visits every other generator in index order
 // with no grid and no early exit (used only to pin the grid walk).
 void compute_cell(const Tess &T, size_t i, Poly &cell, ClipScratch &s, Work *w, int brute) {
@@ -873,6 +908,10 @@ int or_tess_add_wall_host(void *t, int32_t id, void *ctx, HostContains contains,
 
 void or_tess_set_generators(void *t, const double *xyz, size_t n) {
     tess_set_generators(*(Tess *)t, xyz, n);
+}
+
+int or_tess_set_generator(void *t, size_t index, const double q[3]) {
+    return tess_set_generator(*(Tess *)t, index, q);
 }
 
 size_t or_tess_count_generators(void *t) { return ((Tess *)t)->gen.size() / 3; }

@@ -6,8 +6,8 @@
 //                         and where no GPU is visible, handle creation failing (its cleanup)
 //   host_check --device   full handle lifecycles on the current GPU: a tessellation with built-in
 //                         and caller-evaluated walls through calculate, sealed calculate, prune,
-//                         relax and every getter; a batch; a server with tessellation and Lloyd
-//                         jobs; the static entry twice
+//                         relax, single-generator moves and every getter; a batch; a server with
+//                         tessellation and Lloyd jobs; the static entry twice
 //
 // Every call's return code is checked; any sanitizer report or failed check ends the program
 // with a non-zero status.
@@ -84,6 +84,7 @@ void host_checks() {
     CHECK(vd_tess_create(kLo, kHi, 4, 4, 4, nullptr) == VD_EINVAL);
     CHECK(vd_tess_add_wall_plane(nullptr, -1000, kLo, kHi) == VD_EINVAL);
     CHECK(vd_tess_set_generators(nullptr, nullptr, 0) == VD_EINVAL);
+    CHECK(vd_tess_set_generator(nullptr, 0, kLo) == VD_EINVAL);
     CHECK(vd_tess_calculate(nullptr) == VD_EINVAL);
     CHECK(vd_tess_count_cells(nullptr) == 0);
     vd_tess_destroy(nullptr);
@@ -204,6 +205,18 @@ void tess_checks() {
         OK(vd_tess_relax(t));
         OK(vd_tess_calculate(t));
     }
+    // single moves: across bins in both directions, within a bin, rejected by a wall, out of range
+    for (size_t k = 0; k < 400; ++k) {
+        const double q[3] = {p[3 * ((k * 7) % n)], p[3 * ((k * 13) % n) + 1], p[3 * ((k * 31) % n) + 2]};
+        OK(vd_tess_set_generator(t, (k * 11) % m, q));
+    }
+    const double far[3] = {0.02, 0.02, 0.02};
+    OK(vd_tess_set_generator(t, 0, far));   // outside the sphere wall: ignored
+    CHECK(vd_tess_set_generator(t, m, c) == VD_ERANGE);
+    CHECK(vd_tess_set_generator(t, 0, nullptr) == VD_EINVAL);
+    OK(vd_tess_calculate(t));
+    OK(vd_tess_set_generator(t, 1, c));
+    OK(vd_tess_calculate(t));
     int picks[4];
     OK(vd_tess_autotune_launch(t, picks));
     OK(vd_tess_set_option(t, VD_OPT_KEEP_CELLS, 0));

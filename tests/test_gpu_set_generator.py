@@ -1,0 +1,186 @@
+"""set_generator (src/tessellation.rs:71-92) on the device against the oracle, bit for bit.
+
+The bins after a move follow update_generator (src/algorithm/algo_3d_grid.rs:132-143): the old
+bin loses the generator by swap_remove and the new bin pushes it. The order of a bin is the order
+in which a cell's walk visits its generators, which fixes the order of the cuts and so the bits of
+every vertex: each case checks that the moved tessellation differs from the same positions set
+afresh, so a library that rebinned instead would fail.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_same_cell, bits, oracle_wall
+from oracle import OracleTess
+from vorothree_amd import (Algorithm3DGrid, BoundingBox, PlaneGeometry, SphereGeometry, Tessellation,
+                           TrefoilKnotGeometry, Wall)
+from vorothree_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+UNIT = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+
+
+def make(grid, walls=(), keep_cells=True, graphs=True):
+    g = Tessellation(UNIT, Algorithm3DGrid(*grid, UNIT), keep_cells=keep_cells, graphs=graphs)
+    o = OracleTess((0, 0, 0), (1, 1, 1), *grid)
+    for w in walls:
+        g.add_wall(w)
+        oracle_wall(o, w)
+    return g, o
+
+
+def compare(g, o, cells=True):
+    g.calculate()
+    o.calculate(threads=8)
+    n = o.count_cells()
+    assert g.count_cells() == n
+    assert np.array_equal(bits(g.generators()), bits(o.generators()))
+    if cells:
+        for i in range(n):
+            assert_same_cell(g.get_cell(i), o.get_cell(i), i)
+    vol, cen, nf, _ = o.reduced()
+    assert np.array_equal(bits(g.volumes()), bits(vol))
+    assert np.array_equal(bits(g.centroids()), bits(cen))
+    assert np.array_equal(g.face_counts(), nf)
+    return vol
+
+
+def differs_from_fresh(o, grid, walls=(), host=()):
+    """True when the same positions and walls set afresh (bins in index order) give other bits."""
+    f = OracleTess((0, 0, 0), (1, 1, 1), *grid)
+    for w in walls:
+        oracle_wall(f, w)
+    for wid, geometry in host:
+        f.add_wall_host(wid, geometry)
+    f.set_generators(o.generators().reshape(-1, 3))
+    f.calculate(threads=8)
+    return not np.array_equal(bits(f.reduced()[0]), bits(o.reduced()[0]))
+
+
+def move_both(g, o, index, q):
+    rc = o.set_generator(index, q)
+    g.set_generator(index, q)
+    return rc
+
+
+def test_moves_across_and_within_bins_bit_exact():
+    grid = (8, 8, 8)
+    walls = [Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.48))]
+    g, o = make(grid, walls)
+    rng = W.PCG32(1401)
+    pts = rng.uniform(3000, (0, 0, 0), (1, 1, 1))
+    g.set_generators(pts)
+    o.set_generators(pts)
+    n = o.count_generators()
+    codes = []
+    for rnd in range(3):
+        targets = rng.uniform(400, (0, 0, 0), (1, 1, 1))
+        for k, q in enumerate(targets):
+            codes.append(move_both(g, o, (13 * k + 7 * rnd) % n, q))
+        # small moves that stay in their bins
+        gen = o.generators().reshape(-1, 3)
+        for k in range(0, n, 37):
+            codes.append(move_both(g, o, k, gen[k] + 1e-6))
+        codes.append(move_both(g, o, n, (0.5, 0.5, 0.5)))            # past the generators: ignored
+        codes.append(move_both(g, o, n + 10**6, (0.5, 0.5, 0.5)))
+        compare(g, o)
+    assert codes.count(1) == 6 and codes.count(2) > 300 and codes.count(0) > 500
+    assert differs_from_fresh(o, grid, walls)
+
+
+def test_lloyd_by_moves_matches_oracle():
+    # relax written with set_generator: every generator moves to its centroid one at a time, so
+    # the bins keep the swapped orders instead of being rebuilt as relax does
+    grid = (6, 6, 6)
+    g, o = make(grid)
+    pts = W.PCG32(1402).uniform(1500, (0, 0, 0), (1, 1, 1))
+    g.set_generators(pts)
+    o.set_generators(pts)
+    for _ in range(3):
+        compare(g, o, cells=False)
+        cen = o.reduced()[1]
+        for i in range(len(cen)):
+            move_both(g, o, i, cen[i])
+    vol = compare(g, o)
+    assert abs(vol.sum() - 1.0) < 1e-12
+    assert differs_from_fresh(o, grid)
+    g.relax()
+    o.relax()
+    compare(g, o)
+
+
+def test_wall_added_after_moves_keeps_or_rebuilds_bins():
+    # prune_outside_generators (tessellation.rs:178-194) rebins only when the new wall removes a
+    # generator: a wall outside every generator keeps the moved bins, a cutting one rebuilds them
+    grid = (7, 7, 7)
+    g, o = make(grid)
+    rng = W.PCG32(1403)
+    pts = rng.uniform(2500, (0.1, 0.1, 0.1), (0.9, 0.9, 0.9))
+    g.set_generators(pts)
+    o.set_generators(pts)
+    for k, q in enumerate(rng.uniform(500, (0.1, 0.1, 0.1), (0.9, 0.9, 0.9))):
+        move_both(g, o, (11 * k) % 2500, q)
+    keep = Wall(-1000, PlaneGeometry((0.95, 0.5, 0.5), (-1.0, 0.0, 0.0)))
+    g.add_wall(keep)
+    oracle_wall(o, keep)
+    assert o.count_generators() == 2500
+    compare(g, o)
+    assert differs_from_fresh(o, grid, [keep])
+    for k, q in enumerate(rng.uniform(200, (0.1, 0.1, 0.1), (0.9, 0.9, 0.9))):
+        move_both(g, o, (17 * k) % 2500, q)
+    cut = Wall(-1001, SphereGeometry((0.5, 0.5, 0.5), 0.42))
+    g.add_wall(cut)
+    oracle_wall(o, cut)
+    assert o.count_generators() < 2500
+    compare(g, o)
+    assert not differs_from_fresh(o, grid, [keep, cut])
+
+
+def test_tube_and_host_walls_follow_the_moved_generator():
+    # per-generator planes of a tube wall and of the caller's walls are evaluated anew for the
+    # moved generator: PySphere always emits one plane (overwritten in place), PyNearFacets 1 to 8
+    # depending on the position (spliced into the plane lists); sealing reads the plane counts
+    from host_geometries import PyNearFacets, PySphere
+    grid = (10, 10, 10)
+    knot = TrefoilKnotGeometry((0.5, 0.5, 0.5), 0.12, 0.07, 120)
+    host = [(-1001, PySphere((0.5, 0.5, 0.5), 0.47)), (-1002, PyNearFacets((0.5, 0.5, 0.5), 0.62))]
+    g, o = make(grid, [Wall(-1000, knot)], keep_cells=False)
+    for wid, geometry in host:
+        g.add_wall(Wall(wid, geometry))
+        o.add_wall_host(wid, geometry)
+    rng = W.PCG32(1404)
+    pts = rng.uniform(6000, (0, 0, 0), (1, 1, 1))
+    g.set_generators(pts)
+    o.set_generators(pts)
+    n = o.count_generators()
+    for rnd in range(2):
+        compare(g, o, cells=False)
+        for k, q in enumerate(rng.uniform(600, (0, 0, 0), (1, 1, 1))):
+            move_both(g, o, (29 * k + rnd) % n, q)
+    compare(g, o, cells=False)
+    assert differs_from_fresh(o, grid, [Wall(-1000, knot)], host)
+    for k, q in enumerate(rng.uniform(300, (0, 0, 0), (1, 1, 1))):
+        move_both(g, o, (31 * k) % n, q)
+    g.calculate_sealed()
+    o.calculate_sealed(threads=8)
+    assert np.array_equal(bits(g.volumes()), bits(o.reduced()[0]))
+    assert np.array_equal(g.seal_log(), o.seal_log()) and len(o.seal_log()) > 0
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_repeated_calculates_between_moves(graphs):
+    # calculates with the same shapes replay one captured first round; the moves between them
+    # change only the bin contents and positions the replay reads
+    grid = (12, 12, 12)
+    g, o = make(grid, keep_cells=False, graphs=graphs)
+    rng = W.PCG32(1405)
+    pts = rng.uniform(20000, (0, 0, 0), (1, 1, 1))
+    g.set_generators(pts)
+    o.set_generators(pts)
+    for rnd in range(4):
+        compare(g, o, cells=False)
+        for k, q in enumerate(rng.uniform(300, (0, 0, 0), (1, 1, 1))):
+            move_both(g, o, (101 * k + 3 * rnd) % 20000, q)
+    compare(g, o, cells=False)
+    if graphs:
+        assert g.stats()[8] >= 1

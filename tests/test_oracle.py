@@ -339,3 +339,74 @@ def test_generator_outside_the_box_has_an_empty_cell_and_relax_keeps_it():
     g = t.generators().reshape(-1, 3)
     assert np.array_equal(g[0], pts[0])
     assert np.array_equal(g[1:], cen[1:])
+
+
+# --- set_generator ---------------------------------------------------------------------------
+
+def _bin_lists(t):
+    bs, bi, gb = t.bins()
+    return [list(bi[bs[b]:bs[b + 1]]) for b in range(len(bs) - 1)], list(gb)
+
+
+def test_set_generator_bin_order_is_swap_remove_then_push():
+    """update_generator (algo_3d_grid.rs:132-143) uses Vec::swap_remove on the old bin (the last
+    entry takes the vacated slot) and Vec::push on the new one. Grid 2x1x1: bin 0 holds 0, 1, 2 and
+    bin 1 holds 3; moving 0, then 1, then 3 gives the orders Vec's documented semantics fix."""
+    t = OracleTess((0, 0, 0), (1, 1, 1), 2, 1, 1)
+    t.set_generators([[0.1, 0.5, 0.5], [0.2, 0.5, 0.5], [0.3, 0.5, 0.5], [0.7, 0.5, 0.5]])
+    assert _bin_lists(t) == ([[0, 1, 2], [3]], [0, 0, 0, 1])
+    assert t.set_generator(0, (0.8, 0.5, 0.5)) == 0
+    assert _bin_lists(t) == ([[2, 1], [3, 0]], [1, 0, 0, 1])
+    assert t.set_generator(1, (0.9, 0.5, 0.5)) == 0
+    assert _bin_lists(t) == ([[2], [3, 0, 1]], [1, 1, 0, 1])
+    assert t.set_generator(3, (0.4, 0.5, 0.5)) == 0
+    assert _bin_lists(t) == ([[2, 3], [1, 0]], [1, 1, 0, 0])
+    # a move inside its bin changes the position only
+    assert t.set_generator(2, (0.05, 0.25, 0.75)) == 0
+    assert _bin_lists(t) == ([[2, 3], [1, 0]], [1, 1, 0, 0])
+    assert np.array_equal(t.generators().reshape(-1, 3)[2], [0.05, 0.25, 0.75])
+    # a generator outside the box lands in the clamped bin (get_bin_index, algo_3d_grid.rs:98-107)
+    assert t.set_generator(2, (7.0, -3.0, 0.5)) == 0
+    assert _bin_lists(t) == ([[3], [1, 0, 2]], [1, 1, 1, 0])
+
+
+def test_set_generator_ignores_bad_index_and_rejected_position():
+    """tessellation.rs:72-81: an index past the generators or a position that a wall does not
+    contain leaves positions and bins unchanged."""
+    pts = W.PCG32(51).uniform(40, (0, 0, 0), (1, 1, 1))
+    t = OracleTess((0, 0, 0), (1, 1, 1), 3, 3, 3)
+    t.set_generators(pts)
+    t.add_wall_sphere(-1000, (0.5, 0.5, 0.5), 0.6)
+    n = t.count_generators()
+    g0, b0 = t.generators().copy(), _bin_lists(t)
+    assert t.set_generator(n, (0.5, 0.5, 0.5)) == 1
+    assert t.set_generator(0, (0.02, 0.02, 0.02)) == 2   # outside the sphere
+    assert np.array_equal(t.generators(), g0) and _bin_lists(t) == b0
+
+
+def test_set_generator_cells_match_brute_force_and_a_fresh_set():
+    """After moves across bins the grid walk over the swapped bin orders still clips by the same
+    neighbours as clipping by every generator; the same positions set afresh (bins in index order)
+    give the same volumes within 1e-9 (clip order differs, so bits may)."""
+    rng = W.PCG32(52)
+    pts = rng.uniform(200, (0, 0, 0), (1, 1, 1))
+    moves = rng.uniform(30, (0, 0, 0), (1, 1, 1))
+    a = OracleTess((0, 0, 0), (1, 1, 1), 4, 4, 4)
+    a.set_generators(pts)
+    for k, q in enumerate(moves):
+        assert a.set_generator((7 * k) % 200, q) == 0
+    g = a.generators().reshape(-1, 3)
+    b = OracleTess((0, 0, 0), (1, 1, 1), 4, 4, 4)
+    b.set_generators(g)
+    c = OracleTess((0, 0, 0), (1, 1, 1), 4, 4, 4)
+    c.set_generators(g)
+    a.calculate()
+    b.calculate()
+    c.calculate(brute=True)
+    va, _, _, _ = a.reduced()
+    for t in (b, c):
+        v, _, _, _ = t.reduced()
+        assert np.allclose(va, v, rtol=1e-9, atol=1e-15)
+        for i in range(a.count_cells()):
+            assert sorted(a.get_cell(i).face_neighbors) == sorted(t.get_cell(i).face_neighbors), i
+    assert abs(va.sum() - 1.0) < 1e-12

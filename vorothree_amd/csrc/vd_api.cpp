@@ -485,11 +485,20 @@ struct Engine {
     DevBuf d_hflags, d_host_off, d_host_planes;
     DevBuf d_gmem;                     // polytope slots of the global-memory tier
     std::vector<uint32_t> host_max;    // per such wall: most planes it emits for one generator
+    std::vector<uint64_t> h_host_off;  // host copies of d_host_off and d_host_planes
+    std::vector<double> h_host_planes;
     uint32_t seal_slots = vd::kMaxWalls;   // seal-log entries per neighbour
     uint32_t *h_counts = nullptr;      // pinned: per-tier outcome counts of a round
     uint64_t nraw = 0, ngen = 0, nbins = 0;
     bool prepared = false, calculated = false;
     bool binned = false;               // bins of the current generators are built
+    // set_generator (tessellation.rs:71-92) on a single tessellation: its bins as update_generator
+    // leaves them (algo_3d_grid.rs:132-143: swap_remove from the old bin, push to the new one), not
+    // in the sorted order a binning gives; kept on the host as a CSR and uploaded at the next
+    // calculate (bins_dirty). set_generators and relax end it (they bin afresh), as does an
+    // add_wall that removes generators; one that removes none keeps it, as the reference does.
+    bool custom_bins = false, bins_dirty = false;
+    std::vector<uint32_t> h_bin_start, h_items, h_gen_bin;
     // First-round launch sequence (binning, tier-0 launch, collect, counts readback) of a shape
     // seen twice in a row is captured into a graph and replayed while the shape repeats.
     int graphs = 1;
@@ -760,8 +769,10 @@ int host_planes(Engine &E) {
     std::vector<double> gen(3 * ng);
     if (ng) VD_HIP(hipMemcpyAsync(gen.data(), E.d_gen.p, ng * 24, hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
-    std::vector<uint64_t> off(ng * nh + 1, 0);
-    std::vector<double> planes;
+    std::vector<uint64_t> &off = E.h_host_off;
+    std::vector<double> &planes = E.h_host_planes;
+    off.assign(ng * nh + 1, 0);
+    planes.clear();
     std::vector<double> buf(6 * 16);
     E.host_max.assign(nh, 0u);
     for (const Member &m : E.mem) {
@@ -793,10 +804,61 @@ int host_planes(Engine &E) {
     return VD_OK;
 }
 
+// The caller's walls' cut planes for generator g of a single tessellation, now at q: written over
+// its old planes when every wall emits as many as before (one small upload), else spliced into the
+// CSR, which is uploaded whole.
+int host_planes_one(Engine &E, uint64_t g, const double q[3]) {
+    const uint64_t nh = E.nhost;
+    const Member &m = E.mem[0];
+    std::vector<uint64_t> &off = E.h_host_off;
+    std::vector<double> &planes = E.h_host_planes;
+    std::vector<double> mine;
+    std::vector<uint64_t> cnt(nh, 0);
+    std::vector<double> buf(6 * 16);
+    for (uint64_t k = 0; k < nh && k < m.host.size(); ++k) {
+        const vd_wall_host &cb = m.host[k];
+        int c = cb.cut(cb.ctx, q, buf.data(), (int)(buf.size() / 6));
+        if (c > (int)(buf.size() / 6)) {
+            if (c > VD_WALL_MAX_POINTS) return fail(VD_EINVAL, "a wall's cut emitted too many planes");
+            buf.resize(6 * (size_t)c);
+            c = cb.cut(cb.ctx, q, buf.data(), c);
+        }
+        if (c < 0 || c > (int)(buf.size() / 6)) return fail(VD_EINVAL, "a wall's cut returned a bad count");
+        mine.insert(mine.end(), buf.begin(), buf.begin() + 6 * (size_t)c);
+        cnt[k] = (uint64_t)c;
+        E.host_max[k] = std::max<uint32_t>(E.host_max[k], (uint32_t)c);
+    }
+    const uint64_t a = off[g * nh], b = off[g * nh + nh];
+    bool same = true;
+    for (uint64_t k = 0; k < nh; ++k) same &= off[g * nh + k + 1] - off[g * nh + k] == cnt[k];
+    if (same) {
+        if (b > a) {
+            std::copy(mine.begin(), mine.end(), planes.begin() + 6 * a);
+            VD_HIP(hipMemcpyAsync(E.d_host_planes.as<double>() + 6 * a, mine.data(), mine.size() * 8,
+                                  hipMemcpyHostToDevice, E.stream));
+            VD_HIP(hipStreamSynchronize(E.stream));
+        }
+        return VD_OK;
+    }
+    planes.erase(planes.begin() + 6 * a, planes.begin() + 6 * b);
+    planes.insert(planes.begin() + 6 * a, mine.begin(), mine.end());
+    const int64_t shift = (int64_t)(mine.size() / 6) - (int64_t)(b - a);
+    for (uint64_t k = 0; k < nh; ++k) off[g * nh + k + 1] = off[g * nh + k] + cnt[k];
+    for (uint64_t j = g * nh + nh + 1; j < off.size(); ++j) off[j] = (uint64_t)((int64_t)off[j] + shift);
+    VD_TRY(ensure(E.d_host_planes, std::max<size_t>(planes.size(), 6) * 8));
+    VD_HIP(hipMemcpyAsync(E.d_host_off.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, E.stream));
+    if (!planes.empty())
+        VD_HIP(hipMemcpyAsync(E.d_host_planes.p, planes.data(), planes.size() * 8, hipMemcpyHostToDevice,
+                              E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    return VD_OK;
+}
+
 // set_generators for every member from E.d_raw (tessellation.rs:43-67): wall filter in input order,
 // compaction, then the stable counting-sort binning (algo_3d_grid.rs:111-130).
 int engine_prepare(Engine &E) {
     E.prepared = E.calculated = false;
+    E.custom_bins = E.bins_dirty = false;
     const size_t M = E.mem.size();
     const uint64_t n = E.nraw;
     if (E.nbins + 1 >= 0xffffffffull || n >= 0xffffffffull) return fail(VD_EINVAL, "too many bins or points");
@@ -1161,6 +1223,81 @@ int run_round0(Engine &E, uint32_t n, float *ms) {
     return VD_OK;
 }
 
+// get_bin_index (algo_3d_grid.rs:98-107) on the host, the expression of k_bin_count.
+uint32_t host_axis_bin(double x, double lo, double s, double lim) {
+    double v = (x - lo) * s;
+    if (v < 0.0) v = 0.0;
+    if (v > lim) v = lim;
+    if (!(v == v)) return 0;
+    return (uint32_t)v;
+}
+
+uint32_t host_bin_of(const vd::GridGeom &G, const double q[3]) {
+    const uint32_t ix = host_axis_bin(q[0], G.lo[0], G.s[0], G.lim[0]);
+    const uint32_t iy = host_axis_bin(q[1], G.lo[1], G.s[1], G.lim[1]);
+    const uint32_t iz = host_axis_bin(q[2], G.lo[2], G.s[2], G.lim[2]);
+    return ix + iy * (uint32_t)G.n[0] + iz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
+}
+
+// The bins of a single tessellation's current generators on the host, as the binning builds them
+// (algo_3d_grid.rs:111-130: every bin lists its generators in index order).
+int host_bins(Engine &E) {
+    const Member &m = E.mem[0];
+    const uint64_t n = E.ngen, nb = m.nbins;
+    std::vector<double> g(3 * n);
+    if (n) VD_HIP(hipMemcpyAsync(g.data(), E.d_gen.p, n * 24, hipMemcpyDeviceToHost, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    E.h_gen_bin.resize(n);
+    E.h_bin_start.assign(nb + 1, 0u);
+    for (uint64_t i = 0; i < n; ++i) {
+        E.h_gen_bin[i] = host_bin_of(m.grid, &g[3 * i]);
+        E.h_bin_start[E.h_gen_bin[i] + 1]++;
+    }
+    for (uint64_t b = 0; b < nb; ++b) E.h_bin_start[b + 1] += E.h_bin_start[b];
+    std::vector<uint32_t> fill(E.h_bin_start.begin(), E.h_bin_start.end() - 1);
+    E.h_items.resize(n);
+    for (uint64_t i = 0; i < n; ++i) E.h_items[fill[E.h_gen_bin[i]]++] = (uint32_t)i;
+    return VD_OK;
+}
+
+// update_generator (algo_3d_grid.rs:132-143) on the host CSR: generator i leaves bin `from` by
+// swap_remove (its slot takes the bin's last entry) and is pushed at the end of bin `to`; the
+// entries between the two bins shift by one slot.
+void host_move(Engine &E, uint32_t i, uint32_t from, uint32_t to) {
+    std::vector<uint32_t> &bs = E.h_bin_start, &it = E.h_items;
+    const uint32_t e0 = bs[from + 1];
+    uint32_t p = bs[from];
+    while (it[p] != i) ++p;   // the generator is in its bin
+    it[p] = it[e0 - 1];       // slot e0 - 1 is free now
+    if (to > from) {
+        const uint32_t e1 = bs[to + 1];
+        std::memmove(&it[e0 - 1], &it[e0], (size_t)(e1 - e0) * 4);
+        it[e1 - 1] = i;
+        for (uint32_t b = from + 1; b <= to; ++b) bs[b] -= 1;
+    } else {
+        const uint32_t e1 = bs[to + 1];
+        std::memmove(&it[e1 + 1], &it[e1], (size_t)(e0 - 1 - e1) * 4);
+        it[e1] = i;
+        for (uint32_t b = to + 1; b <= from; ++b) bs[b] += 1;
+    }
+    E.h_gen_bin[i] = to;
+}
+
+// Bins kept by set_generator to the device: the CSR, the bin-ordered positions, the tube planes.
+int upload_custom_bins(Engine &E) {
+    const uint64_t n = E.ngen, nb = E.mem[0].nbins;
+    VD_HIP(hipMemcpyAsync(E.d_bin_start.p, E.h_bin_start.data(), (nb + 1) * 4, hipMemcpyHostToDevice, E.stream));
+    if (n) VD_HIP(hipMemcpyAsync(E.d_items.p, E.h_items.data(), n * 4, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(vd::launch_fill_spos(E.d_gen.as<double>(), E.d_items.as<uint32_t>(), n, E.d_desc.as<vd::TessDesc>(),
+                                E.d_gen_tess.as<uint32_t>(), E.d_spos.as<double4>(), E.stream));
+    if (E.ntube)
+        VD_HIP(vd::launch_tube_planes(E.d_gen.as<double>(), n, E.d_desc.as<vd::TessDesc>(),
+                                      E.d_gen_tess.as<uint32_t>(), E.ntube, E.d_tube.as<double>(), E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    E.bins_dirty = false;
+    return VD_OK;
+}
+
 // calculate() of every member (tessellation.rs:213-230).
 int engine_calculate(Engine &E) {
     using clock = std::chrono::steady_clock;
@@ -1195,6 +1332,7 @@ int engine_calculate(Engine &E) {
         E.trunc0 = far;
         VD_TRY(upload_desc(E));
     }
+    if (E.custom_bins && E.bins_dirty) VD_TRY(upload_custom_bins(E));
     // every cell computed within the attempts and rounds; otherwise calculate() fails and
     // leaves no results (cells could be left unfinished, with arena offsets never written)
     bool done = false;
@@ -2033,6 +2171,7 @@ static int tess_add_wall_dev(vd_tess *t, vd::WallDev d, std::vector<double> data
     }
     m.walls.push_back(d);
     m.wdata.push_back(std::move(data));
+    const bool custom = E.custom_bins;
     const int rc = n ? engine_prepare(E) : upload_desc(E);
     if (rc) {
         const std::string msg = g_err;
@@ -2040,8 +2179,12 @@ static int tess_add_wall_dev(vd_tess *t, vd::WallDev d, std::vector<double> data
         m.wdata.pop_back();
         if (host) m.host.pop_back();
         (void)(n ? engine_prepare(E) : upload_desc(E));
+        if (custom && E.ngen == n) E.custom_bins = E.bins_dirty = E.binned = true;
         return fail(rc, msg);
     }
+    // prune_outside_generators (tessellation.rs:178-194) bins afresh only when the wall removed a
+    // generator; otherwise the bins set_generator left stay as they are
+    if (custom && E.ngen == n) E.custom_bins = E.bins_dirty = E.binned = true;
     return VD_OK;
 }
 
@@ -2115,6 +2258,36 @@ int vd_tess_set_generators_device(vd_tess *t, const double *d_xyz, size_t n) {
     E.mem[0].nraw = n;
     engine_layout(E);
     return engine_prepare(E);
+}
+
+int vd_tess_set_generator(vd_tess *t, size_t index, const double xyz[3]) {
+    if (!t || !xyz) return fail(VD_EINVAL, "NULL argument");
+    Engine &E = t->E;
+    if (index >= E.ngen) return fail(VD_ERANGE, "generator index out of range (the reference ignores the call)");
+    Member &m = E.mem[0];
+    // every wall must contain the new position, or nothing changes (tessellation.rs:76-80)
+    for (size_t k = 0; k < m.walls.size(); ++k) {
+        const vd::WallDev &w = m.walls[k];
+        bool in;
+        if (w.kind == vd::kWallHost) {
+            const vd_wall_host &cb = m.host[w.npts];
+            in = cb.contains(cb.ctx, xyz) != 0;
+        } else {
+            vd::WallDev hw = w;
+            hw.pts = m.wdata[k].empty() ? nullptr : m.wdata[k].data();
+            in = vd::wall_contains(hw, xyz[0], xyz[1], xyz[2]);
+        }
+        if (!in) return VD_OK;
+    }
+    if (!E.custom_bins) VD_TRY(host_bins(E));
+    const uint32_t to = host_bin_of(m.grid, xyz), from = E.h_gen_bin[index];
+    if (to != from) host_move(E, (uint32_t)index, from, to);   // algo_3d_grid.rs:136-142
+    VD_HIP(hipMemcpyAsync(E.d_gen.as<double>() + 3 * index, xyz, 24, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    E.custom_bins = E.bins_dirty = E.binned = true;
+    E.calculated = false;
+    if (E.nhost) VD_TRY(host_planes_one(E, index, xyz));   // the caller's walls cut the moved generator anew
+    return VD_OK;
 }
 
 int vd_tess_calculate(vd_tess *t) {

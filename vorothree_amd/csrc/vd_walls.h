@@ -27,15 +27,15 @@ __host__ __device__ inline bool wall_planar(const WallDev &w) {
 }
 
 // Segments of a tube's sample polyline: n in a closed loop, n - 1 open (wall_3d.rs:412, 769, 915).
-__device__ inline uint32_t tube_segments(const WallDev &w) {
+__host__ __device__ inline uint32_t tube_segments(const WallDev &w) {
     if (w.npts == 0) return 0u;
     return w.closed ? w.npts : w.npts - 1u;
 }
 
 // Projection of q on segment i (samples i and (i + 1) % n) and its squared distance
 // (get_closest_point, wall_3d.rs:415-436, 771-795, 917-941).
-__device__ inline double tube_segment(const WallDev &w, uint32_t i, double qx, double qy, double qz, double &px,
-                                      double &py, double &pz) {
+__host__ __device__ inline double tube_segment(const WallDev &w, uint32_t i, double qx, double qy, double qz,
+                                               double &px, double &py, double &pz) {
     const uint32_t j = (i + 1u) % w.npts;
     const double *p0 = w.pts + 3 * (size_t)i, *p1 = w.pts + 3 * (size_t)j;
     const double v0 = p1[0] - p0[0], v1 = p1[1] - p0[1], v2 = p1[2] - p0[2];
@@ -56,8 +56,8 @@ __device__ inline double tube_segment(const WallDev &w, uint32_t i, double qx, d
 }
 
 // get_closest_point, one thread: the first segment with the smallest distance (strict <).
-__device__ inline void tube_closest(const WallDev &w, double qx, double qy, double qz, double &cx, double &cy,
-                                    double &cz) {
+__host__ __device__ inline void tube_closest(const WallDev &w, double qx, double qy, double qz, double &cx,
+                                             double &cy, double &cz) {
     cx = cy = cz = 0.0;
     if (w.npts == 0) return;
     cx = w.pts[0]; cy = w.pts[1]; cz = w.pts[2];
@@ -71,8 +71,8 @@ __device__ inline void tube_closest(const WallDev &w, double qx, double qy, doub
 }
 
 // WallGeometry::contains (wall_3d.rs:31-36, 73-78, 132-143, 202-214, 293-309, 442-446, 691-701,
-// 801-805, 947-951).
-__device__ inline bool wall_contains(const WallDev &w, double x, double y, double z) {
+// 801-805, 947-951). Also evaluated on the host (set_generator), with pts in host memory.
+__host__ __device__ inline bool wall_contains(const WallDev &w, double x, double y, double z) {
     const double dx = x - w.a[0], dy = y - w.a[1], dz = z - w.a[2];
     switch (w.kind) {
     case kWallPlane:

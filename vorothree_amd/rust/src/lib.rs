@@ -10,9 +10,10 @@
 //! Walls: `add_wall` takes any `Wall<3>`. The library prunes the generators with the wall's
 //! `contains` and clips every cell with the planes of its `cut`, both evaluated here on the host
 //! through the wall's public methods (src/wall/mod.rs:34-48) at every `set_generators`, `add_wall`
-//! and `relax`; `is_planar` decides what `calculate_sealed` and `prune_boundaries` treat as
-//! curved. `add_wall_native` takes the built-in geometries in the device's own form instead
-//! (`sys::vd_wall`), which skips the host round trip.
+//! and `relax` (and for the moved generator at `set_generator`); `is_planar` decides what
+//! `calculate_sealed` and `prune_boundaries` treat as curved. `add_wall_native` takes the
+//! built-in geometries in the device's own form instead (`sys::vd_wall`), which skips the host
+//! round trip.
 //!
 //! Differences from the reference, all documented in include/vd.h:
 //! - cells are computed on the device; `get_cell` copies one cell back, `volumes`,
@@ -222,6 +223,17 @@ impl Tessellation<3, Cell3DFaces, Algorithm3DGrid> {
     pub fn set_generators(&mut self, generators: &[f64]) {
         let rc = unsafe { sys::vd_tess_set_generators(self.raw, generators.as_ptr(), generators.len() / 3) };
         check(rc, "set_generators");
+    }
+
+    /// set_generator (src/tessellation.rs:71-92): moves one generator when every wall contains the
+    /// new position; the grid bins follow update_generator (src/algorithm/algo_3d_grid.rs:132-143,
+    /// swap_remove from the old bin, push to the new one). An index past the generators is ignored,
+    /// as in the reference.
+    pub fn set_generator(&mut self, index: usize, generator: &[f64; 3]) {
+        let rc = unsafe { sys::vd_tess_set_generator(self.raw, index, generator.as_ptr()) };
+        if rc != sys::VD_ERANGE {
+            check(rc, "set_generator");
+        }
     }
 
     /// read_generators (src/tessellation.rs:153-171): `id x y z` lines, the id ignored, missing or

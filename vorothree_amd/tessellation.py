@@ -497,6 +497,16 @@ class Tessellation:
         self._host_errors()
         check(rc)
 
+    def set_generator(self, index: int, generator) -> None:
+        """set_generator (src/tessellation.rs:71-92): moves one generator if every wall contains
+        the new position; the bins follow update_generator (swap_remove, push). An index past the
+        generators is ignored, as in the reference."""
+        g = np.ascontiguousarray(np.asarray(generator, dtype=np.float64).reshape(3))
+        rc = lib().vd_tess_set_generator(self._h, int(index), _ptr(g))
+        self._host_errors()
+        if rc != _lib.VD_ERANGE:
+            check(rc)
+
     def add_wall(self, wall: Wall) -> None:
         g = wall.geometry
         if not isinstance(g, _GEOMETRIES):
