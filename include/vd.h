@@ -126,6 +126,16 @@ int vd_tess_set_generators(vd_tess *t, const double *xyz, size_t n);
 /* The same with the points already in device memory (HBM); d_xyz is not retained. */
 int vd_tess_set_generators_device(vd_tess *t, const double *d_xyz, size_t n);
 
+/* random_generators(count), src/tessellation.rs:96-122: candidates uniform in the bounding box,
+ * kept when every wall contains them, until `count` are kept or count * 1000 candidates were drawn;
+ * the kept points become the generators (then as set_generators). The reference seeds its
+ * generator from thread_rng, so its points are not reproducible; here candidate i is drawn from
+ * outputs 3i, 3i + 1, 3i + 2 of the SplitMix64 stream of `seed` (u = (x >> 11) * 2^-53 per axis,
+ * then min + u * (max - min) as in the reference), so equal seeds give equal generators. The
+ * candidates are drawn and filtered on the device in batches; the caller's walls (add_wall_planes_
+ * per_generator) are asked on the host for every candidate of a batch. */
+int vd_tess_random_generators(vd_tess *t, size_t count, uint64_t seed);
+
 /* set_generator(index, &generator), src/tessellation.rs:71-92: moves generator `index` to xyz if
  * every wall contains xyz (otherwise nothing changes and the call returns VD_OK, as the reference
  * ignores it); the bins follow update_generator (src/algorithm/algo_3d_grid.rs:132-143: swap_remove

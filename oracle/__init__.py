@@ -46,6 +46,8 @@ def _load():
         "or_tess_add_wall_host": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _HOST_CONTAINS, _HOST_CUT, ctypes.c_int]),
         "or_tess_set_generators": (None, [_vp, _dp, ctypes.c_size_t]),
         "or_tess_set_generator": (ctypes.c_int, [_vp, ctypes.c_size_t, _dp]),
+        "or_tess_random_generators": (None, [_vp, ctypes.c_size_t, ctypes.c_uint64]),
+        "or_splitmix64_at": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64]),
         "or_tess_count_generators": (ctypes.c_size_t, [_vp]),
         "or_tess_get_generators": (None, [_vp, _dp]),
         "or_tess_get_bins": (None, [_vp, _vp, _vp, _vp]),
@@ -87,6 +89,11 @@ def _d3(x):
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def splitmix64_at(seed: int, k: int) -> int:
+    """Output k of the SplitMix64 stream of seed (the candidates of random_generators)."""
+    return int(_load().or_splitmix64_at(int(seed), int(k)))
 
 
 class OracleCell:
@@ -205,6 +212,10 @@ class OracleTess:
     def set_generators(self, xyz):
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))
         self._lib.or_tess_set_generators(self._h, a.ctypes.data_as(_dp), a.size // 3)
+
+    def random_generators(self, count: int, seed: int) -> None:
+        """random_generators (tessellation.rs:96-122) over the SplitMix64 stream of seed."""
+        self._lib.or_tess_random_generators(self._h, int(count), int(seed))
 
     def set_generator(self, index: int, q) -> int:
         """set_generator (tessellation.rs:71-92): 0 moved, 1 index past the generators, 2 a wall

@@ -703,6 +703,47 @@ void tess_set_generators(Tess &T, const double *xyz, size_t n) {  // tessellatio
     T.computed.clear();
 }
 
+// Output k of the SplitMix64 stream of `seed` (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014): the state advances by 0x9e3779b97f4a7c15 per
+// output and each output is the finaliser of the new state.
+uint64_t splitmix64_at(uint64_t seed, uint64_t k) {
+    uint64_t z = seed + (k + 1) * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// random_generators, tessellation.rs:96-122, one candidate at a time: the reference draws
+// rng.gen::<f64>() per axis from a StdRng seeded by thread_rng; here the doubles are
+// (x >> 11) * 2^-53 of consecutive SplitMix64 outputs of `seed`, three per candidate.
+void tess_random_generators(Tess &T, size_t count, uint64_t seed) {
+    std::vector<double> points;
+    points.reserve(3 * count);
+    size_t found = 0;
+    const size_t max_attempts = count * 1000;                                // :101
+    size_t attempts = 0;
+    uint64_t k = 0;
+    while (found < count && attempts < max_attempts) {                       // :104
+        attempts += 1;
+        double point[3];
+        for (int i = 0; i < 3; ++i) {                                        // :107-111
+            const double u = (double)(splitmix64_at(seed, k++) >> 11) * 0x1.0p-53;
+            point[i] = T.lo[i] + u * (T.hi[i] - T.lo[i]);
+        }
+        bool in = true;                                                      // :113
+        for (const Wall &w : T.walls)
+            if (!wall_contains(w, point)) { in = false; break; }
+        if (in) {
+            points.insert(points.end(), point, point + 3);
+            found += 1;
+        }
+    }
+    T.gen.swap(points);                                                      // :119
+    grid_set_generators(T.grid, T.gen);                                      // :120
+    T.cells.clear();
+    T.computed.clear();
+}
+
 void tess_add_wall(Tess &T, const Wall &w) {  // tessellation.rs:175-200
     T.walls.push_back(w);
     std::vector<double> keep;
@@ -909,6 +950,12 @@ int or_tess_add_wall_host(void *t, int32_t id, void *ctx, HostContains contains,
 void or_tess_set_generators(void *t, const double *xyz, size_t n) {
     tess_set_generators(*(Tess *)t, xyz, n);
 }
+
+void or_tess_random_generators(void *t, size_t count, uint64_t seed) {
+    tess_random_generators(*(Tess *)t, count, seed);
+}
+
+uint64_t or_splitmix64_at(uint64_t seed, uint64_t k) { return splitmix64_at(seed, k); }
 
 int or_tess_set_generator(void *t, size_t index, const double q[3]) {
     return tess_set_generator(*(Tess *)t, index, q);

@@ -6,8 +6,8 @@
 //                         and where no GPU is visible, handle creation failing (its cleanup)
 //   host_check --device   full handle lifecycles on the current GPU: a tessellation with built-in
 //                         and caller-evaluated walls through calculate, sealed calculate, prune,
-//                         relax, single-generator moves and every getter; a batch; a server with
-//                         tessellation and Lloyd jobs; the static entry twice
+//                         relax, single-generator moves, random generators and every getter; a
+//                         batch; a server with tessellation and Lloyd jobs; the static entry twice
 //
 // Every call's return code is checked; any sanitizer report or failed check ends the program
 // with a non-zero status.
@@ -216,6 +216,11 @@ void tess_checks() {
     CHECK(vd_tess_set_generator(t, 0, nullptr) == VD_EINVAL);
     OK(vd_tess_calculate(t));
     OK(vd_tess_set_generator(t, 1, c));
+    OK(vd_tess_calculate(t));
+    OK(vd_tess_random_generators(t, 5000, 12345));   // a few device batches through both walls
+    CHECK(vd_tess_count_generators(t) == 5000);
+    OK(vd_tess_calculate(t));
+    OK(vd_tess_set_generators(t, p.data(), n));
     OK(vd_tess_calculate(t));
     int picks[4];
     OK(vd_tess_autotune_launch(t, picks));

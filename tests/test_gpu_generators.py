@@ -1,4 +1,5 @@
-"""set_generator (src/tessellation.rs:71-92) on the device against the oracle, bit for bit.
+"""set_generator (src/tessellation.rs:71-92) and random_generators (:96-122) on the device against
+the oracle, bit for bit.
 
 The bins after a move follow update_generator (src/algorithm/algo_3d_grid.rs:132-143): the old
 bin loses the generator by swap_remove and the new bin pushes it. The order of a bin is the order
@@ -184,3 +185,62 @@ def test_repeated_calculates_between_moves(graphs):
     compare(g, o, cells=False)
     if graphs:
         assert g.stats()[8] >= 1
+
+
+# --- random_generators ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["none", "sphere", "trefoil", "host", "mixed"])
+def test_random_generators_equal_oracle(name):
+    # the kept candidates of each batch are the oracle's, in stream order, then a full calculate
+    from host_geometries import PyNearFacets
+    lo, hi = (-1.0, 0.0, 2.0), (1.0, 3.0, 3.5)
+    b = BoundingBox(lo, hi)
+    g = Tessellation(b, Algorithm3DGrid(9, 11, 7, b))
+    o = OracleTess(lo, hi, 9, 11, 7)
+    c = (0.0, 1.5, 2.75)
+    walls, host = [], []
+    if name in ("sphere", "mixed"):
+        walls.append(Wall(-1000, SphereGeometry(c, 0.74)))
+    if name in ("trefoil", "mixed"):
+        walls.append(Wall(-1001, TrefoilKnotGeometry(c, 0.2, 0.09, 100)))
+    if name in ("host", "mixed"):
+        host.append((-1002, PyNearFacets(c, 0.9)))
+    for w in walls:
+        g.add_wall(w)
+        oracle_wall(o, w)
+    for wid, geometry in host:
+        g.add_wall(Wall(wid, geometry))
+        o.add_wall_host(wid, geometry)
+    count = 20000 if name != "mixed" else 3000
+    g.random_generators(count, seed=4242 + len(name))
+    o.random_generators(count, 4242 + len(name))
+    assert g.count_generators() == o.count_generators() == count
+    compare(g, o, cells=False)
+
+
+def test_random_generators_run_out_of_candidates_like_the_oracle():
+    # a wall that keeps about one candidate in 4,000 (an eighth of a ball of radius 0.0778): the
+    # count * 1000 = 40,000 candidates give fewer than count points, drawn over several batches
+    g, o = make((3, 3, 3), [Wall(-1000, SphereGeometry((0.0, 0.0, 0.0), 0.0778))])
+    g.random_generators(40, seed=77)
+    o.random_generators(40, 77)
+    n = o.count_generators()
+    assert 0 < n < 40
+    assert np.array_equal(bits(g.generators()), bits(o.generators()))
+    compare(g, o)
+    g.random_generators(0, seed=1)
+    assert g.count_generators() == 0
+
+
+def test_random_generators_unseeded_differ_and_stay_inside():
+    g, _ = make((5, 5, 5), [Wall(-1000, SphereGeometry((0.5, 0.5, 0.5), 0.5))], keep_cells=False)
+    g.random_generators(5000)
+    a = g.generators().reshape(-1, 3)
+    g.random_generators(5000)
+    b = g.generators().reshape(-1, 3)
+    assert len(a) == len(b) == 5000 and not np.array_equal(a, b)
+    for p in (a, b):
+        assert np.all(((p - 0.5) ** 2).sum(axis=1) <= 0.25 + 1e-15)
+    g.calculate()
+    # sphere walls clip with tangent planes: the cells cover the ball and a thin shell beyond it
+    assert np.pi / 6 < g.volumes().sum() < 1.01 * np.pi / 6

@@ -410,3 +410,68 @@ def test_set_generator_cells_match_brute_force_and_a_fresh_set():
         for i in range(a.count_cells()):
             assert sorted(a.get_cell(i).face_neighbors) == sorted(t.get_cell(i).face_neighbors), i
     assert abs(va.sum() - 1.0) < 1e-12
+
+
+# --- random_generators ----------------------------------------------------------------------
+
+def test_splitmix64_known_answers():
+    """The first outputs of SplitMix64 seeded with 1234567, as published with Vigna's reference
+    implementation (splitmix64.c); seed 0's first output is the finaliser of the golden gamma."""
+    from oracle import splitmix64_at
+    assert [splitmix64_at(1234567, k) for k in range(5)] == [
+        6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+        16408922859458223821]
+    assert splitmix64_at(0, 0) == 16294208416658607535
+
+
+def _stream_points(seed, n, lo, hi):
+    """Candidates by the published SplitMix64 recurrence (state += gamma, then the finaliser),
+    stepped here as a sequential generator rather than indexed, in exact integer arithmetic."""
+    mask = (1 << 64) - 1
+    state = seed
+    out = np.empty((n, 3))
+    for i in range(n):
+        for a in range(3):
+            state = (state + 0x9E3779B97F4A7C15) & mask
+            z = state
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+            z ^= z >> 31
+            out[i, a] = lo[a] + ((z >> 11) * 2.0 ** -53) * (hi[a] - lo[a])
+    return out
+
+
+@pytest.mark.parametrize("seed,count", [(7, 300), (2 ** 63 + 11, 50)])
+def test_random_generators_are_the_first_kept_candidates(seed, count):
+    """tessellation.rs:96-122: candidates in stream order, kept when every wall contains them
+    (checked here with the sphere and plane inequalities directly), until count are kept."""
+    lo, hi = (1.0, -2.0, 0.5), (4.0, 1.0, 2.5)
+    t = OracleTess(lo, hi, 4, 4, 4)
+    t.add_wall_sphere(-1000, (2.5, -0.5, 1.5), 1.2)
+    t.add_wall_plane(-1001, (2.5, -0.5, 1.5), (0.3, -1.0, 0.2))
+    t.random_generators(count, seed)
+    cand = _stream_points(seed, 40 * count, lo, hi)
+    c = np.array([2.5, -0.5, 1.5])
+    d = cand - c
+    keep = (np.einsum("ij,ij->i", d, d) <= 1.2 * 1.2) & (d @ np.array([0.3, -1.0, 0.2]) >= 0.0)
+    want = cand[keep][:count]
+    assert len(want) == count
+    assert np.array_equal(t.generators().reshape(-1, 3), want)
+    assert t.count_generators() == count
+
+
+def test_random_generators_stop_after_count_times_1000_candidates():
+    """max_attempts = count * 1000 (tessellation.rs:101): a wall that keeps about one candidate
+    in 5,000 leaves fewer than count points; with no walls every candidate is kept."""
+    t = OracleTess((0, 0, 0), (1, 1, 1), 2, 2, 2)
+    t.add_wall_sphere(-1000, (0.0, 0.0, 0.0), 0.1)   # an eighth of a ball: 5.2e-4 of the box
+    t.random_generators(4, 99)
+    n = t.count_generators()
+    cand = _stream_points(99, 4000, (0, 0, 0), (1, 1, 1))
+    assert n == int((np.einsum("ij,ij->i", cand, cand) <= 0.01).sum()) < 4
+    u = OracleTess((0, 0, 0), (1, 1, 1), 2, 2, 2)
+    u.random_generators(1000, 5)
+    assert np.array_equal(u.generators().reshape(-1, 3), _stream_points(5, 1000, (0, 0, 0), (1, 1, 1)))
+    z = OracleTess((0, 0, 0), (1, 1, 1), 2, 2, 2)
+    z.random_generators(0, 5)
+    assert z.count_generators() == 0

@@ -78,6 +78,7 @@ SIGNATURES = {
     "vd_tess_set_generators": (ctypes.c_int, [_vp, _vp, _sz]),
     "vd_tess_set_generators_device": (ctypes.c_int, [_vp, _vp, _sz]),
     "vd_tess_set_generator": (ctypes.c_int, [_vp, _sz, _vp]),
+    "vd_tess_random_generators": (ctypes.c_int, [_vp, _sz, ctypes.c_uint64]),
     "vd_tess_calculate": (ctypes.c_int, [_vp]),
     "vd_tess_relax": (ctypes.c_int, [_vp]),
     "vd_tess_calculate_sealed": (ctypes.c_int, [_vp]),

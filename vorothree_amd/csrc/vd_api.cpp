@@ -854,9 +854,9 @@ int host_planes_one(Engine &E, uint64_t g, const double q[3]) {
     return VD_OK;
 }
 
-// set_generators for every member from E.d_raw (tessellation.rs:43-67): wall filter in input order,
-// compaction, then the stable counting-sort binning (algo_3d_grid.rs:111-130).
-int engine_prepare(Engine &E) {
+// The wall filter of set_generators for every member from E.d_raw (tessellation.rs:43-67): the
+// points every wall of their member contains, compacted in input order into E.d_gen.
+int engine_filter(Engine &E) {
     E.prepared = E.calculated = false;
     E.custom_bins = E.bins_dirty = false;
     const size_t M = E.mem.size();
@@ -930,6 +930,13 @@ int engine_prepare(Engine &E) {
         E.ngen = kept[M];
         VD_TRY(upload_desc(E));
     }
+    return VD_OK;
+}
+
+// set_generators for every member from E.d_raw (tessellation.rs:43-67): wall filter in input order,
+// compaction, then the stable counting-sort binning (algo_3d_grid.rs:111-130).
+int engine_prepare(Engine &E) {
+    VD_TRY(engine_filter(E));
     // the caller's walls: cut planes of every kept generator, as CSR over (generator, wall)
     if (E.nhost) VD_TRY(host_planes(E));
     // buffers of the binning (it runs at the start of calculate, see enqueue_binning)
@@ -2256,6 +2263,51 @@ int vd_tess_set_generators_device(vd_tess *t, const double *d_xyz, size_t n) {
     VD_TRY(ensure(E.d_raw, (n + 1) * 24));
     if (n) VD_HIP(hipMemcpyAsync(E.d_raw.p, d_xyz, n * 24, hipMemcpyDeviceToDevice, E.stream));
     E.mem[0].nraw = n;
+    engine_layout(E);
+    return engine_prepare(E);
+}
+
+// random_generators (tessellation.rs:96-122): candidates in batches on the device, each batch
+// through the wall filter of set_generators; the first `count` kept candidates in stream order
+// among the first count * 1000 become the generators.
+int vd_tess_random_generators(vd_tess *t, size_t count, uint64_t seed) {
+    if (!t) return fail(VD_EINVAL, "NULL handle");
+    if (count >= 0x7fffffffull) return fail(VD_EINVAL, "too many generators");
+    Engine &E = t->E;
+    Member &m = E.mem[0];
+    const uint64_t limit = (uint64_t)count * 1000;   // max_attempts (:101)
+    constexpr uint64_t kMaxBatch = uint64_t(1) << 24;
+    struct Acc {
+        DevBuf b;
+        ~Acc() { release(b); }
+    } acc;
+    VD_TRY(ensure(acc.b, ((uint64_t)count + 1) * 24));
+    uint64_t found = 0, tried = 0;
+    uint64_t batch = std::min<uint64_t>(std::max<uint64_t>(count + count / 4, 4096), kMaxBatch);
+    while (found < count && tried < limit) {
+        const uint64_t b = std::min(batch, limit - tried);
+        VD_TRY(ensure(E.d_raw, (b + 1) * 24));
+        VD_HIP(vd::launch_random_points(seed, tried, b, m.lo, m.hi, E.d_raw.as<double>(), E.stream));
+        m.nraw = b;
+        engine_layout(E);
+        VD_TRY(engine_filter(E));
+        const uint64_t take = std::min<uint64_t>(E.ngen, count - found);
+        if (take)
+            VD_HIP(hipMemcpyAsync(acc.b.as<double>() + 3 * found, E.d_gen.p, take * 24, hipMemcpyDeviceToDevice,
+                                  E.stream));
+        found += take;
+        tried += b;
+        // the next batch from the share kept so far, with a quarter more for its spread
+        const double rate = tried ? (double)std::max<uint64_t>(found, 1) / (double)tried : 1.0;
+        const double want = (double)(count - found) / rate * 1.25 + 4096.0;
+        batch = (uint64_t)std::min<double>(want, (double)kMaxBatch);
+    }
+    VD_HIP(hipStreamSynchronize(E.stream));
+    // the kept points become the generators; every wall contains them, so the filter keeps all
+    VD_TRY(ensure(E.d_raw, (found + 1) * 24));
+    if (found)
+        VD_HIP(hipMemcpyAsync(E.d_raw.p, acc.b.p, found * 24, hipMemcpyDeviceToDevice, E.stream));
+    m.nraw = found;
     engine_layout(E);
     return engine_prepare(E);
 }

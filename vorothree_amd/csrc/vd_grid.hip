@@ -299,6 +299,30 @@ __global__ void k_relax(const double *__restrict__ gen, const double *__restrict
     out[3 * i] = src[3 * i]; out[3 * i + 1] = src[3 * i + 1]; out[3 * i + 2] = src[3 * i + 2];
 }
 
+// Candidate points of random_generators (tessellation.rs:96-122): point i takes outputs 3i, 3i + 1
+// and 3i + 2 of the SplitMix64 stream of `seed` (Steele, Lea and Flood 2014: output k is the
+// finaliser of seed + (k + 1) * 0x9e3779b97f4a7c15), each made a double in [0, 1) from its top 53
+// bits and mapped to min + u * (max - min) per axis, the reference's expression.
+__device__ inline uint64_t splitmix64_at(uint64_t seed, uint64_t k) {
+    uint64_t z = seed + (k + 1) * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_random_points(uint64_t seed, uint64_t start, uint64_t n, double lox, double loy, double loz,
+                                double hix, double hiy, double hiz, double *__restrict__ out) {
+    const uint64_t i = gid();
+    if (i >= n) return;
+    const uint64_t k = 3 * (start + i);
+    const double lo[3] = {lox, loy, loz}, hi[3] = {hix, hiy, hiz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double u = (double)(splitmix64_at(seed, k + a) >> 11) * 0x1.0p-53;
+        out[3 * i + a] = lo[a] + u * (hi[a] - lo[a]);
+    }
+}
+
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
     const uint64_t i = gid();
     if (i < n) p[i] = v;
@@ -473,6 +497,13 @@ hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n
                             const uint32_t *tess, double4 *spos, hipStream_t st) {
     if (n == 0) return hipSuccess;
     k_fill_spos<<<blocks_for(n), kBlock, 0, st>>>(xyz, items, n, desc, tess, spos);
+    return hipGetLastError();
+}
+
+hipError_t launch_random_points(uint64_t seed, uint64_t start, uint64_t n, const double lo[3], const double hi[3],
+                               double *out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_random_points<<<blocks_for(n), kBlock, 0, st>>>(seed, start, n, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], out);
     return hipGetLastError();
 }
 

@@ -198,6 +198,8 @@ hipError_t launch_fill_spos(const double *xyz, const uint32_t *items, uint64_t n
                             const uint32_t *tess, double4 *spos, hipStream_t st);
 hipError_t launch_tube_planes(const double *gen, uint64_t n, const TessDesc *desc, const uint32_t *tess,
                               uint32_t ntube, double *out, hipStream_t st);   // 7 doubles per slot
+hipError_t launch_random_points(uint64_t seed, uint64_t start, uint64_t n, const double lo[3], const double hi[3],
+                               double *out, hipStream_t st);   // n points, 24 B each
 // Post-pass topology of the cells of the last launch (tessellation.rs:244-264, 330-350): per cell
 // the wall mask of its curved-wall set and its count of neighbour faces (0 <= id < member count).
 // cnt[i] is 0 for a cell whose mask is empty.

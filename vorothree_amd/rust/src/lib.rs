@@ -225,6 +225,22 @@ impl Tessellation<3, Cell3DFaces, Algorithm3DGrid> {
         check(rc, "set_generators");
     }
 
+    /// random_generators (src/tessellation.rs:96-122): `count` points uniform in the bounding box
+    /// that every wall contains. Seeded per call from the standard library's per-process random
+    /// hasher keys, as the reference seeds from thread_rng; `random_generators_seeded` repeats.
+    pub fn random_generators(&mut self, count: usize) {
+        use std::hash::{BuildHasher, Hasher};
+        let mut h = std::collections::hash_map::RandomState::new().build_hasher();
+        h.write_usize(count);
+        self.random_generators_seeded(count, h.finish());
+    }
+
+    /// random_generators with the candidates drawn from the SplitMix64 stream of `seed`.
+    pub fn random_generators_seeded(&mut self, count: usize, seed: u64) {
+        let rc = unsafe { sys::vd_tess_random_generators(self.raw, count, seed) };
+        check(rc, "random_generators");
+    }
+
     /// set_generator (src/tessellation.rs:71-92): moves one generator when every wall contains the
     /// new position; the grid bins follow update_generator (src/algorithm/algo_3d_grid.rs:132-143,
     /// swap_remove from the old bin, push to the new one). An index past the generators is ignored,

@@ -73,6 +73,7 @@ unsafe extern "C" {
     pub fn vd_tess_set_generators(t: *mut vd_tess, xyz: *const f64, n: usize) -> c_int;
     pub fn vd_tess_set_generators_device(t: *mut vd_tess, d_xyz: *const f64, n: usize) -> c_int;
     pub fn vd_tess_set_generator(t: *mut vd_tess, index: usize, xyz: *const f64) -> c_int;
+    pub fn vd_tess_random_generators(t: *mut vd_tess, count: usize, seed: u64) -> c_int;
     pub fn vd_tess_calculate(t: *mut vd_tess) -> c_int;
     pub fn vd_tess_relax(t: *mut vd_tess) -> c_int;
     pub fn vd_tess_calculate_sealed(t: *mut vd_tess) -> c_int;

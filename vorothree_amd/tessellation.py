@@ -20,6 +20,8 @@ from __future__ import annotations
 
 import ctypes
 import math
+import secrets
+from typing import Optional
 
 import numpy as np
 
@@ -494,6 +496,17 @@ class Tessellation:
     def set_generators_device(self, ptr: int, n: int) -> None:
         """Points already in device memory (e.g. a torch CUDA tensor's data_ptr())."""
         rc = lib().vd_tess_set_generators_device(self._h, ctypes.c_void_p(int(ptr)), int(n))
+        self._host_errors()
+        check(rc)
+
+    def random_generators(self, count: int, seed: Optional[int] = None) -> None:
+        """random_generators (src/tessellation.rs:96-122): `count` points uniform in the bounding
+        box that every wall contains (fewer if count * 1000 candidates do not yield them). Without
+        a seed, one is drawn from the operating system, as the reference seeds from thread_rng;
+        with one, the candidates are the SplitMix64 stream of that seed (vd_tess_random_generators)."""
+        if seed is None:
+            seed = secrets.randbits(64)
+        rc = lib().vd_tess_random_generators(self._h, int(count), int(seed) & 0xFFFFFFFFFFFFFFFF)
         self._host_errors()
         check(rc)
 
