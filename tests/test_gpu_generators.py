@@ -244,3 +244,22 @@ def test_random_generators_unseeded_differ_and_stay_inside():
     g.calculate()
     # sphere walls clip with tangent planes: the cells cover the ball and a thin shell beyond it
     assert np.pi / 6 < g.volumes().sum() < 1.01 * np.pi / 6
+
+
+def test_import_generators_reads_the_voropp_input():
+    # tests/comparisons.rs:109-140: import_generators of the Voro++ input behind a sphere wall
+    from conftest import GOLDEN, read_generators
+    path = f"{GOLDEN}/voropp/vertex_positions/generators.txt"
+    b = BoundingBox((-10.0,) * 3, (10.0,) * 3)
+    g = Tessellation(b, Algorithm3DGrid(6, 6, 6, b))
+    o = OracleTess((-10.0,) * 3, (10.0,) * 3, 6, 6, 6)
+    w = Wall(-1000, SphereGeometry((0.0, 0.0, 0.0), 8.0))
+    g.add_wall(w)
+    oracle_wall(o, w)
+    g.import_generators(path)
+    o.set_generators(read_generators(path))
+    assert g.count_generators() == o.count_generators() > 0
+    compare(g, o)
+    with open(path) as fh:
+        g.read_generators(fh.read())
+    compare(g, o)

@@ -25,5 +25,6 @@ from .tessellation import (  # noqa: F401
     calculate_static,
     grid_for,
     grid_for_points,
+    parse_generators,
     shell_prefix,
 )

@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import re
 import secrets
 from typing import Optional
 
@@ -305,6 +306,34 @@ _GEOMETRIES = (PlaneGeometry, SphereGeometry, CylinderGeometry, ConeGeometry, To
                ConvexPolyhedronGeometry, _Tube)
 
 
+# White_Space characters of Unicode, which Rust's str::split_whitespace and str::trim split on (no
+# \x1c-\x1f, which Python's str.split also takes)
+_RUST_WS = re.compile("[\t\n\x0b\x0c\r \x85\xa0\u1680\u2000-\u200a\u2028\u2029\u202f\u205f\u3000]+")
+_RUST_F64 = re.compile(r"[+-]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf|infinity|nan)", re.IGNORECASE)
+
+
+def _rust_f64(tok: str) -> float:
+    """str::parse::<f64> or 0.0 where it fails (tessellation.rs:163: .ok().unwrap_or(0.0)):
+    decimal and exponent forms, inf, infinity and nan in any case, with an optional sign; no
+    underscores or hexadecimal, which Python's float() would take or reject differently."""
+    return float(tok) if _RUST_F64.fullmatch(tok) else 0.0
+
+
+def parse_generators(text: str) -> np.ndarray:
+    """read_generators' parser (src/tessellation.rs:153-171): one point per line (lines split at
+    \n, a trailing \r dropped, as str::lines does) that is not blank; its first token, the id, is
+    skipped and the next three are x, y and z, 0.0 where missing or unparsable. Returns (n, 3)."""
+    pts = []
+    for line in text.split("\n"):
+        if line.endswith("\r"):
+            line = line[:-1]
+        parts = [p for p in _RUST_WS.split(line) if p]
+        if not parts:
+            continue
+        pts.append([_rust_f64(parts[k]) if k < len(parts) else 0.0 for k in (1, 2, 3)])
+    return np.asarray(pts, dtype=np.float64).reshape(-1, 3)
+
+
 def _is_host_geometry(g) -> bool:
     """A geometry of the caller's (the WallGeometry trait, src/wall/mod.rs:53-71): ``contains(p)``
     and ``cut(g)`` returning the (point, normal) planes for generator g, normals pointing out of
@@ -498,6 +527,17 @@ class Tessellation:
         rc = lib().vd_tess_set_generators_device(self._h, ctypes.c_void_p(int(ptr)), int(n))
         self._host_errors()
         check(rc)
+
+    def read_generators(self, text: str) -> None:
+        """read_generators (src/tessellation.rs:153-171): `id x y z` lines (parse_generators),
+        then set_generators."""
+        self.set_generators(parse_generators(text))
+
+    def import_generators(self, path) -> None:
+        """import_generators (src/tessellation.rs:126-148): read_generators of a UTF-8 text file;
+        an unreadable file raises OSError (the reference returns its io::Error)."""
+        with open(path, "r", encoding="utf-8", newline="") as fh:
+            self.read_generators(fh.read())
 
     def random_generators(self, count: int, seed: Optional[int] = None) -> None:
         """random_generators (src/tessellation.rs:96-122): `count` points uniform in the bounding
