@@ -117,3 +117,59 @@ def test_full_config4_batch_members_bit_exact():
                 and np.array_equal(goff.astype(np.uint64), ooff) and np.array_equal(gids, oids)):
             bad.append(t)
     assert not bad, f"members differing from the oracle: {bad[:20]}"
+
+
+def test_scale_100m_uniform_one_tessellation():
+    # one tessellation of 100M uniform generators with full cells kept (about 165 GB of device
+    # memory: the vertex arena alone holds 4.0e9 entries, 96 GB): entry offsets past 2^31 and
+    # byte offsets past 2^32 in every arena. The generators come from random_generators on the
+    # device; the oracle computes a sample of cells spread over the index range (the last ones
+    # included) from the same points.
+    from vorothree_amd import grid_for
+    n = 100_000_000
+    b = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    grid = grid_for(b, n, 1.0)
+    g = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=True)
+    g.random_generators(n, seed=2026)
+    assert g.count_generators() == n
+    g.calculate()
+    assert g.count_cells() == n
+    gv = g.volumes()
+    # the fan triangulation is origin-relative (cell_3d_faces.rs:106-143): a cell of volume 1e-8
+    # near (1, 1, 1) sums determinants of order 1 and keeps about 1e-15 of rounding, so over 1e8
+    # cells the sum is 1 to about 1e-6 (it is 1 + 4.3e-7 here; the sample below is bit-exact)
+    assert abs(gv.sum() - 1.0) < 2e-6 and gv.min() > 0.0
+    assert g.face_counts().min() >= 4
+    pts = g.generators()
+    o = OracleTess((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), *grid)
+    o.set_generators(pts)
+    del pts
+    rng = np.random.default_rng(11)
+    idx = np.unique(np.concatenate([rng.choice(n, 3000, replace=False), np.arange(n - 50, n),
+                                    np.arange(0, 50), np.arange(n // 2 - 25, n // 2 + 25)]))
+    o.calculate(subset=idx, threads=THREADS)
+    vol, cen, _, _ = o.reduced(idx)
+    assert np.array_equal(bits(gv[idx]), bits(vol))
+    assert np.array_equal(bits(g.centroids()[idx]), bits(cen))
+    for i in idx:
+        assert_same_cell(g.get_cell(int(i)), o.get_cell(int(i)), int(i))
+    # reciprocity across the sample. The reference's clip keeps vertices within 1e-9 of a plane
+    # (cell_3d_faces.rs:303-326), so near-cospherical generators can leave a sliver face on one
+    # side that the other side's clip does not make: such one-sided pairs must be rare, their face
+    # must be a sliver, and the neighbour's cell must still be the oracle's bit for bit
+    pairs, one_sided = 0, []
+    for i in idx[::5]:
+        ci = g.get_cell(int(i))
+        for k, j in enumerate(ci.face_neighbors()):
+            if j >= 0:
+                pairs += 1
+                if int(i) not in set(g.get_cell(int(j)).face_neighbors()):
+                    one_sided.append((int(i), int(j), ci.face_area(k), ci.volume()))
+    assert len(one_sided) <= max(1, pairs // 1000), one_sided[:10]
+    if one_sided:
+        js = np.array(sorted({j for _, j, _, _ in one_sided}), dtype=np.int64)
+        o.calculate(subset=js, threads=THREADS)
+        for j in js:
+            assert_same_cell(g.get_cell(int(j)), o.get_cell(int(j)), int(j))
+        for i, j, area, v in one_sided:
+            assert area < 1e-6 * v ** (2.0 / 3.0), (i, j, area, v)
