@@ -55,12 +55,14 @@ def parse():
     p.add_argument("--serve-sweep-requests", type=int, default=30)
     p.add_argument("--serve-sweep-rates", type=str, default="5,50,500")
     p.add_argument("--no-clustered", action="store_true", help="skip the 4M clustered workload")
-    p.add_argument("--only", choices=["batched", "clustered", "serve", "cpu"], default=None,
+    p.add_argument("--only", choices=["batched", "clustered", "serve", "cpu", "scale"], default=None,
                    help="run one secondary workload alone and print its JSON object (for profiling)")
     p.add_argument("--no-plain", action="store_true",
                    help="--only batched / --only serve: skip the plain comparison legs")
     p.add_argument("--no-launch-tune", action="store_true",
                    help="keep the default launch shape instead of vd_*_autotune_launch's pick")
+    p.add_argument("--no-scale", action="store_true", help="skip the 100M-generator tessellation")
+    p.add_argument("--scale-n", type=int, default=100_000_000)
     p.add_argument("--clustered-n", type=int, default=4_000_000)
     p.add_argument("--clustered-targets", type=str, default="0.25,0.35,0.7",
                    help="generators-per-bin targets of the density rule tried for the clustered workload")
@@ -261,6 +263,45 @@ def clustered(n: int, targets, steps: int, launch_tune: bool = True) -> dict:
     return out
 
 
+def scale(n: int, steps: int) -> dict:
+    """One tessellation at HBM scale: n uniform generators drawn on the device by
+    random_generators (seed 2026) in the unit cube, no walls, a grid of about one generator per
+    bin, full cells kept. One calculate() allocates the arenas, then ``steps`` are timed; the
+    device memory the handle holds is the drop in free device memory."""
+    from vorothree_amd import Algorithm3DGrid, BoundingBox, Tessellation, grid_for, _hip
+
+    b = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    grid = grid_for(b, n, 1.0)
+    _hip.synchronize()
+    free0 = _hip.mem_free()
+    t = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=True, graphs=False)
+    c0 = time.perf_counter()
+    t.random_generators(n, seed=2026)
+    _hip.synchronize()
+    gen_s = time.perf_counter() - c0
+    c0 = time.perf_counter()
+    t.calculate()
+    _hip.synchronize()
+    first_s = time.perf_counter() - c0
+    c0 = time.perf_counter()
+    for _ in range(steps):
+        t.calculate()
+    _hip.synchronize()
+    dt = (time.perf_counter() - c0) / steps
+    st = t.stats()
+    cells = t.count_cells()
+    held = free0 - _hip.mem_free()
+    out = {"workload": f"unit cube, {n} uniform f64 generators from random_generators (seed 2026), no walls, "
+                       f"grid {grid[0]}^3, full cells kept",
+           "generators": n, "cells": cells, "grid": list(grid), "steps": steps,
+           "ms_per_step": dt * 1e3, "cells_per_s": cells / dt, "first_calculate_ms": first_s * 1e3,
+           "random_generators_ms": gen_s * 1e3, "device_bytes_held": held,
+           "arena_vertices": st[6], "arena_face_indices": st[7],
+           "volume_sum": float(np.sum(t.volumes()))}
+    t.close()
+    return out
+
+
 BATCH_TUNE = (0.25, 0.35, 0.5, 1.0, 2.0)
 
 
@@ -452,6 +493,9 @@ def main():
     if a.only == "clustered":
         print(json.dumps(clustered(a.clustered_n, targets_of(a), 3, not a.no_launch_tune)), flush=True)
         return
+    if a.only == "scale":
+        print(json.dumps(scale(a.scale_n, 2)), flush=True)
+        return
     if a.only == "serve":
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
         print(json.dumps(served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates,
@@ -629,6 +673,9 @@ def main():
     if not a.no_clustered:
         progress(f"clustered: {a.clustered_n} points")
         out["clustered"] = clustered(a.clustered_n, targets_of(a), 3, not a.no_launch_tune)
+    if not a.no_scale:
+        progress(f"scale: {a.scale_n} points")
+        out["scale"] = scale(a.scale_n, 2)
     if not a.no_serve:
         rates = [float(x) for x in a.serve_sweep_rates.split(",") if x]
         out["serve"] = served(a.serve_requests, a.serve_rate, a.serve_budget_gb, a.serve_sweep_requests, rates)
