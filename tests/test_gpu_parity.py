@@ -441,3 +441,41 @@ def test_face_of_256_vertices_is_an_error():
         t.calculate()
     assert e.value.code == VD_ECAPACITY and "256" in str(e.value)
     assert t.count_cells() == 0
+
+
+def test_bin_boundaries_and_box_faces():
+    # coordinates exactly on bin boundaries and on the box faces (x = max maps to the last bin
+    # through grid_limit = n - 1e-5, algo_3d_grid.rs:85-87,98-107), with random points around them
+    n = 6
+    k = np.arange(n + 1) / n
+    edge = np.stack(np.meshgrid(k, k[::2], k[1::2], indexing="ij"), axis=-1).reshape(-1, 3)
+    pts = np.concatenate([edge, W.PCG32(1501).uniform(400, (0, 0, 0), (1, 1, 1))])
+    g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (n, n, n), pts)
+    vol = check_all_cells(g, o)
+    assert abs(vol.sum() - 1.0) < 1e-12
+
+
+def test_box_far_from_the_origin():
+    # the volume and centroid fans are origin-relative (cell_3d_faces.rs:106-205): a unit box at
+    # 1e6 cancels most of each determinant's digits, the same way on both sides
+    lo = (1e6, -2e6, 5e5)
+    hi = (1e6 + 1.0, -2e6 + 1.0, 5e5 + 1.0)
+    pts = W.PCG32(1502).uniform(2000, lo, hi)
+    g, o = build_pair(lo, hi, (12, 12, 12), pts, [("sphere", -1000, (1e6 + 0.5, -2e6 + 0.5, 5e5 + 0.5), 0.5)])
+    check_all_cells(g, o)
+
+
+@pytest.mark.parametrize("shape", ["plane", "line"])
+def test_coplanar_and_collinear_generators(shape):
+    # every cell spans the box across the degenerate directions: two large faces per cell on the
+    # box sides, and ties in every bisector distance across the plane or along the line
+    rng = W.PCG32(1503 if shape == "plane" else 1504)
+    if shape == "plane":
+        xy = rng.uniform(500, (0, 0, 0), (1, 1, 1))
+        pts = np.stack([xy[:, 0], xy[:, 1], np.full(500, 0.5)], axis=1)
+    else:
+        t = rng.uniform(200, (0, 0, 0), (1, 1, 1))[:, 0]
+        pts = np.stack([t, 0.3 + 0.4 * t, np.full(200, 0.25)], axis=1)
+    g, o = build_pair((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (8, 8, 8), pts)
+    vol = check_all_cells(g, o)
+    assert abs(vol.sum() - 1.0) < 1e-12
