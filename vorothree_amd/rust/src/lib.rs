@@ -136,6 +136,43 @@ impl Cell3DFaces {
         (volume / 6.0).abs()
     }
 
+    /// Area of face `face_index` (cell_3d_faces.rs:207-254): half the cross-product norm of each
+    /// fan triangle from the face's first vertex, summed in fan order; 0.0 past the last face or
+    /// for a face of fewer than 3 vertices.
+    pub fn face_area(&self, face_index: usize) -> f64 {
+        if face_index >= self.face_counts.len() {
+            return 0.0;
+        }
+        let off: usize = self.face_counts[..face_index].iter().map(|&c| c as usize).sum();
+        let count = self.face_counts[face_index] as usize;
+        if count < 3 {
+            return 0.0;
+        }
+        let p0 = self.vertex(self.face_indices[off]);
+        let mut area = 0.0;
+        for i in 1..count - 1 {
+            let p1 = self.vertex(self.face_indices[off + i]);
+            let p2 = self.vertex(self.face_indices[off + i + 1]);
+            let u = [p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]];
+            let w = [p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]];
+            let n = [u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]];
+            area += 0.5 * (n[0] * n[0] + n[1] * n[1] + n[2] * n[2]).sqrt();
+        }
+        area
+    }
+
+    /// The vertex indices of every face, in face order (cell_3d_faces.rs:256-269).
+    pub fn faces(&self) -> Vec<Vec<usize>> {
+        let mut out = Vec::with_capacity(self.face_counts.len());
+        let mut off: usize = 0;
+        for &count in &self.face_counts {
+            let end = off + count as usize;
+            out.push(self.face_indices[off..end].iter().map(|&i| i as usize).collect());
+            off = end;
+        }
+        out
+    }
+
     /// Centroid from the same triangulation (cell_3d_faces.rs:145-205): [0, 0, 0] when the sum of
     /// the determinants (6V) is below 1e-9 in magnitude.
     pub fn centroid(&self) -> [f64; 3] {
