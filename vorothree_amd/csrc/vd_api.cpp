@@ -1270,11 +1270,12 @@ int host_bins(Engine &E) {
 // update_generator (algo_3d_grid.rs:132-143) on the host CSR: generator i leaves bin `from` by
 // swap_remove (its slot takes the bin's last entry) and is pushed at the end of bin `to`; the
 // entries between the two bins shift by one slot.
-void host_move(Engine &E, uint32_t i, uint32_t from, uint32_t to) {
+int host_move(Engine &E, uint32_t i, uint32_t from, uint32_t to) {
     std::vector<uint32_t> &bs = E.h_bin_start, &it = E.h_items;
     const uint32_t e0 = bs[from + 1];
     uint32_t p = bs[from];
-    while (it[p] != i) ++p;   // the generator is in its bin
+    while (p < e0 && it[p] != i) ++p;
+    if (p == e0) return fail(VD_ESTATE, "generator missing from its bin list");
     it[p] = it[e0 - 1];       // slot e0 - 1 is free now
     if (to > from) {
         const uint32_t e1 = bs[to + 1];
@@ -1288,6 +1289,7 @@ void host_move(Engine &E, uint32_t i, uint32_t from, uint32_t to) {
         for (uint32_t b = to + 1; b <= from; ++b) bs[b] += 1;
     }
     E.h_gen_bin[i] = to;
+    return VD_OK;
 }
 
 // Bins kept by set_generator to the device: the CSR, the bin-ordered positions, the tube planes.
@@ -2333,7 +2335,7 @@ int vd_tess_set_generator(vd_tess *t, size_t index, const double xyz[3]) {
     }
     if (!E.custom_bins) VD_TRY(host_bins(E));
     const uint32_t to = host_bin_of(m.grid, xyz), from = E.h_gen_bin[index];
-    if (to != from) host_move(E, (uint32_t)index, from, to);   // algo_3d_grid.rs:136-142
+    if (to != from) VD_TRY(host_move(E, (uint32_t)index, from, to));   // algo_3d_grid.rs:136-142
     VD_HIP(hipMemcpyAsync(E.d_gen.as<double>() + 3 * index, xyz, 24, hipMemcpyHostToDevice, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
     E.custom_bins = E.bins_dirty = E.binned = true;
