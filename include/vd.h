@@ -142,9 +142,11 @@ int vd_tess_random_generators(vd_tess *t, size_t count, uint64_t seed);
  * from the old bin, push to the end of the new one), which changes the visiting order of later
  * calculate() calls exactly as in the reference. An index past the generators returns VD_ERANGE
  * (the reference ignores it). As after set_generators, count_cells() is 0 until the next
- * calculate(). Each call copies 24 bytes to the device and synchronises; the first one after a
- * binning reads the generators back once to rebuild the bins on the host (O(n)), and a move
- * between bins shifts the host bin lists (O(n) memmove); the bins go up once, at calculate(). */
+ * calculate(). The first call after a binning reads the generators back once and holds bins and
+ * positions on the host (O(n)); each call then changes only those (O(size of the two bins)), with
+ * no device work unless the caller's walls (add_wall_planes_per_generator) are present, whose
+ * planes for the moved generator go up at once; the next calculate() uploads bins and positions
+ * in one pass (O(n + bins)). */
 int vd_tess_set_generator(vd_tess *t, size_t index, const double xyz[3]);
 
 /* calculate(), src/tessellation.rs:213-230: every cell, results kept on the device. */

@@ -263,3 +263,54 @@ def test_import_generators_reads_the_voropp_input():
     with open(path) as fh:
         g.read_generators(fh.read())
     compare(g, o)
+
+
+def test_moves_through_the_first_and_last_bins_and_back():
+    # the per-bin lists merged into the bin offsets at calculate: moves into and out of bin 0 and
+    # the last bin, a generator moved many times, moves back to where a generator started
+    grid = (5, 5, 5)
+    g, o = make(grid)
+    rng = W.PCG32(1406)
+    pts = rng.uniform(300, (0, 0, 0), (1, 1, 1))
+    g.set_generators(pts)
+    o.set_generators(pts)
+    corners = [(0.01, 0.02, 0.03), (0.99, 0.98, 0.97)]
+    for rnd in range(3):
+        for k in range(0, 300, 7):
+            move_both(g, o, k, corners[(k + rnd) % 2])
+        for k in range(0, 300, 14):
+            move_both(g, o, k, pts[k])
+        for q in rng.uniform(40, (0, 0, 0), (1, 1, 1)):
+            move_both(g, o, 150, q)
+        assert np.array_equal(bits(g.generators()), bits(o.generators()))   # before the upload
+        compare(g, o)
+    assert differs_from_fresh(o, grid)
+
+
+def test_many_moves_are_cheap_and_match_the_oracle():
+    # 5,000 moves on 50,000 generators against the oracle, then 100,000 moves on 1M generators,
+    # which must take under 10 s: a move does no device work and shifts no whole array
+    import time
+    grid = (24, 24, 24)
+    g, o = make(grid, keep_cells=False)
+    rng = W.PCG32(1407)
+    pts = rng.uniform(50_000, (0, 0, 0), (1, 1, 1))
+    g.set_generators(pts)
+    o.set_generators(pts)
+    compare(g, o, cells=False)
+    for k, q in enumerate(rng.uniform(5000, (0, 0, 0), (1, 1, 1))):
+        move_both(g, o, (7919 * k) % 50_000, q)
+    compare(g, o, cells=False)
+    assert differs_from_fresh(o, grid)
+
+    big = Tessellation(UNIT, Algorithm3DGrid(100, 100, 100, UNIT), keep_cells=False)
+    big.set_generators(W.PCG32(1408).uniform(1_000_000, (0, 0, 0), (1, 1, 1)))
+    big.calculate()
+    targets = W.PCG32(1409).uniform(100_000, (0, 0, 0), (1, 1, 1))
+    t0 = time.perf_counter()
+    for k in range(100_000):
+        big.set_generator((104_729 * k) % 1_000_000, targets[k])
+    dt = time.perf_counter() - t0
+    big.calculate()
+    assert dt < 10.0, f"100,000 moves took {dt:.1f} s"
+    assert abs(big.volumes().sum() - 1.0) < 1e-9

@@ -22,6 +22,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -494,11 +495,16 @@ struct Engine {
     bool binned = false;               // bins of the current generators are built
     // set_generator (tessellation.rs:71-92) on a single tessellation: its bins as update_generator
     // leaves them (algo_3d_grid.rs:132-143: swap_remove from the old bin, push to the new one), not
-    // in the sorted order a binning gives; kept on the host as a CSR and uploaded at the next
-    // calculate (bins_dirty). set_generators and relax end it (they bin afresh), as does an
-    // add_wall that removes generators; one that removes none keeps it, as the reference does.
-    bool custom_bins = false, bins_dirty = false;
+    // in the sorted order a binning gives. Kept on the host: a CSR (h_bin_start, h_items) with the
+    // bins moves have changed as lists beside it (h_over), and the positions (h_gen); a move touches
+    // only those, and the next calculate merges the lists into the CSR and uploads bins and
+    // positions once (bins_dirty, pos_dirty). set_generators and relax end it (they bin afresh), as
+    // does an add_wall that removes generators; one that removes none keeps it, as the reference
+    // does.
+    bool custom_bins = false, bins_dirty = false, pos_dirty = false;
     std::vector<uint32_t> h_bin_start, h_items, h_gen_bin;
+    std::unordered_map<uint32_t, std::vector<uint32_t>> h_over;
+    std::vector<double> h_gen;
     // First-round launch sequence (binning, tier-0 launch, collect, counts readback) of a shape
     // seen twice in a row is captured into a graph and replayed while the shape repeats.
     int graphs = 1;
@@ -858,7 +864,7 @@ int host_planes_one(Engine &E, uint64_t g, const double q[3]) {
 // points every wall of their member contains, compacted in input order into E.d_gen.
 int engine_filter(Engine &E) {
     E.prepared = E.calculated = false;
-    E.custom_bins = E.bins_dirty = false;
+    E.custom_bins = E.bins_dirty = E.pos_dirty = false;
     const size_t M = E.mem.size();
     const uint64_t n = E.nraw;
     if (E.nbins + 1 >= 0xffffffffull || n >= 0xffffffffull) return fail(VD_EINVAL, "too many bins or points");
@@ -1247,11 +1253,12 @@ uint32_t host_bin_of(const vd::GridGeom &G, const double q[3]) {
 }
 
 // The bins of a single tessellation's current generators on the host, as the binning builds them
-// (algo_3d_grid.rs:111-130: every bin lists its generators in index order).
+// (algo_3d_grid.rs:111-130: every bin lists its generators in index order), and their positions.
 int host_bins(Engine &E) {
     const Member &m = E.mem[0];
     const uint64_t n = E.ngen, nb = m.nbins;
-    std::vector<double> g(3 * n);
+    std::vector<double> &g = E.h_gen;
+    g.resize(3 * n);
     if (n) VD_HIP(hipMemcpyAsync(g.data(), E.d_gen.p, n * 24, hipMemcpyDeviceToHost, E.stream));
     VD_HIP(hipStreamSynchronize(E.stream));
     E.h_gen_bin.resize(n);
@@ -1264,37 +1271,79 @@ int host_bins(Engine &E) {
     std::vector<uint32_t> fill(E.h_bin_start.begin(), E.h_bin_start.end() - 1);
     E.h_items.resize(n);
     for (uint64_t i = 0; i < n; ++i) E.h_items[fill[E.h_gen_bin[i]]++] = (uint32_t)i;
+    E.h_over.clear();
+    E.pos_dirty = false;
     return VD_OK;
 }
 
-// update_generator (algo_3d_grid.rs:132-143) on the host CSR: generator i leaves bin `from` by
-// swap_remove (its slot takes the bin's last entry) and is pushed at the end of bin `to`; the
-// entries between the two bins shift by one slot.
+// The list of bin b as moves left it: its CSR slice until a move first touches it.
+std::vector<uint32_t> &over_list(Engine &E, uint32_t b) {
+    auto ins = E.h_over.try_emplace(b);
+    if (ins.second) ins.first->second.assign(E.h_items.begin() + E.h_bin_start[b], E.h_items.begin() + E.h_bin_start[b + 1]);
+    return ins.first->second;
+}
+
+// update_generator (algo_3d_grid.rs:132-143): generator i leaves bin `from` by swap_remove (its
+// slot takes the bin's last entry) and is pushed at the end of bin `to`.
 int host_move(Engine &E, uint32_t i, uint32_t from, uint32_t to) {
-    std::vector<uint32_t> &bs = E.h_bin_start, &it = E.h_items;
-    const uint32_t e0 = bs[from + 1];
-    uint32_t p = bs[from];
-    while (p < e0 && it[p] != i) ++p;
-    if (p == e0) return fail(VD_ESTATE, "generator missing from its bin list");
-    it[p] = it[e0 - 1];       // slot e0 - 1 is free now
-    if (to > from) {
-        const uint32_t e1 = bs[to + 1];
-        std::memmove(&it[e0 - 1], &it[e0], (size_t)(e1 - e0) * 4);
-        it[e1 - 1] = i;
-        for (uint32_t b = from + 1; b <= to; ++b) bs[b] -= 1;
-    } else {
-        const uint32_t e1 = bs[to + 1];
-        std::memmove(&it[e1 + 1], &it[e1], (size_t)(e0 - 1 - e1) * 4);
-        it[e1] = i;
-        for (uint32_t b = to + 1; b <= from; ++b) bs[b] += 1;
-    }
+    std::vector<uint32_t> &src = over_list(E, from);
+    const auto it = std::find(src.begin(), src.end(), i);
+    if (it == src.end()) return fail(VD_ESTATE, "generator missing from its bin list");
+    *it = src.back();
+    src.pop_back();
+    over_list(E, to).push_back(i);
     E.h_gen_bin[i] = to;
     return VD_OK;
 }
 
-// Bins kept by set_generator to the device: the CSR, the bin-ordered positions, the tube planes.
+// The moved bins' lists merged into the CSR: bins between two moved ones keep their entries (one
+// block copy) and shift by the count change so far.
+void merge_over(Engine &E) {
+    if (E.h_over.empty()) return;
+    const std::vector<uint32_t> &bs = E.h_bin_start, &it = E.h_items;
+    const uint32_t nb = (uint32_t)(bs.size() - 1);
+    std::vector<uint32_t> keys;
+    keys.reserve(E.h_over.size());
+    for (const auto &kv : E.h_over) keys.push_back(kv.first);
+    std::sort(keys.begin(), keys.end());
+    std::vector<uint32_t> ns(bs.size()), ni(it.size());
+    int64_t delta = 0;
+    uint32_t next = 0;   // first bin not written yet
+    auto copy_run = [&](uint32_t b0, uint32_t b1) {   // unmoved bins [b0, b1)
+        for (uint32_t b = b0; b < b1; ++b) ns[b] = (uint32_t)((int64_t)bs[b] + delta);
+        if (bs[b1] > bs[b0])
+            std::memcpy(&ni[(size_t)((int64_t)bs[b0] + delta)], &it[bs[b0]], (size_t)(bs[b1] - bs[b0]) * 4);
+    };
+    for (const uint32_t k : keys) {
+        copy_run(next, k);
+        const std::vector<uint32_t> &l = E.h_over[k];
+        ns[k] = (uint32_t)((int64_t)bs[k] + delta);
+        if (!l.empty()) std::memcpy(&ni[ns[k]], l.data(), l.size() * 4);
+        delta += (int64_t)l.size() - (int64_t)(bs[k + 1] - bs[k]);
+        next = k + 1;
+    }
+    copy_run(next, nb);
+    ns[nb] = (uint32_t)((int64_t)bs[nb] + delta);   // delta is 0: moves keep the count
+    E.h_bin_start.swap(ns);
+    E.h_items.swap(ni);
+    E.h_over.clear();
+}
+
+// Positions written by set_generator to the device, in one copy.
+int flush_positions(Engine &E) {
+    if (!E.custom_bins || !E.pos_dirty) return VD_OK;
+    if (E.ngen) VD_HIP(hipMemcpyAsync(E.d_gen.p, E.h_gen.data(), E.ngen * 24, hipMemcpyHostToDevice, E.stream));
+    VD_HIP(hipStreamSynchronize(E.stream));
+    E.pos_dirty = false;
+    return VD_OK;
+}
+
+// Bins and positions kept by set_generator to the device: the CSR, the positions, the bin-ordered
+// positions and the tube planes.
 int upload_custom_bins(Engine &E) {
     const uint64_t n = E.ngen, nb = E.mem[0].nbins;
+    merge_over(E);
+    VD_TRY(flush_positions(E));
     VD_HIP(hipMemcpyAsync(E.d_bin_start.p, E.h_bin_start.data(), (nb + 1) * 4, hipMemcpyHostToDevice, E.stream));
     if (n) VD_HIP(hipMemcpyAsync(E.d_items.p, E.h_items.data(), n * 4, hipMemcpyHostToDevice, E.stream));
     VD_HIP(vd::launch_fill_spos(E.d_gen.as<double>(), E.d_items.as<uint32_t>(), n, E.d_desc.as<vd::TessDesc>(),
@@ -2163,6 +2212,7 @@ static int tess_add_wall_dev(vd_tess *t, vd::WallDev d, std::vector<double> data
     // already passed keeps them all, and re-binning an unchanged set gives the same bins), so a
     // failure below leaves the handle as it was, without the new wall.
     const uint64_t n = E.ngen;
+    VD_TRY(flush_positions(E));
     if (n) {
         DevBuf keep;
         const ScopedBufs keep_g{&keep, 1};
@@ -2336,9 +2386,8 @@ int vd_tess_set_generator(vd_tess *t, size_t index, const double xyz[3]) {
     if (!E.custom_bins) VD_TRY(host_bins(E));
     const uint32_t to = host_bin_of(m.grid, xyz), from = E.h_gen_bin[index];
     if (to != from) VD_TRY(host_move(E, (uint32_t)index, from, to));   // algo_3d_grid.rs:136-142
-    VD_HIP(hipMemcpyAsync(E.d_gen.as<double>() + 3 * index, xyz, 24, hipMemcpyHostToDevice, E.stream));
-    VD_HIP(hipStreamSynchronize(E.stream));
-    E.custom_bins = E.bins_dirty = E.binned = true;
+    std::memcpy(&E.h_gen[3 * index], xyz, 24);                          // tessellation.rs:89-91
+    E.custom_bins = E.bins_dirty = E.pos_dirty = E.binned = true;
     E.calculated = false;
     if (E.nhost) VD_TRY(host_planes_one(E, index, xyz));   // the caller's walls cut the moved generator anew
     return VD_OK;
@@ -2400,7 +2449,13 @@ size_t vd_tess_count_cells(const vd_tess *t) { return (t && t->E.calculated) ? t
 
 int vd_tess_get_generators(const vd_tess *t, double *out, size_t n) {
     if (!t || (!out && t->E.ngen)) return fail(VD_EINVAL, "NULL argument");
-    return copy_slice(t->E.d_gen, 24, t->E.mem[0], out, n);
+    const Engine &E = t->E;
+    if (E.custom_bins && E.pos_dirty) {   // moves not uploaded yet: the host copy is current
+        if (n < E.ngen) return fail(VD_EINVAL, "output buffer smaller than the cell count");
+        if (E.ngen) std::memcpy(out, E.h_gen.data(), E.ngen * 24);
+        return VD_OK;
+    }
+    return copy_slice(E.d_gen, 24, E.mem[0], out, n);
 }
 
 int vd_tess_get_volumes(const vd_tess *t, double *out, size_t n) {
