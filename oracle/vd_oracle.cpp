@@ -16,10 +16,14 @@
 // would hold 8.6e8 entries); walks over it visit exactly the entries a walk over the whole table
 // visits.
 //
+// A second one, in random_generators: the reference draws from a StdRng seeded by thread_rng, so
+// no run of it can be reproduced; the candidates here are the SplitMix64 stream of a given seed.
+//
 // Pinned by: the Voro++ fixtures (vertex counts and positions, face counts), closed forms (box,
 // lattices, plane-clipped boxes), a brute-force all-pairs clipping on small inputs, topology
-// invariants (reciprocity, Euler characteristic), and the grid unit-test values of the reference.
-// See tests/test_oracle.py.
+// invariants (reciprocity, Euler characteristic), the grid unit-test values of the reference, the
+// bin orders Vec::swap_remove and Vec::push fix for set_generator, and SplitMix64's published
+// outputs with a direct rejection check for random_generators. See tests/test_oracle.py.
 
 #include <algorithm>
 #include <atomic>
