@@ -102,8 +102,9 @@ int vd_tess_add_wall(vd_tess *t, const vd_wall *wall);
  *   planar: WallGeometry::is_planar (calculate_sealed and prune_boundaries treat non-planar walls
  *     as curved).
  * The callbacks run on the calling thread during vd_tess_add_wall_planes_per_generator,
- * vd_tess_set_generators(_device) and vd_tess_relax: contains for every input point, cut for every
- * kept generator. ctx must stay valid while the handle holds the wall. */
+ * vd_tess_set_generators(_device), vd_tess_random_generators and vd_tess_relax (contains for every
+ * input point or candidate, cut for every kept generator) and vd_tess_set_generator (both for the
+ * moved generator). ctx must stay valid while the handle holds the wall. */
 typedef struct vd_wall_host {
     void *ctx;
     int (*contains)(void *ctx, const double p[3]);
