@@ -1083,19 +1083,31 @@ constexpr int kPretestAfter = VD_PRETEST_AFTER;
 #endif
 constexpr int kPretestMinDiv = VD_PRETEST_MIN_DIV;
 
-// Target bin of shell offset (dx, dy, dz) for a generator in bin (ix, iy, iz) at relative position
-// (rx, ry, rz): false if outside the grid, else its index and the squared distance from the
-// generator to the bin (algo_3d_grid.rs:175-190).
-__device__ inline bool entry_bin(int dx, int dy, int dz, const GridGeom &G, int ix, int iy, int iz, double rx,
-                                 double ry, double rz, uint32_t &b, double &bdist) {
+// Target bin of shell offset (dx, dy, dz) for a generator in bin (ix, iy, iz): false if outside
+// the grid, else its index (algo_3d_grid.rs:175-180,192).
+__device__ inline bool entry_bin_index(int dx, int dy, int dz, const GridGeom &G, int ix, int iy, int iz,
+                                       uint32_t &b) {
     const int bx = ix + dx, by = iy + dy, bz = iz + dz;
     if (!(bx >= 0 && bx < G.n[0] && by >= 0 && by < G.n[1] && bz >= 0 && bz < G.n[2])) return false;
     b = (uint32_t)bx + (uint32_t)by * (uint32_t)G.n[0] + (uint32_t)bz * (uint32_t)G.n[0] * (uint32_t)G.n[1];
+    return true;
+}
+
+// Squared distance from a generator at relative position (rx, ry, rz) in its bin to the bin at
+// offset (dx, dy, dz) (algo_3d_grid.rs:182-190).
+__device__ inline double entry_bin_dist(int dx, int dy, int dz, const GridGeom &G, double rx, double ry, double rz) {
     double ex = dx > 0 ? ((double)dx - rx) * G.cs[0] : dx < 0 ? ((double)(-(dx + 1)) + rx) * G.cs[0] : 0.0;
     double ey = dy > 0 ? ((double)dy - ry) * G.cs[1] : dy < 0 ? ((double)(-(dy + 1)) + ry) * G.cs[1] : 0.0;
     double ez = dz > 0 ? ((double)dz - rz) * G.cs[2] : dz < 0 ? ((double)(-(dz + 1)) + rz) * G.cs[2] : 0.0;
     ex = fmax(ex, 0.0); ey = fmax(ey, 0.0); ez = fmax(ez, 0.0);
-    bdist = ex * ex + ey * ey + ez * ez;
+    return ex * ex + ey * ey + ez * ez;
+}
+
+// Both: false if outside the grid, else the bin's index and distance.
+__device__ inline bool entry_bin(int dx, int dy, int dz, const GridGeom &G, int ix, int iy, int iz, double rx,
+                                 double ry, double rz, uint32_t &b, double &bdist) {
+    if (!entry_bin_index(dx, dy, dz, G, ix, iy, iz, b)) return false;
+    bdist = entry_bin_dist(dx, dy, dz, G, rx, ry, rz);
     return true;
 }
 
@@ -1264,12 +1276,12 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                     binr = false;
                     bc = 0;
                     bdist = 0.0;
+                    int dx = 0, dy = 0, dz = 0;
                     if (!bend) {
-                        int dx, dy, dz;
                         shell_unpack(shell[e], pk, dx, dy, dz);
                         bd2 = shell_key(dx, dy, dz, G.cs);
                         uint32_t b = 0;
-                        binr = entry_bin(dx, dy, dz, G, ix, iy, iz, rx, ry, rz, b, bdist);
+                        binr = entry_bin_index(dx, dy, dz, G, ix, iy, iz, b);
                         if (binr) {
                             bs = bin_start[b];
                             bc = bin_start[b + 1] - bs;
@@ -1277,6 +1289,9 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                     } else {
                         bd2 = (e == nshell) ? shell_next : DBL_MAX;
                     }
+                    // the bin distance decides only whether a bin's generators are taken: empty
+                    // bins (most of a batch where generators are sparse) skip it
+                    if (bc) bdist = entry_bin_dist(dx, dy, dz, G, rx, ry, rz);
                     load = false;
                 }
                 // gather the next window of candidates under the current radius
