@@ -20,7 +20,8 @@
 //   sface[S]         per face slot: face number
 //   fstart[F+1]      first slot of each face (fstart[F] = S); fnb[F] neighbour id per face
 // "Slots" are the flattened face_indices of the reference; sedge carries the successor so a clip
-// reads each edge with one LDS access. While V <= 64, lane v also keeps vertex v in registers.
+// reads each edge with one LDS access. In a fast clip (V <= 64 NV), lane v reads vertex v (and
+// v + 64, ...) into registers once, at the clip's start.
 //
 // Parallel clip (one plane): lanes over vertices compute the signed distances and the kept-vertex
 // numbering (ballot prefix counts); lanes over slots classify each edge, prefix-sum the output
@@ -346,28 +347,12 @@ __device__ inline uint64_t arena_take(const CellArgs &A, ArenaChunks &ac, int k,
     return o;
 }
 
-// Lane l's register copies of vertices l, 64 + l, ... (NV per lane; the fast tiers only).
+// Lane l's copies of vertices l, 64 + l, ... during a fast clip (NV per lane).
 template <int NV>
 struct VRegN { double x[NV], y[NV], z[NV]; };
 
 template <class C>
 using VReg = VRegN<C::NV>;
-
-template <class C>
-__device__ inline void vreg_load(Mem<C> &m, const Poly &P, VReg<C> &r) {
-    if constexpr (C::FAST) {
-        const int l = lane_id_here();
-#pragma unroll
-        for (int k = 0; k < C::NV; ++k) {
-            const int v = 64 * k + l;
-            if (v < P.V) {
-                r.x[k] = m.vx[0][v];   // single buffer (static_assert in clip_fast)
-                r.y[k] = m.vy[0][v];
-                r.z[k] = m.vz[0][v];
-            }
-        }
-    }
-}
 
 // A vertex set as NV 64-bit ballots (vertex v in word v / 64, bit v % 64).
 template <int NV>
@@ -666,14 +651,14 @@ __device__ int clip_generic(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, boo
 }
 
 
-// Fast clip for V <= 64 * NV, S <= 192 * NV, F <= 64: vertex distances from the register copies
-// (NV vertices per lane), slot data of the clip held in registers (3 * NV chunks of 64 slots),
-// crossings and lid cycle handled with lane shuffles. Same result as clip_generic. Returns kGeneric
+// Fast clip for V <= 64 * NV, S <= 192 * NV, F <= 64: vertex distances from register copies read
+// at its start (NV vertices per lane), slot data of the clip held in registers (3 * NV chunks of
+// 64 slots), crossings and lid cycle handled with lane shuffles. Same result as clip_generic. Returns kGeneric
 // (with the current polytope untouched) when the faces, crossings or the lid do not fit its
 // register layout.
 template <class C>
-__device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t nid, bool withgen,
-                         double gx, double gy, double gz, double &r2) {
+__device__ int clip_fast(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, bool withgen, double gx, double gy,
+                         double gz, double &r2) {
     constexpr int NV = C::NV;
     constexpr int NCH = 3 * NV;                 // slot chunks
     constexpr uint32_t SH = NV == 1 ? 6 : 7;    // bits of a vertex index in the packed slot word
@@ -687,11 +672,21 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     const int c = 0, o = 0;
     if (NV > 1 && P.F > 64) return kGeneric;
 
-    // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362). Lanes past
-    //    the polytope compute a distance from stale register copies; every ballot is masked with
-    //    the polytope's lanes (a scalar mask), so those lanes take no part. Ballots are of single
+    // 1. signed distances (:303-315), early outs (:317-326), kept vertices (:345-362). Each lane
+    //    reads its vertices from LDS in whole 64-lane chunks of the VM-entry arrays (read here
+    //    rather than kept in registers across the walk, which frees registers there). Lanes past
+    //    the polytope compute a distance from stale entries; every ballot is masked with the
+    //    polytope's lanes (a scalar mask), so those lanes take no part. Ballots are of single
     //    compares, combined with scalar operations (a ballot of a combined predicate costs two more
     //    vector instructions).
+    static_assert(64 * NV <= C::VM, "whole vertex chunks stay inside the vertex arrays");
+    VReg<C> r;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        r.x[k] = m.vx[0][64 * k + l];
+        r.y[k] = m.vy[0][64 * k + l];
+        r.z[k] = m.vz[0][64 * k + l];
+    }
     double d[NV];
     VMask<NV> gtm, ltm, inm, vam;
     bool anyg = false, anyl = false;
@@ -962,16 +957,15 @@ __device__ int clip_fast(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t
     P.S = Spre;
     P.nk = nk;
     r2 = withgen ? wave_max(md2) : 0.0;
-    vreg_load(m, P, r);
     return kModified;
 }
 
 template <class C>
-__device__ inline int clip_any(Mem<C> &m, Poly &P, VReg<C> &r, const Plane pl, int32_t nid, bool withgen,
-                               double gx, double gy, double gz, double &r2) {
+__device__ inline int clip_any(Mem<C> &m, Poly &P, const Plane pl, int32_t nid, bool withgen, double gx, double gy,
+                               double gz, double &r2) {
     if constexpr (C::FAST) {
         // a polytope that leaves the register layout moves the cell to a generic tier
-        const int rc = clip_fast<C>(m, P, r, pl, nid, withgen, gx, gy, gz, r2);
+        const int rc = clip_fast<C>(m, P, pl, nid, withgen, gx, gy, gz, r2);
         return rc == kGeneric ? kOverflow : rc;
     } else {
         return clip_generic<C>(m, P, pl, nid, withgen, gx, gy, gz, r2);
@@ -1176,8 +1170,6 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
     const int nwalls = D.nwalls;
     Poly P;
     poly_box<C>(m, P, D.box_lo, D.box_hi);
-    VReg<C> r{};   // lanes past the polytope keep defined (stale) values
-    vreg_load(m, P, r);
     int status = kCellOk;
     double r2 = 0.0;
 
@@ -1231,7 +1223,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
             }
             if (!has) break;
             double wr2 = 0.0;
-            const int rc = clip_any<C>(m, P, r, Plane{q.px, q.py, q.pz, q.nx, q.ny, q.nz}, W.id, false,
+            const int rc = clip_any<C>(m, P, Plane{q.px, q.py, q.pz, q.nx, q.ny, q.nz}, W.id, false,
                                        gx, gy, gz, wr2);
             if (rc < 0) {
                 status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
@@ -1378,7 +1370,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
         }
 
         double nr2 = 0.0;
-        const int rc = clip_any<C>(m, P, r, pl, nid, withgen, gx, gy, gz, nr2);   // tessellation.rs:462-463
+        const int rc = clip_any<C>(m, P, pl, nid, withgen, gx, gy, gz, nr2);   // tessellation.rs:462-463
         if (rc < 0) {
             status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
             break;
@@ -1461,7 +1453,7 @@ __device__ void process_cell(Mem<C> &m, const CellArgs &A, uint32_t i, uint32_t 
                     dot += (gz - q.pz) * q.nz;
                     if (!(dot <= 1e-9)) continue;
                     double wr2 = 0.0;
-                    const int rc = clip_any<C>(m, P, r, Plane{q.px, q.py, q.pz, q.nx, q.ny, q.nz}, W.id,
+                    const int rc = clip_any<C>(m, P, Plane{q.px, q.py, q.pz, q.nx, q.ny, q.nz}, W.id,
                                                false, gx, gy, gz, wr2);
                     if (rc < 0) {
                         status = (rc == kFace255) ? kCellFace255 : kCellCapacity;
