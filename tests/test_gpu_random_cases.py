@@ -93,3 +93,35 @@ def test_random_case_bit_exact(seed):
         g.relax()
         o.relax()
         compare(g, o)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_batch_bit_exact(seed):
+    # 1 to 8 random cases as members of one batch (walls given with the points), each member
+    # against the oracle run on it alone, then one batch relax step
+    from vorothree_amd import Batch
+    rng = np.random.default_rng(5000 + seed)
+    cases = [random_case(int(s)) for s in rng.integers(0, 10**6, int(rng.integers(1, 9)))]
+    b = Batch(keep_cells=True)
+    oracles = []
+    for lo, hi, grid, pts, walls, _ in cases:
+        b.add(BoundingBox(lo, hi), grid, walls, pts)
+        o = OracleTess(lo, hi, *grid)
+        for w in walls:
+            oracle_wall(o, w)
+        o.set_generators(pts)
+        oracles.append(o)
+    for step in range(2):
+        b.calculate()
+        for t, o in enumerate(oracles):
+            o.calculate(threads=8)
+            vol, cen, nf, _ = o.reduced()
+            assert np.array_equal(bits(b.generators(t)), bits(o.generators())), (step, t)
+            assert np.array_equal(bits(b.volumes(t)), bits(vol)), (step, t)
+            assert np.array_equal(bits(b.centroids(t)), bits(cen)), (step, t)
+            assert np.array_equal(b.face_counts(t), nf), (step, t)
+            for i in range(o.count_cells()):
+                assert_same_cell(b.get_cell(t, i), o.get_cell(i), (step, t, i))
+        b.relax()
+        for o in oracles:
+            o.relax()
