@@ -560,7 +560,20 @@ def main():
     assert np.array_equal(tg.volumes().view(np.uint64), t.volumes().view(np.uint64))
     tg.close()
 
-    progress(f"plain: {value:.4g} cells/s, graph replay {graph_rate:.4g} cells/s")
+    # the same steps with the points handed over in host memory (PCIe upload inside the step):
+    # reported beside the headline, never as it
+    for _ in range(max(1, a.warmup)):
+        t.set_generators(w.points)
+        t.calculate()
+    _hip.synchronize()
+    h0 = time.perf_counter()
+    for _ in range(a.steps):
+        t.set_generators(w.points)
+        t.calculate()
+    _hip.synchronize()
+    host_rate = t.count_cells() / ((time.perf_counter() - h0) / a.steps)
+
+    progress(f"plain: {value:.4g} cells/s, graph replay {graph_rate:.4g} cells/s, host upload {host_rate:.4g} cells/s")
     # algorithmic work of one cell launch, counted by the oracle over the same cells
     cpu = None
     flops_per_cell = None
@@ -658,6 +671,7 @@ def main():
                                      if launch else "the library defaults"),
                    "cells_retried_large_tier": st[1], "shell_prefix_entries": st[3],
                    "graph_replay_cells_per_s": graph_rate, "graph_replays": graph_replays,
+                   "host_upload_cells_per_s": host_rate,
                    "volume_sum": float(vol.sum())},
         "roofline": roof,
         "cpu_baseline": cpu,
