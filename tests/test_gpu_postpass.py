@@ -89,3 +89,29 @@ def test_prune_needs_full_cells():
     with pytest.raises(_lib.VdError) as e:
         t.prune_boundaries()
     assert e.value.code == _lib.VD_ESTATE
+
+
+@pytest.mark.parametrize("name", ["sphere", "trefoil"])
+def test_post_passes_after_moves(name):
+    # set_generator moves (bins in update_generator's order), then calculate_sealed and
+    # prune_boundaries: the sealed cells, both logs and the prune positions against the oracle
+    t, o = pair(name, n=6000)
+    n = o.count_generators()
+    for k, q in enumerate(W.PCG32(1020).uniform(800, (0, 0, 0), (1, 1, 1))):
+        o.set_generator((37 * k) % n, q)
+        t.set_generator((37 * k) % n, q)
+    t.calculate_sealed()
+    o.calculate_sealed(threads=8)
+    vol, cen, nf, _ = o.reduced()
+    assert np.array_equal(bits(t.generators()), bits(o.generators()))
+    assert np.array_equal(bits(t.volumes()), bits(vol))
+    assert np.array_equal(bits(t.centroids()), bits(cen))
+    assert np.array_equal(t.seal_log(), o.seal_log()) and len(o.seal_log()) > 0
+    for i in range(0, o.count_cells(), 29):
+        assert_same_cell(t.get_cell(i), o.get_cell(i), i)
+    t.prune_boundaries()
+    o.prune_boundaries()
+    ti, tp = t.prune_log()
+    oi, op = o.prune_log()
+    assert np.array_equal(ti, oi) and len(oi) > 0
+    assert np.array_equal(bits(tp.reshape(-1)), bits(op.reshape(-1)))
