@@ -120,7 +120,7 @@ def test_full_config4_batch_members_bit_exact():
 
 
 def test_scale_100m_uniform_one_tessellation():
-    # one tessellation of 100M uniform generators with full cells kept (about 165 GB of device
+    # one tessellation of 100M uniform generators with full cells kept (about 150 GB of device
     # memory: the vertex arena alone holds 4.0e9 entries, 96 GB): entry offsets past 2^31 and
     # byte offsets past 2^32 in every arena. The generators come from random_generators on the
     # device; the oracle computes a sample of cells spread over the index range (the last ones
@@ -130,6 +130,13 @@ def test_scale_100m_uniform_one_tessellation():
     b = BoundingBox((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
     grid = grid_for(b, n, 1.0)
     g = Tessellation(b, Algorithm3DGrid(*grid, b), keep_cells=True)
+    try:
+        scale_checks(g, n, grid)
+    finally:
+        g.close()   # 150 GB of device memory: released even when a check fails
+
+
+def scale_checks(g, n, grid):
     g.random_generators(n, seed=2026)
     assert g.count_generators() == n
     g.calculate()
